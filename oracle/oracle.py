@@ -1,0 +1,295 @@
+"""ctypes wrapper over oracle/build/liboracle.so -- TEST INFRASTRUCTURE ONLY.
+
+The oracle is the CPU restatement of ferrolearn's f64 squared-Euclidean
+path (ref_cpu.c).  Only tests/, the smoke check and bench.py's CPU-baseline
+leg may import this module; the product never does.
+
+Every function takes numpy arrays; f32 inputs are widened to f64, which is
+exact, exactly as the reference's f64 path sees f32 data converted to f64.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "build", "liboracle.so")
+_lib = None
+
+_i64 = C.c_int64
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int64)
+
+
+def build() -> None:
+    """Compile ref_cpu.c with the committed Makefile."""
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        L.or_sqeuclid.restype = C.c_double
+        L.or_sqeuclid.argtypes = [_dp, _dp, _i64]
+        L.or_euclid.restype = C.c_double
+        L.or_euclid.argtypes = [_dp, _dp, _i64]
+        L.or_nearest_center.restype = _i64
+        L.or_nearest_center.argtypes = [_dp, _dp, _i64, _i64, _dp]
+        L.or_assign.restype = C.c_double
+        L.or_assign.argtypes = [_dp, _i64, _i64, _dp, _i64, _ip, C.c_int]
+        L.or_assign_rows.restype = None
+        L.or_assign_rows.argtypes = [_dp, _i64, _i64, _dp, _i64, _ip, _dp, C.c_int]
+        L.or_recompute_centroids.restype = C.c_double
+        L.or_recompute_centroids.argtypes = [_dp, _i64, _i64, _ip, _dp, _i64, _dp, _dp]
+        L.or_kmeans_lloyd.restype = None
+        L.or_kmeans_lloyd.argtypes = [_dp, _i64, _i64, _i64, _dp, _i64, C.c_double,
+                                      C.c_int, _dp, _ip, _dp, _ip]
+        L.or_kmeans_fit_from.restype = _i64
+        L.or_kmeans_fit_from.argtypes = [_dp, _i64, _i64, _i64, _dp, _i64, _i64,
+                                         C.c_double, C.c_int, _dp, _ip, _dp, _ip]
+        L.or_transform.restype = None
+        L.or_transform.argtypes = [_dp, _i64, _i64, _dp, _i64, _dp, C.c_int]
+        L.or_rng_stream.restype = None
+        L.or_rng_stream.argtypes = [C.c_uint64, _i64, C.POINTER(C.c_uint64)]
+        L.or_blocked_sum.restype = C.c_double
+        L.or_blocked_sum.argtypes = [_dp, _i64]
+        L.or_blocked_search.restype = _i64
+        L.or_blocked_search.argtypes = [_dp, _i64, C.c_double]
+        L.or_kmeanspp.restype = None
+        L.or_kmeanspp.argtypes = [_dp, _i64, _i64, _i64, C.c_uint64, _dp, C.c_int]
+        L.or_minibatch_update.restype = C.c_double
+        L.or_minibatch_update.argtypes = [_dp, _i64, _ip, _ip, _i64, _dp, _i64, _ip]
+        L.or_minibatch_run.restype = None
+        L.or_minibatch_run.argtypes = [_dp, _i64, _i64, _i64, _dp, _i64, _i64,
+                                       C.c_double, C.c_uint64, C.c_int, _dp, _ip, _dp, _ip]
+        L.or_knn_query.restype = _i64
+        L.or_knn_query.argtypes = [_dp, _i64, _i64, _dp, _i64, _ip, _dp]
+        L.or_kneighbors.restype = None
+        L.or_kneighbors.argtypes = [_dp, _i64, _i64, _dp, _i64, _i64, _ip, _dp, C.c_int]
+        L.or_class_scores.restype = None
+        L.or_class_scores.argtypes = [_ip, _dp, _i64, _ip, _i64, C.c_int, _dp]
+        L.or_vote.restype = _i64
+        L.or_vote.argtypes = [_dp, _i64]
+        L.or_proba.restype = None
+        L.or_proba.argtypes = [_dp, _i64, _dp]
+        L.or_weighted_mean.restype = C.c_double
+        L.or_weighted_mean.argtypes = [_ip, _dp, _i64, _dp, C.c_int]
+        L.or_knn_classify.restype = None
+        L.or_knn_classify.argtypes = [_dp, _i64, _i64, _ip, _i64, _dp, _i64, _i64,
+                                      C.c_int, _ip, _dp, C.c_int]
+        L.or_knn_regress.restype = None
+        L.or_knn_regress.argtypes = [_dp, _i64, _i64, _dp, _dp, _i64, _i64, C.c_int,
+                                     _dp, C.c_int]
+        L.or_max_threads.restype = C.c_int
+        L.or_max_threads.argtypes = []
+        _lib = L
+    return _lib
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+def _i64a(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.int64))
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _pi(a):
+    return a.ctypes.data_as(_ip)
+
+
+def max_threads() -> int:
+    return int(lib().or_max_threads())
+
+
+def sqeuclid(a, b) -> float:
+    a, b = _f64(a), _f64(b)
+    return float(lib().or_sqeuclid(_p(a), _p(b), a.shape[0]))
+
+
+def euclid(a, b) -> float:
+    a, b = _f64(a), _f64(b)
+    return float(lib().or_euclid(_p(a), _p(b), a.shape[0]))
+
+
+def nearest_center(row, centers):
+    row, centers = _f64(row), _f64(centers)
+    dist = C.c_double()
+    lab = lib().or_nearest_center(_p(row), _p(centers), centers.shape[0],
+                                  centers.shape[1], C.byref(dist))
+    return int(lab), float(dist.value)
+
+
+def assign(X, centers, nthreads=0):
+    X, centers = _f64(X), _f64(centers)
+    n, d = X.shape
+    labels = np.zeros(n, np.int64)
+    inertia = lib().or_assign(_p(X), n, d, _p(centers), centers.shape[0], _pi(labels), nthreads)
+    return labels, float(inertia)
+
+
+def assign_rows(X, centers, nthreads=0):
+    X, centers = _f64(X), _f64(centers)
+    n, d = X.shape
+    labels = np.zeros(n, np.int64)
+    dist = np.zeros(n, np.float64)
+    lib().or_assign_rows(_p(X), n, d, _p(centers), centers.shape[0], _pi(labels), _p(dist),
+                         nthreads)
+    return labels, dist
+
+
+def recompute_centroids(X, labels, old_centers):
+    X, labels, old = _f64(X), _i64a(labels), _f64(old_centers)
+    n, d = X.shape
+    k = old.shape[0]
+    new = np.zeros((k, d), np.float64)
+    counts = np.zeros(k, np.float64)
+    shift = lib().or_recompute_centroids(_p(X), n, d, _pi(labels), _p(old), k, _p(new),
+                                         _p(counts))
+    return new, counts, float(shift)
+
+
+def kmeans_lloyd(X, init, max_iter, tol, nthreads=0):
+    X, init = _f64(X), _f64(init)
+    n, d = X.shape
+    k = init.shape[0]
+    centers = np.zeros((k, d), np.float64)
+    labels = np.zeros(n, np.int64)
+    inertia = C.c_double()
+    n_iter = C.c_int64()
+    lib().or_kmeans_lloyd(_p(X), n, d, k, _p(init), max_iter, tol, nthreads, _p(centers),
+                          _pi(labels), C.byref(inertia), C.byref(n_iter))
+    return centers, labels, float(inertia.value), int(n_iter.value)
+
+
+def kmeans_fit_from(X, inits, max_iter, tol, nthreads=0):
+    """inits: (n_init, k, d)."""
+    X, inits = _f64(X), _f64(inits)
+    n, d = X.shape
+    n_init, k, _ = inits.shape
+    centers = np.zeros((k, d), np.float64)
+    labels = np.zeros(n, np.int64)
+    inertia = C.c_double()
+    n_iter = C.c_int64()
+    best = lib().or_kmeans_fit_from(_p(X), n, d, k, _p(inits), n_init, max_iter, tol,
+                                    nthreads, _p(centers), _pi(labels), C.byref(inertia),
+                                    C.byref(n_iter))
+    return centers, labels, float(inertia.value), int(n_iter.value), int(best)
+
+
+def transform(X, centers, nthreads=0):
+    X, centers = _f64(X), _f64(centers)
+    n, d = X.shape
+    k = centers.shape[0]
+    out = np.zeros((n, k), np.float64)
+    lib().or_transform(_p(X), n, d, _p(centers), k, _p(out), nthreads)
+    return out
+
+
+def rng_stream(seed, count):
+    out = np.zeros(count, np.uint64)
+    lib().or_rng_stream(C.c_uint64(seed), count, out.ctypes.data_as(C.POINTER(C.c_uint64)))
+    return out
+
+
+def blocked_sum(v) -> float:
+    v = _f64(v)
+    return float(lib().or_blocked_sum(_p(v), v.shape[0]))
+
+
+def blocked_search(v, threshold) -> int:
+    v = _f64(v)
+    return int(lib().or_blocked_search(_p(v), v.shape[0], threshold))
+
+
+def kmeanspp(X, k, seed, nthreads=0):
+    X = _f64(X)
+    n, d = X.shape
+    out = np.zeros((k, d), np.float64)
+    lib().or_kmeanspp(_p(X), n, d, k, C.c_uint64(seed), _p(out), nthreads)
+    return out
+
+
+def minibatch_update(X, batch_idx, batch_lab, centers, counts):
+    X = _f64(X)
+    bi, bl = _i64a(batch_idx), _i64a(batch_lab)
+    C_ = _f64(centers).copy()
+    cnt = _i64a(counts).copy()
+    shift = lib().or_minibatch_update(_p(X), X.shape[1], _pi(bi), _pi(bl), bi.shape[0], _p(C_),
+                                      C_.shape[0], _pi(cnt))
+    return C_, cnt, float(shift)
+
+
+def minibatch_run(X, init, batch, max_iter, tol, seed, nthreads=0):
+    X, init = _f64(X), _f64(init)
+    n, d = X.shape
+    k = init.shape[0]
+    centers = np.zeros((k, d), np.float64)
+    labels = np.zeros(n, np.int64)
+    inertia = C.c_double()
+    n_iter = C.c_int64()
+    lib().or_minibatch_run(_p(X), n, d, k, _p(init), batch, max_iter, tol, C.c_uint64(seed),
+                           nthreads, _p(centers), _pi(labels), C.byref(inertia),
+                           C.byref(n_iter))
+    return centers, labels, float(inertia.value), int(n_iter.value)
+
+
+def knn_query(data, q, k):
+    data, q = _f64(data), _f64(q)
+    N, d = data.shape
+    kk = min(k, N)
+    idx = np.zeros(max(kk, 1), np.int64)
+    dist = np.zeros(max(kk, 1), np.float64)
+    got = lib().or_knn_query(_p(data), N, d, _p(q), k, _pi(idx), _p(dist))
+    return idx[:got], dist[:got]
+
+
+def kneighbors(data, Q, k, nthreads=0):
+    data, Q = _f64(data), _f64(Q)
+    N, d = data.shape
+    nq = Q.shape[0]
+    idx = np.zeros((nq, k), np.int64)
+    dist = np.zeros((nq, k), np.float64)
+    lib().or_kneighbors(_p(data), N, d, _p(Q), nq, k, _pi(idx), _p(dist), nthreads)
+    return dist, idx
+
+
+def class_positions(y):
+    """Sorted unique classes and the position of each label among them
+    (knn.rs:304-306: sort_unstable + dedup)."""
+    y = _i64a(y)
+    classes = np.unique(y)
+    return classes, np.searchsorted(classes, y).astype(np.int64)
+
+
+def knn_classify(data, y, Q, k, weights=0, nthreads=0, proba=False):
+    data, Q = _f64(data), _f64(Q)
+    classes, ypos = class_positions(y)
+    N, d = data.shape
+    nq = Q.shape[0]
+    pred = np.zeros(nq, np.int64)
+    pr = np.zeros((nq, classes.shape[0]), np.float64) if proba else None
+    lib().or_knn_classify(_p(data), N, d, _pi(ypos), classes.shape[0], _p(Q), nq, k, weights,
+                          _pi(pred), _p(pr) if proba else None, nthreads)
+    if proba:
+        return classes[pred], pr
+    return classes[pred]
+
+
+def knn_regress(data, y, Q, k, weights=0, nthreads=0):
+    data, Q, y = _f64(data), _f64(Q), _f64(y)
+    N, d = data.shape
+    nq = Q.shape[0]
+    pred = np.zeros(nq, np.float64)
+    lib().or_knn_regress(_p(data), N, d, _p(y), _p(Q), nq, k, weights, _p(pred), nthreads)
+    return pred

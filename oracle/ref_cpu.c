@@ -1,0 +1,662 @@
+/*
+ * ref_cpu.c -- CPU restatement of ferrolearn's squared-Euclidean hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the oracle: tests/, the smoke check
+ * and bench.py's CPU-baseline leg load it to check and to time the reference
+ * algorithm.  The product (ferrolearn_amd/libferrolearn_hip.so) never links,
+ * loads or calls it.
+ *
+ * Arithmetic contract (Rust semantics of the reference's f64 path):
+ *   - every + - * / sqrt is one IEEE-754 binary64 op, round-to-nearest;
+ *   - no FMA contraction, no reassociation (built with -O2 -ffp-contract=off,
+ *     no -ffast-math);
+ *   - folds run left to right from +0.0, exactly as Iterator::fold does.
+ *
+ * Each function cites the reference file:line it restates.  Where the
+ * reference's behaviour is implementation-defined (rayon's reduction tree,
+ * select_nth_unstable's tie order, rand's StdRng stream) this file picks a
+ * definition and says so; those parts are "parity unpinned" against the real
+ * reference and pinned only against this definition.
+ */
+#include <float.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* a1: distance fold                                                         */
+/* ------------------------------------------------------------------------ */
+
+/* ferrolearn-cluster/src/kmeans.rs:159-163 (squared_euclidean) and its twin
+ * mini_batch_kmeans.rs:216-220: acc = acc + (a_j - b_j) * (a_j - b_j). */
+double or_sqeuclid(const double *a, const double *b, int64_t d) {
+    double acc = 0.0;
+    for (int64_t j = 0; j < d; ++j) {
+        double t = a[j] - b[j];
+        acc = acc + t * t;
+    }
+    return acc;
+}
+
+/* ferrolearn-neighbors/src/kdtree.rs:342-348 (euclidean_distance): the same
+ * fold (map then fold from zero) followed by a correctly rounded sqrt. */
+double or_euclid(const double *a, const double *b, int64_t d) {
+    return sqrt(or_sqeuclid(a, b, d));
+}
+
+/* ------------------------------------------------------------------------ */
+/* a2/a3: nearest centre, assignment, inertia                               */
+/* ------------------------------------------------------------------------ */
+
+/* kmeans.rs:295-309 (nearest_center): init (0, F::MAX); strict `<` over
+ * centres in index order, so the lowest index wins an exact tie and a NaN
+ * distance never wins. */
+int64_t or_nearest_center(const double *row, const double *C, int64_t k,
+                          int64_t d, double *dist_out) {
+    int64_t best = 0;
+    double bd = DBL_MAX;
+    for (int64_t c = 0; c < k; ++c) {
+        double v = or_sqeuclid(row, C + c * d, d);
+        if (v < bd) {
+            bd = v;
+            best = c;
+        }
+    }
+    if (dist_out) *dist_out = bd;
+    return best;
+}
+
+static int or_threads(int nthreads) {
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#else
+    nthreads = 1;
+#endif
+    return nthreads < 1 ? 1 : nthreads;
+}
+
+/* kmeans.rs:267-352 (assign_clusters_into / assign_serial / assign_parallel).
+ * Serial below n*d < 100000 (PARALLEL_WORK_THRESHOLD, :261).  Otherwise rows
+ * are cut into chunks of max(n/T, 64) (:333), each chunk folds its own
+ * inertia sequentially, and the chunk sums are added in chunk order.  rayon
+ * combines chunk sums with an unspecified tree (:351), so the last bits of
+ * the reference's inertia are not reproducible; labels are. */
+double or_assign(const double *X, int64_t n, int64_t d, const double *C,
+                 int64_t k, int64_t *labels, int nthreads) {
+    nthreads = or_threads(nthreads);
+    if (n * d < 100000) {
+        double inertia = 0.0;
+        for (int64_t i = 0; i < n; ++i) {
+            double dist;
+            labels[i] = or_nearest_center(X + i * d, C, k, d, &dist);
+            inertia = inertia + dist;
+        }
+        return inertia;
+    }
+    int64_t chunk = n / nthreads;
+    if (chunk < 64) chunk = 64;
+    int64_t nchunks = (n + chunk - 1) / chunk;
+    double *part = (double *)calloc((size_t)nchunks, sizeof(double));
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+    for (int64_t ch = 0; ch < nchunks; ++ch) {
+        int64_t lo = ch * chunk, hi = lo + chunk < n ? lo + chunk : n;
+        double local = 0.0;
+        for (int64_t i = lo; i < hi; ++i) {
+            double dist;
+            labels[i] = or_nearest_center(X + i * d, C, k, d, &dist);
+            local = local + dist;
+        }
+        part[ch] = local;
+    }
+    double inertia = 0.0;
+    for (int64_t ch = 0; ch < nchunks; ++ch) inertia = inertia + part[ch];
+    free(part);
+    return inertia;
+}
+
+/* Per-row minimum squared distance (the value nearest_center returns), for
+ * exact-sum inertia checks. */
+void or_assign_rows(const double *X, int64_t n, int64_t d, const double *C,
+                    int64_t k, int64_t *labels, double *dist, int nthreads) {
+    nthreads = or_threads(nthreads);
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+    for (int64_t i = 0; i < n; ++i)
+        labels[i] = or_nearest_center(X + i * d, C, k, d, &dist[i]);
+}
+
+/* ------------------------------------------------------------------------ */
+/* a4: centroid update                                                       */
+/* ------------------------------------------------------------------------ */
+
+/* kmeans.rs:358-401 (recompute_centroids_into): zero sums and counts; for
+ * rows in ascending index order count[l] += 1 and sum[l][j] += x[i][j];
+ * divide by the count, or keep the old centre when the count is 0; return
+ * sqrt(max_c fold(new_c - old_c)). */
+double or_recompute_centroids(const double *X, int64_t n, int64_t d,
+                              const int64_t *labels, const double *old_c,
+                              int64_t k, double *new_c, double *counts) {
+    for (int64_t i = 0; i < k * d; ++i) new_c[i] = 0.0;
+    for (int64_t c = 0; c < k; ++c) counts[c] = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        int64_t l = labels[i];
+        counts[l] = counts[l] + 1.0;
+        for (int64_t j = 0; j < d; ++j)
+            new_c[l * d + j] = new_c[l * d + j] + X[i * d + j];
+    }
+    for (int64_t c = 0; c < k; ++c) {
+        if (counts[c] > 0.0) {
+            for (int64_t j = 0; j < d; ++j)
+                new_c[c * d + j] = new_c[c * d + j] / counts[c];
+        } else {
+            for (int64_t j = 0; j < d; ++j) new_c[c * d + j] = old_c[c * d + j];
+        }
+    }
+    double max_shift = 0.0;
+    for (int64_t c = 0; c < k; ++c) {
+        double s = or_sqeuclid(new_c + c * d, old_c + c * d, d);
+        if (s > max_shift) max_shift = s;
+    }
+    return sqrt(max_shift);
+}
+
+/* ------------------------------------------------------------------------ */
+/* a5: one Lloyd run and n_init restarts from given initial centres         */
+/* ------------------------------------------------------------------------ */
+
+/* kmeans.rs:459-495 (the body of one run): assign -> recompute -> swap ->
+ * n_iter = iter + 1 -> break if max_shift < tol.  labels and inertia come
+ * from the last assignment (made against the pre-update centres); the
+ * returned centres are the post-update ones.  With max_iter == 0 the labels
+ * stay 0, the inertia stays F::MAX and the centres are the initial ones. */
+void or_kmeans_lloyd(const double *X, int64_t n, int64_t d, int64_t k,
+                     const double *init, int64_t max_iter, double tol,
+                     int nthreads, double *centers_out, int64_t *labels_out,
+                     double *inertia_out, int64_t *n_iter_out) {
+    double *cur = (double *)malloc((size_t)(k * d) * sizeof(double));
+    double *nxt = (double *)malloc((size_t)(k * d) * sizeof(double));
+    double *cnt = (double *)malloc((size_t)k * sizeof(double));
+    memcpy(cur, init, (size_t)(k * d) * sizeof(double));
+    for (int64_t i = 0; i < n; ++i) labels_out[i] = 0;
+    double inertia = DBL_MAX;
+    int64_t n_iter = 0;
+    for (int64_t it = 0; it < max_iter; ++it) {
+        inertia = or_assign(X, n, d, cur, k, labels_out, nthreads);
+        double shift = or_recompute_centroids(X, n, d, labels_out, cur, k, nxt, cnt);
+        double *t = cur;
+        cur = nxt;
+        nxt = t;
+        n_iter = it + 1;
+        if (shift < tol) break;
+    }
+    memcpy(centers_out, cur, (size_t)(k * d) * sizeof(double));
+    *inertia_out = inertia;
+    *n_iter_out = n_iter;
+    free(cur);
+    free(nxt);
+    free(cnt);
+}
+
+/* kmeans.rs:459-503: n_init runs, each from its own initial centres
+ * (init is n_init*k*d); a later run replaces the kept one only on strictly
+ * smaller inertia.  Returns the index of the kept run. */
+int64_t or_kmeans_fit_from(const double *X, int64_t n, int64_t d, int64_t k,
+                           const double *inits, int64_t n_init,
+                           int64_t max_iter, double tol, int nthreads,
+                           double *centers_out, int64_t *labels_out,
+                           double *inertia_out, int64_t *n_iter_out) {
+    double *c = (double *)malloc((size_t)(k * d) * sizeof(double));
+    int64_t *l = (int64_t *)malloc((size_t)n * sizeof(int64_t));
+    int64_t best = -1;
+    for (int64_t r = 0; r < n_init; ++r) {
+        double in;
+        int64_t it;
+        or_kmeans_lloyd(X, n, d, k, inits + r * k * d, max_iter, tol, nthreads,
+                        c, l, &in, &it);
+        if (best < 0 || in < *inertia_out) {
+            best = r;
+            *inertia_out = in;
+            *n_iter_out = it;
+            memcpy(centers_out, c, (size_t)(k * d) * sizeof(double));
+            memcpy(labels_out, l, (size_t)n * sizeof(int64_t));
+        }
+    }
+    free(c);
+    free(l);
+    return best;
+}
+
+/* kmeans.rs:555-618 (Transform): dist[i][c] = sqrt(fold(x_i, c)). */
+void or_transform(const double *X, int64_t n, int64_t d, const double *C,
+                  int64_t k, double *out, int nthreads) {
+    nthreads = or_threads(nthreads);
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t c = 0; c < k; ++c)
+            out[i * k + c] = sqrt(or_sqeuclid(X + i * d, C + c * d, d));
+}
+
+/* ------------------------------------------------------------------------ */
+/* a6: k-means++ with the library's own generator (parity unpinned vs rand) */
+/* ------------------------------------------------------------------------ */
+
+/* The reference seeds rand 0.9's StdRng (ChaCha12); that stream is not
+ * vendored, so this restatement uses a documented generator instead:
+ * SplitMix64 expands the seed into xoshiro256** state; random_f64 takes the
+ * top 53 bits; random_range(n) is Lemire's multiply-shift (no rejection).
+ * The library's host code uses the same definition so the two agree. */
+typedef struct { uint64_t s[4]; } or_rng;
+
+static uint64_t or_splitmix(uint64_t *x) {
+    uint64_t z = (*x += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+static void or_rng_seed(or_rng *r, uint64_t seed) {
+    uint64_t x = seed;
+    for (int i = 0; i < 4; ++i) r->s[i] = or_splitmix(&x);
+}
+static uint64_t or_rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+static uint64_t or_rng_next(or_rng *r) {
+    uint64_t *s = r->s;
+    uint64_t out = or_rotl(s[1] * 5, 7) * 9;
+    uint64_t t = s[1] << 17;
+    s[2] ^= s[0];
+    s[3] ^= s[1];
+    s[1] ^= s[2];
+    s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = or_rotl(s[3], 45);
+    return out;
+}
+static double or_rng_f64(or_rng *r) {
+    return (double)(or_rng_next(r) >> 11) * 0x1.0p-53;
+}
+static int64_t or_rng_range(or_rng *r, int64_t n) {
+    return (int64_t)(((__uint128_t)or_rng_next(r) * (uint64_t)n) >> 64);
+}
+
+/* Exported so the tests can pin the generator's first outputs. */
+void or_rng_stream(uint64_t seed, int64_t count, uint64_t *out) {
+    or_rng r;
+    or_rng_seed(&r, seed);
+    for (int64_t i = 0; i < count; ++i) out[i] = or_rng_next(&r);
+}
+
+/* The reference's sums over min_dists are plain left folds (kmeans.rs:200,
+ * :207-213, :222-228).  This restatement uses a fixed two-level order
+ * instead: blocks of OR_BLK consecutive values are folded left to right,
+ * then the block sums are folded left to right.  The GPU runs the same two
+ * levels, so both sides agree bit for bit. */
+#define OR_BLK 4096
+double or_blocked_sum(const double *v, int64_t n) {
+    double tot = 0.0;
+    for (int64_t b = 0; b < n; b += OR_BLK) {
+        int64_t e = b + OR_BLK < n ? b + OR_BLK : n;
+        double s = 0.0;
+        for (int64_t i = b; i < e; ++i) s = s + v[i];
+        tot = tot + s;
+    }
+    return tot;
+}
+
+/* First index i whose two-level running sum reaches threshold; n-1 if none
+ * does (kmeans.rs:205-214 with the two-level order above). */
+int64_t or_blocked_search(const double *v, int64_t n, double threshold) {
+    double run = 0.0;
+    for (int64_t b = 0; b < n; b += OR_BLK) {
+        int64_t e = b + OR_BLK < n ? b + OR_BLK : n;
+        double s = 0.0;
+        for (int64_t i = b; i < e; ++i) s = s + v[i];
+        if (run + s >= threshold) {
+            double c = run;
+            for (int64_t i = b; i < e; ++i) {
+                c = c + v[i];
+                if (c >= threshold) return i;
+            }
+        }
+        run = run + s;
+    }
+    return n - 1;
+}
+
+/* kmeans.rs:169-253 (kmeans_plus_plus), greedy multi-trial k-means++:
+ * first centre uniform; n_trials = max(2 + floor(ln k), 1); for each new
+ * centre, if the total potential is <= 0 pick uniformly, else draw n_trials
+ * candidates with probability proportional to D(x)^2 and keep the one with
+ * the strictly smallest resulting potential (first wins ties). */
+void or_kmeanspp(const double *X, int64_t n, int64_t d, int64_t k,
+                 uint64_t seed, double *centers, int nthreads) {
+    nthreads = or_threads(nthreads);
+    or_rng rng;
+    or_rng_seed(&rng, seed);
+    memset(centers, 0, (size_t)(k * d) * sizeof(double));
+    if (n == 0) return;
+    int64_t first = or_rng_range(&rng, n);
+    memcpy(centers, X + first * d, (size_t)d * sizeof(double));
+    if (k == 1) return;
+    double *md = (double *)malloc((size_t)n * sizeof(double));
+    double *nd = (double *)malloc((size_t)n * sizeof(double));
+    double *bd = (double *)malloc((size_t)n * sizeof(double));
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+    for (int64_t i = 0; i < n; ++i) md[i] = or_sqeuclid(X + i * d, centers, d);
+    int64_t n_trials = 2 + (int64_t)floor(log((double)k));
+    if (n_trials < 1) n_trials = 1;
+    for (int64_t c = 1; c < k; ++c) {
+        double total = or_blocked_sum(md, n);
+        if (total <= 0.0) {
+            int64_t idx = or_rng_range(&rng, n);
+            memcpy(centers + c * d, X + idx * d, (size_t)d * sizeof(double));
+            continue;
+        }
+        int64_t best = 0;
+        double best_pot = 0.0;
+        int have = 0;
+        for (int64_t t = 0; t < n_trials; ++t) {
+            double thr = or_rng_f64(&rng) * total;
+            int64_t cand = or_blocked_search(md, n, thr);
+            const double *cr = X + cand * d;
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+            for (int64_t i = 0; i < n; ++i) {
+                double v = or_sqeuclid(X + i * d, cr, d);
+                nd[i] = v < md[i] ? v : md[i];
+            }
+            double pot = or_blocked_sum(nd, n);
+            if (!have || pot < best_pot) {
+                have = 1;
+                best_pot = pot;
+                best = cand;
+                double *tmp = bd;
+                bd = nd;
+                nd = tmp;
+            }
+        }
+        memcpy(centers + c * d, X + best * d, (size_t)d * sizeof(double));
+        double *tmp = md;
+        md = bd;
+        bd = tmp;
+    }
+    free(md);
+    free(nd);
+    free(bd);
+}
+
+/* ------------------------------------------------------------------------ */
+/* a8: MiniBatchKMeans pieces                                               */
+/* ------------------------------------------------------------------------ */
+
+/* mini_batch_kmeans.rs:410-447 (update_centers_mini_batch): for each batch
+ * sample in order, count[l] += 1, lr = 1/count, c_l[j] += lr * (x[j] - c_l[j]);
+ * returns max_c sqrt(fold(new_c - old_c)). */
+double or_minibatch_update(const double *X, int64_t d, const int64_t *batch_idx,
+                           const int64_t *batch_lab, int64_t b, double *C,
+                           int64_t k, int64_t *counts) {
+    double *old = (double *)malloc((size_t)(k * d) * sizeof(double));
+    memcpy(old, C, (size_t)(k * d) * sizeof(double));
+    for (int64_t s = 0; s < b; ++s) {
+        int64_t l = batch_lab[s];
+        const double *x = X + batch_idx[s] * d;
+        counts[l] += 1;
+        double lr = 1.0 / (double)counts[l];
+        for (int64_t j = 0; j < d; ++j)
+            C[l * d + j] = C[l * d + j] + lr * (x[j] - C[l * d + j]);
+    }
+    double mx = 0.0;
+    for (int64_t c = 0; c < k; ++c) {
+        double s = sqrt(or_sqeuclid(C + c * d, old + c * d, d));
+        if (s > mx) mx = s;
+    }
+    free(old);
+    return mx;
+}
+
+/* mini_batch_kmeans.rs:451-472 (sample_batch_indices) with the generator
+ * above: a full shuffle when batch >= n, else a partial Fisher-Yates. */
+void or_sample_batch(int64_t n, int64_t batch, or_rng *rng, int64_t *pool,
+                     int64_t *out, int64_t *out_len) {
+    for (int64_t i = 0; i < n; ++i) pool[i] = i;
+    if (batch >= n) {
+        for (int64_t i = 0; i < n; ++i) {
+            int64_t j = i + or_rng_range(rng, n - i);
+            int64_t t = pool[i];
+            pool[i] = pool[j];
+            pool[j] = t;
+        }
+        memcpy(out, pool, (size_t)n * sizeof(int64_t));
+        *out_len = n;
+        return;
+    }
+    for (int64_t i = 0; i < batch; ++i) {
+        int64_t j = i + or_rng_range(rng, n - i);
+        int64_t t = pool[i];
+        pool[i] = pool[j];
+        pool[j] = t;
+        out[i] = pool[i];
+    }
+    *out_len = batch;
+}
+
+/* mini_batch_kmeans.rs:478-592 for one run from given initial centres:
+ * counts start at 1; per iteration sample a batch, assign it (serial,
+ * :381-404), update online, n_iter += 1, stop if shift < tol; then assign
+ * every row (parallel per row, sequential inertia fold, :343-376). */
+void or_minibatch_run(const double *X, int64_t n, int64_t d, int64_t k,
+                      const double *init, int64_t batch, int64_t max_iter,
+                      double tol, uint64_t seed, int nthreads,
+                      double *centers_out, int64_t *labels_out,
+                      double *inertia_out, int64_t *n_iter_out) {
+    or_rng rng;
+    or_rng_seed(&rng, seed);
+    memcpy(centers_out, init, (size_t)(k * d) * sizeof(double));
+    int64_t *counts = (int64_t *)malloc((size_t)k * sizeof(int64_t));
+    for (int64_t c = 0; c < k; ++c) counts[c] = 1;
+    int64_t *pool = (int64_t *)malloc((size_t)n * sizeof(int64_t));
+    int64_t bmax = batch < n ? batch : n;
+    int64_t *bidx = (int64_t *)malloc((size_t)bmax * sizeof(int64_t));
+    int64_t *blab = (int64_t *)malloc((size_t)bmax * sizeof(int64_t));
+    int64_t it = 0;
+    for (int64_t r = 0; r < max_iter; ++r) {
+        int64_t blen;
+        or_sample_batch(n, batch, &rng, pool, bidx, &blen);
+        for (int64_t s = 0; s < blen; ++s)
+            blab[s] = or_nearest_center(X + bidx[s] * d, centers_out, k, d, NULL);
+        double shift = or_minibatch_update(X, d, bidx, blab, blen, centers_out, k, counts);
+        it += 1;
+        if (shift < tol) break;
+    }
+    double *dist = (double *)malloc((size_t)n * sizeof(double));
+    or_assign_rows(X, n, d, centers_out, k, labels_out, dist, nthreads);
+    double in = 0.0;
+    for (int64_t i = 0; i < n; ++i) in = in + dist[i];
+    *inertia_out = in;
+    *n_iter_out = it;
+    free(dist);
+    free(counts);
+    free(pool);
+    free(bidx);
+    free(blab);
+}
+
+/* ------------------------------------------------------------------------ */
+/* a9/a10/a15: brute-force kNN                                               */
+/* ------------------------------------------------------------------------ */
+
+/* kdtree.rs:364-391 (brute_force_knn): every distance sqrt(fold), then the k
+ * smallest sorted ascending.  The reference breaks distance ties through
+ * select_nth_unstable_by + a stable sort, an implementation-defined order;
+ * this restatement defines ties by ascending index, i.e. it sorts by the
+ * pair (distance, index).  k is clamped to N (:384). */
+static int or_pair_less(double da, int64_t ia, double db, int64_t ib) {
+    return da < db || (da == db && ia < ib);
+}
+
+int64_t or_knn_query(const double *data, int64_t N, int64_t d,
+                     const double *q, int64_t k, int64_t *idx_out,
+                     double *dist_out) {
+    if (k > N) k = N;
+    if (k <= 0) return 0;
+    int64_t cnt = 0;
+    for (int64_t i = 0; i < N; ++i) {
+        double v = sqrt(or_sqeuclid(data + i * d, q, d));
+        if (cnt == k && !or_pair_less(v, i, dist_out[k - 1], idx_out[k - 1]))
+            continue;
+        int64_t p = cnt < k ? cnt++ : k - 1;
+        while (p > 0 && or_pair_less(v, i, dist_out[p - 1], idx_out[p - 1])) {
+            dist_out[p] = dist_out[p - 1];
+            idx_out[p] = idx_out[p - 1];
+            --p;
+        }
+        dist_out[p] = v;
+        idx_out[p] = i;
+    }
+    return k;
+}
+
+/* knn.rs:875-917 (kneighbors_impl), parallel over queries here. */
+void or_kneighbors(const double *data, int64_t N, int64_t d, const double *Q,
+                   int64_t nq, int64_t k, int64_t *idx, double *dist,
+                   int nthreads) {
+    nthreads = or_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+    for (int64_t i = 0; i < nq; ++i)
+        or_knn_query(data, N, d, Q + i * d, k, idx + i * k, dist + i * k);
+}
+
+/* ------------------------------------------------------------------------ */
+/* a12-a14: vote, probabilities, regression mean                             */
+/* ------------------------------------------------------------------------ */
+
+/* knn.rs:388-423 (class_score_vec).  ypos[i] is the position of y_train[i]
+ * in the sorted unique class list.  Uniform: +1 per neighbour.  Distance: if
+ * any neighbour distance is < 1e-15 only those count, +1 each; otherwise
+ * +1/d, accumulated in neighbour order. */
+void or_class_scores(const int64_t *nidx, const double *ndist, int64_t k,
+                     const int64_t *ypos, int64_t n_classes, int weights,
+                     double *scores) {
+    const double eps = 1e-15;
+    for (int64_t c = 0; c < n_classes; ++c) scores[c] = 0.0;
+    if (weights == 0) {
+        for (int64_t m = 0; m < k; ++m) {
+            int64_t ci = ypos[nidx[m]];
+            scores[ci] = scores[ci] + 1.0;
+        }
+        return;
+    }
+    int zero = 0;
+    for (int64_t m = 0; m < k; ++m)
+        if (ndist[m] < eps) zero = 1;
+    for (int64_t m = 0; m < k; ++m) {
+        int64_t ci = ypos[nidx[m]];
+        if (zero) {
+            if (ndist[m] < eps) scores[ci] = scores[ci] + 1.0;
+        } else {
+            scores[ci] = scores[ci] + 1.0 / ndist[m];
+        }
+    }
+}
+
+/* knn.rs:427-440 (weighted_vote): argmax with strict `>` over the sorted
+ * classes, so the smallest class position wins a tie. */
+int64_t or_vote(const double *scores, int64_t n_classes) {
+    int64_t b = 0;
+    double bs = scores[0];
+    for (int64_t c = 1; c < n_classes; ++c)
+        if (scores[c] > bs) {
+            bs = scores[c];
+            b = c;
+        }
+    return b;
+}
+
+/* knn.rs:452-486 (predict_proba): scores / fold(scores); uniform 1/C when
+ * the total is not positive. */
+void or_proba(const double *scores, int64_t n_classes, double *out) {
+    double tot = 0.0;
+    for (int64_t c = 0; c < n_classes; ++c) tot = tot + scores[c];
+    if (tot > 0.0) {
+        for (int64_t c = 0; c < n_classes; ++c) out[c] = scores[c] / tot;
+    } else {
+        double u = 1.0 / (double)n_classes;
+        for (int64_t c = 0; c < n_classes; ++c) out[c] = u;
+    }
+}
+
+/* knn.rs:952-988 (weighted_mean), neighbour order throughout. */
+double or_weighted_mean(const int64_t *nidx, const double *ndist, int64_t k,
+                        const double *y, int weights) {
+    const double eps = 1e-15;
+    if (weights == 0) {
+        double s = 0.0;
+        for (int64_t m = 0; m < k; ++m) s = s + y[nidx[m]];
+        return s / (double)k;
+    }
+    int64_t nz = 0;
+    for (int64_t m = 0; m < k; ++m)
+        if (ndist[m] < eps) nz++;
+    if (nz > 0) {
+        double s = 0.0;
+        for (int64_t m = 0; m < k; ++m)
+            if (ndist[m] < eps) s = s + y[nidx[m]];
+        return s / (double)nz;
+    }
+    double ws = 0.0, wt = 0.0;
+    for (int64_t m = 0; m < k; ++m) {
+        double w = 1.0 / ndist[m];
+        ws = ws + w * y[nidx[m]];
+        wt = wt + w;
+    }
+    return ws / wt;
+}
+
+/* knn.rs:322-383 (classifier predict) with BruteForce (knn.rs:148):
+ * neighbours, class scores, vote; returns class positions. */
+void or_knn_classify(const double *data, int64_t N, int64_t d,
+                     const int64_t *ypos, int64_t n_classes, const double *Q,
+                     int64_t nq, int64_t k, int weights, int64_t *pred_pos,
+                     double *proba_or_null, int nthreads) {
+    nthreads = or_threads(nthreads);
+    int64_t kk = k < N ? k : N;
+#pragma omp parallel num_threads(nthreads)
+    {
+        int64_t *ni = (int64_t *)malloc((size_t)kk * sizeof(int64_t));
+        double *nd = (double *)malloc((size_t)kk * sizeof(double));
+        double *sc = (double *)malloc((size_t)n_classes * sizeof(double));
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t i = 0; i < nq; ++i) {
+            or_knn_query(data, N, d, Q + i * d, k, ni, nd);
+            or_class_scores(ni, nd, kk, ypos, n_classes, weights, sc);
+            pred_pos[i] = or_vote(sc, n_classes);
+            if (proba_or_null) or_proba(sc, n_classes, proba_or_null + i * n_classes);
+        }
+        free(ni);
+        free(nd);
+        free(sc);
+    }
+}
+
+/* knn.rs:720-781 (regressor predict) with BruteForce. */
+void or_knn_regress(const double *data, int64_t N, int64_t d, const double *y,
+                    const double *Q, int64_t nq, int64_t k, int weights,
+                    double *pred, int nthreads) {
+    nthreads = or_threads(nthreads);
+    int64_t kk = k < N ? k : N;
+#pragma omp parallel num_threads(nthreads)
+    {
+        int64_t *ni = (int64_t *)malloc((size_t)kk * sizeof(int64_t));
+        double *nd = (double *)malloc((size_t)kk * sizeof(double));
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t i = 0; i < nq; ++i) {
+            or_knn_query(data, N, d, Q + i * d, k, ni, nd);
+            pred[i] = or_weighted_mean(ni, nd, kk, y, weights);
+        }
+        free(ni);
+        free(nd);
+    }
+}
+
+/* Thread count the OpenMP runtime would use by default (for reports). */
+int or_max_threads(void) { return or_threads(0); }

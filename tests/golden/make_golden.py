@@ -1,0 +1,292 @@
+"""Writes tests/golden/golden_vectors.json with pure-Python float arithmetic.
+
+Python floats are IEEE-754 binary64 and every `+ - * / math.sqrt` on them is
+one correctly rounded op, evaluated left to right, with no FMA contraction --
+the same contract as the reference's Rust f64 code.  So these loops compute
+the values the reference computes, independently of the C oracle, and the
+expected outputs are stored exactly (float.hex).
+
+Sources of each case:
+  fold            ferrolearn-cluster/src/kmeans.rs:159-163, kdtree.rs:342-348
+  argmin/ties     kmeans.rs:295-309
+  lloyd           kmeans.rs:358-401, :459-495 (serial path: n*d < 100000,
+                  so the inertia is a plain sequential fold and fully pinned)
+  knn             kdtree.rs:364-391 with the (distance, index) tie order
+  vote / mean     knn.rs:388-440, :952-988
+  minibatch       mini_batch_kmeans.rs:410-447
+  rng             SplitMix64 + xoshiro256** (the library's own generator)
+
+Run:  python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+import random
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+M64 = (1 << 64) - 1
+
+
+def hx(v: float) -> str:
+    return float(v).hex()
+
+
+def fold(a, b):
+    acc = 0.0
+    for x, y in zip(a, b):
+        t = x - y
+        acc = acc + t * t
+    return acc
+
+
+def nearest(row, centers):
+    best, bd = 0, 1.7976931348623157e308
+    for c, ctr in enumerate(centers):
+        v = fold(row, ctr)
+        if v < bd:
+            bd, best = v, c
+    return best, bd
+
+
+def lloyd(X, init, max_iter, tol):
+    k, d = len(init), len(init[0])
+    cur = [list(r) for r in init]
+    labels = [0] * len(X)
+    inertia = 1.7976931348623157e308
+    n_iter = 0
+    for it in range(max_iter):
+        inertia = 0.0
+        for i, row in enumerate(X):
+            labels[i], v = nearest(row, cur)
+            inertia = inertia + v
+        sums = [[0.0] * d for _ in range(k)]
+        cnt = [0.0] * k
+        for i, row in enumerate(X):
+            l = labels[i]
+            cnt[l] = cnt[l] + 1.0
+            for j in range(d):
+                sums[l][j] = sums[l][j] + row[j]
+        new = []
+        for c in range(k):
+            if cnt[c] > 0.0:
+                new.append([s / cnt[c] for s in sums[c]])
+            else:
+                new.append(list(cur[c]))
+        mx = 0.0
+        for c in range(k):
+            s = fold(new[c], cur[c])
+            if s > mx:
+                mx = s
+        shift = math.sqrt(mx)
+        cur = new
+        n_iter = it + 1
+        if shift < tol:
+            break
+    return cur, labels, inertia, n_iter
+
+
+def knn(data, q, k):
+    ds = [(math.sqrt(fold(row, q)), i) for i, row in enumerate(data)]
+    ds.sort()
+    return ds[: min(k, len(data))]
+
+
+def class_scores(neigh, ypos, n_classes, weights):
+    sc = [0.0] * n_classes
+    if weights == 0:
+        for d, i in neigh:
+            sc[ypos[i]] = sc[ypos[i]] + 1.0
+        return sc
+    zero = any(d < 1e-15 for d, _ in neigh)
+    for d, i in neigh:
+        if zero:
+            if d < 1e-15:
+                sc[ypos[i]] = sc[ypos[i]] + 1.0
+        else:
+            sc[ypos[i]] = sc[ypos[i]] + 1.0 / d
+    return sc
+
+
+def vote(sc):
+    b, bs = 0, sc[0]
+    for c in range(1, len(sc)):
+        if sc[c] > bs:
+            b, bs = c, sc[c]
+    return b
+
+
+def wmean(neigh, y, weights):
+    if weights == 0:
+        s = 0.0
+        for _, i in neigh:
+            s = s + y[i]
+        return s / float(len(neigh))
+    z = [(d, i) for d, i in neigh if d < 1e-15]
+    if z:
+        s = 0.0
+        for _, i in z:
+            s = s + y[i]
+        return s / float(len(z))
+    ws = wt = 0.0
+    for d, i in neigh:
+        w = 1.0 / d
+        ws = ws + w * y[i]
+        wt = wt + w
+    return ws / wt
+
+
+def mb_update(X, bidx, blab, C, counts):
+    C = [list(r) for r in C]
+    old = [list(r) for r in C]
+    counts = list(counts)
+    for i, l in zip(bidx, blab):
+        counts[l] += 1
+        lr = 1.0 / float(counts[l])
+        for j in range(len(C[l])):
+            C[l][j] = C[l][j] + lr * (X[i][j] - C[l][j])
+    mx = 0.0
+    for c in range(len(C)):
+        s = math.sqrt(fold(C[c], old[c]))
+        if s > mx:
+            mx = s
+    return C, counts, mx
+
+
+def rng_stream(seed, count):
+    x = seed & M64
+    s = []
+    for _ in range(4):
+        x = (x + 0x9E3779B97F4A7C15) & M64
+        z = x
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M64
+        s.append(z ^ (z >> 31))
+    rotl = lambda v, k: ((v << k) | (v >> (64 - k))) & M64
+    out = []
+    for _ in range(count):
+        out.append(rotl((s[1] * 5) & M64, 7) * 9 & M64)
+        t = (s[1] << 17) & M64
+        s[2] ^= s[0]
+        s[3] ^= s[1]
+        s[1] ^= s[2]
+        s[0] ^= s[3]
+        s[2] ^= t
+        s[3] = rotl(s[3], 45)
+    return out
+
+
+def main():
+    rnd = random.Random(20260101)
+    out = {}
+
+    # --- folds: random and adversarial ---------------------------------
+    folds = []
+    for d in (1, 2, 3, 5, 8, 31, 32, 64, 127, 128):
+        a = [rnd.uniform(-10, 10) for _ in range(d)]
+        b = [rnd.uniform(-10, 10) for _ in range(d)]
+        folds.append((a, b))
+    # cancellation: huge common offset
+    folds.append(([1e16 + 2.0, 3.0, -1e16], [1e16, 3.5, -1e16 + 4.0]))
+    # subnormal differences
+    folds.append(([5e-324, 1e-310, 0.0], [0.0, -1e-310, 5e-324]))
+    # order-sensitive: big term then many small ones
+    folds.append(([1.0] + [2.0 ** -27] * 40, [0.0] * 41))
+    folds.append(([2.0 ** -27] * 40 + [1.0], [0.0] * 41))
+    # overflow to +inf
+    folds.append(([1e200, 0.0], [-1e200, 0.0]))
+    # exact 3-4-5 (kdtree.rs:486-491)
+    folds.append(([0.0, 0.0], [3.0, 4.0]))
+    out["fold"] = [
+        {"a": [hx(v) for v in a], "b": [hx(v) for v in b], "sq": hx(fold(a, b)),
+         "dist": hx(math.sqrt(fold(a, b)))}
+        for a, b in folds
+    ]
+
+    # --- argmin with planted exact ties --------------------------------
+    ties = []
+    for trial in range(6):
+        d = rnd.choice([2, 3, 7])
+        k = rnd.choice([3, 5, 9])
+        centers = [[float(rnd.randint(-3, 3)) for _ in range(d)] for _ in range(k)]
+        centers[k - 1] = list(centers[0])          # duplicate centre -> exact tie
+        if k > 3:
+            centers[2] = list(centers[1])
+        rows = [[float(rnd.randint(-4, 4)) for _ in range(d)] for _ in range(40)]
+        rows += [list(centers[0]), list(centers[1])]
+        labs, dists = zip(*[nearest(r, centers) for r in rows])
+        ties.append({"centers": [[hx(v) for v in c] for c in centers],
+                     "rows": [[hx(v) for v in r] for r in rows],
+                     "labels": list(labs), "dist": [hx(v) for v in dists]})
+    out["argmin"] = ties
+
+    # --- Lloyd runs (serial path) --------------------------------------
+    runs = []
+    for (n, d, k, max_iter, tol, spread) in ((120, 3, 4, 50, 1e-4, 1.0),
+                                              (300, 5, 6, 100, 1e-4, 2.5),
+                                              (64, 2, 8, 3, 0.0, 4.0),
+                                              (50, 4, 5, 0, 1e-4, 1.0),
+                                              (90, 3, 3, 100, 1e-4, 0.2)):
+        cen = [[rnd.uniform(-10, 10) for _ in range(d)] for _ in range(k)]
+        X = [[cen[i % k][j] + rnd.gauss(0, spread) for j in range(d)] for i in range(n)]
+        init = [list(X[rnd.randrange(n)]) for _ in range(k)]
+        # one centre far away so a cluster can empty out
+        init[-1] = [1e3] * d
+        C, L, inertia, it = lloyd(X, init, max_iter, tol)
+        runs.append({"X": [[hx(v) for v in r] for r in X],
+                     "init": [[hx(v) for v in r] for r in init],
+                     "max_iter": max_iter, "tol": tol,
+                     "centers": [[hx(v) for v in r] for r in C], "labels": L,
+                     "inertia": hx(inertia), "n_iter": it})
+    out["lloyd"] = runs
+
+    # --- kNN with duplicate points (exact ties) -------------------------
+    knns = []
+    for (N, d, nq, k) in ((40, 2, 12, 5), (60, 3, 10, 7), (25, 4, 6, 25), (30, 1, 8, 4)):
+        data = [[float(rnd.randint(-3, 3)) for _ in range(d)] for _ in range(N)]
+        Q = [[rnd.choice([float(rnd.randint(-3, 3)), rnd.uniform(-3, 3)]) for _ in range(d)]
+             for _ in range(nq)]
+        Q[0] = list(data[3])                                  # exact match
+        y = [rnd.randrange(4) * 3 for _ in range(N)]          # classes {0,3,6,9}
+        yr = [rnd.uniform(-50, 50) for _ in range(N)]
+        classes = sorted(set(y))
+        ypos = [classes.index(v) for v in y]
+        res = []
+        for q in Q:
+            nb = knn(data, q, k)
+            r = {"idx": [i for _, i in nb], "dist": [hx(dv) for dv, _ in nb]}
+            for w in (0, 1):
+                sc = class_scores(nb, ypos, len(classes), w)
+                r[f"vote_w{w}"] = classes[vote(sc)]
+                r[f"mean_w{w}"] = hx(wmean(nb, yr, w))
+            res.append(r)
+        knns.append({"data": [[hx(v) for v in r] for r in data],
+                     "Q": [[hx(v) for v in r] for r in Q], "k": k,
+                     "y": y, "yr": [hx(v) for v in yr], "results": res})
+    out["knn"] = knns
+
+    # --- MiniBatch online update ----------------------------------------
+    mbs = []
+    for (n, d, k, b) in ((80, 3, 4, 20), (50, 2, 3, 50)):
+        X = [[rnd.uniform(-5, 5) for _ in range(d)] for _ in range(n)]
+        C = [list(X[i]) for i in range(k)]
+        counts = [1] * k
+        bidx = [rnd.randrange(n) for _ in range(b)]
+        blab = [nearest(X[i], C)[0] for i in bidx]
+        C2, cnt2, sh = mb_update(X, bidx, blab, C, counts)
+        mbs.append({"X": [[hx(v) for v in r] for r in X], "C": [[hx(v) for v in r] for r in C],
+                    "counts": counts, "bidx": bidx, "blab": blab,
+                    "C_out": [[hx(v) for v in r] for r in C2], "counts_out": cnt2,
+                    "shift": hx(sh)})
+    out["minibatch"] = mbs
+
+    # --- generator ------------------------------------------------------
+    out["rng"] = {str(s): [str(v) for v in rng_stream(s, 8)] for s in (0, 1, 42, 2 ** 63 + 5)}
+
+    with open(os.path.join(HERE, "golden_vectors.json"), "w") as f:
+        json.dump(out, f, indent=0, separators=(",", ":"))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,389 @@
+"""Pins the CPU oracle (oracle/ref_cpu.c) before anything is checked against it.
+
+Three independent sources:
+  * golden vectors written by pure-Python binary64 loops (tests/golden/
+    make_golden.py) -- bit-exact expectations;
+  * the reference's own sklearn fixtures (fixtures/*.json and
+    tests/fixtures/sklearn_reference/*.json of ferrolearn), with the
+    tolerances the reference's oracle tests use;
+  * closed forms and known answers from the reference's unit tests.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests._golden import arr, bits, golden, ref_fixture
+
+G = golden()
+
+
+# ----------------------------------------------------------------- fold -----
+def test_fold_golden_bit_exact():
+    for case in G["fold"]:
+        a, b = arr(case["a"]), arr(case["b"])
+        sq = O.sqeuclid(a, b)
+        want = float.fromhex(case["sq"])
+        assert bits([sq])[0] == bits([want])[0], (case, sq, want)
+        assert bits([O.euclid(a, b)])[0] == bits([float.fromhex(case["dist"])])[0]
+
+
+def test_fold_order_matters():
+    # The two order-sensitive cases hold the same multiset of terms; a
+    # restatement that reassociated would give them equal sums.
+    c1, c2 = G["fold"][12], G["fold"][13]
+    assert c1["sq"] != c2["sq"]
+
+
+def test_fold_known_answers():
+    # kdtree.rs:486-497: 3-4-5 triangle, self distance 0
+    assert O.euclid([0.0, 0.0], [3.0, 4.0]) == 5.0
+    assert O.euclid([1.0, 2.0, 3.0], [1.0, 2.0, 3.0]) == 0.0
+
+
+def test_fold_is_not_fma():
+    # (1+2^-30)^2 with an FMA keeps the 2^-60 term; two rounded ops lose it.
+    a = [1.0 + 2.0 ** -30, 2.0 ** -60 * 0]
+    b = [0.0, 0.0]
+    t = 1.0 + 2.0 ** -30
+    no_fma = t * t + 0.0
+    assert O.sqeuclid(a, b) == no_fma
+
+
+# --------------------------------------------------------------- argmin -----
+def test_argmin_lowest_index_on_ties():
+    for case in G["argmin"]:
+        C = arr(case["centers"])
+        for r, lab, dist in zip(case["rows"], case["labels"], case["dist"]):
+            l, dv = O.nearest_center(arr(r), C)
+            assert l == lab
+            assert dv == float.fromhex(dist)
+    # explicit: duplicated centre 0 and k-1 -> label 0
+    C = np.array([[1.0, 1.0], [5.0, 5.0], [1.0, 1.0]])
+    assert O.nearest_center([1.0, 1.0], C)[0] == 0
+
+
+def test_argmin_nan_never_wins():
+    # kmeans.rs:303: a NaN distance fails `<`, so the row keeps label 0 and
+    # F::MAX when every distance is NaN.
+    C = np.array([[np.nan, 0.0], [np.nan, 1.0]])
+    lab, dist = O.nearest_center([0.0, 0.0], C)
+    assert lab == 0 and dist == np.finfo(np.float64).max
+
+
+# ---------------------------------------------------------------- Lloyd -----
+def test_lloyd_golden_bit_exact():
+    for run in G["lloyd"]:
+        X, init = arr(run["X"]), arr(run["init"])
+        C, L, inertia, it = O.kmeans_lloyd(X, init, run["max_iter"], run["tol"], nthreads=1)
+        assert it == run["n_iter"]
+        assert list(L) == run["labels"]
+        assert (bits(C) == bits(arr(run["centers"]))).all()
+        assert bits([inertia])[0] == bits([float.fromhex(run["inertia"])])[0]
+
+
+def test_lloyd_zero_iterations_keeps_init():
+    run = [r for r in G["lloyd"] if r["max_iter"] == 0][0]
+    X, init = arr(run["X"]), arr(run["init"])
+    C, L, inertia, it = O.kmeans_lloyd(X, init, 0, 1e-4)
+    assert it == 0 and (L == 0).all() and inertia == np.finfo(np.float64).max
+    assert (C == init).all()
+
+
+def test_empty_cluster_keeps_old_centre():
+    X = np.array([[0.0, 0.0], [0.1, 0.0], [10.0, 10.0]])
+    old = np.array([[0.0, 0.0], [10.0, 10.0], [1e6, 1e6]])
+    labels, _ = O.assign(X, old)
+    new, counts, shift = O.recompute_centroids(X, labels, old)
+    assert counts[2] == 0.0 and (new[2] == old[2]).all()
+    assert new[0, 0] == 0.05
+
+
+def test_centroid_sum_order_is_index_order():
+    # Values chosen so that summing in a different order rounds differently.
+    X = np.array([[1e16], [1.0], [1.0], [-1e16]])
+    labels = np.zeros(4, np.int64)
+    new, _, _ = O.recompute_centroids(X, labels, np.zeros((1, 1)))
+    s = 0.0
+    for v in (1e16, 1.0, 1.0, -1e16):
+        s = s + v
+    assert new[0, 0] == s / 4.0
+
+
+def test_parallel_assign_labels_match_serial():
+    rng = np.random.default_rng(3)
+    X = rng.normal(size=(5000, 40))
+    C = rng.normal(size=(9, 40))
+    l1, i1 = O.assign(X, C, nthreads=1)
+    l8, i8 = O.assign(X, C, nthreads=8)
+    assert (l1 == l8).all()
+    assert abs(i1 - i8) <= 1e-12 * i1
+    labels, dist = O.assign_rows(X, C)
+    assert (labels == l1).all()
+    assert abs(math.fsum(dist) - i1) <= 1e-12 * i1
+
+
+def test_single_cluster_centre_is_mean():
+    # kmeans.rs:800-817: centre of (1,1),(2,2),(3,3),(4,4) is (2.5, 2.5)
+    X = np.array([[1.0, 1.0], [2.0, 2.0], [3.0, 3.0], [4.0, 4.0]])
+    C, L, inertia, it = O.kmeans_lloyd(X, X[:1], 300, 1e-4)
+    assert (L == 0).all() and C[0, 0] == 2.5 and C[0, 1] == 2.5
+
+
+def test_k_equals_n_inertia_zero():
+    # kmeans.rs:783-798
+    X = np.array([[0.0, 0.0], [5.0, 5.0], [10.0, 10.0]])
+    C, L, inertia, it = O.kmeans_lloyd(X, X.copy(), 300, 1e-4)
+    assert inertia == 0.0 and len(set(L.tolist())) == 3
+
+
+def test_identical_points_inertia_zero():
+    X = np.ones((4, 2))
+    C, L, inertia, it = O.kmeans_lloyd(X, X[:1], 300, 1e-4)
+    assert inertia == 0.0
+
+
+def test_transform_is_sqrt_fold():
+    rng = np.random.default_rng(5)
+    X = rng.normal(size=(20, 3))
+    C = rng.normal(size=(4, 3))
+    T = O.transform(X, C)
+    for i in range(20):
+        for c in range(4):
+            assert T[i, c] == math.sqrt(O.sqeuclid(X[i], C[c]))
+
+
+# ------------------------------------------------------ sklearn fixtures ----
+def _relabel_agreement(a, b):
+    """Pairwise co-membership agreement (label permutation invariant)."""
+    a, b = np.asarray(a), np.asarray(b)
+    same_a = a[:, None] == a[None, :]
+    same_b = b[:, None] == b[None, :]
+    return (same_a == same_b).mean()
+
+
+@pytest.mark.parametrize("name,nested", [("kmeans.json", True), ("sklearn_kmeans_50x5.json", False)])
+def test_kmeans_sklearn_fixture_fixed_point(name, nested):
+    fx = ref_fixture(name)
+    if nested:
+        X = np.array(fx["input"]["X"])
+        exp = fx["expected"]
+        centers, labels, inertia = (np.array(exp["cluster_centers"]), np.array(exp["labels"]),
+                                    exp["inertia"])
+    else:
+        X = np.array(fx["X_train"])
+        centers, labels, inertia = (np.array(fx["cluster_centers"]), np.array(fx["labels"]),
+                                    fx["inertia"])
+    # Assignment against sklearn's centres reproduces sklearn's labels ...
+    L, _ = O.assign(X, centers)
+    assert (L == labels).all()
+    # ... the inertia to 1e-12 relative (sum of exact folds) ...
+    _, dist = O.assign_rows(X, centers)
+    assert abs(math.fsum(dist) - inertia) <= 1e-12 * inertia
+    # ... and sklearn's centres are the means of their members: a Lloyd run
+    # started there stops after one iteration with the same labels.
+    C, L2, inertia2, it = O.kmeans_lloyd(X, centers, 300, 1e-4)
+    assert it == 1 and (L2 == labels).all()
+    assert np.abs(C - centers).max() <= 1e-12 * np.abs(centers).max()
+
+
+@pytest.mark.parametrize("name,nested", [("kmeans.json", True), ("sklearn_kmeans_50x5.json", False)])
+def test_kmeans_sklearn_fixture_statistical(name, nested):
+    # ferrolearn-cluster/tests/oracle_tests.rs:94-112: inertia ratio in
+    # [0.95, 1.05] and pairwise label agreement >= 0.99 from k-means++ init.
+    fx = ref_fixture(name)
+    if nested:
+        X = np.array(fx["input"]["X"])
+        k = fx["params"]["n_clusters"]
+        exp_l, exp_i = fx["expected"]["labels"], fx["expected"]["inertia"]
+    else:
+        X = np.array(fx["X_train"])
+        k = fx["params"]["n_clusters"]
+        exp_l, exp_i = fx["labels"], fx["inertia"]
+    inits = np.stack([O.kmeanspp(X, k, 42 + r) for r in range(10)])
+    C, L, inertia, it, best = O.kmeans_fit_from(X, inits, 300, 1e-4)
+    assert 0.95 <= inertia / exp_i <= 1.05
+    assert _relabel_agreement(L, exp_l) >= 0.99
+
+
+def test_knn_classifier_iris_fixture():
+    # ferrolearn-neighbors/tests/oracle_tests.rs:51-95: >= 95 % prediction
+    # match and accuracy within 0.05 of sklearn's.
+    fx = ref_fixture("kneighbors_classifier.json")
+    X, y = np.array(fx["input"]["X"]), np.array(fx["input"]["y"])
+    k = fx["params"]["n_neighbors"]
+    pred = O.knn_classify(X, y, X, k)
+    exp = np.array(fx["expected"]["predictions"])
+    assert (pred == exp).mean() >= 0.95
+    assert abs((pred == y).mean() - fx["expected"]["accuracy"]) <= 0.05
+
+
+def test_knn_regressor_diabetes_fixture():
+    # oracle_tests.rs:97-138: max |diff| < 1.0 and R^2 within 0.1.
+    fx = ref_fixture("kneighbors_regressor.json")
+    X, y = np.array(fx["input"]["X"]), np.array(fx["input"]["y"])
+    k = fx["params"]["n_neighbors"]
+    pred = O.knn_regress(X, y, X, k)
+    exp = np.array(fx["expected"]["predictions"])
+    assert np.abs(pred - exp).max() < 1.0
+    r2 = 1 - ((y - pred) ** 2).sum() / ((y - y.mean()) ** 2).sum()
+    assert abs(r2 - fx["expected"]["r2"]) <= 0.1
+
+
+def test_knn_sklearn_equivalence_50x5():
+    fc = ref_fixture("sklearn_kneighbors_classifier_50x5.json")
+    X, y = np.array(fc["X_train"]), np.array(fc["y_train"])
+    k = fc["params"]["n_neighbors"]
+    assert (O.knn_classify(X, y, X, k) == np.array(fc["predictions"])).all()
+    assert (O.knn_classify(X, y, np.array(fc["X_test"]), k) ==
+            np.array(fc["predictions_test"])).all()
+    fr = ref_fixture("sklearn_kneighbors_regressor_50x5.json")
+    X, y = np.array(fr["X_train"]), np.array(fr["y_train"])
+    k = fr["params"]["n_neighbors"]
+    np.testing.assert_allclose(O.knn_regress(X, y, X, k), fr["predictions"], rtol=1e-12)
+    np.testing.assert_allclose(O.knn_regress(X, y, np.array(fr["X_test"]), k),
+                               fr["predictions_test"], rtol=1e-12)
+
+
+# -------------------------------------------------------------- kNN golden --
+def test_knn_golden_bit_exact():
+    for case in G["knn"]:
+        data, Q, k = arr(case["data"]), arr(case["Q"]), case["k"]
+        y = np.array(case["y"])
+        yr = arr(case["yr"])
+        dist, idx = O.kneighbors(data, Q, k)
+        for qi, res in enumerate(case["results"]):
+            assert list(idx[qi]) == res["idx"]
+            assert (bits(dist[qi]) == bits(arr(res["dist"]))).all()
+        for w in (0, 1):
+            pred = O.knn_classify(data, y, Q, k, weights=w)
+            assert list(pred) == [r[f"vote_w{w}"] for r in case["results"]]
+            pr = O.knn_regress(data, yr, Q, k, weights=w)
+            assert (bits(pr) == bits(arr([r[f"mean_w{w}"] for r in case["results"]]))).all()
+
+
+def test_knn_known_answers():
+    # knn.rs:1050-1064 k=1 memorises; :1066-1080 k=n predicts the mode
+    X = np.array([[0.0, 0.0], [1.0, 0.0], [0.0, 1.0], [1.0, 1.0]])
+    assert list(O.knn_classify(X, [0, 1, 2, 3], X, 1)) == [0, 1, 2, 3]
+    X = np.array([[1.0], [2.0], [3.0], [4.0], [5.0]])
+    assert list(O.knn_classify(X, [0, 0, 0, 1, 1], X, 5)) == [0] * 5
+    # knn.rs:1262-1275 mean of 3 nearest = 10.0
+    X = np.array([[0.0], [1.0], [2.0], [3.0], [100.0]])
+    assert O.knn_regress(X, [0.0, 10.0, 20.0, 30.0, 1000.0], [[1.0]], 3)[0] == 10.0
+    # knn.rs:1277-1296 inverse-distance weights
+    got = O.knn_regress(np.array([[0.0], [10.0]]), [0.0, 100.0], [[1.0]], 2, weights=1)[0]
+    assert got == (1.0 * 0.0 + (1.0 / 9.0) * 100.0) / (1.0 + 1.0 / 9.0)
+    # knn.rs:1395-1411 exact match takes all the weight
+    got = O.knn_regress(np.array([[0.0], [1.0], [2.0]]), [10.0, 20.0, 30.0], [[1.0]], 3, 1)[0]
+    assert got == 20.0
+    # kdtree.rs:445-453 brute force simple
+    idx, _ = O.knn_query(np.array([[0.0, 0.0], [1.0, 0.0], [0.0, 1.0], [1.0, 1.0]]),
+                         [0.1, 0.1], 2)
+    assert idx[0] == 0
+
+
+def test_knn_k_clamped_to_n():
+    data = np.arange(6.0).reshape(3, 2)
+    idx, dist = O.knn_query(data, [0.0, 0.0], 10)
+    assert list(idx) == [0, 1, 2]
+
+
+def test_knn_matches_full_sort():
+    rng = np.random.default_rng(11)
+    data = rng.integers(-2, 3, size=(300, 3)).astype(np.float64)   # many ties
+    Q = rng.integers(-2, 3, size=(20, 3)).astype(np.float64)
+    dist, idx = O.kneighbors(data, Q, 17)
+    for qi in range(20):
+        full = sorted((math.sqrt(O.sqeuclid(data[i], Q[qi])), i) for i in range(300))[:17]
+        assert [i for _, i in full] == list(idx[qi])
+
+
+def test_proba_rows_sum_to_one():
+    rng = np.random.default_rng(2)
+    X = rng.normal(size=(50, 3))
+    y = rng.integers(0, 4, size=50)
+    _, pr = O.knn_classify(X, y, X[:10], 5, proba=True)
+    np.testing.assert_allclose(pr.sum(1), 1.0, rtol=1e-15)
+
+
+# ----------------------------------------------------------- MiniBatch ------
+def test_minibatch_update_golden():
+    for case in G["minibatch"]:
+        X, C = arr(case["X"]), arr(case["C"])
+        C2, cnt, sh = O.minibatch_update(X, case["bidx"], case["blab"], C, case["counts"])
+        assert (bits(C2) == bits(arr(case["C_out"]))).all()
+        assert list(cnt) == case["counts_out"]
+        assert sh == float.fromhex(case["shift"])
+
+
+def test_minibatch_run_reasonable():
+    fx = ref_fixture("mini_batch_kmeans.json")
+    X = np.array(fx["input"]["X"])
+    k = fx["params"]["n_clusters"]
+    best = None
+    for r in range(3):
+        init = O.kmeanspp(X, k, 42 + r * 1_000_003)
+        C, L, inertia, it = O.minibatch_run(X, init, 50, 100, 0.0, 42 + r * 1_000_003)
+        assert it == 100                # tol = 0 -> every run does max_iter
+        if best is None or inertia < best:
+            best = inertia
+    # ferrolearn-cluster/tests/oracle_tests.rs:199-250: ratio in [0.9, 1.1]
+    assert 0.9 <= best / fx["expected"]["inertia"] <= 1.1
+
+
+# ----------------------------------------------------------- generator ------
+def test_rng_known_answer_and_stream():
+    # SplitMix64 seeded with 0 gives 0xe220a8397b1dcdaf first (published
+    # known answer); the xoshiro256** stream must match the Python restatement.
+    from tests.golden.make_golden import rng_stream
+    x = 0x9E3779B97F4A7C15
+    z = x
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & ((1 << 64) - 1)
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & ((1 << 64) - 1)
+    assert z ^ (z >> 31) == 0xE220A8397B1DCDAF
+    for seed, vals in G["rng"].items():
+        got = O.rng_stream(int(seed), 8)
+        assert [int(v) for v in got] == [int(v) for v in vals]
+        assert rng_stream(int(seed), 8) == [int(v) for v in vals]
+
+
+def test_blocked_sum_and_search():
+    rng = np.random.default_rng(1)
+    v = rng.random(10000)
+    s = O.blocked_sum(v)
+    assert abs(s - math.fsum(v)) <= 1e-12 * s
+    # two-level definition, restated
+    ref = 0.0
+    for b in range(0, 10000, 4096):
+        p = 0.0
+        for x in v[b:b + 4096]:
+            p = p + x
+        ref = ref + p
+    assert s == ref
+    thr = 0.37 * s
+    i = O.blocked_search(v, thr)
+    assert v[:i].sum() < thr * (1 + 1e-12) and v[:i + 1].sum() >= thr * (1 - 1e-12)
+    assert O.blocked_search(v, 2 * s) == 9999
+
+
+def test_kmeanspp_deterministic_and_picks_rows():
+    rng = np.random.default_rng(0)
+    X = rng.normal(size=(500, 4))
+    a = O.kmeanspp(X, 7, 123)
+    b = O.kmeanspp(X, 7, 123)
+    assert (a == b).all()
+    rows = {tuple(r) for r in X}
+    assert all(tuple(r) in rows for r in a)
+    assert not (O.kmeanspp(X, 7, 124) == a).all()
+
+
+def test_kmeanspp_spreads_over_blobs():
+    # D^2 sampling on well separated blobs picks one centre per blob.
+    rng = np.random.default_rng(4)
+    cen = rng.uniform(-100, 100, size=(6, 3))
+    X = np.concatenate([c + rng.normal(scale=0.1, size=(100, 3)) for c in cen])
+    C = O.kmeanspp(X, 6, 7)
+    blob = [int(np.argmin(((cen - c) ** 2).sum(1))) for c in C]
+    assert sorted(blob) == list(range(6))
