@@ -1,0 +1,19 @@
+"""ferrolearn_amd -- MI355X (gfx950) squared-Euclidean path of ferrolearn.
+
+KMeans Lloyd (assignment, inertia, centroid update) and brute-force
+KNeighborsClassifier / KNeighborsRegressor predict on hand-written HIP
+kernels behind the C ABI in include/ferrolearn_hip.h.  Labels, iteration
+counts and neighbour indices match the reference's f64 CPU path exactly.
+"""
+from .cluster import FittedKMeans, KMeans, kmeans_plusplus
+from .errors import (DeviceError, FerroError, InsufficientSamples, InvalidParameter,
+                     NumericalInstability, ShapeMismatch)
+from .neighbors import (Algorithm, FittedKNeighborsClassifier, FittedKNeighborsRegressor,
+                        KNeighborsClassifier, KNeighborsRegressor, Weights)
+
+__all__ = [
+    "KMeans", "FittedKMeans", "kmeans_plusplus", "KNeighborsClassifier",
+    "KNeighborsRegressor", "FittedKNeighborsClassifier", "FittedKNeighborsRegressor",
+    "Algorithm", "Weights", "FerroError", "ShapeMismatch", "InsufficientSamples",
+    "InvalidParameter", "NumericalInstability", "DeviceError",
+]

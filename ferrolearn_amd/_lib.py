@@ -1,0 +1,126 @@
+"""ctypes binding of libferrolearn_hip.so (the C ABI in include/ferrolearn_hip.h).
+
+The library is the product: there is no CPU fallback.  If the shared object
+is missing the import fails with an error that says how to build it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libferrolearn_hip.so")
+
+FL_OK = 0
+FL_SHAPE_MISMATCH = 1
+FL_INSUFFICIENT_SAMPLES = 2
+FL_INVALID_PARAMETER = 3
+FL_NUMERICAL_INSTABILITY = 4
+FL_DEVICE_ERROR = 5
+
+FL_F32 = 0
+FL_F64 = 1
+
+FL_PREC_FP64 = 0
+FL_PREC_BF16 = 1
+FL_PREC_BF16X3 = 2
+FL_PREC_EXACT = 3
+
+FL_UNIFORM = 0
+FL_DISTANCE = 1
+
+PRECISIONS = {"fp64": FL_PREC_FP64, "bf16": FL_PREC_BF16, "bf16x3": FL_PREC_BF16X3,
+              "exact": FL_PREC_EXACT}
+
+
+class fl_report(C.Structure):
+    _fields_ = [
+        ("ms_total", C.c_double),
+        ("ms_distance", C.c_double),
+        ("ms_finalize", C.c_double),
+        ("ms_fallback", C.c_double),
+        ("ms_update", C.c_double),
+        ("n_rows", C.c_int64),
+        ("n_candidates", C.c_int64),
+        ("n_fallback", C.c_int64),
+        ("n_iter", C.c_int64),
+        ("n_dist_launches", C.c_int64),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class fl_kmeans_params(C.Structure):
+    _fields_ = [
+        ("n_clusters", C.c_int64),
+        ("max_iter", C.c_int64),
+        ("tol", C.c_double),
+        ("n_init", C.c_int64),
+        ("random_state", C.c_uint64),
+        ("has_random_state", C.c_int32),
+        ("precision", C.c_int32),
+    ]
+
+
+_vp = C.c_void_p
+_i64 = C.c_int64
+_i32 = C.c_int32
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int64)
+_rp = C.POINTER(fl_report)
+
+# name -> (restype, argtypes); every symbol the header declares.
+SIGNATURES = {
+    "fl_last_error": (C.c_char_p, []),
+    "fl_version": (C.c_int, []),
+    "fl_device_count": (C.c_int, []),
+    "fl_kmeans_params_default": (None, [C.POINTER(fl_kmeans_params), _i64]),
+    "fl_kmeans_fit": (C.c_int, [_vp, _i32, _i64, _i64, C.POINTER(fl_kmeans_params), _vp, _vp,
+                                _vp, _dp, _ip, _rp]),
+    "fl_kmeans_fit_device": (C.c_int, [_vp, _i32, _i64, _i64, C.POINTER(fl_kmeans_params), _vp,
+                                       _vp, _vp, _dp, _ip, _rp]),
+    "fl_kmeans_predict": (C.c_int, [_vp, _i64, _i64, _vp, _i32, _i64, _i64, _vp]),
+    "fl_kmeans_transform": (C.c_int, [_vp, _i64, _i64, _vp, _i32, _i64, _i64, _vp]),
+    "fl_kmeans_plusplus": (C.c_int, [_vp, _i32, _i64, _i64, _i64, C.c_uint64, _vp]),
+    "fl_knn_fit_classifier": (C.c_int, [_vp, _i32, _i64, _i64, _vp, _i64, _i64, _i32, _i32,
+                                        C.POINTER(_vp)]),
+    "fl_knn_fit_regressor": (C.c_int, [_vp, _i32, _i64, _i64, _vp, _i64, _i64, _i32, _i32,
+                                       C.POINTER(_vp)]),
+    "fl_knn_predict_classify": (C.c_int, [_vp, _vp, _i32, _i64, _i64, _vp, _rp]),
+    "fl_knn_predict_regress": (C.c_int, [_vp, _vp, _i32, _i64, _i64, _vp, _rp]),
+    "fl_knn_predict_proba": (C.c_int, [_vp, _vp, _i32, _i64, _i64, _vp, _rp]),
+    "fl_knn_kneighbors": (C.c_int, [_vp, _vp, _i32, _i64, _i64, _i64, _vp, _vp, _rp]),
+    "fl_knn_kneighbors_device": (C.c_int, [_vp, _vp, _i32, _i64, _i64, _i64, _vp, _vp, _rp]),
+    "fl_knn_predict_classify_device": (C.c_int, [_vp, _vp, _i32, _i64, _i64, _vp, _rp]),
+    "fl_knn_predict_regress_device": (C.c_int, [_vp, _vp, _i32, _i64, _i64, _vp, _rp]),
+    "fl_knn_set_precision": (C.c_int, [_vp, _i32]),
+    "fl_knn_n_classes": (_i64, [_vp]),
+    "fl_knn_classes": (C.c_int, [_vp, _vp]),
+    "fl_knn_n_samples_fit": (_i64, [_vp]),
+    "fl_knn_free": (None, [_vp]),
+    "fl_backend_gemm_f64": (C.c_int, [_vp, _i64, _i64, _vp, _i64, _i64, _vp]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library once; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `make -C ferrolearn_amd/csrc` "
+                "(or __graft_entry__.build()); there is no CPU fallback")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    return lib().fl_last_error().decode()

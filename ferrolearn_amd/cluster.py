@@ -1,0 +1,201 @@
+"""KMeans with the reference's builder API, running on the gfx950 library.
+
+Mirrors ferrolearn-cluster/src/kmeans.rs: `KMeans(n_clusters)` with
+`with_max_iter / with_tol / with_n_init / with_random_state` (kmeans.rs:86-112),
+`fit(x)` -> `FittedKMeans` (Fit, kmeans.rs:403-513) exposing
+`cluster_centers() / labels() / inertia() / n_iter()` (:121-156), `predict`
+(:528-553), `transform` (:555-618) and `KMeans.fit_predict` (:522-525).
+
+Inputs may be numpy arrays (copied to the GPU) or CUDA tensors already in
+HBM (float32 or float64, C-contiguous, row-major).
+"""
+from __future__ import annotations
+
+import copy
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from .errors import check
+
+
+def _dtype_code(dt) -> int:
+    name = str(dt)
+    if name.endswith("float32"):
+        return L.FL_F32
+    if name.endswith("float64"):
+        return L.FL_F64
+    raise TypeError(f"unsupported dtype {dt}: use float32 or float64")
+
+
+def _host2d(x) -> np.ndarray:
+    x = np.asarray(x)
+    if x.dtype not in (np.float32, np.float64):
+        x = x.astype(np.float64)
+    if x.ndim != 2:
+        raise ValueError("expected a 2-D array")
+    return np.ascontiguousarray(x)
+
+
+def _is_device(x) -> bool:
+    return hasattr(x, "is_cuda") and bool(x.is_cuda)
+
+
+class KMeans:
+    """kmeans.rs:55-113.  Defaults from KMeans::new: max_iter 300, tol 1e-4,
+    n_init 10, random_state None.  `precision` picks the candidate filter
+    ('fp64' GEMM form, 'exact' direct folds); labels are exact either way."""
+
+    def __init__(self, n_clusters: int):
+        self.n_clusters = int(n_clusters)
+        self.max_iter = 300
+        self.tol = 1e-4
+        self.n_init = 10
+        self.random_state = None
+        self.precision = "fp64"
+
+    def _with(self, **kw):
+        o = copy.copy(self)
+        for k, v in kw.items():
+            setattr(o, k, v)
+        return o
+
+    def with_max_iter(self, max_iter: int):
+        return self._with(max_iter=int(max_iter))
+
+    def with_tol(self, tol: float):
+        return self._with(tol=float(tol))
+
+    def with_n_init(self, n_init: int):
+        return self._with(n_init=int(n_init))
+
+    def with_random_state(self, seed: int):
+        return self._with(random_state=int(seed))
+
+    def with_precision(self, precision: str):
+        return self._with(precision=precision)
+
+    def _params(self):
+        p = L.fl_kmeans_params()
+        L.lib().fl_kmeans_params_default(C.byref(p), self.n_clusters)
+        p.max_iter = self.max_iter
+        p.tol = self.tol
+        p.n_init = self.n_init
+        if self.random_state is not None:
+            p.random_state = self.random_state
+            p.has_random_state = 1
+        p.precision = L.PRECISIONS[self.precision]
+        return p
+
+    def fit(self, x, y=None, init_centers=None) -> "FittedKMeans":
+        """Fit<Array2<F>, ()>.  init_centers: optional (n_init, k, d) or
+        (k, d) array of initial centres (otherwise the library's k-means++)."""
+        p = self._params()
+        rep = L.fl_report()
+        inertia = C.c_double()
+        n_iter = C.c_int64()
+        if _is_device(x):
+            import torch
+            n, d = x.shape
+            dt = _dtype_code(x.dtype)
+            k = self.n_clusters
+            centers = torch.empty((max(k, 0), d), dtype=torch.float64, device=x.device)
+            labels = torch.empty((n,), dtype=torch.int64, device=x.device)
+            init_t = None
+            if init_centers is not None:
+                init_t = torch.as_tensor(init_centers, dtype=torch.float64,
+                                         device=x.device).contiguous()
+            st = L.lib().fl_kmeans_fit_device(
+                C.c_void_p(x.data_ptr()), dt, n, d, C.byref(p),
+                C.c_void_p(init_t.data_ptr()) if init_t is not None else None,
+                C.c_void_p(centers.data_ptr()), C.c_void_p(labels.data_ptr()),
+                C.byref(inertia), C.byref(n_iter), C.byref(rep))
+            check(st)
+            return FittedKMeans(centers, labels, inertia.value, n_iter.value, rep.as_dict(),
+                                self.precision)
+        xh = _host2d(x)
+        n, d = xh.shape
+        k = self.n_clusters
+        init = None
+        if init_centers is not None:
+            init = np.ascontiguousarray(np.asarray(init_centers, dtype=np.float64))
+            if init.size != self.n_init * k * d:
+                raise ValueError("init_centers must hold n_init * n_clusters * n_features values")
+        centers = np.zeros((max(k, 0), d), np.float64)
+        labels = np.zeros(n, np.int64)
+        st = L.lib().fl_kmeans_fit(
+            xh.ctypes.data_as(C.c_void_p), _dtype_code(xh.dtype), n, d, C.byref(p),
+            init.ctypes.data_as(C.c_void_p) if init is not None else None,
+            centers.ctypes.data_as(C.c_void_p), labels.ctypes.data_as(C.c_void_p),
+            C.byref(inertia), C.byref(n_iter), C.byref(rep))
+        check(st)
+        return FittedKMeans(centers, labels, inertia.value, n_iter.value, rep.as_dict(),
+                            self.precision)
+
+    def fit_predict(self, x):
+        """kmeans.rs:522-525: fit, then predict against the final centres."""
+        return self.fit(x).predict(x)
+
+
+class FittedKMeans:
+    """kmeans.rs:121-156."""
+
+    def __init__(self, centers, labels, inertia, n_iter, report, precision="fp64"):
+        self.cluster_centers_ = centers
+        self.labels_ = labels
+        self.inertia_ = float(inertia)
+        self.n_iter_ = int(n_iter)
+        self.report = report
+        self.precision = precision
+
+    def cluster_centers(self):
+        return self.cluster_centers_
+
+    def labels(self):
+        return self.labels_
+
+    def inertia(self) -> float:
+        return self.inertia_
+
+    def n_iter(self) -> int:
+        return self.n_iter_
+
+    def _centers_host(self):
+        c = self.cluster_centers_
+        if _is_device(c):
+            c = c.cpu().numpy()
+        return np.ascontiguousarray(c, dtype=np.float64)
+
+    def predict(self, x):
+        """Predict (kmeans.rs:528-553): ShapeMismatch on the feature count."""
+        c = self._centers_host()
+        xh = _host2d(x.cpu().numpy() if _is_device(x) else x)
+        n, d = xh.shape
+        out = np.zeros(n, np.int64)
+        check(L.lib().fl_kmeans_predict(c.ctypes.data_as(C.c_void_p), c.shape[0], c.shape[1],
+                                        xh.ctypes.data_as(C.c_void_p), _dtype_code(xh.dtype),
+                                        n, d, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def transform(self, x):
+        """Transform (kmeans.rs:555-618): n x k Euclidean distances."""
+        c = self._centers_host()
+        xh = _host2d(x.cpu().numpy() if _is_device(x) else x)
+        n, d = xh.shape
+        out = np.zeros((n, c.shape[0]), np.float64)
+        check(L.lib().fl_kmeans_transform(c.ctypes.data_as(C.c_void_p), c.shape[0], c.shape[1],
+                                          xh.ctypes.data_as(C.c_void_p), _dtype_code(xh.dtype),
+                                          n, d, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+
+def kmeans_plusplus(x, n_clusters: int, seed: int) -> np.ndarray:
+    """The library's k-means++ (kmeans.rs:169-253 with its own generator)."""
+    xh = _host2d(x)
+    n, d = xh.shape
+    out = np.zeros((n_clusters, d), np.float64)
+    check(L.lib().fl_kmeans_plusplus(xh.ctypes.data_as(C.c_void_p), _dtype_code(xh.dtype), n, d,
+                                     n_clusters, C.c_uint64(seed),
+                                     out.ctypes.data_as(C.c_void_p)))
+    return out

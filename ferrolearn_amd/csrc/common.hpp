@@ -1,0 +1,79 @@
+// common.hpp -- shared host/device helpers for the gfx950 distance path.
+//
+// Every translation unit is compiled with -ffp-contract=off, so `a + b * c`
+// is two rounded ops, never an FMA: the exact folds below reproduce the
+// reference's Rust arithmetic (ferrolearn-cluster/src/kmeans.rs:159-163).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/ferrolearn_hip.h"
+
+namespace fl {
+
+typedef double double4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+constexpr int kWave = 64;
+
+// ---------------------------------------------------------------- errors --
+struct Error {
+    fl_status code;
+    std::string msg;
+};
+
+void set_error(fl_status code, const std::string &msg);
+void clear_error();
+fl_status fail(fl_status code, const std::string &msg);
+
+#define FL_HIP(call)                                                         \
+    do {                                                                     \
+        hipError_t e_ = (call);                                              \
+        if (e_ != hipSuccess)                                                \
+            throw ::fl::DeviceError(std::string(#call) + ": " +             \
+                                    hipGetErrorString(e_));                  \
+    } while (0)
+
+struct DeviceError {
+    std::string msg;
+    explicit DeviceError(std::string m) : msg(std::move(m)) {}
+};
+
+// A FerroError-style failure detected after work has started (e.g. a NaN
+// distance found on the device).
+struct StatusError {
+    fl_status code;
+    std::string msg;
+};
+
+// ------------------------------------------------------------ device math --
+template <typename T>
+__device__ __forceinline__ double ld64(const T *p) {
+    return (double)(*p);
+}
+
+// Exact reference fold acc = acc + (a_j - b_j)^2, left to right from +0.
+template <typename TA, typename TB>
+__device__ __forceinline__ double fold_exact(const TA *a, const TB *b, int d) {
+    double acc = 0.0;
+    for (int j = 0; j < d; ++j) {
+        double t = (double)a[j] - (double)b[j];
+        acc = acc + t * t;
+    }
+    return acc;
+}
+
+// (key, idx) lexicographic order; NaN keys compare false like Rust's `<`.
+__device__ __forceinline__ bool pair_less(double ka, int64_t ia, double kb,
+                                          int64_t ib) {
+    return ka < kb || (ka == kb && ia < ib);
+}
+
+__host__ __device__ __forceinline__ int64_t ceil_div(int64_t a, int64_t b) {
+    return (a + b - 1) / b;
+}
+
+}  // namespace fl

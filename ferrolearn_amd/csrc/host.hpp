@@ -1,0 +1,183 @@
+// host.hpp -- host-side runtime pieces shared by the KMeans and kNN entry
+// points: device buffers, a growable workspace, stream-ordered event timing,
+// error-bound constants and the filter-kernel launcher.
+#pragma once
+
+#include <chrono>
+#include <cmath>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "dist_topm.hpp"
+
+namespace fl {
+
+// Device buffer owned by the host object (hipMalloc / hipFree).
+class DevBuf {
+  public:
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void *get(size_t bytes) {
+        if (bytes > cap_) {
+            release();
+            size_t want = bytes < 256 ? 256 : bytes;
+            FL_HIP(hipMalloc(&p_, want));
+            cap_ = want;
+        }
+        return p_;
+    }
+    void *ptr() const { return p_; }
+    void release() {
+        if (p_) (void)hipFree(p_);
+        p_ = nullptr;
+        cap_ = 0;
+    }
+
+  private:
+    void *p_ = nullptr;
+    size_t cap_ = 0;
+};
+
+// Named scratch buffers that only grow; reused across calls on one object.
+class Workspace {
+  public:
+    template <typename T>
+    T *get(const std::string &name, int64_t count) {
+        return static_cast<T *>(bufs_[name].get((size_t)(count > 0 ? count : 1) * sizeof(T)));
+    }
+    void clear() { bufs_.clear(); }
+
+  private:
+    std::map<std::string, DevBuf> bufs_;
+};
+
+// Accumulates kernel time between two events recorded on one stream.
+class PhaseTimer {
+  public:
+    explicit PhaseTimer(hipStream_t s) : s_(s) {}
+    ~PhaseTimer() {
+        for (auto &p : pairs_) {
+            (void)hipEventDestroy(p.first);
+            (void)hipEventDestroy(p.second);
+        }
+    }
+    int begin() {
+        hipEvent_t a, b;
+        FL_HIP(hipEventCreate(&a));
+        FL_HIP(hipEventCreate(&b));
+        pairs_.push_back({a, b});
+        FL_HIP(hipEventRecord(a, s_));
+        return (int)pairs_.size() - 1;
+    }
+    void end(int id) { FL_HIP(hipEventRecord(pairs_[id].second, s_)); }
+    double ms(int id) {
+        float t = 0.f;
+        FL_HIP(hipEventSynchronize(pairs_[id].second));
+        FL_HIP(hipEventElapsedTime(&t, pairs_[id].first, pairs_[id].second));
+        return (double)t;
+    }
+
+  private:
+    hipStream_t s_;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pairs_;
+};
+
+struct WallClock {
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    double ms() const {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+            .count();
+    }
+};
+
+// Rigorous bound coefficients for the f64 GEMM form (the filter in
+// dist_topm.hpp): |value - fold| <= alpha*|a||b| + beta*(|a|^2 + |b|^2),
+// covering the MFMA dot product, the row norms, the combination, the
+// reference fold's own rounding and the key's rounding, times 2 for margin.
+struct Bound {
+    double alpha, beta;
+};
+inline double gamma_n(double m, double u) { return m * u / (1.0 - m * u); }
+inline Bound bound_fp64(int d) {
+    const double u = std::ldexp(1.0, -53);
+    const double gd = gamma_n(d, u), gd2 = gamma_n(d + 2, u);
+    return {2.0 * (2.0 * gd + 6.0 * u + 2.0 * gd2), 2.0 * (gd + 4.0 * u + gd2)};
+}
+
+inline void check_launch() { FL_HIP(hipGetLastError()); }
+
+template <typename TA, typename TB, int M, int MODE>
+void launch_topm_t(const TA *A, const TB *B, const TopmArgs &a, int S, hipStream_t st) {
+    const size_t shmem = sizeof(TopmShared<M, MODE>);
+    auto kern = dist_topm_kernel<TA, TB, M, MODE>;
+    FL_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)shmem));
+    dim3 grid((unsigned)ceil_div(a.nA, TM_ROWS), (unsigned)S);
+    hipLaunchKernelGGL(kern, grid, dim3(256), shmem, st, A, B, a);
+    check_launch();
+}
+
+template <typename TA, typename TB>
+void launch_topm(int M, int mode, const TA *A, const TB *B, const TopmArgs &a, int S,
+                 hipStream_t st) {
+    if (mode == MODE_MFMA64) {
+        switch (M) {
+            case 4: return launch_topm_t<TA, TB, 4, MODE_MFMA64>(A, B, a, S, st);
+            case 16: return launch_topm_t<TA, TB, 16, MODE_MFMA64>(A, B, a, S, st);
+            case 32: return launch_topm_t<TA, TB, 32, MODE_MFMA64>(A, B, a, S, st);
+            case 64: return launch_topm_t<TA, TB, 64, MODE_MFMA64>(A, B, a, S, st);
+        }
+    } else {
+        switch (M) {
+            case 16: return launch_topm_t<TA, TB, 16, MODE_EXACT>(A, B, a, S, st);
+            case 32: return launch_topm_t<TA, TB, 32, MODE_EXACT>(A, B, a, S, st);
+            case 64: return launch_topm_t<TA, TB, 64, MODE_EXACT>(A, B, a, S, st);
+        }
+    }
+    throw DeviceError("internal: unsupported candidate list size");
+}
+
+inline unsigned grid1(int64_t n, int block) { return (unsigned)ceil_div(n, block); }
+
+// Convert to f64 (exact widening) when needed; returns the f64 view.
+template <typename T>
+const double *as_f64(const T *X, int64_t count, Workspace &ws, const char *slot,
+                     hipStream_t st);
+
+template <>
+inline const double *as_f64<double>(const double *X, int64_t, Workspace &, const char *,
+                                    hipStream_t) {
+    return X;
+}
+template <>
+inline const double *as_f64<float>(const float *X, int64_t count, Workspace &ws, const char *slot,
+                                   hipStream_t st) {
+    double *Y = ws.get<double>(slot, count);
+    hipLaunchKernelGGL(to_f64_kernel<float>, dim3(grid1(count, 256)), dim3(256), 0, st, X, count,
+                       Y);
+    check_launch();
+    return Y;
+}
+
+// Library entry guard: turns exceptions into status codes.
+template <typename F>
+fl_status guarded(F &&f) {
+    try {
+        clear_error();
+        return f();
+    } catch (const DeviceError &e) {
+        return fail(FL_DEVICE_ERROR, e.msg);
+    } catch (const StatusError &e) {
+        return fail(e.code, e.msg);
+    } catch (const std::bad_alloc &) {
+        return fail(FL_DEVICE_ERROR, "host allocation failed");
+    }
+}
+
+size_t dtype_size(int32_t dtype);
+
+}  // namespace fl

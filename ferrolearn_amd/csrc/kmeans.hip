@@ -1,0 +1,561 @@
+// kmeans.hip -- KMeans fit / predict / transform on gfx950.
+//
+// Reference: ferrolearn-cluster/src/kmeans.rs.  Fit (:403-513) runs n_init
+// Lloyd runs of assign (:267-352) -> recompute centroids (:358-401) -> swap
+// -> n_iter = iter + 1 -> stop when max_shift < tol, and keeps the run with
+// the strictly smallest inertia.  Labels and inertia come from the last
+// assignment, the centres are the post-update ones (:468-495).
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "host.hpp"
+#include "kmeans_kernels.hpp"
+
+using namespace fl;
+
+namespace {
+
+constexpr int64_t kMaxClusters = 8192;
+constexpr int kKmM = 4;   // candidates kept per row by the filter
+
+// The library's generator (SplitMix64 seeding xoshiro256**), identical to
+// the oracle's restatement; kmeans.rs:178,203,213 draw from rand's StdRng,
+// whose stream is not reproduced.
+struct Rng {
+    uint64_t s[4];
+    explicit Rng(uint64_t seed) {
+        uint64_t x = seed;
+        for (int i = 0; i < 4; ++i) {
+            uint64_t z = (x += 0x9E3779B97F4A7C15ULL);
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+            s[i] = z ^ (z >> 31);
+        }
+    }
+    static uint64_t rotl(uint64_t v, int k) { return (v << k) | (v >> (64 - k)); }
+    uint64_t next() {
+        const uint64_t out = rotl(s[1] * 5, 7) * 9;
+        const uint64_t t = s[1] << 17;
+        s[2] ^= s[0];
+        s[3] ^= s[1];
+        s[1] ^= s[2];
+        s[0] ^= s[3];
+        s[2] ^= t;
+        s[3] = rotl(s[3], 45);
+        return out;
+    }
+    double f64() { return (double)(next() >> 11) * 0x1.0p-53; }
+    int64_t range(int64_t n) {
+        return (int64_t)(((unsigned __int128)next() * (uint64_t)n) >> 64);
+    }
+};
+
+struct KmAcc {
+    double ms_dist = 0, ms_fin = 0, ms_fb = 0, ms_upd = 0;
+    int64_t n_rows = 0, n_cand = 0, n_fb = 0, n_launch = 0;
+};
+
+template <typename TX>
+__global__ void copy_row_f64_kernel(const TX *__restrict__ X, int64_t row, int d,
+                                    double *__restrict__ out) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < d) out[j] = (double)X[row * d + j];
+}
+
+__global__ void labels_to_i64_kernel(const int32_t *__restrict__ a, int64_t n,
+                                     int64_t *__restrict__ b) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
+// One KMeans problem on the device: X stays resident, scratch is reused.
+template <typename TX>
+class Lloyd {
+  public:
+    Lloyd(const TX *X, int64_t n, int d, int64_t k, int32_t precision, hipStream_t st)
+        : X_(X), n_(n), d_(d), k_(k), st_(st) {
+        // The direct exact scan is cheaper than filter + re-score when there
+        // are few centres.
+        direct_ = precision == FL_PREC_EXACT || k <= 32;
+        if (!direct_) {
+            xn_ = ws_.get<double>("xn", n);
+            xs_ = ws_.get<double>("xs", n);
+            hipLaunchKernelGGL(row_norms_kernel<TX>, dim3(grid1(n, 256)), dim3(256), 0, st, X, n,
+                               d, xn_, xs_);
+            check_launch();
+        }
+    }
+
+    // labels/best for every row against centres C (k*d f64, device).
+    void assign(const double *C, int32_t *labels, double *best, KmAcc &acc) {
+        PhaseTimer tm(st_);
+        acc.n_rows += n_;
+        if (direct_) {
+            const int t = tm.begin();
+            hipLaunchKernelGGL(km_assign_exact_kernel<TX>, dim3(grid1(n_, 256)), dim3(256), 0, st_,
+                               X_, C, n_, d_, k_, labels, best);
+            check_launch();
+            tm.end(t);
+            acc.ms_dist += tm.ms(t);
+            acc.n_launch += 1;
+            return;
+        }
+        double *cn = ws_.get<double>("cn", k_);
+        double *cs = ws_.get<double>("cs", k_);
+        hipLaunchKernelGGL(row_norms_kernel<double>, dim3(grid1(k_, 256)), dim3(256), 0, st_, C,
+                           k_, d_, cn, cs);
+        check_launch();
+        Cand cand;
+        cand.key = ws_.get<double>("ckey", n_ * kKmM);
+        cand.idx = ws_.get<int32_t>("cidx", n_ * kKmM);
+        cand.cnt = ws_.get<int32_t>("ccnt", n_);
+        cand.theta = ws_.get<double>("cth", n_);
+        cand.theta_idx = ws_.get<int32_t>("cthi", n_);
+        TopmArgs a;
+        a.nA = n_;
+        a.nB = k_;
+        a.d = d_;
+        a.b_per_split = k_;
+        a.an = xn_;
+        a.as = xs_;
+        a.bn = cn;
+        a.bs = cs;
+        const Bound bd = bound_fp64(d_);
+        a.alpha = bd.alpha;
+        a.beta = bd.beta;
+        a.sqrt_key = 0;
+        a.out = cand;
+        const int t0 = tm.begin();
+        launch_topm<TX, double>(kKmM, MODE_MFMA64, X_, C, a, 1, st_);
+        tm.end(t0);
+        acc.n_launch += 1;
+        int32_t *flags = ws_.get<int32_t>("flags", n_);
+        int32_t *fcnt = ws_.get<int32_t>("fcnt", 1);
+        FL_HIP(hipMemsetAsync(fcnt, 0, sizeof(int32_t), st_));
+        const int t1 = tm.begin();
+        hipLaunchKernelGGL(km_finalize_kernel<TX>, dim3(grid1(n_, 256)), dim3(256), 0, st_, X_, C,
+                           n_, d_, k_, kKmM, cand, labels, best, flags, fcnt);
+        check_launch();
+        tm.end(t1);
+        int32_t nflag = 0;
+        FL_HIP(hipMemcpyAsync(&nflag, fcnt, sizeof(int32_t), hipMemcpyDeviceToHost, st_));
+        FL_HIP(hipStreamSynchronize(st_));
+        acc.ms_dist += tm.ms(t0);
+        acc.ms_fin += tm.ms(t1);
+        acc.n_cand += n_ * (int64_t)(k_ < kKmM ? k_ : kKmM);
+        if (nflag > 0) {
+            acc.n_fb += nflag;
+            const int t2 = tm.begin();
+            hipLaunchKernelGGL(km_fallback_kernel<TX>, dim3(grid1(nflag, 256)), dim3(256), 0, st_,
+                               X_, C, d_, k_, flags, fcnt, labels, best);
+            check_launch();
+            tm.end(t2);
+            acc.ms_fb += tm.ms(t2);
+        }
+    }
+
+    // Deterministic sum of best[0..n) into *out (device).
+    void inertia(const double *best, double *out) {
+        const int64_t nb = ceil_div(n_, SUM_BLK);
+        double *part = ws_.get<double>("part", nb);
+        hipLaunchKernelGGL(block_sum_kernel, dim3((unsigned)nb), dim3(256), 0, st_, best, n_, part);
+        hipLaunchKernelGGL(seq_sum_kernel, dim3(1), dim3(64), 0, st_, part, nb, out);
+        check_launch();
+    }
+
+    // Reference-order centroid update; shift (device) = max centre movement.
+    void update(const int32_t *labels, const double *oldc, double *newc, double *shift,
+                KmAcc &acc) {
+        PhaseTimer tm(st_);
+        const int t = tm.begin();
+        const int64_t nblk = ceil_div(n_, UPD_RB);
+        int32_t *blockcnt = ws_.get<int32_t>("blockcnt", nblk * k_);
+        int32_t *blockbase = ws_.get<int32_t>("blockbase", nblk * k_);
+        int32_t *count = ws_.get<int32_t>("count", k_);
+        int64_t *offset = ws_.get<int64_t>("offset", k_);
+        int32_t *order = ws_.get<int32_t>("order", n_);
+        const size_t hist_b = (size_t)k_ * 4, ctr_b = (size_t)k_ * 8;
+        FL_HIP(hipFuncSetAttribute((const void *)km_block_hist_kernel,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)hist_b));
+        FL_HIP(hipFuncSetAttribute((const void *)km_scatter_kernel,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)ctr_b));
+        hipLaunchKernelGGL(km_block_hist_kernel, dim3((unsigned)nblk), dim3(256), hist_b, st_,
+                           labels, n_, k_, blockcnt);
+        hipLaunchKernelGGL(km_col_scan_kernel, dim3(grid1(k_, 256)), dim3(256), 0, st_, blockcnt,
+                           nblk, k_, blockbase, count);
+        hipLaunchKernelGGL(km_offsets_kernel, dim3(1), dim3(64), 0, st_, count, k_, offset);
+        hipLaunchKernelGGL(km_scatter_kernel, dim3((unsigned)nblk), dim3(64), ctr_b, st_, labels,
+                           n_, k_, blockbase, offset, order);
+        hipLaunchKernelGGL(km_sum_kernel<TX>, dim3((unsigned)k_, (unsigned)ceil_div(d_, 64)),
+                           dim3(64), 0, st_, X_, d_, order, offset, count, oldc, newc);
+        hipLaunchKernelGGL(km_shift_kernel, dim3(1), dim3(256), 0, st_, newc, oldc, k_, d_,
+                           shift);
+        check_launch();
+        tm.end(t);
+        acc.ms_upd += tm.ms(t);
+    }
+
+    // The library's k-means++ into out (k*d f64, device).
+    void kmeanspp(uint64_t seed, double *out) {
+        Rng rng(seed);
+        const int64_t first = rng.range(n_);
+        hipLaunchKernelGGL(copy_row_f64_kernel<TX>, dim3(grid1(d_, 64)), dim3(64), 0, st_, X_,
+                           first, d_, out);
+        check_launch();
+        if (k_ == 1) {
+            FL_HIP(hipStreamSynchronize(st_));
+            return;
+        }
+        double *md = ws_.get<double>("kpp_md", n_);
+        double *nd = ws_.get<double>("kpp_nd", n_);
+        double *bd = ws_.get<double>("kpp_bd", n_);
+        const int64_t nblk = ceil_div(n_, KPP_BLK);
+        double *bsum = ws_.get<double>("kpp_bsum", nblk);
+        double *tot = ws_.get<double>("kpp_tot", 1);
+        double *thr = ws_.get<double>("kpp_thr", 64);
+        int64_t *cand = ws_.get<int64_t>("kpp_cand", 64);
+        hipLaunchKernelGGL(kpp_first_kernel<TX>, dim3(grid1(n_, 256)), dim3(256), 0, st_, X_, n_,
+                           d_, out, md);
+        check_launch();
+        int64_t n_trials = 2 + (int64_t)std::floor(std::log((double)k_));
+        if (n_trials < 1) n_trials = 1;
+        if (n_trials > 64) n_trials = 64;
+        std::vector<double> hthr((size_t)n_trials);
+        std::vector<int64_t> hcand((size_t)n_trials);
+        auto total_of = [&](const double *v) {
+            hipLaunchKernelGGL(kpp_block_sums_kernel, dim3(grid1(nblk, 64)), dim3(64), 0, st_, v,
+                               n_, bsum);
+            hipLaunchKernelGGL(seq_sum_kernel, dim3(1), dim3(64), 0, st_, bsum, nblk, tot);
+            check_launch();
+            double h = 0.0;
+            FL_HIP(hipMemcpyAsync(&h, tot, 8, hipMemcpyDeviceToHost, st_));
+            FL_HIP(hipStreamSynchronize(st_));
+            return h;
+        };
+        for (int64_t c = 1; c < k_; ++c) {
+            const double total = total_of(md);   // leaves md's block sums in bsum
+            if (total <= 0.0) {
+                const int64_t idx = rng.range(n_);
+                hipLaunchKernelGGL(copy_row_f64_kernel<TX>, dim3(grid1(d_, 64)), dim3(64), 0, st_,
+                                   X_, idx, d_, out + c * d_);
+                check_launch();
+                continue;
+            }
+            for (int64_t t = 0; t < n_trials; ++t) hthr[(size_t)t] = rng.f64() * total;
+            FL_HIP(hipMemcpyAsync(thr, hthr.data(), (size_t)n_trials * 8, hipMemcpyHostToDevice,
+                                  st_));
+            hipLaunchKernelGGL(kpp_search_kernel, dim3(1), dim3(64), 0, st_, md, n_, bsum, nblk,
+                               thr, (int)n_trials, cand);
+            check_launch();
+            FL_HIP(hipMemcpyAsync(hcand.data(), cand, (size_t)n_trials * 8,
+                                  hipMemcpyDeviceToHost, st_));
+            FL_HIP(hipStreamSynchronize(st_));
+            int64_t best = 0;
+            double best_pot = 0.0;
+            bool have = false;
+            for (int64_t t = 0; t < n_trials; ++t) {
+                hipLaunchKernelGGL(kpp_trial_kernel<TX>, dim3(grid1(n_, 256)), dim3(256), 0, st_,
+                                   X_, n_, d_, hcand[(size_t)t], md, nd);
+                check_launch();
+                const double pot = total_of(nd);
+                if (!have || pot < best_pot) {
+                    have = true;
+                    best_pot = pot;
+                    best = hcand[(size_t)t];
+                    std::swap(nd, bd);
+                }
+            }
+            hipLaunchKernelGGL(copy_row_f64_kernel<TX>, dim3(grid1(d_, 64)), dim3(64), 0, st_, X_,
+                               best, d_, out + c * d_);
+            check_launch();
+            std::swap(md, bd);
+        }
+        FL_HIP(hipStreamSynchronize(st_));
+    }
+
+    Workspace &ws() { return ws_; }
+
+  private:
+    const TX *X_;
+    int64_t n_;
+    int d_;
+    int64_t k_;
+    hipStream_t st_;
+    bool direct_ = true;
+    double *xn_ = nullptr, *xs_ = nullptr;
+    Workspace ws_;
+};
+
+struct StreamGuard {
+    hipStream_t s = nullptr;
+    StreamGuard() { FL_HIP(hipStreamCreate(&s)); }
+    ~StreamGuard() {
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+
+// kmeans.rs:421-449, same order and conditions.
+fl_status validate_fit(int64_t n, int64_t d, const fl_kmeans_params *p, int32_t dtype) {
+    if (!p) return fail(FL_INVALID_PARAMETER, "InvalidParameter: params: must not be NULL");
+    if (p->n_clusters == 0)
+        return fail(FL_INVALID_PARAMETER, "InvalidParameter: n_clusters: must be at least 1");
+    if (n == 0)
+        return fail(FL_INSUFFICIENT_SAMPLES,
+                    "InsufficientSamples: required " + std::to_string(p->n_clusters) +
+                        ", actual 0: KMeans requires at least n_clusters samples");
+    if (n < p->n_clusters)
+        return fail(FL_INSUFFICIENT_SAMPLES,
+                    "InsufficientSamples: required " + std::to_string(p->n_clusters) +
+                        ", actual " + std::to_string(n) +
+                        ": KMeans requires at least n_clusters samples");
+    if (p->n_init == 0)
+        return fail(FL_INVALID_PARAMETER, "InvalidParameter: n_init: must be at least 1");
+    // Limits of this implementation (no reference counterpart).
+    if (p->n_clusters < 0 || p->n_clusters > kMaxClusters)
+        return fail(FL_INVALID_PARAMETER,
+                    "InvalidParameter: n_clusters: the GPU path supports 1..8192 clusters");
+    if (p->n_init < 0 || p->max_iter < 0)
+        return fail(FL_INVALID_PARAMETER, "InvalidParameter: n_init/max_iter: negative");
+    if (dtype != FL_F32 && dtype != FL_F64)
+        return fail(FL_INVALID_PARAMETER, "InvalidParameter: dtype: must be FL_F32 or FL_F64");
+    if (d <= 0 || d > (1 << 20))
+        return fail(FL_INVALID_PARAMETER, "InvalidParameter: n_features: must be in 1..2^20");
+    if (n > 0x7fffffffLL)
+        return fail(FL_INVALID_PARAMETER, "InvalidParameter: n_samples: at most 2^31-1 rows");
+    if (p->precision != FL_PREC_FP64 && p->precision != FL_PREC_EXACT &&
+        p->precision != FL_PREC_BF16 && p->precision != FL_PREC_BF16X3)
+        return fail(FL_INVALID_PARAMETER, "InvalidParameter: precision: unknown mode");
+    return FL_OK;
+}
+
+template <typename TX>
+fl_status fit_device(const TX *X, int64_t n, int d, const fl_kmeans_params *p,
+                     const double *d_init, double *d_centers, int64_t *d_labels,
+                     double *inertia_out, int64_t *n_iter_out, fl_report *rep) {
+    WallClock wc;
+    StreamGuard sg;
+    hipStream_t st = sg.s;
+    const int64_t k = p->n_clusters;
+    Lloyd<TX> L(X, n, d, k, p->precision, st);
+    Workspace &ws = L.ws();
+    double *cur = ws.get<double>("cur", k * d);
+    double *nxt = ws.get<double>("nxt", k * d);
+    int32_t *labels = ws.get<int32_t>("labels", n);
+    double *best = ws.get<double>("best", n);
+    double *dsc = ws.get<double>("scalars", 2);   // [inertia, shift]
+    FL_HIP(hipMemsetAsync(labels, 0, (size_t)n * 4, st));
+    const uint64_t base = p->has_random_state ? p->random_state : 0;
+    KmAcc acc;
+    double best_inertia = 0.0;
+    int64_t best_iter = 0;
+    for (int64_t run = 0; run < p->n_init; ++run) {
+        if (d_init)
+            FL_HIP(hipMemcpyAsync(cur, d_init + run * k * d, (size_t)(k * d) * 8,
+                                  hipMemcpyDeviceToDevice, st));
+        else
+            L.kmeanspp(base + (uint64_t)run, cur);
+        double inertia = 1.7976931348623157e308;
+        int64_t n_iter = 0;
+        for (int64_t it = 0; it < p->max_iter; ++it) {
+            L.assign(cur, labels, best, acc);
+            L.inertia(best, dsc);
+            L.update(labels, cur, nxt, dsc + 1, acc);
+            double h[2];
+            FL_HIP(hipMemcpyAsync(h, dsc, 16, hipMemcpyDeviceToHost, st));
+            FL_HIP(hipStreamSynchronize(st));
+            inertia = h[0];
+            std::swap(cur, nxt);
+            n_iter = it + 1;
+            if (h[1] < p->tol) break;
+        }
+        if (run == 0 || inertia < best_inertia) {
+            best_inertia = inertia;
+            best_iter = n_iter;
+            FL_HIP(hipMemcpyAsync(d_centers, cur, (size_t)(k * d) * 8, hipMemcpyDeviceToDevice,
+                                  st));
+            hipLaunchKernelGGL(labels_to_i64_kernel, dim3(grid1(n, 256)), dim3(256), 0, st, labels,
+                               n, d_labels);
+            check_launch();
+        }
+    }
+    FL_HIP(hipStreamSynchronize(st));
+    *inertia_out = best_inertia;
+    *n_iter_out = best_iter;
+    if (rep) {
+        std::memset(rep, 0, sizeof(*rep));
+        rep->ms_total = wc.ms();
+        rep->ms_distance = acc.ms_dist;
+        rep->ms_finalize = acc.ms_fin;
+        rep->ms_fallback = acc.ms_fb;
+        rep->ms_update = acc.ms_upd;
+        rep->n_rows = acc.n_rows;
+        rep->n_candidates = acc.n_cand;
+        rep->n_fallback = acc.n_fb;
+        rep->n_iter = best_iter;
+        rep->n_dist_launches = acc.n_launch;
+    }
+    return FL_OK;
+}
+
+fl_status fit_dispatch(const void *dX, int32_t dtype, int64_t n, int64_t d,
+                       const fl_kmeans_params *p, const double *d_init, double *d_centers,
+                       int64_t *d_labels, double *inertia, int64_t *n_iter, fl_report *rep) {
+    if (dtype == FL_F32)
+        return fit_device<float>((const float *)dX, n, (int)d, p, d_init, d_centers, d_labels,
+                                 inertia, n_iter, rep);
+    return fit_device<double>((const double *)dX, n, (int)d, p, d_init, d_centers, d_labels,
+                              inertia, n_iter, rep);
+}
+
+fl_status check_predict(int64_t k, int64_t d_centers, int64_t n, int64_t d, int32_t dtype) {
+    if (d != d_centers)
+        return fail(FL_SHAPE_MISMATCH,
+                    "ShapeMismatch: expected [" + std::to_string(d_centers) + "], actual [" +
+                        std::to_string(d) +
+                        "]: number of features must match fitted KMeans model");
+    if (dtype != FL_F32 && dtype != FL_F64)
+        return fail(FL_INVALID_PARAMETER, "InvalidParameter: dtype: must be FL_F32 or FL_F64");
+    if (k <= 0 || k > kMaxClusters)
+        return fail(FL_INVALID_PARAMETER,
+                    "InvalidParameter: n_clusters: the GPU path supports 1..8192 clusters");
+    if (n < 0 || n > 0x7fffffffLL)
+        return fail(FL_INVALID_PARAMETER, "InvalidParameter: n_samples: out of range");
+    return FL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void fl_kmeans_params_default(fl_kmeans_params *p, int64_t n_clusters) {
+    std::memset(p, 0, sizeof(*p));
+    p->n_clusters = n_clusters;
+    p->max_iter = 300;
+    p->tol = 1e-4;
+    p->n_init = 10;
+    p->has_random_state = 0;
+    p->precision = FL_PREC_FP64;
+}
+
+fl_status fl_kmeans_fit_device(const void *dX, int32_t dtype, int64_t n, int64_t d,
+                               const fl_kmeans_params *params, const double *d_init,
+                               double *d_centers_out, int64_t *d_labels_out, double *inertia_out,
+                               int64_t *n_iter_out, fl_report *report) {
+    return guarded([&]() -> fl_status {
+        if (fl_status s = validate_fit(n, d, params, dtype)) return s;
+        return fit_dispatch(dX, dtype, n, d, params, d_init, d_centers_out, d_labels_out,
+                            inertia_out, n_iter_out, report);
+    });
+}
+
+fl_status fl_kmeans_fit(const void *X, int32_t dtype, int64_t n, int64_t d,
+                        const fl_kmeans_params *params, const double *init_centers,
+                        double *centers_out, int64_t *labels_out, double *inertia_out,
+                        int64_t *n_iter_out, fl_report *report) {
+    return guarded([&]() -> fl_status {
+        if (fl_status s = validate_fit(n, d, params, dtype)) return s;
+        const int64_t k = params->n_clusters;
+        DevBuf bx, bi, bc, bl;
+        const size_t xb = (size_t)(n * d) * dtype_size(dtype);
+        void *dX = bx.get(xb);
+        FL_HIP(hipMemcpy(dX, X, xb, hipMemcpyHostToDevice));
+        double *dI = nullptr;
+        if (init_centers) {
+            const size_t ib = (size_t)(params->n_init * k * d) * 8;
+            dI = (double *)bi.get(ib);
+            FL_HIP(hipMemcpy(dI, init_centers, ib, hipMemcpyHostToDevice));
+        }
+        double *dC = (double *)bc.get((size_t)(k * d) * 8);
+        int64_t *dL = (int64_t *)bl.get((size_t)n * 8);
+        fl_status s = fit_dispatch(dX, dtype, n, d, params, dI, dC, dL, inertia_out, n_iter_out,
+                                   report);
+        if (s) return s;
+        FL_HIP(hipMemcpy(centers_out, dC, (size_t)(k * d) * 8, hipMemcpyDeviceToHost));
+        FL_HIP(hipMemcpy(labels_out, dL, (size_t)n * 8, hipMemcpyDeviceToHost));
+        return FL_OK;
+    });
+}
+
+fl_status fl_kmeans_predict(const double *centers, int64_t k, int64_t d_centers, const void *X,
+                            int32_t dtype, int64_t n, int64_t d, int64_t *labels_out) {
+    return guarded([&]() -> fl_status {
+        if (fl_status s = check_predict(k, d_centers, n, d, dtype)) return s;
+        if (n == 0) return FL_OK;
+        StreamGuard sg;
+        DevBuf bx, bc, bl, bb, b64;
+        const size_t xb = (size_t)(n * d) * dtype_size(dtype);
+        void *dX = bx.get(xb);
+        FL_HIP(hipMemcpy(dX, X, xb, hipMemcpyHostToDevice));
+        double *dC = (double *)bc.get((size_t)(k * d) * 8);
+        FL_HIP(hipMemcpy(dC, centers, (size_t)(k * d) * 8, hipMemcpyHostToDevice));
+        int32_t *dl = (int32_t *)bl.get((size_t)n * 4);
+        double *db = (double *)bb.get((size_t)n * 8);
+        int64_t *d64 = (int64_t *)b64.get((size_t)n * 8);
+        KmAcc acc;
+        if (dtype == FL_F32) {
+            Lloyd<float> L((const float *)dX, n, (int)d, k, FL_PREC_FP64, sg.s);
+            L.assign(dC, dl, db, acc);
+        } else {
+            Lloyd<double> L((const double *)dX, n, (int)d, k, FL_PREC_FP64, sg.s);
+            L.assign(dC, dl, db, acc);
+        }
+        hipLaunchKernelGGL(labels_to_i64_kernel, dim3(grid1(n, 256)), dim3(256), 0, sg.s, dl, n,
+                           d64);
+        check_launch();
+        FL_HIP(hipStreamSynchronize(sg.s));
+        FL_HIP(hipMemcpy(labels_out, d64, (size_t)n * 8, hipMemcpyDeviceToHost));
+        return FL_OK;
+    });
+}
+
+fl_status fl_kmeans_transform(const double *centers, int64_t k, int64_t d_centers, const void *X,
+                              int32_t dtype, int64_t n, int64_t d, double *dist_out) {
+    return guarded([&]() -> fl_status {
+        if (fl_status s = check_predict(k, d_centers, n, d, dtype)) return s;
+        if (n == 0) return FL_OK;
+        DevBuf bx, bc, bo;
+        const size_t xb = (size_t)(n * d) * dtype_size(dtype);
+        void *dX = bx.get(xb);
+        FL_HIP(hipMemcpy(dX, X, xb, hipMemcpyHostToDevice));
+        double *dC = (double *)bc.get((size_t)(k * d) * 8);
+        FL_HIP(hipMemcpy(dC, centers, (size_t)(k * d) * 8, hipMemcpyHostToDevice));
+        double *dO = (double *)bo.get((size_t)(n * k) * 8);
+        if (dtype == FL_F32)
+            hipLaunchKernelGGL(km_transform_kernel<float>, dim3(grid1(n * k, 256)), dim3(256), 0,
+                               0, (const float *)dX, dC, n, (int)d, k, dO);
+        else
+            hipLaunchKernelGGL(km_transform_kernel<double>, dim3(grid1(n * k, 256)), dim3(256), 0,
+                               0, (const double *)dX, dC, n, (int)d, k, dO);
+        check_launch();
+        FL_HIP(hipMemcpy(dist_out, dO, (size_t)(n * k) * 8, hipMemcpyDeviceToHost));
+        return FL_OK;
+    });
+}
+
+fl_status fl_kmeans_plusplus(const void *X, int32_t dtype, int64_t n, int64_t d, int64_t k,
+                             uint64_t seed, double *centers_out) {
+    return guarded([&]() -> fl_status {
+        fl_kmeans_params p;
+        fl_kmeans_params_default(&p, k);
+        if (fl_status s = validate_fit(n, d, &p, dtype)) return s;
+        StreamGuard sg;
+        DevBuf bx, bc;
+        const size_t xb = (size_t)(n * d) * dtype_size(dtype);
+        void *dX = bx.get(xb);
+        FL_HIP(hipMemcpy(dX, X, xb, hipMemcpyHostToDevice));
+        double *dC = (double *)bc.get((size_t)(k * d) * 8);
+        if (dtype == FL_F32) {
+            Lloyd<float> L((const float *)dX, n, (int)d, k, FL_PREC_EXACT, sg.s);
+            L.kmeanspp(seed, dC);
+        } else {
+            Lloyd<double> L((const double *)dX, n, (int)d, k, FL_PREC_EXACT, sg.s);
+            L.kmeanspp(seed, dC);
+        }
+        FL_HIP(hipMemcpy(centers_out, dC, (size_t)(k * d) * 8, hipMemcpyDeviceToHost));
+        return FL_OK;
+    });
+}
+
+}  // extern "C"

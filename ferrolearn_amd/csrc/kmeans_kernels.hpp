@@ -1,0 +1,345 @@
+// kmeans_kernels.hpp -- KMeans assignment decision, inertia, reference-order
+// centroid update and k-means++ passes on gfx950.
+//
+// Reference: ferrolearn-cluster/src/kmeans.rs.  Every decision uses the
+// exact fold (kmeans.rs:159-163) and the strict `<` argmin that keeps the
+// lowest centre index on ties (kmeans.rs:295-309).
+#pragma once
+
+#include "common.hpp"
+#include "dist_topm.hpp"
+
+namespace fl {
+
+// -------------------------------------------------------------- decision --
+// After the filter kernel: fold the kept candidates exactly, pick the
+// smallest (fold, index), and certify it: every centre that was not kept has
+// key >= theta and key <= fold, so theta > best fold proves that no other
+// centre has a smaller fold or an equal fold with a lower index.
+template <typename TX>
+__global__ void km_finalize_kernel(const TX *__restrict__ X, const double *__restrict__ C,
+                                   int64_t n, int d, int64_t k, int M, Cand cand,
+                                   int32_t *__restrict__ labels, double *__restrict__ best,
+                                   int32_t *__restrict__ flag_list,
+                                   int32_t *__restrict__ flag_count) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = cand.cnt[i];
+    const TX *x = X + i * d;
+    double bf = 0.0;
+    int32_t bl = -1;
+    for (int m = 0; m < c; ++m) {
+        const int32_t ix = cand.idx[i * M + m];
+        const double f = fold_exact(x, C + (int64_t)ix * d, d);
+        if (bl < 0 || pair_less(f, ix, bf, bl)) {
+            bf = f;
+            bl = ix;
+        }
+    }
+    bool ok = bl >= 0 && (c == M || (int64_t)c == k);
+    if (ok && c == M) ok = cand.theta[i] > bf;
+    if (!ok) {
+        flag_list[atomicAdd(flag_count, 1)] = (int32_t)i;
+        return;
+    }
+    labels[i] = bl;
+    best[i] = bf;
+}
+
+// Exact scan over every centre for one row: kmeans.rs:295-309 verbatim
+// (init (0, F::MAX), update iff fold < best).
+template <typename TX>
+__device__ __forceinline__ void km_exact_row(const TX *__restrict__ x,
+                                             const double *__restrict__ C, int d, int64_t k,
+                                             int32_t &lab, double &dist) {
+    int32_t bl = 0;
+    double bd = 1.7976931348623157e308;
+    for (int64_t c = 0; c < k; ++c) {
+        const double f = fold_exact(x, C + c * d, d);
+        if (f < bd) {
+            bd = f;
+            bl = (int32_t)c;
+        }
+    }
+    lab = bl;
+    dist = bd;
+}
+
+// Direct exact assignment of every row (used when k*d is small).
+template <typename TX>
+__global__ void km_assign_exact_kernel(const TX *__restrict__ X, const double *__restrict__ C,
+                                       int64_t n, int d, int64_t k,
+                                       int32_t *__restrict__ labels,
+                                       double *__restrict__ best) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int32_t l;
+    double v;
+    km_exact_row(X + i * d, C, d, k, l, v);
+    labels[i] = l;
+    best[i] = v;
+}
+
+// Exact scan for the rows the certificate could not settle.
+template <typename TX>
+__global__ void km_fallback_kernel(const TX *__restrict__ X, const double *__restrict__ C,
+                                   int d, int64_t k, const int32_t *__restrict__ list,
+                                   const int32_t *__restrict__ count,
+                                   int32_t *__restrict__ labels, double *__restrict__ best) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= *count) return;
+    const int64_t i = list[t];
+    int32_t l;
+    double v;
+    km_exact_row(X + i * d, C, d, k, l, v);
+    labels[i] = l;
+    best[i] = v;
+}
+
+// ---------------------------------------------------------------- inertia --
+// Deterministic two-level sum of the per-row best folds: 1024-row blocks are
+// reduced by a fixed tree, then the block sums in block order.  The
+// reference's own parallel inertia is order dependent (kmeans.rs:326-352);
+// both are within a few ulps of the exact sum.
+constexpr int SUM_BLK = 1024;
+
+static __global__ __attribute__((unused)) __launch_bounds__(256) void block_sum_kernel(const double *__restrict__ v, int64_t n,
+                                                        double *__restrict__ part) {
+    __shared__ double s[256];
+    const int64_t b0 = (int64_t)blockIdx.x * SUM_BLK;
+    double acc = 0.0;
+    for (int r = 0; r < SUM_BLK / 256; ++r) {
+        const int64_t i = b0 + r * 256 + threadIdx.x;
+        if (i < n) acc = acc + v[i];
+    }
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) s[threadIdx.x] = s[threadIdx.x] + s[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+static __global__ __attribute__((unused)) void seq_sum_kernel(const double *__restrict__ part, int64_t m,
+                               double *__restrict__ out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double acc = 0.0;
+    for (int64_t i = 0; i < m; ++i) acc = acc + part[i];
+    *out = acc;
+}
+
+// ---------------------------------------------------------- centroid update --
+// kmeans.rs:358-401 sums x[i][j] into sum[label][j] for i ascending.  The
+// GPU reproduces that order exactly: a stable counting sort groups the rows
+// of each cluster in ascending index order, then one lane per (cluster,
+// feature) folds its cluster's rows left to right from +0.
+constexpr int UPD_RB = 4096;   // rows per counting block
+
+static __global__ __attribute__((unused)) __launch_bounds__(256) void km_block_hist_kernel(const int32_t *__restrict__ labels,
+                                                            int64_t n, int64_t k,
+                                                            int32_t *__restrict__ blockcnt) {
+    extern __shared__ int32_t hist[];
+    for (int64_t c = threadIdx.x; c < k; c += blockDim.x) hist[c] = 0;
+    __syncthreads();
+    const int64_t b0 = (int64_t)blockIdx.x * UPD_RB;
+    for (int r = threadIdx.x; r < UPD_RB; r += blockDim.x) {
+        const int64_t i = b0 + r;
+        if (i < n) atomicAdd(&hist[labels[i]], 1);
+    }
+    __syncthreads();
+    for (int64_t c = threadIdx.x; c < k; c += blockDim.x)
+        blockcnt[(int64_t)blockIdx.x * k + c] = hist[c];
+}
+
+// Per cluster: running position of each block's first member, and the
+// cluster's total count.
+static __global__ __attribute__((unused)) void km_col_scan_kernel(const int32_t *__restrict__ blockcnt, int64_t nblk, int64_t k,
+                                   int32_t *__restrict__ blockbase, int32_t *__restrict__ count) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= k) return;
+    int32_t run = 0;
+    for (int64_t b = 0; b < nblk; ++b) {
+        blockbase[b * k + c] = run;
+        run += blockcnt[b * k + c];
+    }
+    count[c] = run;
+}
+
+static __global__ __attribute__((unused)) void km_offsets_kernel(const int32_t *__restrict__ count, int64_t k,
+                                  int64_t *__restrict__ offset) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int64_t run = 0;
+    for (int64_t c = 0; c < k; ++c) {
+        offset[c] = run;
+        run += count[c];
+    }
+}
+
+// One wave per counting block: lanes that share a label in a 64-row chunk
+// are grouped with ballots; ranks follow lane (= row) order, so the scatter
+// is stable.
+static __global__ __attribute__((unused)) __launch_bounds__(64) void km_scatter_kernel(const int32_t *__restrict__ labels,
+                                                        int64_t n, int64_t k,
+                                                        const int32_t *__restrict__ blockbase,
+                                                        const int64_t *__restrict__ offset,
+                                                        int32_t *__restrict__ order) {
+    extern __shared__ int64_t ctr[];
+    const int lane = threadIdx.x;
+    const int64_t b = blockIdx.x;
+    for (int64_t c = lane; c < k; c += 64) ctr[c] = offset[c] + blockbase[b * k + c];
+    wave_sync_lds();
+    const int64_t b0 = b * UPD_RB;
+    for (int r0 = 0; r0 < UPD_RB; r0 += 64) {
+        const int64_t i = b0 + r0 + lane;
+        const bool valid = i < n;
+        const int32_t lab = valid ? labels[i] : -1;
+        uint64_t todo = __ballot(valid);
+        while (todo) {
+            const int leader = __ffsll((unsigned long long)todo) - 1;
+            const int32_t L = __shfl(lab, leader);
+            const uint64_t m = __ballot(valid && lab == L);
+            const int64_t base = ctr[L];
+            if (valid && lab == L) {
+                const int rank = __popcll(m & ((1ull << lane) - 1ull));
+                order[base + rank] = (int32_t)i;
+            }
+            wave_sync_lds();
+            if (lane == leader) ctr[L] = base + __popcll(m);
+            wave_sync_lds();
+            todo &= ~m;
+        }
+    }
+}
+
+// sum[c][j] = fold over the cluster's rows in ascending index order; then
+// divide by the count or keep the old centre for an empty cluster.
+template <typename TX>
+__global__ __launch_bounds__(64) void km_sum_kernel(const TX *__restrict__ X, int d,
+                                                    const int32_t *__restrict__ order,
+                                                    const int64_t *__restrict__ offset,
+                                                    const int32_t *__restrict__ count,
+                                                    const double *__restrict__ oldc,
+                                                    double *__restrict__ newc) {
+    const int64_t c = blockIdx.x;
+    const int j = blockIdx.y * 64 + threadIdx.x;
+    if (j >= d) return;
+    const int64_t off = offset[c];
+    const int32_t cnt = count[c];
+    double acc = 0.0;
+    int32_t m = 0;
+    constexpr int U = 16;
+    for (; m + U <= cnt; m += U) {
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = (double)X[(int64_t)order[off + m + u] * d + j];
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc = acc + v[u];
+    }
+    for (; m < cnt; ++m) acc = acc + (double)X[(int64_t)order[off + m] * d + j];
+    newc[c * d + j] = cnt > 0 ? acc / (double)cnt : oldc[c * d + j];
+}
+
+// max_shift = sqrt(max_c fold(new_c, old_c)) (kmeans.rs:389-400); the max
+// ignores NaN folds the way `if shift > max_shift` does.
+static __global__ __attribute__((unused)) __launch_bounds__(256) void km_shift_kernel(const double *__restrict__ newc,
+                                                       const double *__restrict__ oldc,
+                                                       int64_t k, int d,
+                                                       double *__restrict__ shift) {
+    __shared__ double s[256];
+    double mx = 0.0;
+    for (int64_t c = threadIdx.x; c < k; c += 256) {
+        const double f = fold_exact(newc + c * d, oldc + c * d, d);
+        if (f > mx) mx = f;
+    }
+    s[threadIdx.x] = mx;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h && s[threadIdx.x + h] > s[threadIdx.x])
+            s[threadIdx.x] = s[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *shift = sqrt(s[0]);
+}
+
+// --------------------------------------------------------------- transform --
+template <typename TX>
+__global__ void km_transform_kernel(const TX *__restrict__ X, const double *__restrict__ C,
+                                    int64_t n, int d, int64_t k, double *__restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * k) return;
+    const int64_t i = t / k, c = t % k;
+    out[t] = sqrt(fold_exact(X + i * d, C + c * d, d));
+}
+
+// ---------------------------------------------------------------- k-means++ --
+// The library's k-means++ (kmeans.rs:169-253 with two-level sums, see
+// KPP_BLK): the distance passes run here, the generator on the host.
+constexpr int KPP_BLK = 4096;
+
+template <typename TX>
+__global__ void kpp_first_kernel(const TX *__restrict__ X, int64_t n, int d,
+                                 const double *__restrict__ c0, double *__restrict__ md) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) md[i] = fold_exact(X + i * d, c0, d);
+}
+
+// nd[i] = min(md[i], fold(x_i, cand)) with the reference's `if d < nd`.
+template <typename TX>
+__global__ void kpp_trial_kernel(const TX *__restrict__ X, int64_t n, int d, int64_t cand,
+                                 const double *__restrict__ md, double *__restrict__ nd) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double v = fold_exact(X + i * d, X + cand * d, d);
+    const double m = md[i];
+    nd[i] = v < m ? v : m;
+}
+
+// Sequential fold of each KPP_BLK block (one lane per block).
+static __global__ __attribute__((unused)) void kpp_block_sums_kernel(const double *__restrict__ v, int64_t n,
+                                      double *__restrict__ bsum) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t lo = b * KPP_BLK;
+    if (lo >= n) return;
+    const int64_t hi = lo + KPP_BLK < n ? lo + KPP_BLK : n;
+    double s = 0.0;
+    for (int64_t i = lo; i < hi; ++i) s = s + v[i];
+    bsum[b] = s;
+}
+
+// out[0] = total (block sums folded in order); for each of nthr thresholds,
+// the first index whose two-level running sum reaches it (n-1 if none).
+static __global__ __attribute__((unused)) void kpp_search_kernel(const double *__restrict__ v, int64_t n,
+                                  const double *__restrict__ bsum, int64_t nblk,
+                                  const double *__restrict__ thr, int nthr,
+                                  int64_t *__restrict__ cand) {
+    const int t = threadIdx.x;
+    if (t >= nthr) return;
+    const double th = thr[t];
+    double run = 0.0;
+    int64_t res = n - 1;
+    for (int64_t b = 0; b < nblk; ++b) {
+        const double s = bsum[b];
+        if (run + s >= th) {
+            const int64_t lo = b * KPP_BLK;
+            const int64_t hi = lo + KPP_BLK < n ? lo + KPP_BLK : n;
+            double c = run;
+            int64_t found = -1;
+            for (int64_t i = lo; i < hi; ++i) {
+                c = c + v[i];
+                if (c >= th) {
+                    found = i;
+                    break;
+                }
+            }
+            if (found >= 0) {
+                res = found;
+                break;
+            }
+        }
+        run = run + s;
+    }
+    cand[t] = res;
+}
+
+}  // namespace fl

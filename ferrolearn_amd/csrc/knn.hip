@@ -1,0 +1,565 @@
+// knn.hip -- brute-force KNeighborsClassifier / KNeighborsRegressor on gfx950.
+//
+// Reference: ferrolearn-neighbors/src/knn.rs (fit :256-320 / :660-718,
+// predict :322-383 / :720-781, predict_proba :452-486, kneighbors
+// :823-834 / :875-917) with Algorithm::BruteForce (knn.rs:148) whose search
+// is kdtree.rs:364-391.  The fitted handle owns device copies of X and y, as
+// the fitted Rust model owns clones of them (knn.rs:311-318).
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "host.hpp"
+
+using namespace fl;
+
+struct fl_knn {
+    int is_classifier = 0;
+    int32_t dtype = FL_F64;
+    int64_t n = 0, d = 0, n_neighbors = 5;
+    int32_t weights = FL_UNIFORM;
+    int32_t precision = FL_PREC_FP64;
+    DevBuf X;                      // n*d in dtype
+    DevBuf xn, xs;                 // row squared norms and norms (f64)
+    std::vector<int64_t> classes;  // sorted unique labels (classifier)
+    DevBuf ypos, classes_dev;      // label position per row, classes
+    DevBuf yreg;                   // regression targets
+    hipStream_t stream = nullptr;
+    Workspace ws;
+    ~fl_knn() {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+constexpr int64_t kMaxNeighbors = 64;
+
+std::string shape_msg(int64_t expected, int64_t actual, const char *ctx) {
+    return "ShapeMismatch: expected [" + std::to_string(expected) + "], actual [" +
+           std::to_string(actual) + "]: " + ctx;
+}
+
+int pick_m_filter(int64_t k) {
+    for (int m : {16, 32, 64})
+        if (k + 6 <= m) return m;
+    return 64;
+}
+
+int pick_m_exact(int64_t k) {
+    for (int m : {16, 32, 64})
+        if (k <= m) return m;
+    return 64;
+}
+
+struct Acc {
+    double ms_dist = 0, ms_fin = 0, ms_fb = 0, ms_upd = 0;
+    int64_t n_cand = 0, n_fb = 0, n_launch = 0;
+};
+
+__global__ void gather_rows_kernel(const double *__restrict__ Q, int d,
+                                   const int32_t *__restrict__ list, int64_t cnt,
+                                   double *__restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cnt * d) return;
+    const int64_t r = t / d, j = t % d;
+    out[t] = Q[(int64_t)list[r] * d + j];
+}
+
+__global__ void scatter_results_kernel(const double *__restrict__ dsrc,
+                                       const int64_t *__restrict__ isrc,
+                                       const int32_t *__restrict__ list, int64_t cnt, int k,
+                                       double *__restrict__ dist, int64_t *__restrict__ idx) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cnt * k) return;
+    const int64_t r = t / k, m = t % k;
+    dist[(int64_t)list[r] * k + m] = dsrc[t];
+    idx[(int64_t)list[r] * k + m] = isrc[t];
+}
+
+// Exact top-k of nq f64 queries against the fitted rows.  mode selects the
+// candidate filter; queries the certificate cannot settle are re-run in the
+// exact mode, whose kept lists are exact by construction.
+template <typename TB>
+void knn_search(fl_knn *h, const double *Q, int64_t nq, int k, int mode, double *dist,
+                int64_t *idx, Acc &acc, int depth) {
+    if (nq == 0) return;
+    hipStream_t st = h->stream;
+    Workspace &ws = h->ws;
+    const std::string p = depth ? "fb_" : "";
+    const TB *B = (const TB *)h->X.ptr();
+    const int64_t nB = h->n;
+    const int d = (int)h->d;
+
+    double *qn = ws.get<double>(p + "qn", nq);
+    double *qs = ws.get<double>(p + "qs", nq);
+    hipLaunchKernelGGL(row_norms_kernel<double>, dim3(grid1(nq, 256)), dim3(256), 0, st, Q, nq,
+                       d, qn, qs);
+    check_launch();
+
+    const int M = mode == MODE_EXACT ? pick_m_exact(k) : pick_m_filter(k);
+    const int64_t nblk = ceil_div(nq, TM_ROWS);
+    int64_t S = ceil_div(2048, nblk);
+    S = std::min<int64_t>(S, FIN_MAXC / M);
+    S = std::min<int64_t>(S, std::max<int64_t>(1, nB / 256));
+    S = std::max<int64_t>(S, 1);
+    const int64_t bps = ceil_div(nB, S);
+    S = ceil_div(nB, bps);
+
+    Cand cand;
+    cand.key = ws.get<double>(p + "ckey", S * nq * M);
+    cand.idx = ws.get<int32_t>(p + "cidx", S * nq * M);
+    cand.cnt = ws.get<int32_t>(p + "ccnt", S * nq);
+    cand.theta = ws.get<double>(p + "cth", S * nq);
+    cand.theta_idx = ws.get<int32_t>(p + "cthi", S * nq);
+
+    TopmArgs a;
+    a.nA = nq;
+    a.nB = nB;
+    a.d = d;
+    a.b_per_split = bps;
+    a.an = qn;
+    a.as = qs;
+    a.bn = (const double *)h->xn.ptr();
+    a.bs = (const double *)h->xs.ptr();
+    const Bound bd = bound_fp64(d);
+    a.alpha = bd.alpha;
+    a.beta = bd.beta;
+    a.sqrt_key = 1;
+    a.out = cand;
+
+    PhaseTimer tm(st);
+    const int t0 = tm.begin();
+    launch_topm<double, TB>(M, mode, Q, B, a, (int)S, st);
+    tm.end(t0);
+    acc.n_launch += 1;
+
+    int32_t *flags = ws.get<int32_t>(p + "flags", nq);
+    int32_t *fcnt = ws.get<int32_t>(p + "fcnt", 1);
+    unsigned long long *ncand = ws.get<unsigned long long>(p + "ncand", 1);
+    FL_HIP(hipMemsetAsync(fcnt, 0, sizeof(int32_t), st));
+    FL_HIP(hipMemsetAsync(ncand, 0, sizeof(unsigned long long), st));
+    const int t1 = tm.begin();
+    hipLaunchKernelGGL(knn_finalize_kernel<TB>, dim3((unsigned)nq), dim3(64), 0, st, Q, B, d, nq,
+                       nB, (int)S, M, bps, cand, k, mode == MODE_EXACT ? 0 : 1, dist, idx, flags,
+                       fcnt, ncand);
+    check_launch();
+    tm.end(t1);
+    int32_t nflag = 0;
+    unsigned long long nc = 0;
+    FL_HIP(hipMemcpyAsync(&nflag, fcnt, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    FL_HIP(hipMemcpyAsync(&nc, ncand, sizeof(nc), hipMemcpyDeviceToHost, st));
+    FL_HIP(hipStreamSynchronize(st));
+    if (depth == 0) {
+        acc.ms_dist += tm.ms(t0);
+        acc.ms_fin += tm.ms(t1);
+    } else {
+        acc.ms_fb += tm.ms(t0) + tm.ms(t1);
+    }
+    acc.n_cand += (int64_t)nc;
+    if (nflag == 0) return;
+    if (mode == MODE_EXACT)
+        throw StatusError{FL_NUMERICAL_INSTABILITY,
+                          "NumericalInstability: a distance is NaN (the reference panics "
+                          "here, kdtree.rs:386)"};
+    acc.n_fb += nflag;
+    double *Qf = ws.get<double>("fbq", (int64_t)nflag * d);
+    hipLaunchKernelGGL(gather_rows_kernel, dim3(grid1((int64_t)nflag * d, 256)), dim3(256), 0, st,
+                       Q, d, flags, (int64_t)nflag, Qf);
+    check_launch();
+    double *df = ws.get<double>("fbd", (int64_t)nflag * k);
+    int64_t *xf = ws.get<int64_t>("fbi", (int64_t)nflag * k);
+    knn_search<TB>(h, Qf, nflag, k, MODE_EXACT, df, xf, acc, depth + 1);
+    hipLaunchKernelGGL(scatter_results_kernel, dim3(grid1((int64_t)nflag * k, 256)), dim3(256), 0,
+                       st, df, xf, flags, (int64_t)nflag, k, dist, idx);
+    check_launch();
+}
+
+int filter_mode(int32_t precision) {
+    return precision == FL_PREC_EXACT ? MODE_EXACT : MODE_MFMA64;
+}
+
+// Top-k search for device queries of either dtype into device outputs.
+void search_device(fl_knn *h, const void *dQ, int32_t dtype, int64_t nq, int k, double *dist,
+                   int64_t *idx, Acc &acc) {
+    const double *Q = dtype == FL_F32
+                          ? as_f64<float>((const float *)dQ, nq * h->d, h->ws, "q64", h->stream)
+                          : (const double *)dQ;
+    const int mode = filter_mode(h->precision);
+    if (h->dtype == FL_F32)
+        knn_search<float>(h, Q, nq, k, mode, dist, idx, acc, 0);
+    else
+        knn_search<double>(h, Q, nq, k, mode, dist, idx, acc, 0);
+}
+
+void fill_report(fl_report *r, const Acc &a, int64_t nq, double wall) {
+    if (!r) return;
+    std::memset(r, 0, sizeof(*r));
+    r->ms_total = wall;
+    r->ms_distance = a.ms_dist;
+    r->ms_finalize = a.ms_fin;
+    r->ms_fallback = a.ms_fb;
+    r->ms_update = a.ms_upd;
+    r->n_rows = nq;
+    r->n_candidates = a.n_cand;
+    r->n_fallback = a.n_fb;
+    r->n_dist_launches = a.n_launch;
+}
+
+fl_status check_dtype(int32_t dtype) {
+    if (dtype != FL_F32 && dtype != FL_F64)
+        return fail(FL_INVALID_PARAMETER, "InvalidParameter: dtype: must be FL_F32 or FL_F64");
+    return FL_OK;
+}
+
+fl_status check_precision(int32_t p) {
+    if (p != FL_PREC_FP64 && p != FL_PREC_EXACT && p != FL_PREC_BF16 && p != FL_PREC_BF16X3)
+        return fail(FL_INVALID_PARAMETER, "InvalidParameter: precision: unknown mode");
+    return FL_OK;
+}
+
+// knn.rs:277-301 validation order, then the device copy.
+fl_status knn_fit_common(const void *X, int32_t dtype, int64_t n, int64_t d, int64_t y_len,
+                         int64_t n_neighbors, int32_t weights, int32_t precision,
+                         const char *what, fl_knn **out) {
+    if (!out) return fail(FL_INVALID_PARAMETER, "InvalidParameter: out: must not be NULL");
+    *out = nullptr;
+    if (fl_status s = check_dtype(dtype)) return s;
+    if (n != y_len)
+        return fail(FL_SHAPE_MISMATCH,
+                    shape_msg(n, y_len, "y length must match number of samples in X"));
+    if (n_neighbors == 0)
+        return fail(FL_INVALID_PARAMETER, "InvalidParameter: n_neighbors: must be at least 1");
+    if (n < n_neighbors)
+        return fail(FL_INSUFFICIENT_SAMPLES,
+                    "InsufficientSamples: required " + std::to_string(n_neighbors) + ", actual " +
+                        std::to_string(n) + ": " + what + " requires at least n_neighbors=" +
+                        std::to_string(n_neighbors) + " samples");
+    if (n_neighbors < 0 || n_neighbors > kMaxNeighbors)
+        return fail(FL_INVALID_PARAMETER,
+                    "InvalidParameter: n_neighbors: the GPU brute-force path supports 1..64");
+    if (d <= 0 || d > (1 << 20))
+        return fail(FL_INVALID_PARAMETER, "InvalidParameter: n_features: must be in 1..2^20");
+    if (n > 0x7fffffffLL)
+        return fail(FL_INVALID_PARAMETER, "InvalidParameter: n_samples: at most 2^31-1 rows");
+    if (weights != FL_UNIFORM && weights != FL_DISTANCE)
+        return fail(FL_INVALID_PARAMETER, "InvalidParameter: weights: unknown scheme");
+    if (fl_status s = check_precision(precision)) return s;
+    if (!X) return fail(FL_INVALID_PARAMETER, "InvalidParameter: X: must not be NULL");
+
+    fl_knn *h = new fl_knn();
+    try {
+        h->dtype = dtype;
+        h->n = n;
+        h->d = d;
+        h->n_neighbors = n_neighbors;
+        h->weights = weights;
+        h->precision = precision;
+        FL_HIP(hipStreamCreate(&h->stream));
+        const size_t bytes = (size_t)(n * d) * dtype_size(dtype);
+        void *dX = h->X.get(bytes);
+        FL_HIP(hipMemcpyAsync(dX, X, bytes, hipMemcpyHostToDevice, h->stream));
+        double *xn = (double *)h->xn.get((size_t)n * 8);
+        double *xs = (double *)h->xs.get((size_t)n * 8);
+        if (dtype == FL_F32)
+            hipLaunchKernelGGL(row_norms_kernel<float>, dim3(grid1(n, 256)), dim3(256), 0,
+                               h->stream, (const float *)dX, n, (int)d, xn, xs);
+        else
+            hipLaunchKernelGGL(row_norms_kernel<double>, dim3(grid1(n, 256)), dim3(256), 0,
+                               h->stream, (const double *)dX, n, (int)d, xn, xs);
+        check_launch();
+        FL_HIP(hipStreamSynchronize(h->stream));
+    } catch (...) {
+        delete h;
+        throw;
+    }
+    *out = h;
+    return FL_OK;
+}
+
+fl_status check_query(fl_knn *h, const void *Q, int32_t dtype, int64_t nq, int64_t d) {
+    if (!h) return fail(FL_INVALID_PARAMETER, "InvalidParameter: handle: must not be NULL");
+    if (fl_status s = check_dtype(dtype)) return s;
+    if (d != h->d)
+        return fail(FL_SHAPE_MISMATCH,
+                    shape_msg(h->d, d, "number of features must match training data"));
+    if (nq < 0) return fail(FL_INVALID_PARAMETER, "InvalidParameter: n_queries: negative");
+    if (nq > 0 && !Q) return fail(FL_INVALID_PARAMETER, "InvalidParameter: Q: must not be NULL");
+    return FL_OK;
+}
+
+// Host-pointer wrapper: copy Q in, run on device buffers, copy out.
+template <typename Run>
+fl_status host_call(fl_knn *h, const void *Q, int32_t dtype, int64_t nq, Run &&run) {
+    void *dQ = nullptr;
+    if (nq > 0) {
+        const size_t bytes = (size_t)(nq * h->d) * dtype_size(dtype);
+        dQ = h->ws.get<unsigned char>("qhost", (int64_t)bytes);
+        FL_HIP(hipMemcpyAsync(dQ, Q, bytes, hipMemcpyHostToDevice, h->stream));
+    }
+    return run(dQ);
+}
+
+fl_status kneighbors_dev(fl_knn *h, const void *dQ, int32_t dtype, int64_t nq, int64_t k,
+                         double *d_dist, int64_t *d_idx, fl_report *rep) {
+    WallClock wc;
+    Acc acc;
+    search_device(h, dQ, dtype, nq, (int)k, d_dist, d_idx, acc);
+    FL_HIP(hipStreamSynchronize(h->stream));
+    fill_report(rep, acc, nq, wc.ms());
+    return FL_OK;
+}
+
+fl_status classify_dev(fl_knn *h, const void *dQ, int32_t dtype, int64_t nq, int64_t *d_labels,
+                       double *d_proba, fl_report *rep) {
+    WallClock wc;
+    Acc acc;
+    const int k = (int)h->n_neighbors;
+    double *dist = h->ws.get<double>("kdist", nq * k);
+    int64_t *idx = h->ws.get<int64_t>("kidx", nq * k);
+    search_device(h, dQ, dtype, nq, k, dist, idx, acc);
+    if (nq > 0) {
+        PhaseTimer tm(h->stream);
+        const int t = tm.begin();
+        if (d_labels)
+            hipLaunchKernelGGL(knn_vote_kernel, dim3(grid1(nq, 256)), dim3(256), 0, h->stream,
+                               dist, idx, nq, k, (const int32_t *)h->ypos.ptr(),
+                               (const int64_t *)h->classes_dev.ptr(), h->weights, d_labels);
+        else
+            hipLaunchKernelGGL(knn_proba_kernel, dim3(grid1(nq, 256)), dim3(256), 0, h->stream,
+                               dist, idx, nq, k, (const int32_t *)h->ypos.ptr(),
+                               (int)h->classes.size(), h->weights, d_proba);
+        check_launch();
+        tm.end(t);
+        acc.ms_upd += tm.ms(t);
+    }
+    FL_HIP(hipStreamSynchronize(h->stream));
+    fill_report(rep, acc, nq, wc.ms());
+    return FL_OK;
+}
+
+fl_status regress_dev(fl_knn *h, const void *dQ, int32_t dtype, int64_t nq, double *d_y,
+                      fl_report *rep) {
+    WallClock wc;
+    Acc acc;
+    const int k = (int)h->n_neighbors;
+    double *dist = h->ws.get<double>("kdist", nq * k);
+    int64_t *idx = h->ws.get<int64_t>("kidx", nq * k);
+    search_device(h, dQ, dtype, nq, k, dist, idx, acc);
+    if (nq > 0) {
+        PhaseTimer tm(h->stream);
+        const int t = tm.begin();
+        hipLaunchKernelGGL(knn_mean_kernel, dim3(grid1(nq, 256)), dim3(256), 0, h->stream, dist,
+                           idx, nq, k, (const double *)h->yreg.ptr(), h->weights, d_y);
+        check_launch();
+        tm.end(t);
+        acc.ms_upd += tm.ms(t);
+    }
+    FL_HIP(hipStreamSynchronize(h->stream));
+    fill_report(rep, acc, nq, wc.ms());
+    return FL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+fl_status fl_knn_fit_classifier(const void *X, int32_t dtype, int64_t n, int64_t d,
+                                const int64_t *y, int64_t y_len, int64_t n_neighbors,
+                                int32_t weights, int32_t precision, fl_knn_handle *out) {
+    return guarded([&]() -> fl_status {
+        fl_knn *h = nullptr;
+        fl_status s = knn_fit_common(X, dtype, n, d, y_len, n_neighbors, weights, precision,
+                                     "KNeighborsClassifier", &h);
+        if (s) return s;
+        try {
+            h->is_classifier = 1;
+            // knn.rs:304-306: classes = sort + dedup of y.
+            h->classes.assign(y, y + n);
+            std::sort(h->classes.begin(), h->classes.end());
+            h->classes.erase(std::unique(h->classes.begin(), h->classes.end()),
+                             h->classes.end());
+            std::vector<int32_t> pos((size_t)n);
+            for (int64_t i = 0; i < n; ++i)
+                pos[(size_t)i] = (int32_t)(std::lower_bound(h->classes.begin(),
+                                                            h->classes.end(), y[i]) -
+                                           h->classes.begin());
+            void *dp = h->ypos.get((size_t)n * 4);
+            FL_HIP(hipMemcpy(dp, pos.data(), (size_t)n * 4, hipMemcpyHostToDevice));
+            void *dc = h->classes_dev.get(h->classes.size() * 8);
+            FL_HIP(hipMemcpy(dc, h->classes.data(), h->classes.size() * 8,
+                             hipMemcpyHostToDevice));
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+        return FL_OK;
+    });
+}
+
+fl_status fl_knn_fit_regressor(const void *X, int32_t dtype, int64_t n, int64_t d,
+                               const double *y, int64_t y_len, int64_t n_neighbors,
+                               int32_t weights, int32_t precision, fl_knn_handle *out) {
+    return guarded([&]() -> fl_status {
+        fl_knn *h = nullptr;
+        fl_status s = knn_fit_common(X, dtype, n, d, y_len, n_neighbors, weights, precision,
+                                     "KNeighborsRegressor", &h);
+        if (s) return s;
+        try {
+            void *dy = h->yreg.get((size_t)n * 8);
+            FL_HIP(hipMemcpy(dy, y, (size_t)n * 8, hipMemcpyHostToDevice));
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+        return FL_OK;
+    });
+}
+
+fl_status fl_knn_kneighbors_device(fl_knn_handle h, const void *dQ, int32_t dtype, int64_t nq,
+                                   int64_t d, int64_t k, double *d_dist_out, int64_t *d_idx_out,
+                                   fl_report *report) {
+    return guarded([&]() -> fl_status {
+        if (fl_status s = check_query(h, dQ, dtype, nq, d)) return s;
+        if (k == 0) k = h->n_neighbors;
+        if (k <= 0)
+            return fail(FL_INVALID_PARAMETER, "InvalidParameter: n_neighbors: must be at least 1");
+        if (k > h->n)
+            return fail(FL_INSUFFICIENT_SAMPLES,
+                        "InsufficientSamples: required " + std::to_string(k) + ", actual " +
+                            std::to_string(h->n) +
+                            ": n_neighbors exceeds number of training samples");
+        if (k > kMaxNeighbors)
+            return fail(FL_INVALID_PARAMETER,
+                        "InvalidParameter: n_neighbors: the GPU brute-force path supports 1..64");
+        return kneighbors_dev(h, dQ, dtype, nq, k, d_dist_out, d_idx_out, report);
+    });
+}
+
+fl_status fl_knn_kneighbors(fl_knn_handle h, const void *Q, int32_t dtype, int64_t nq, int64_t d,
+                            int64_t k, double *dist_out, int64_t *idx_out, fl_report *report) {
+    return guarded([&]() -> fl_status {
+        if (fl_status s = check_query(h, Q, dtype, nq, d)) return s;
+        if (k == 0) k = h->n_neighbors;
+        if (k <= 0)
+            return fail(FL_INVALID_PARAMETER, "InvalidParameter: n_neighbors: must be at least 1");
+        if (k > h->n)
+            return fail(FL_INSUFFICIENT_SAMPLES,
+                        "InsufficientSamples: required " + std::to_string(k) + ", actual " +
+                            std::to_string(h->n) +
+                            ": n_neighbors exceeds number of training samples");
+        if (k > kMaxNeighbors)
+            return fail(FL_INVALID_PARAMETER,
+                        "InvalidParameter: n_neighbors: the GPU brute-force path supports 1..64");
+        return host_call(h, Q, dtype, nq, [&](const void *dQ) -> fl_status {
+            double *dd = h->ws.get<double>("odist", nq * k);
+            int64_t *di = h->ws.get<int64_t>("oidx", nq * k);
+            fl_status s = kneighbors_dev(h, dQ, dtype, nq, k, dd, di, report);
+            if (s) return s;
+            if (nq > 0) {
+                FL_HIP(hipMemcpy(dist_out, dd, (size_t)(nq * k) * 8, hipMemcpyDeviceToHost));
+                FL_HIP(hipMemcpy(idx_out, di, (size_t)(nq * k) * 8, hipMemcpyDeviceToHost));
+            }
+            return FL_OK;
+        });
+    });
+}
+
+fl_status fl_knn_predict_classify_device(fl_knn_handle h, const void *dQ, int32_t dtype,
+                                         int64_t nq, int64_t d, int64_t *d_labels_out,
+                                         fl_report *report) {
+    return guarded([&]() -> fl_status {
+        if (fl_status s = check_query(h, dQ, dtype, nq, d)) return s;
+        if (!h->is_classifier)
+            return fail(FL_INVALID_PARAMETER, "InvalidParameter: handle: not a classifier");
+        return classify_dev(h, dQ, dtype, nq, d_labels_out, nullptr, report);
+    });
+}
+
+fl_status fl_knn_predict_classify(fl_knn_handle h, const void *Q, int32_t dtype, int64_t nq,
+                                  int64_t d, int64_t *labels_out, fl_report *report) {
+    return guarded([&]() -> fl_status {
+        if (fl_status s = check_query(h, Q, dtype, nq, d)) return s;
+        if (!h->is_classifier)
+            return fail(FL_INVALID_PARAMETER, "InvalidParameter: handle: not a classifier");
+        return host_call(h, Q, dtype, nq, [&](const void *dQ) -> fl_status {
+            int64_t *dl = h->ws.get<int64_t>("olab", nq);
+            fl_status s = classify_dev(h, dQ, dtype, nq, dl, nullptr, report);
+            if (s) return s;
+            if (nq > 0)
+                FL_HIP(hipMemcpy(labels_out, dl, (size_t)nq * 8, hipMemcpyDeviceToHost));
+            return FL_OK;
+        });
+    });
+}
+
+fl_status fl_knn_predict_proba(fl_knn_handle h, const void *Q, int32_t dtype, int64_t nq,
+                               int64_t d, double *proba_out, fl_report *report) {
+    return guarded([&]() -> fl_status {
+        if (fl_status s = check_query(h, Q, dtype, nq, d)) return s;
+        if (!h->is_classifier)
+            return fail(FL_INVALID_PARAMETER, "InvalidParameter: handle: not a classifier");
+        const int64_t C = (int64_t)h->classes.size();
+        return host_call(h, Q, dtype, nq, [&](const void *dQ) -> fl_status {
+            double *dp = h->ws.get<double>("oproba", nq * C);
+            fl_status s = classify_dev(h, dQ, dtype, nq, nullptr, dp, report);
+            if (s) return s;
+            if (nq > 0)
+                FL_HIP(hipMemcpy(proba_out, dp, (size_t)(nq * C) * 8, hipMemcpyDeviceToHost));
+            return FL_OK;
+        });
+    });
+}
+
+fl_status fl_knn_predict_regress_device(fl_knn_handle h, const void *dQ, int32_t dtype,
+                                        int64_t nq, int64_t d, double *d_y_out,
+                                        fl_report *report) {
+    return guarded([&]() -> fl_status {
+        if (fl_status s = check_query(h, dQ, dtype, nq, d)) return s;
+        if (h->is_classifier)
+            return fail(FL_INVALID_PARAMETER, "InvalidParameter: handle: not a regressor");
+        return regress_dev(h, dQ, dtype, nq, d_y_out, report);
+    });
+}
+
+fl_status fl_knn_predict_regress(fl_knn_handle h, const void *Q, int32_t dtype, int64_t nq,
+                                 int64_t d, double *y_out, fl_report *report) {
+    return guarded([&]() -> fl_status {
+        if (fl_status s = check_query(h, Q, dtype, nq, d)) return s;
+        if (h->is_classifier)
+            return fail(FL_INVALID_PARAMETER, "InvalidParameter: handle: not a regressor");
+        return host_call(h, Q, dtype, nq, [&](const void *dQ) -> fl_status {
+            double *dy = h->ws.get<double>("oy", nq);
+            fl_status s = regress_dev(h, dQ, dtype, nq, dy, report);
+            if (s) return s;
+            if (nq > 0) FL_HIP(hipMemcpy(y_out, dy, (size_t)nq * 8, hipMemcpyDeviceToHost));
+            return FL_OK;
+        });
+    });
+}
+
+fl_status fl_knn_set_precision(fl_knn_handle h, int32_t precision) {
+    return guarded([&]() -> fl_status {
+        if (!h) return fail(FL_INVALID_PARAMETER, "InvalidParameter: handle: must not be NULL");
+        if (fl_status s = check_precision(precision)) return s;
+        h->precision = precision;
+        return FL_OK;
+    });
+}
+
+int64_t fl_knn_n_classes(fl_knn_handle h) { return h ? (int64_t)h->classes.size() : 0; }
+
+fl_status fl_knn_classes(fl_knn_handle h, int64_t *classes_out) {
+    if (!h) return fail(FL_INVALID_PARAMETER, "InvalidParameter: handle: must not be NULL");
+    std::copy(h->classes.begin(), h->classes.end(), classes_out);
+    return FL_OK;
+}
+
+int64_t fl_knn_n_samples_fit(fl_knn_handle h) { return h ? h->n : 0; }
+
+void fl_knn_free(fl_knn_handle h) { delete h; }
+
+}  // extern "C"

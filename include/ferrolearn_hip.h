@@ -1,0 +1,234 @@
+/*
+ * ferrolearn_hip.h -- C ABI of the MI355X (gfx950) squared-Euclidean path.
+ *
+ * Drop-in boundary for ferrolearn's KMeans / MiniBatchKMeans Lloyd step and
+ * brute-force KNeighborsClassifier / KNeighborsRegressor predict.  Every
+ * entry point cites the reference interface it replaces.  Conventions:
+ *
+ *   - matrices are row-major, C-contiguous; labels and indices are int64_t
+ *     (Rust usize); output buffers are caller-allocated;
+ *   - host-pointer entry points copy inputs to the GPU and results back;
+ *     the *_device entry points take device pointers already in HBM;
+ *   - arguments are validated in the same order and with the same
+ *     conditions as the Rust code before anything is launched; on error no
+ *     output is written and fl_last_error() holds the context string;
+ *   - labels, iteration counts and neighbour indices equal the reference's
+ *     f64 CPU path bit for bit, whatever the precision mode: the reduced
+ *     precision forms only filter candidates, and every decision is taken on
+ *     exact f64 folds (kmeans.rs:159-163) with the reference's tie rules;
+ *   - a handle is not shared between threads without external locking;
+ *     distinct handles may run concurrently (one HIP stream each).
+ */
+#ifndef FERROLEARN_HIP_H
+#define FERROLEARN_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ferrolearn-core/src/error.rs:30-91 (FerroError), one code per variant used
+ * on this path, plus FL_DEVICE_ERROR for HIP failures. */
+typedef enum fl_status {
+    FL_OK = 0,
+    FL_SHAPE_MISMATCH = 1,        /* FerroError::ShapeMismatch          */
+    FL_INSUFFICIENT_SAMPLES = 2,  /* FerroError::InsufficientSamples    */
+    FL_INVALID_PARAMETER = 3,     /* FerroError::InvalidParameter       */
+    FL_NUMERICAL_INSTABILITY = 4, /* FerroError::NumericalInstability   */
+    FL_DEVICE_ERROR = 5           /* HIP runtime error (no Rust analogue) */
+} fl_status;
+
+/* Storage type of an input matrix.  Arithmetic is f64 either way: f32
+ * values are widened exactly, as the reference's f64 path sees them. */
+typedef enum fl_dtype { FL_F32 = 0, FL_F64 = 1 } fl_dtype;
+
+/* How candidate distances are filtered before the exact f64 decision.
+ *   FL_PREC_FP64  : f64 MFMA GEMM form ||a||^2 + ||b||^2 - 2 a.b (plain form)
+ *   FL_PREC_BF16  : a.b on BF16 MFMA, FP32 accumulation
+ *   FL_PREC_BF16X3: a.b as hi*hi + hi*lo + lo*hi BF16 MFMA products
+ *   FL_PREC_EXACT : direct f64 fold for every pair (no filter)
+ * Results are identical in every mode; only speed and the share of rows
+ * that need an exact re-score differ. */
+typedef enum fl_precision {
+    FL_PREC_FP64 = 0,
+    FL_PREC_BF16 = 1,
+    FL_PREC_BF16X3 = 2,
+    FL_PREC_EXACT = 3
+} fl_precision;
+
+/* knn.rs:72-79 (Weights) */
+typedef enum fl_weights { FL_UNIFORM = 0, FL_DISTANCE = 1 } fl_weights;
+
+/* Per-call measurements, filled when a non-NULL pointer is passed.  Kernel
+ * times come from HIP events recorded on the stream the kernels run on. */
+typedef struct fl_report {
+    double ms_total;        /* whole call, host wall clock                  */
+    double ms_distance;     /* candidate-filter (distance) kernels          */
+    double ms_finalize;     /* exact re-score, selection and certificates   */
+    double ms_fallback;     /* exact full scans for uncertified rows        */
+    double ms_update;       /* centroid update / vote / mean epilogues      */
+    int64_t n_rows;         /* rows (queries or samples x iterations)       */
+    int64_t n_candidates;   /* candidates re-scored exactly in f64          */
+    int64_t n_fallback;     /* rows whose certificate failed (full scan)    */
+    int64_t n_iter;         /* Lloyd iterations run (KMeans)                */
+    int64_t n_dist_launches;/* launches of the distance kernel              */
+} fl_report;
+
+/* Last error context string of the calling thread ("" after success). */
+const char *fl_last_error(void);
+
+/* Library version: major*10000 + minor*100 + patch. */
+int fl_version(void);
+
+/* Number of visible HIP devices (0 when none; never fails). */
+int fl_device_count(void);
+
+/* ------------------------------------------------------------------ */
+/* KMeans -- ferrolearn-cluster/src/kmeans.rs                          */
+/* ------------------------------------------------------------------ */
+
+/* kmeans.rs:55-113 (KMeans builder).  Defaults from KMeans::new
+ * (kmeans.rs:76-84): max_iter 300, tol 1e-4, n_init 10, no seed. */
+typedef struct fl_kmeans_params {
+    int64_t n_clusters;
+    int64_t max_iter;
+    double tol;
+    int64_t n_init;
+    uint64_t random_state;  /* used when has_random_state != 0 */
+    int32_t has_random_state;
+    int32_t precision;      /* fl_precision for the assignment filter */
+} fl_kmeans_params;
+
+/* kmeans.rs:76-84 (KMeans::new) */
+void fl_kmeans_params_default(fl_kmeans_params *p, int64_t n_clusters);
+
+/* Fit<Array2<F>, ()> for KMeans (kmeans.rs:403-513).
+ * init_centers: NULL runs the library's k-means++ per restart (seed
+ * random_state + run, kmeans.rs:451,460, with the library's own generator);
+ * otherwise n_init*k*d f64 initial centres, one block per restart, which
+ * makes the run comparable with any other implementation started from the
+ * same centres.  Outputs: centers_out k*d (post-update centres of the kept
+ * run), labels_out n, inertia_out, n_iter_out (kmeans.rs:468-503). */
+fl_status fl_kmeans_fit(const void *X, int32_t dtype, int64_t n, int64_t d,
+                        const fl_kmeans_params *params,
+                        const double *init_centers, double *centers_out,
+                        int64_t *labels_out, double *inertia_out,
+                        int64_t *n_iter_out, fl_report *report);
+
+/* Same as fl_kmeans_fit with X, init_centers and every output in device
+ * memory (inertia_out and n_iter_out stay host pointers). */
+fl_status fl_kmeans_fit_device(const void *dX, int32_t dtype, int64_t n,
+                               int64_t d, const fl_kmeans_params *params,
+                               const double *d_init_centers,
+                               double *d_centers_out, int64_t *d_labels_out,
+                               double *inertia_out, int64_t *n_iter_out,
+                               fl_report *report);
+
+/* Predict<Array2<F>> for FittedKMeans (kmeans.rs:528-553): labels of X
+ * against centres (k*d_centers f64).  ShapeMismatch if d != d_centers. */
+fl_status fl_kmeans_predict(const double *centers, int64_t k,
+                            int64_t d_centers, const void *X, int32_t dtype,
+                            int64_t n, int64_t d, int64_t *labels_out);
+
+/* Transform<Array2<F>> for FittedKMeans (kmeans.rs:555-618):
+ * dist_out[i*k + c] = sqrt(fold(x_i, c_c)). */
+fl_status fl_kmeans_transform(const double *centers, int64_t k,
+                              int64_t d_centers, const void *X, int32_t dtype,
+                              int64_t n, int64_t d, double *dist_out);
+
+/* The library's k-means++ (kmeans.rs:169-253 restated with the library's
+ * generator; see fl_kmeans_fit).  centers_out: k*d f64. */
+fl_status fl_kmeans_plusplus(const void *X, int32_t dtype, int64_t n,
+                             int64_t d, int64_t k, uint64_t seed,
+                             double *centers_out);
+
+/* ------------------------------------------------------------------ */
+/* k-nearest neighbours -- ferrolearn-neighbors/src/knn.rs             */
+/* ------------------------------------------------------------------ */
+
+typedef struct fl_knn *fl_knn_handle;
+
+/* Fit<Array2<F>, Array1<usize>> for KNeighborsClassifier (knn.rs:256-320)
+ * with Algorithm::BruteForce (knn.rs:148).  Validation order: y_len != n
+ * -> ShapeMismatch; n_neighbors == 0 -> InvalidParameter; n < n_neighbors
+ * -> InsufficientSamples.  The handle owns a device copy of X and y, as the
+ * fitted Rust model owns clones (knn.rs:311-318).  Supported n_neighbors:
+ * 1..64. */
+fl_status fl_knn_fit_classifier(const void *X, int32_t dtype, int64_t n,
+                                int64_t d, const int64_t *y, int64_t y_len,
+                                int64_t n_neighbors, int32_t weights,
+                                int32_t precision, fl_knn_handle *out);
+
+/* Fit<Array2<F>, Array1<F>> for KNeighborsRegressor (knn.rs:660-718). */
+fl_status fl_knn_fit_regressor(const void *X, int32_t dtype, int64_t n,
+                               int64_t d, const double *y, int64_t y_len,
+                               int64_t n_neighbors, int32_t weights,
+                               int32_t precision, fl_knn_handle *out);
+
+/* Predict for FittedKNeighborsClassifier (knn.rs:322-383): class labels. */
+fl_status fl_knn_predict_classify(fl_knn_handle h, const void *Q,
+                                  int32_t dtype, int64_t nq, int64_t d,
+                                  int64_t *labels_out, fl_report *report);
+
+/* Predict for FittedKNeighborsRegressor (knn.rs:720-781). */
+fl_status fl_knn_predict_regress(fl_knn_handle h, const void *Q,
+                                 int32_t dtype, int64_t nq, int64_t d,
+                                 double *y_out, fl_report *report);
+
+/* FittedKNeighborsClassifier::predict_proba (knn.rs:452-486):
+ * proba_out is nq*n_classes, classes in fl_knn_classes order. */
+fl_status fl_knn_predict_proba(fl_knn_handle h, const void *Q, int32_t dtype,
+                               int64_t nq, int64_t d, double *proba_out,
+                               fl_report *report);
+
+/* kneighbors (knn.rs:823-834, 875-917): k == 0 means the fitted
+ * n_neighbors.  Errors: ShapeMismatch on feature count, InvalidParameter
+ * for k == 0 after defaulting, InsufficientSamples for k > n_fit.
+ * dist_out / idx_out are nq*k, sorted by (distance, index). */
+fl_status fl_knn_kneighbors(fl_knn_handle h, const void *Q, int32_t dtype,
+                            int64_t nq, int64_t d, int64_t k,
+                            double *dist_out, int64_t *idx_out,
+                            fl_report *report);
+
+/* Device-memory forms of the three predict calls: Q and the outputs are
+ * device pointers. */
+fl_status fl_knn_kneighbors_device(fl_knn_handle h, const void *dQ,
+                                   int32_t dtype, int64_t nq, int64_t d,
+                                   int64_t k, double *d_dist_out,
+                                   int64_t *d_idx_out, fl_report *report);
+fl_status fl_knn_predict_classify_device(fl_knn_handle h, const void *dQ,
+                                         int32_t dtype, int64_t nq, int64_t d,
+                                         int64_t *d_labels_out,
+                                         fl_report *report);
+fl_status fl_knn_predict_regress_device(fl_knn_handle h, const void *dQ,
+                                        int32_t dtype, int64_t nq, int64_t d,
+                                        double *d_y_out, fl_report *report);
+
+/* Switch the candidate filter of a fitted handle (fl_precision). */
+fl_status fl_knn_set_precision(fl_knn_handle h, int32_t precision);
+
+/* HasClasses (ferrolearn-core/src/introspection.rs:44-50) and
+ * n_samples_fit (knn.rs:838-842): sorted unique classes. */
+int64_t fl_knn_n_classes(fl_knn_handle h);
+fl_status fl_knn_classes(fl_knn_handle h, int64_t *classes_out);
+int64_t fl_knn_n_samples_fit(fl_knn_handle h);
+
+void fl_knn_free(fl_knn_handle h);
+
+/* ------------------------------------------------------------------ */
+/* Backend -- ferrolearn-core/src/backend.rs:37-123                    */
+/* ------------------------------------------------------------------ */
+
+/* Backend::gemm (backend.rs:41-44; faer impl backend_faer.rs:61-78):
+ * C (m*n) = A (m*k) . B (k2*n), f64, host pointers.  ShapeMismatch when
+ * k != k2. */
+fl_status fl_backend_gemm_f64(const double *A, int64_t m, int64_t k,
+                              const double *B, int64_t k2, int64_t n,
+                              double *C);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FERROLEARN_HIP_H */

@@ -1,0 +1,104 @@
+"""C ABI checks that need no GPU: the library loads, exports every symbol the
+header declares, and validates arguments on the host in the reference's
+order (kmeans.rs:421-449, knn.rs:277-301) before touching the device."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import ferrolearn_amd as F
+from ferrolearn_amd import _lib as L
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    txt = open(os.path.join(ROOT, "include", "ferrolearn_hip.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(fl_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = L.lib()
+    names = _declared()
+    assert len(names) >= 20
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(names) == set(L.SIGNATURES), set(names) ^ set(L.SIGNATURES)
+
+
+def test_version_and_device_count_never_fail():
+    assert L.lib().fl_version() >= 100
+    assert L.lib().fl_device_count() >= 0
+
+
+def test_kmeans_validation_order():
+    x = np.ones((3, 2))
+    with pytest.raises(F.InvalidParameter, match="n_clusters"):
+        F.KMeans(0).fit(x)
+    with pytest.raises(F.InsufficientSamples, match="actual 0"):
+        F.KMeans(2).fit(np.zeros((0, 2)))
+    with pytest.raises(F.InsufficientSamples, match="required 4, actual 3"):
+        F.KMeans(4).fit(x)
+    with pytest.raises(F.InvalidParameter, match="n_init"):
+        F.KMeans(2).with_n_init(0).fit(x)
+    # k == 0 is reported before the empty input (kmeans.rs:424 before :431)
+    with pytest.raises(F.InvalidParameter):
+        F.KMeans(0).fit(np.zeros((0, 2)))
+
+
+def test_kmeans_predict_shape_mismatch():
+    fitted = F.FittedKMeans(np.zeros((2, 2)), np.zeros(4, np.int64), 0.0, 1, {})
+    with pytest.raises(F.ShapeMismatch, match="expected \\[2\\], actual \\[3\\]"):
+        fitted.predict(np.ones((2, 3)))
+    with pytest.raises(F.ShapeMismatch):
+        fitted.transform(np.ones((2, 3)))
+
+
+def test_knn_validation_order():
+    x = np.ones((3, 2))
+    with pytest.raises(F.ShapeMismatch):
+        F.KNeighborsClassifier().fit(x, [0, 1])
+    with pytest.raises(F.InvalidParameter, match="n_neighbors"):
+        F.KNeighborsClassifier().with_n_neighbors(0).fit(x, [0, 1, 2])
+    with pytest.raises(F.InsufficientSamples, match="required 5, actual 3"):
+        F.KNeighborsClassifier().fit(x, [0, 1, 2])
+    # y length is checked before n_neighbors (knn.rs:280 before :288)
+    with pytest.raises(F.ShapeMismatch):
+        F.KNeighborsRegressor().with_n_neighbors(0).fit(x, [0.0])
+    with pytest.raises(F.InsufficientSamples):
+        F.KNeighborsRegressor().fit(x, [0.0, 1.0, 2.0])
+
+
+def test_knn_gpu_limit_is_reported_after_reference_checks():
+    x = np.ones((100, 2))
+    with pytest.raises(F.InvalidParameter, match="1..64"):
+        F.KNeighborsClassifier().with_n_neighbors(65).fit(x, np.zeros(100))
+
+
+def test_gemm_shape_mismatch():
+    a = np.ones((2, 3))
+    b = np.ones((4, 2))
+    st = L.lib().fl_backend_gemm_f64(a.ctypes.data_as(C.c_void_p), 2, 3,
+                                     b.ctypes.data_as(C.c_void_p), 4, 2,
+                                     np.zeros(4).ctypes.data_as(C.c_void_p))
+    assert st == L.FL_SHAPE_MISMATCH
+    assert "inner dimensions" in L.last_error()
+
+
+def test_no_device_gives_device_error_not_a_crash():
+    if L.lib().fl_device_count() > 0:
+        pytest.skip("a GPU is present")
+    with pytest.raises(F.DeviceError):
+        F.KMeans(2).with_n_init(1).fit(np.ones((4, 2)))
+
+
+def test_product_never_imports_the_oracle():
+    pkg = os.path.join(ROOT, "ferrolearn_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".hip", ".hpp", ".cpp", ".h")) or f == "Makefile":
+                txt = open(os.path.join(dirpath, f), errors="ignore").read()
+                assert not re.search(r"(from|import)\s+oracle|liboracle|oracle/|ref_cpu", txt), f

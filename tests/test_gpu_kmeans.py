@@ -1,0 +1,170 @@
+"""KMeans parity on the GPU: the library against the oracle and the golden
+vectors.  Labels, n_iter and centres must match bit for bit (centres follow
+the reference's index-order sums); inertia within 1e-12 relative."""
+import math
+
+import numpy as np
+import pytest
+
+import ferrolearn_amd as F
+from oracle import oracle as O
+from tests._golden import arr, bits, golden, ref_fixture
+
+pytestmark = pytest.mark.gpu
+
+G = golden()
+
+
+def blobs(n, d, k, seed, spread=1.0, box=10.0, dtype=np.float64):
+    rng = np.random.default_rng(seed)
+    cen = rng.uniform(-box, box, size=(k, d))
+    X = cen[np.arange(n) % k] + rng.normal(scale=spread, size=(n, d))
+    return X.astype(dtype)
+
+
+def fit_lib(X, init, max_iter, tol, precision="fp64"):
+    init = np.asarray(init, np.float64)
+    k = init.shape[-2]
+    m = (F.KMeans(k).with_n_init(1).with_max_iter(max_iter).with_tol(tol)
+         .with_precision(precision))
+    return m.fit(X, init_centers=init.reshape(1, k, -1))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "exact"])
+def test_golden_lloyd_runs(precision):
+    for run in G["lloyd"]:
+        X, init = arr(run["X"]), arr(run["init"])
+        f = fit_lib(X, init, run["max_iter"], run["tol"], precision)
+        assert f.n_iter() == run["n_iter"]
+        assert list(f.labels()) == run["labels"]
+        assert (bits(f.cluster_centers()) == bits(arr(run["centers"]))).all()
+        want = float.fromhex(run["inertia"])
+        assert abs(f.inertia() - want) <= 1e-12 * abs(want)
+
+
+def test_golden_argmin_ties_predict():
+    for case in G["argmin"]:
+        C = arr(case["centers"])
+        rows = arr(case["rows"])
+        fitted = F.FittedKMeans(C, np.zeros(1, np.int64), 0.0, 0, {})
+        assert list(fitted.predict(rows)) == case["labels"]
+
+
+@pytest.mark.parametrize("k", [40, 100])
+def test_filter_path_ties_lowest_index(k):
+    # More than 32 centres takes the f64 MFMA filter + certificate path.
+    rng = np.random.default_rng(k)
+    C = rng.integers(-3, 4, size=(k, 6)).astype(np.float64)
+    C[k - 1] = C[3]
+    C[k - 2] = C[0]
+    X = np.concatenate([rng.integers(-4, 5, size=(3000, 6)).astype(np.float64), C])
+    fitted = F.FittedKMeans(C, np.zeros(1, np.int64), 0.0, 0, {})
+    got = fitted.predict(X)
+    want, _ = O.assign(X, C)
+    assert (got == want).all()
+
+
+def test_sklearn_fixture_assign():
+    fx = ref_fixture("kmeans.json")
+    X = np.array(fx["input"]["X"])
+    C = np.array(fx["expected"]["cluster_centers"])
+    fitted = F.FittedKMeans(C, np.zeros(1, np.int64), 0.0, 0, {})
+    assert (fitted.predict(X) == np.array(fx["expected"]["labels"])).all()
+    f = fit_lib(X, C, 300, 1e-4)
+    assert f.n_iter() == 1
+    assert abs(f.inertia() - fx["expected"]["inertia"]) <= 1e-12 * fx["expected"]["inertia"]
+
+
+@pytest.mark.parametrize("n,d,k,dtype,precision", [
+    (100_000, 32, 16, np.float64, "fp64"),      # config-1 shape, direct exact path
+    (40_000, 32, 64, np.float64, "fp64"),       # MFMA filter path
+    (30_000, 64, 200, np.float32, "fp64"),      # f32 storage, MFMA filter path
+    (20_000, 16, 48, np.float64, "exact"),
+])
+def test_lloyd_matches_oracle(n, d, k, dtype, precision):
+    X = blobs(n, d, k, seed=n + k, spread=1.0, dtype=dtype)
+    init = O.kmeanspp(X.astype(np.float64), k, 7)
+    c_o, l_o, i_o, it_o = O.kmeans_lloyd(X.astype(np.float64), init, 100, 1e-4)
+    f = fit_lib(X, init, 100, 1e-4, precision)
+    assert f.n_iter() == it_o
+    assert (f.labels() == l_o).all()
+    assert (bits(f.cluster_centers()) == bits(c_o)).all()
+    assert abs(f.inertia() - i_o) <= 1e-12 * i_o
+    # inertia against the exact sum of the last assignment's folds, made
+    # with the pre-update centres of the last iteration
+    pre = O.kmeans_lloyd(X.astype(np.float64), init, it_o - 1, 1e-4)[0] if it_o > 1 else init
+    _, dist = O.assign_rows(X.astype(np.float64), pre)
+    assert abs(math.fsum(dist) - f.inertia()) <= 1e-12 * f.inertia()
+
+
+def test_overlapping_blobs_many_iterations():
+    # Overlapping clusters move for many iterations: one label flip anywhere
+    # would change later centres and n_iter.
+    X = blobs(20_000, 8, 40, seed=3, spread=6.0, box=5.0)
+    init = O.kmeanspp(X, 40, 11)
+    c_o, l_o, i_o, it_o = O.kmeans_lloyd(X, init, 60, 0.0)
+    f = fit_lib(X, init, 60, 0.0)
+    assert it_o == 60 and f.n_iter() == 60
+    assert (f.labels() == l_o).all()
+    assert (bits(f.cluster_centers()) == bits(c_o)).all()
+
+
+def test_n_init_keeps_strictly_smaller_inertia():
+    X = blobs(5000, 4, 6, seed=9, spread=3.0)
+    inits = np.stack([O.kmeanspp(X, 6, s) for s in (1, 2, 3)])
+    c_o, l_o, i_o, it_o, best = O.kmeans_fit_from(X, inits, 100, 1e-4)
+    f = F.KMeans(6).with_n_init(3).with_max_iter(100).fit(X, init_centers=inits)
+    assert (f.labels() == l_o).all() and f.n_iter() == it_o
+    assert (bits(f.cluster_centers()) == bits(c_o)).all()
+
+
+def test_empty_cluster_keeps_centre_and_zero_iter():
+    X = np.array([[0.0, 0.0], [0.1, 0.0], [10.0, 10.0], [10.0, 10.1]])
+    init = np.array([[0.0, 0.0], [10.0, 10.0], [1e6, 1e6]])
+    f = fit_lib(X, init, 10, 1e-4)
+    c_o, l_o, i_o, it_o = O.kmeans_lloyd(X, init, 10, 1e-4)
+    assert (bits(f.cluster_centers()) == bits(c_o)).all() and f.n_iter() == it_o
+    f0 = fit_lib(X, init, 0, 1e-4)
+    assert f0.n_iter() == 0 and (f0.labels() == 0).all()
+    assert f0.inertia() == np.finfo(np.float64).max
+    assert (f0.cluster_centers() == init).all()
+
+
+def test_kmeanspp_matches_oracle():
+    for (n, d, k, seed) in ((3000, 5, 9, 42), (10_000, 16, 33, 7), (9000, 3, 1, 5)):
+        X = blobs(n, d, max(k, 2), seed=seed)
+        got = F.kmeans_plusplus(X, k, seed)
+        want = O.kmeanspp(X, k, seed)
+        assert (bits(got) == bits(want)).all()
+
+
+def test_fit_without_init_uses_library_kmeanspp():
+    X = blobs(8000, 5, 7, seed=1, spread=0.5)
+    f = F.KMeans(7).with_n_init(2).with_random_state(5).with_max_iter(50).fit(X)
+    inits = np.stack([O.kmeanspp(X, 7, 5), O.kmeanspp(X, 7, 6)])
+    c_o, l_o, i_o, it_o, best = O.kmeans_fit_from(X, inits, 50, 1e-4)
+    assert (f.labels() == l_o).all() and f.n_iter() == it_o
+    assert (bits(f.cluster_centers()) == bits(c_o)).all()
+
+
+def test_transform_and_predict_match_oracle():
+    X = blobs(3000, 7, 5, seed=2)
+    C = X[:50].copy()
+    fitted = F.FittedKMeans(C, np.zeros(1, np.int64), 0.0, 0, {})
+    assert (bits(fitted.transform(X)) == bits(O.transform(X, C))).all()
+    assert (fitted.predict(X) == O.assign(X, C)[0]).all()
+    assert (fitted.predict(X.astype(np.float32)) ==
+            O.assign(X.astype(np.float32).astype(np.float64), C)[0]).all()
+
+
+def test_device_tensor_fit():
+    torch = pytest.importorskip("torch")
+    X = blobs(50_000, 32, 64, seed=4, dtype=np.float32)
+    init = O.kmeanspp(X.astype(np.float64), 64, 3)
+    c_o, l_o, i_o, it_o = O.kmeans_lloyd(X.astype(np.float64), init, 30, 1e-4)
+    xt = torch.from_numpy(X).cuda()
+    f = (F.KMeans(64).with_n_init(1).with_max_iter(30)
+         .fit(xt, init_centers=init.reshape(1, 64, 32)))
+    assert f.n_iter() == it_o
+    assert (f.labels().cpu().numpy() == l_o).all()
+    assert (bits(f.cluster_centers().cpu().numpy()) == bits(c_o)).all()

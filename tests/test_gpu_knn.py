@@ -1,0 +1,155 @@
+"""Brute-force kNN parity on the GPU: the library against the oracle and the
+golden vectors.  Neighbour indices and distances bit for bit in (distance,
+index) order; votes and means bit for bit."""
+import numpy as np
+import pytest
+
+import ferrolearn_amd as F
+from oracle import oracle as O
+from tests._golden import arr, bits, golden, ref_fixture
+
+pytestmark = pytest.mark.gpu
+
+G = golden()
+PRECISIONS = ["fp64", "exact"]
+
+
+def mixture(N, d, comps, seed, dtype=np.float32):
+    rng = np.random.default_rng(seed)
+    means = rng.uniform(-5, 5, size=(comps, d))
+    sd = rng.uniform(0.5, 2.0, size=comps)
+    c = rng.integers(0, comps, size=N)
+    X = means[c] + rng.normal(size=(N, d)) * sd[c, None]
+    return X.astype(dtype), c
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_golden_knn(precision):
+    for case in G["knn"]:
+        data, Q, k = arr(case["data"]), arr(case["Q"]), case["k"]
+        y, yr = np.array(case["y"]), arr(case["yr"])
+        for w in (0, 1):
+            W = F.Weights.Distance if w else F.Weights.Uniform
+            clf = (F.KNeighborsClassifier().with_n_neighbors(k).with_weights(W)
+                   .with_precision(precision).fit(data, y))
+            dist, idx = clf.kneighbors(Q)
+            for qi, res in enumerate(case["results"]):
+                assert list(idx[qi]) == res["idx"]
+                assert (bits(dist[qi]) == bits(arr(res["dist"]))).all()
+            assert list(clf.predict(Q)) == [r[f"vote_w{w}"] for r in case["results"]]
+            reg = (F.KNeighborsRegressor().with_n_neighbors(k).with_weights(W)
+                   .with_precision(precision).fit(data, yr))
+            want = arr([r[f"mean_w{w}"] for r in case["results"]])
+            assert (bits(reg.predict(Q)) == bits(want)).all()
+
+
+def test_reference_fixtures():
+    fx = ref_fixture("kneighbors_classifier.json")
+    X, y = np.array(fx["input"]["X"]), np.array(fx["input"]["y"])
+    clf = F.KNeighborsClassifier().with_n_neighbors(5).fit(X, y)
+    assert (clf.predict(X) == O.knn_classify(X, y, X, 5)).all()
+    assert (clf.predict(X) == np.array(fx["expected"]["predictions"])).mean() >= 0.95
+    fr = ref_fixture("kneighbors_regressor.json")
+    X, y = np.array(fr["input"]["X"]), np.array(fr["input"]["y"])
+    reg = F.KNeighborsRegressor().with_n_neighbors(5).fit(X, y)
+    assert (bits(reg.predict(X)) == bits(O.knn_regress(X, y, X, 5))).all()
+    fc = ref_fixture("sklearn_kneighbors_classifier_50x5.json")
+    clf = F.KNeighborsClassifier().fit(np.array(fc["X_train"]), np.array(fc["y_train"]))
+    assert (clf.predict(np.array(fc["X_test"])) == np.array(fc["predictions_test"])).all()
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("k", [1, 10, 33, 64])
+def test_mixture_matches_oracle(precision, k):
+    X, comp = mixture(20_000, 128, 64, seed=k)
+    Q, _ = mixture(300, 128, 64, seed=1000 + k)
+    y = comp % 10
+    clf = (F.KNeighborsClassifier().with_n_neighbors(k).with_precision(precision)
+           .fit(X, y))
+    dist, idx = clf.kneighbors(Q)
+    d_o, i_o = O.kneighbors(X.astype(np.float64), Q.astype(np.float64), k)
+    assert (idx == i_o).all()
+    assert (bits(dist) == bits(d_o)).all()
+    assert (clf.predict(Q) == O.knn_classify(X.astype(np.float64), y,
+                                             Q.astype(np.float64), k)).all()
+
+
+def test_duplicates_force_certificate_fallback():
+    # Many exact duplicates: ties beyond the kept list cannot be certified,
+    # so those queries take the exact scan.  Results must still be exact.
+    rng = np.random.default_rng(5)
+    base = rng.integers(-2, 3, size=(400, 4)).astype(np.float64)
+    X = np.concatenate([base] * 30)
+    Q = rng.integers(-2, 3, size=(200, 4)).astype(np.float64)
+    clf = F.KNeighborsClassifier().with_n_neighbors(10).fit(X, np.arange(len(X)) % 3)
+    dist, idx = clf.kneighbors(Q)
+    d_o, i_o = O.kneighbors(X, Q, 10)
+    assert (idx == i_o).all() and (bits(dist) == bits(d_o)).all()
+    assert clf.report["n_fallback"] > 0
+
+
+def test_f64_queries_and_proba():
+    X, comp = mixture(5000, 20, 8, seed=2, dtype=np.float64)
+    Q, _ = mixture(100, 20, 8, seed=3, dtype=np.float64)
+    y = comp * 7
+    clf = (F.KNeighborsClassifier().with_n_neighbors(6).with_weights(F.Weights.Distance)
+           .fit(X, y))
+    pr = clf.predict_proba(Q)
+    _, pr_o = O.knn_classify(X, y, Q, 6, weights=1, proba=True)
+    assert (bits(pr) == bits(pr_o)).all()
+    assert list(clf.classes()) == sorted(set(y.tolist()))
+    assert clf.n_samples_fit() == 5000
+
+
+def test_k_equals_n_and_errors():
+    X = np.array([[1.0], [2.0], [3.0], [4.0], [5.0]])
+    clf = F.KNeighborsClassifier().with_n_neighbors(5).fit(X, [0, 0, 0, 1, 1])
+    assert list(clf.predict(X)) == [0] * 5
+    with pytest.raises(F.InsufficientSamples):
+        clf.kneighbors(X, 6)
+    with pytest.raises(F.ShapeMismatch):
+        clf.predict(np.ones((2, 3)))
+    with pytest.raises(F.ShapeMismatch):
+        clf.kneighbors(np.ones((2, 3)))
+
+
+def test_device_tensors():
+    torch = pytest.importorskip("torch")
+    X, comp = mixture(30_000, 96, 32, seed=8)
+    Q, _ = mixture(500, 96, 32, seed=9)
+    clf = F.KNeighborsClassifier().with_n_neighbors(10).fit(X, comp % 10)
+    qt = torch.from_numpy(Q).cuda()
+    dist, idx = clf.kneighbors(qt)
+    d_o, i_o = O.kneighbors(X.astype(np.float64), Q.astype(np.float64), 10)
+    assert (idx.cpu().numpy() == i_o).all()
+    assert (bits(dist.cpu().numpy()) == bits(d_o)).all()
+    lab = clf.predict(qt).cpu().numpy()
+    assert (lab == O.knn_classify(X.astype(np.float64), comp % 10,
+                                  Q.astype(np.float64), 10)).all()
+
+
+def test_gemm_backend():
+    import ctypes as C
+    from ferrolearn_amd import _lib as L
+    rng = np.random.default_rng(0)
+    A = rng.normal(size=(37, 19))
+    B = rng.normal(size=(19, 23))
+    out = np.zeros((37, 23))
+    st = L.lib().fl_backend_gemm_f64(A.ctypes.data_as(C.c_void_p), 37, 19,
+                                     B.ctypes.data_as(C.c_void_p), 19, 23,
+                                     out.ctypes.data_as(C.c_void_p))
+    assert st == 0
+    np.testing.assert_allclose(out, A @ B, rtol=1e-12, atol=1e-12)
+
+
+def test_sqrt_and_division_correctly_rounded():
+    # The exact paths rely on IEEE sqrt and division on the GPU: distances of
+    # random folds must equal the CPU's correctly rounded sqrt bit for bit.
+    rng = np.random.default_rng(1)
+    X = rng.normal(size=(4000, 3)) * np.exp(rng.uniform(-20, 20, size=(4000, 1)))
+    Q = rng.normal(size=(50, 3))
+    clf = F.KNeighborsClassifier().with_n_neighbors(64).with_precision("exact").fit(
+        X, np.zeros(4000, np.int64))
+    dist, idx = clf.kneighbors(Q)
+    d_o, i_o = O.kneighbors(X, Q, 64)
+    assert (idx == i_o).all() and (bits(dist) == bits(d_o)).all()
