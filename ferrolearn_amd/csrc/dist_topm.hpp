@@ -16,8 +16,9 @@
 //   MODE_EXACT  : the exact reference fold itself (L = fold, or sqrt(fold)
 //                 for kNN), so the kept lists are already exact.
 //
-// Layout of v_mfma_f64_16x16x4f64 on a 64-lane wave: lane l supplies
-// A[l%16][l/16] and B[l/16][l%16] and holds D[4*(l/16)+v][l%16], v=0..3.
+// Layout of v_mfma_f64_16x16x4f64 on a 64-lane wave (measured on gfx950
+// with one-hot inputs): lane l supplies A[l%16][l/16] and B[l/16][l%16] and
+// holds D[l/16 + 4*v][l%16] in accumulator register v = 0..3.
 #pragma once
 
 #include "common.hpp"
@@ -70,6 +71,12 @@ struct TopmShared {
     int32_t sidx[TM_WAVES][STAGE];
     int32_t srow[TM_WAVES][STAGE];
 };
+
+// Row (within the wave's 32 A rows) of accumulator register v of 16-row
+// tile i held by this lane.
+__device__ __forceinline__ int acc_row(int i, int v, int lane) {
+    return 16 * i + (lane >> 4) + 4 * v;
+}
 
 __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
@@ -133,14 +140,14 @@ __global__ __launch_bounds__(256) void dist_topm_kernel(const TA *__restrict__ A
         sh.lthi[w][lane] = 0x7fffffff;
     }
 
-    // The 8 A rows this lane's accumulators belong to: 16*i + 4*(lane/16) + v.
+    // The 8 A rows this lane's accumulators belong to (acc_row).
     double a_n[2][4], a_s[2][4];
     bool a_ok[2][4];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            const int64_t r = row0 + 16 * i + 4 * (lane >> 4) + v;
+            const int64_t r = row0 + acc_row(i, v, lane);
             a_ok[i][v] = r < nA;
             a_n[i][v] = a_ok[i][v] ? args.an[r] : 0.0;
             a_s[i][v] = a_ok[i][v] ? args.as[r] : 0.0;
@@ -199,8 +206,7 @@ __global__ __launch_bounds__(256) void dist_topm_kernel(const TA *__restrict__ A
                     for (int i = 0; i < 2; ++i)
 #pragma unroll
                         for (int v = 0; v < 4; ++v) {
-                            const double av =
-                                sh.as_[w * TM_ROWS_W + 16 * i + 4 * (lane >> 4) + v][kk];
+                            const double av = sh.as_[w * TM_ROWS_W + acc_row(i, v, lane)][kk];
 #pragma unroll
                             for (int j = 0; j < 4; ++j) {
                                 const double t = av - bv[j];
@@ -218,8 +224,8 @@ __global__ __launch_bounds__(256) void dist_topm_kernel(const TA *__restrict__ A
         for (int i = 0; i < 2; ++i)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                th[i][v] = sh.lth[w][16 * i + 4 * (lane >> 4) + v];
-                thi[i][v] = sh.lthi[w][16 * i + 4 * (lane >> 4) + v];
+                th[i][v] = sh.lth[w][acc_row(i, v, lane)];
+                thi[i][v] = sh.lthi[w][acc_row(i, v, lane)];
             }
         int scnt = 0;
 #pragma unroll
@@ -235,7 +241,7 @@ __global__ __launch_bounds__(256) void dist_topm_kernel(const TA *__restrict__ A
             for (int i = 0; i < 2; ++i)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
-                    const int rl = 16 * i + 4 * (lane >> 4) + v;
+                    const int rl = acc_row(i, v, lane);
                     const bool ok = cok && a_ok[i][v];
                     double key;
                     if (MODE == MODE_MFMA64) {
@@ -258,8 +264,8 @@ __global__ __launch_bounds__(256) void dist_topm_kernel(const TA *__restrict__ A
                         for (int i2 = 0; i2 < 2; ++i2)
 #pragma unroll
                             for (int v2 = 0; v2 < 4; ++v2) {
-                                th[i2][v2] = sh.lth[w][16 * i2 + 4 * (lane >> 4) + v2];
-                                thi[i2][v2] = sh.lthi[w][16 * i2 + 4 * (lane >> 4) + v2];
+                                th[i2][v2] = sh.lth[w][acc_row(i2, v2, lane)];
+                                thi[i2][v2] = sh.lthi[w][acc_row(i2, v2, lane)];
                             }
                         pass = ok && pair_less(key, (int32_t)col, th[i][v], thi[i][v]);
                         mask = __ballot(pass);
