@@ -1,0 +1,329 @@
+// dist_bf16.hpp -- BF16 MFMA candidate filter with a fused per-row top-M.
+//
+// Same contract as dist_topm.hpp (every kept key is a rigorous lower bound of
+// the exact fold; the M smallest keys per row are kept; theta is the M-th
+// key), but the dot product a.b runs on v_mfma_f32_16x16x32_bf16 with FP32
+// accumulation:
+//   PASSES = 1 : a.b ~ ah.bh                    (ah = bf16(a))
+//   PASSES = 3 : a.b ~ ah.bh + ah.bl + al.bh    (al = bf16(a - ah))
+// Operands are pre-split into hi / lo bf16 arrays padded to NKC*32 features.
+//
+// Roles: the streamed rows (kNN database / KMeans centres) are the MFMA A
+// operand, staged through LDS 64 rows at a time; the resident rows (kNN
+// queries / KMeans samples) are the B operand and stay in VGPRs for the
+// whole kernel, 32 per wave.
+//
+// Key of a pair, in f32:  key' = fma(-2, acc, fma(-csq, sx_r, xn'_r))
+//   xn'_r = (1 - b) |x_r|^2, sx_r = |x_r|, csq = c |q|;
+// the lower bound itself is key' + (1 - b)|q|^2, a per-row constant that
+// does not change the order and is added back (in f64) for the certificate.
+// c and b bound the bf16 representation error of the split, the FP32
+// accumulation (taken as 2^-23 per addition, whatever order MFMA uses), the
+// f32 epilogue and the exact fold's own rounding (host.hpp, bound_bf16).
+#pragma once
+
+#include "common.hpp"
+#include "dist_topm.hpp"
+
+namespace fl {
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef unsigned short u16x8_t __attribute__((ext_vector_type(8)));
+
+constexpr int BF_WAVES = 4;
+constexpr int BF_QW = 32;                 // resident rows per wave (two 16-col tiles)
+constexpr int BF_Q = BF_WAVES * BF_QW;    // resident rows per block
+constexpr int BF_T = 64;                  // streamed rows per tile (four 16-row tiles)
+constexpr int BF_STAGE = 256;
+
+// Output layout of v_mfma_f32_16x16x32_bf16: lane l holds D[i][l%16] with
+// i = bf_acc_row(l, v) for accumulator register v = 0..3.
+__device__ __forceinline__ int bf_acc_row(int lane, int v) { return 4 * (lane >> 4) + v; }
+
+// f32 -> bf16, round to nearest even (finite inputs; NaN stays NaN).
+__host__ __device__ __forceinline__ unsigned short bf16_bits_rne(float f) {
+    unsigned int u = __builtin_bit_cast(unsigned int, f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)((u >> 16) | ((u & 0xffff) ? 0x40 : 0));
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__host__ __device__ __forceinline__ float bf16_bits_to_f32(unsigned short h) {
+    return __builtin_bit_cast(float, (unsigned int)h << 16);
+}
+
+// Split rows into hi / lo bf16 (zero padded to dpad features).
+template <typename T>
+__global__ void bf16_split_kernel(const T *__restrict__ X, int64_t n, int d, int dpad,
+                                  unsigned short *__restrict__ hi,
+                                  unsigned short *__restrict__ lo) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * dpad) return;
+    const int64_t r = t / dpad;
+    const int j = (int)(t % dpad);
+    unsigned short h = 0, l = 0;
+    if (j < d) {
+        const double x = (double)X[r * d + j];
+        h = bf16_bits_rne((float)x);
+        const double rem = x - (double)bf16_bits_to_f32(h);   // exact (Sterbenz)
+        l = bf16_bits_rne((float)rem);
+    }
+    hi[t] = h;
+    lo[t] = l;
+}
+
+// Per streamed row: xn' = (1-b)|x|^2 and |x| in f32; per resident row:
+// csq = c|q| (f32) and qoff = (1-b)|q|^2 (f64).
+static __global__ __attribute__((unused)) void bf16_row_terms_kernel(
+    const double *__restrict__ n2, const double *__restrict__ n1, int64_t n, double c, double b,
+    float *__restrict__ t0, float *__restrict__ t1, double *__restrict__ off) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (off) {
+        t0[i] = (float)(c * n1[i]);
+        off[i] = (1.0 - b) * n2[i];
+    } else {
+        t0[i] = (float)((1.0 - b) * n2[i]);
+        t1[i] = (float)n1[i];
+    }
+}
+
+struct Bf16Args {
+    int64_t nA, nB;          // resident rows, streamed rows
+    int64_t b_per_split;
+    const unsigned short *ah, *al;   // resident hi/lo [nA][dpad]
+    const unsigned short *bh, *bl;   // streamed hi/lo [nB][dpad]
+    const float *csq;                // resident c|q|
+    const double *qoff;              // resident (1-b)|q|^2
+    const float *xnp, *sx;           // streamed (1-b)|x|^2, |x|
+    Cand out;
+};
+
+template <int M, int NKC>
+struct Bf16Shared {
+    static constexpr int ROWB = NKC * 64 + 16;   // bytes per staged row (hi or lo) + pad
+    unsigned char tile[2][BF_T][ROWB];           // [hi/lo][row][bytes]
+    float xnp[BF_T];
+    float sx[BF_T];
+    float lkey[BF_WAVES][BF_QW][M];
+    int32_t lidx[BF_WAVES][BF_QW][M];
+    int32_t lcnt[BF_WAVES][BF_QW];
+    float lth[BF_WAVES][BF_QW];
+    float skey[BF_WAVES][BF_STAGE];
+    int32_t sidx[BF_WAVES][BF_STAGE];
+    int32_t srow[BF_WAVES][BF_STAGE];
+};
+
+__device__ __forceinline__ bool pair_less_f(float ka, int32_t ia, float kb, int32_t ib) {
+    return ka < kb || (ka == kb && ia < ib);
+}
+
+template <int M, int NKC>
+__device__ void bf16_flush(Bf16Shared<M, NKC> &sh, int w, int lane, int scnt) {
+    wave_sync_lds();
+    if (lane < BF_QW) {
+        const int r = lane;
+        int c = sh.lcnt[w][r];
+        for (int e = 0; e < scnt; ++e) {
+            if (sh.srow[w][e] != r) continue;
+            const float k = sh.skey[w][e];
+            const int32_t ix = sh.sidx[w][e];
+            if (c == M && !pair_less_f(k, ix, sh.lkey[w][r][M - 1], sh.lidx[w][r][M - 1]))
+                continue;
+            int p = c < M ? c++ : M - 1;
+            while (p > 0 && pair_less_f(k, ix, sh.lkey[w][r][p - 1], sh.lidx[w][r][p - 1])) {
+                sh.lkey[w][r][p] = sh.lkey[w][r][p - 1];
+                sh.lidx[w][r][p] = sh.lidx[w][r][p - 1];
+                --p;
+            }
+            sh.lkey[w][r][p] = k;
+            sh.lidx[w][r][p] = ix;
+        }
+        sh.lcnt[w][r] = c;
+        if (c == M) sh.lth[w][r] = sh.lkey[w][r][M - 1];
+    }
+    wave_sync_lds();
+}
+
+template <int M, int NKC, int PASSES>
+__global__ __launch_bounds__(256) void dist_bf16_kernel(Bf16Args args) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    typedef Bf16Shared<M, NKC> Sh;
+    Sh &sh = *reinterpret_cast<Sh *>(smem_raw);
+    constexpr int DPAD = NKC * 32;
+
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int64_t nA = args.nA;
+    const int64_t q0 = (int64_t)blockIdx.x * BF_Q + w * BF_QW;   // this wave's resident rows
+    const int split = blockIdx.y;
+    const int64_t bbeg = (int64_t)split * args.b_per_split;
+    int64_t bend = bbeg + args.b_per_split;
+    if (bend > args.nB) bend = args.nB;
+
+    if (lane < BF_QW) {
+        sh.lcnt[w][lane] = 0;
+        sh.lth[w][lane] = __builtin_huge_valf();
+    }
+
+    // Resident operand fragments: for column tile jt, lane l holds row
+    // q0 + 16*jt + l%16, features kc*32 + 8*(l/16) .. +7.
+    bf16x8_t qh[2][NKC], ql[2][NKC];
+    float csq[2];
+    bool qok[2];
+#pragma unroll
+    for (int jt = 0; jt < 2; ++jt) {
+        const int64_t q = q0 + 16 * jt + (lane & 15);
+        qok[jt] = q < nA;
+        csq[jt] = qok[jt] ? args.csq[q] : 0.f;
+#pragma unroll
+        for (int kc = 0; kc < NKC; ++kc) {
+            u16x8_t vh = (u16x8_t){0, 0, 0, 0, 0, 0, 0, 0}, vl = vh;
+            if (qok[jt]) {
+                const int64_t off = q * DPAD + kc * 32 + 8 * (lane >> 4);
+                vh = *reinterpret_cast<const u16x8_t *>(args.ah + off);
+                if (PASSES == 3) vl = *reinterpret_cast<const u16x8_t *>(args.al + off);
+            }
+            qh[jt][kc] = __builtin_bit_cast(bf16x8_t, vh);
+            ql[jt][kc] = __builtin_bit_cast(bf16x8_t, vl);
+        }
+    }
+
+    for (int64_t b0 = bbeg; b0 < bend; b0 += BF_T) {
+        __syncthreads();
+        // Stage 64 streamed rows (hi and lo) into LDS, 16 bytes per load.
+        constexpr int CH = DPAD / 8;                 // 16-byte chunks per row
+        for (int t = threadIdx.x; t < 2 * BF_T * CH; t += 256) {
+            const int part = t / (BF_T * CH);
+            const int rem = t % (BF_T * CH);
+            const int r = rem / CH, ch = rem % CH;
+            const int64_t gb = b0 + r;
+            u16x8_t v = (u16x8_t){0, 0, 0, 0, 0, 0, 0, 0};
+            if (gb < bend && (PASSES == 3 || part == 0)) {
+                const unsigned short *src = part ? args.bl : args.bh;
+                v = *reinterpret_cast<const u16x8_t *>(src + gb * DPAD + ch * 8);
+            }
+            *reinterpret_cast<u16x8_t *>(&sh.tile[part][r][ch * 16]) = v;
+        }
+        for (int t = threadIdx.x; t < BF_T; t += 256) {
+            const int64_t gb = b0 + t;
+            sh.xnp[t] = gb < bend ? args.xnp[gb] : 0.f;
+            sh.sx[t] = gb < bend ? args.sx[gb] : 0.f;
+        }
+        __syncthreads();
+
+        float4_t acc[4][2];
+#pragma unroll
+        for (int it = 0; it < 4; ++it)
+#pragma unroll
+            for (int jt = 0; jt < 2; ++jt) acc[it][jt] = (float4_t){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kc = 0; kc < NKC; ++kc) {
+#pragma unroll
+            for (int it = 0; it < 4; ++it) {
+                const int r = 16 * it + (lane & 15);
+                const int boff = kc * 64 + 16 * (lane >> 4);
+                const bf16x8_t xh =
+                    __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u16x8_t *>(&sh.tile[0][r][boff]));
+                bf16x8_t xl;
+                if (PASSES == 3)
+                    xl = __builtin_bit_cast(bf16x8_t,
+                                            *reinterpret_cast<const u16x8_t *>(&sh.tile[1][r][boff]));
+#pragma unroll
+                for (int jt = 0; jt < 2; ++jt) {
+                    acc[it][jt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh, qh[jt][kc],
+                                                                          acc[it][jt], 0, 0, 0);
+                    if (PASSES == 3) {
+                        acc[it][jt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh, ql[jt][kc],
+                                                                              acc[it][jt], 0, 0, 0);
+                        acc[it][jt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xl, qh[jt][kc],
+                                                                              acc[it][jt], 0, 0, 0);
+                    }
+                }
+            }
+        }
+
+        // ---- epilogue ----
+        float th[2];
+#pragma unroll
+        for (int jt = 0; jt < 2; ++jt) th[jt] = sh.lth[w][16 * jt + (lane & 15)];
+        float key[4][2][4];
+        bool any = false;
+#pragma unroll
+        for (int it = 0; it < 4; ++it)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int rloc = 16 * it + bf_acc_row(lane, v);
+                const bool rok = b0 + rloc < bend;
+                const float base_xn = sh.xnp[rloc];
+                const float base_sx = sh.sx[rloc];
+#pragma unroll
+                for (int jt = 0; jt < 2; ++jt) {
+                    const float t = __builtin_fmaf(-csq[jt], base_sx, base_xn);
+                    const float kv = __builtin_fmaf(-2.f, acc[it][jt][v], t);
+                    key[it][jt][v] = kv;
+                    any = any || (rok && qok[jt] && kv < th[jt]);
+                }
+            }
+        if (__ballot(any) == 0) continue;
+
+        int scnt = 0;
+#pragma unroll
+        for (int it = 0; it < 4; ++it)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int rloc = 16 * it + bf_acc_row(lane, v);
+                const bool rok = b0 + rloc < bend;
+#pragma unroll
+                for (int jt = 0; jt < 2; ++jt) {
+                    const float kv = key[it][jt][v];
+                    bool pass = rok && qok[jt] && kv < th[jt];
+                    uint64_t mask = __ballot(pass);
+                    if (mask == 0) continue;
+                    int np = __popcll(mask);
+                    if (scnt + np > BF_STAGE) {
+                        bf16_flush<M, NKC>(sh, w, lane, scnt);
+                        scnt = 0;
+                        th[0] = sh.lth[w][lane & 15];
+                        th[1] = sh.lth[w][16 + (lane & 15)];
+                        pass = rok && qok[jt] && kv < th[jt];
+                        mask = __ballot(pass);
+                        np = __popcll(mask);
+                    }
+                    if (pass) {
+                        const int pos = scnt + __popcll(mask & ((1ull << lane) - 1ull));
+                        sh.skey[w][pos] = kv;
+                        sh.sidx[w][pos] = (int32_t)(b0 + rloc);
+                        sh.srow[w][pos] = 16 * jt + (lane & 15);
+                    }
+                    scnt += np;
+                }
+            }
+        if (scnt > 0) bf16_flush<M, NKC>(sh, w, lane, scnt);
+    }
+
+    wave_sync_lds();
+    if (lane < BF_QW) {
+        const int64_t q = q0 + lane;
+        if (q < nA) {
+            const int64_t o = (int64_t)split * nA + q;
+            const int c = sh.lcnt[w][lane];
+            const double qo = args.qoff[q];
+            args.out.cnt[o] = c;
+            if (c == M) {
+                // key' + qoff, rounded down by a few f32 ulps of each part so
+                // that theta stays below every key that was not kept.
+                const double kf = (double)sh.lth[w][lane];
+                args.out.theta[o] = (kf + qo) - (fabs(kf) + fabs(qo)) * 0x1.0p-22;
+            } else {
+                args.out.theta[o] = __builtin_huge_val();
+            }
+            args.out.theta_idx[o] = c == M ? sh.lidx[w][lane][M - 1] : 0x7fffffff;
+            for (int m = 0; m < M; ++m) {
+                args.out.key[o * M + m] = m < c ? (double)sh.lkey[w][lane][m] + qo : __builtin_huge_val();
+                args.out.idx[o * M + m] = m < c ? sh.lidx[w][lane][m] : -1;
+            }
+        }
+    }
+}
+
+}  // namespace fl

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "dist_bf16.hpp"
 #include "dist_topm.hpp"
 
 namespace fl {
@@ -108,7 +109,73 @@ inline Bound bound_fp64(int d) {
     return {2.0 * (2.0 * gd + 6.0 * u + 2.0 * gd2), 2.0 * (gd + 4.0 * u + gd2)};
 }
 
+// Same role for the BF16 forms (dist_bf16.hpp): the key is
+// fma(-2, acc, fma(-c|q|, |x|, (1-b)|x|^2)) + (1-b)|q|^2 <= fold, with
+//   acc error <= cdot |a||b|: split representation (3.2 ub^2 for the 3-product
+//   form, 2 ub + ub^2 for one product, ub = 2^-8 plus the f64->f32 step) and
+//   FP32 accumulation of PASSES*d exact products at <= 2^-23 per addition;
+//   f32 epilogue roundings (u32 per op) and the exact fold's own error;
+// everything doubled for margin.
+inline Bound bound_bf16(int d, int passes) {
+    const double u64 = std::ldexp(1.0, -53), u32 = std::ldexp(1.0, -24);
+    const double ub = std::ldexp(1.0, -8) + std::ldexp(1.0, -23);
+    const double uacc = std::ldexp(1.0, -23);
+    const double gacc = gamma_n((double)passes * d, uacc);
+    const double rep = passes == 3 ? 3.2 * ub * ub : 2.0 * ub + ub * ub;
+    const double cdot = rep + (passes == 3 ? 1.03 : 1.01) * gacc;
+    const double gf = gamma_n(d + 2, u64);
+    return {2.0 * (2.0 * cdot + 3.0 * u32 + 2.0 * gf), 2.0 * (2.0 * u32 + 2.0 * gf)};
+}
+
 inline void check_launch() { FL_HIP(hipGetLastError()); }
+
+template <int M, int NKC, int PASSES>
+void launch_bf16_t(const Bf16Args &a, int S, hipStream_t st) {
+    const size_t shmem = sizeof(Bf16Shared<M, NKC>);
+    auto kern = dist_bf16_kernel<M, NKC, PASSES>;
+    FL_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)shmem));
+    dim3 grid((unsigned)ceil_div(a.nA, BF_Q), (unsigned)S);
+    hipLaunchKernelGGL(kern, grid, dim3(256), shmem, st, a);
+    check_launch();
+}
+
+template <int M, int PASSES>
+void launch_bf16_m(int nkc, const Bf16Args &a, int S, hipStream_t st) {
+    switch (nkc) {
+        case 1: return launch_bf16_t<M, 1, PASSES>(a, S, st);
+        case 2: return launch_bf16_t<M, 2, PASSES>(a, S, st);
+        case 3: return launch_bf16_t<M, 3, PASSES>(a, S, st);
+        case 4: return launch_bf16_t<M, 4, PASSES>(a, S, st);
+        case 8: return launch_bf16_t<M, 8, PASSES>(a, S, st);
+    }
+    throw DeviceError("internal: unsupported feature chunk count");
+}
+
+// NKC (32-feature chunks) used for d features by the BF16 kernels; 0 when
+// d is too large for the register-resident layout (d > 256).
+inline int bf16_nkc(int64_t d) {
+    const int64_t c = (d + 31) / 32;
+    if (c <= 4) return (int)c;
+    if (c <= 8) return 8;
+    return 0;
+}
+
+template <int PASSES>
+void launch_bf16_p(int M, int nkc, const Bf16Args &a, int S, hipStream_t st) {
+    switch (M) {
+        case 4: return launch_bf16_m<4, PASSES>(nkc, a, S, st);
+        case 16: return launch_bf16_m<16, PASSES>(nkc, a, S, st);
+        case 32: return launch_bf16_m<32, PASSES>(nkc, a, S, st);
+        case 64: return launch_bf16_m<64, PASSES>(nkc, a, S, st);
+    }
+    throw DeviceError("internal: unsupported candidate list size");
+}
+
+inline void launch_bf16(int passes, int M, int nkc, const Bf16Args &a, int S, hipStream_t st) {
+    if (passes == 3) return launch_bf16_p<3>(M, nkc, a, S, st);
+    return launch_bf16_p<1>(M, nkc, a, S, st);
+}
 
 template <typename TA, typename TB, int M, int MODE>
 void launch_topm_t(const TA *A, const TB *B, const TopmArgs &a, int S, hipStream_t st) {

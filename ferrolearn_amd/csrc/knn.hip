@@ -25,6 +25,10 @@ struct fl_knn {
     std::vector<int64_t> classes;  // sorted unique labels (classifier)
     DevBuf ypos, classes_dev;      // label position per row, classes
     DevBuf yreg;                   // regression targets
+    DevBuf bh, bl;                 // BF16 hi / lo split of X, padded rows
+    DevBuf bxnp, bsx;              // BF16 per-row key terms
+    int bf_split_ready = 0;        // hi/lo arrays built
+    int bf_terms_passes = 0;       // passes the key terms were built for
     hipStream_t stream = nullptr;
     Workspace ws;
     ~fl_knn() {
@@ -99,7 +103,8 @@ void knn_search(fl_knn *h, const double *Q, int64_t nq, int k, int mode, double 
     check_launch();
 
     const int M = mode == MODE_EXACT ? pick_m_exact(k) : pick_m_filter(k);
-    const int64_t nblk = ceil_div(nq, TM_ROWS);
+    const int rows_per_block = (mode == MODE_BF16 || mode == MODE_BF16X3) ? BF_Q : TM_ROWS;
+    const int64_t nblk = ceil_div(nq, rows_per_block);
     int64_t S = ceil_div(2048, nblk);
     S = std::min<int64_t>(S, FIN_MAXC / M);
     S = std::min<int64_t>(S, std::max<int64_t>(1, nB / 256));
@@ -114,25 +119,74 @@ void knn_search(fl_knn *h, const double *Q, int64_t nq, int k, int mode, double 
     cand.theta = ws.get<double>(p + "cth", S * nq);
     cand.theta_idx = ws.get<int32_t>(p + "cthi", S * nq);
 
-    TopmArgs a;
-    a.nA = nq;
-    a.nB = nB;
-    a.d = d;
-    a.b_per_split = bps;
-    a.an = qn;
-    a.as = qs;
-    a.bn = (const double *)h->xn.ptr();
-    a.bs = (const double *)h->xs.ptr();
-    const Bound bd = bound_fp64(d);
-    a.alpha = bd.alpha;
-    a.beta = bd.beta;
-    a.sqrt_key = 1;
-    a.out = cand;
-
     PhaseTimer tm(st);
-    const int t0 = tm.begin();
-    launch_topm<double, TB>(M, mode, Q, B, a, (int)S, st);
-    tm.end(t0);
+    int t0;
+    if (mode == MODE_BF16 || mode == MODE_BF16X3) {
+        const int passes = mode == MODE_BF16X3 ? 3 : 1;
+        const int nkc = bf16_nkc(d);
+        const int dpad = nkc * 32;
+        const Bound bd = bound_bf16(d, passes);
+        if (!h->bf_split_ready) {
+            unsigned short *bh = (unsigned short *)h->bh.get((size_t)(nB * dpad) * 2);
+            unsigned short *bl = (unsigned short *)h->bl.get((size_t)(nB * dpad) * 2);
+            hipLaunchKernelGGL(bf16_split_kernel<TB>, dim3(grid1(nB * dpad, 256)), dim3(256), 0,
+                               st, B, nB, d, dpad, bh, bl);
+            check_launch();
+            h->bf_split_ready = 1;
+        }
+        if (h->bf_terms_passes != passes) {
+            float *xnp = (float *)h->bxnp.get((size_t)nB * 4);
+            float *sx = (float *)h->bsx.get((size_t)nB * 4);
+            hipLaunchKernelGGL(bf16_row_terms_kernel, dim3(grid1(nB, 256)), dim3(256), 0, st,
+                               (const double *)h->xn.ptr(), (const double *)h->xs.ptr(), nB,
+                               bd.alpha, bd.beta, xnp, sx, (double *)nullptr);
+            check_launch();
+            h->bf_terms_passes = passes;
+        }
+        unsigned short *qh = ws.get<unsigned short>(p + "qh", nq * dpad);
+        unsigned short *ql = ws.get<unsigned short>(p + "ql", nq * dpad);
+        float *csq = ws.get<float>(p + "csq", nq);
+        double *qoff = ws.get<double>(p + "qoff", nq);
+        hipLaunchKernelGGL(bf16_split_kernel<double>, dim3(grid1(nq * dpad, 256)), dim3(256), 0,
+                           st, Q, nq, d, dpad, qh, ql);
+        hipLaunchKernelGGL(bf16_row_terms_kernel, dim3(grid1(nq, 256)), dim3(256), 0, st, qn, qs,
+                           nq, bd.alpha, bd.beta, csq, (float *)nullptr, qoff);
+        check_launch();
+        Bf16Args a;
+        a.nA = nq;
+        a.nB = nB;
+        a.b_per_split = bps;
+        a.ah = qh;
+        a.al = ql;
+        a.bh = (const unsigned short *)h->bh.ptr();
+        a.bl = (const unsigned short *)h->bl.ptr();
+        a.csq = csq;
+        a.qoff = qoff;
+        a.xnp = (const float *)h->bxnp.ptr();
+        a.sx = (const float *)h->bsx.ptr();
+        a.out = cand;
+        t0 = tm.begin();
+        launch_bf16(passes, M, nkc, a, (int)S, st);
+        tm.end(t0);
+    } else {
+        TopmArgs a;
+        a.nA = nq;
+        a.nB = nB;
+        a.d = d;
+        a.b_per_split = bps;
+        a.an = qn;
+        a.as = qs;
+        a.bn = (const double *)h->xn.ptr();
+        a.bs = (const double *)h->xs.ptr();
+        const Bound bd = bound_fp64(d);
+        a.alpha = bd.alpha;
+        a.beta = bd.beta;
+        a.sqrt_key = 1;
+        a.out = cand;
+        t0 = tm.begin();
+        launch_topm<double, TB>(M, mode, Q, B, a, (int)S, st);
+        tm.end(t0);
+    }
     acc.n_launch += 1;
 
     int32_t *flags = ws.get<int32_t>(p + "flags", nq);
@@ -176,8 +230,11 @@ void knn_search(fl_knn *h, const double *Q, int64_t nq, int k, int mode, double 
     check_launch();
 }
 
-int filter_mode(int32_t precision) {
-    return precision == FL_PREC_EXACT ? MODE_EXACT : MODE_MFMA64;
+int filter_mode(int32_t precision, int64_t d) {
+    if (precision == FL_PREC_EXACT) return MODE_EXACT;
+    if ((precision == FL_PREC_BF16 || precision == FL_PREC_BF16X3) && bf16_nkc(d) > 0)
+        return precision == FL_PREC_BF16 ? MODE_BF16 : MODE_BF16X3;
+    return MODE_MFMA64;
 }
 
 // Top-k search for device queries of either dtype into device outputs.
@@ -186,7 +243,7 @@ void search_device(fl_knn *h, const void *dQ, int32_t dtype, int64_t nq, int k, 
     const double *Q = dtype == FL_F32
                           ? as_f64<float>((const float *)dQ, nq * h->d, h->ws, "q64", h->stream)
                           : (const double *)dQ;
-    const int mode = filter_mode(h->precision);
+    const int mode = filter_mode(h->precision, h->d);
     if (h->dtype == FL_F32)
         knn_search<float>(h, Q, nq, k, mode, dist, idx, acc, 0);
     else

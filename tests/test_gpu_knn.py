@@ -11,7 +11,7 @@ from tests._golden import arr, bits, golden, ref_fixture
 pytestmark = pytest.mark.gpu
 
 G = golden()
-PRECISIONS = ["fp64", "exact"]
+PRECISIONS = ["fp64", "exact", "bf16x3", "bf16"]
 
 
 def mixture(N, d, comps, seed, dtype=np.float32):
