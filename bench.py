@@ -72,22 +72,21 @@ def cpu_baseline_knn(X, y, Qx, gpu_labels, gpu_idx, k):
                       "rate extrapolated to the whole query set"}, exact
 
 
-def time_predict(clf, qt, steps, warmup, torch):
+def time_predict(clf, qd, steps, warmup, sync):
     reps = []
     for _ in range(warmup):
-        clf.predict(qt)
-    torch.cuda.synchronize()
+        clf.predict(qd)
+    sync()
     t0 = time.perf_counter()
     for _ in range(steps):
-        out = clf.predict(qt)
+        out = clf.predict(qd)
         reps.append(clf.report)
-    torch.cuda.synchronize()
+    sync()
     dt = time.perf_counter() - t0
     return out, dt, reps
 
 
 def run_knn(args):
-    import torch
     import ferrolearn_amd as F
 
     k = 10
@@ -95,14 +94,14 @@ def run_knn(args):
     X, y, Qx = knn_data()
     N, d = X.shape
     Q = Qx.shape[0]
-    qt = torch.from_numpy(Qx).cuda()
+    qd = F.to_device(Qx)          # queries resident in HBM before timing
     forms = {}
     base = None
     for form in args.forms:
         log(f"form {form}: fit + warmup")
         clf = F.KNeighborsClassifier().with_n_neighbors(k).with_precision(form).fit(X, y)
-        labels, dt, reps = time_predict(clf, qt, args.steps, args.warmup, torch)
-        dist, idx = clf.kneighbors(qt)
+        labels, dt, reps = time_predict(clf, qd, args.steps, args.warmup, F.device.synchronize)
+        dist, idx = clf.kneighbors(qd)
         ms_dist = float(np.mean([r["ms_distance"] for r in reps]))
         forms[form] = {
             "queries_per_s": Q * args.steps / dt,
@@ -112,12 +111,11 @@ def run_knn(args):
             "ms_fallback": float(np.mean([r["ms_fallback"] for r in reps])),
             "candidates_per_query": reps[-1]["n_candidates"] / Q,
             "fallback_queries": int(reps[-1]["n_fallback"]),
-            "labels": labels.cpu().numpy(), "idx": idx.cpu().numpy(),
+            "labels": labels.numpy(), "idx": idx.numpy(),
         }
         log(f"form {form}: {forms[form]['queries_per_s']:.0f} q/s, "
             f"{forms[form]['ms_per_step']:.2f} ms/step, kernel {ms_dist:.2f} ms")
         del clf
-        torch.cuda.empty_cache()
     ref = forms["fp64"] if "fp64" in forms else None
     head = args.forms[0]
     h = forms[head]

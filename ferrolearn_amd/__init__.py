@@ -5,7 +5,9 @@ KNeighborsClassifier / KNeighborsRegressor predict on hand-written HIP
 kernels behind the C ABI in include/ferrolearn_hip.h.  Labels, iteration
 counts and neighbour indices match the reference's f64 CPU path exactly.
 """
+from . import device
 from .cluster import FittedKMeans, KMeans, kmeans_plusplus
+from .device import DeviceArray, to_device
 from .errors import (DeviceError, FerroError, InsufficientSamples, InvalidParameter,
                      NumericalInstability, ShapeMismatch)
 from .neighbors import (Algorithm, FittedKNeighborsClassifier, FittedKNeighborsRegressor,
@@ -15,5 +17,6 @@ __all__ = [
     "KMeans", "FittedKMeans", "kmeans_plusplus", "KNeighborsClassifier",
     "KNeighborsRegressor", "FittedKNeighborsClassifier", "FittedKNeighborsRegressor",
     "Algorithm", "Weights", "FerroError", "ShapeMismatch", "InsufficientSamples",
-    "InvalidParameter", "NumericalInstability", "DeviceError",
+    "InvalidParameter", "NumericalInstability", "DeviceError", "DeviceArray", "to_device",
+    "device",
 ]

@@ -100,6 +100,11 @@ SIGNATURES = {
     "fl_knn_n_samples_fit": (_i64, [_vp]),
     "fl_knn_free": (None, [_vp]),
     "fl_backend_gemm_f64": (C.c_int, [_vp, _i64, _i64, _vp, _i64, _i64, _vp]),
+    "fl_device_malloc": (C.c_int, [C.POINTER(_vp), C.c_uint64]),
+    "fl_device_free": (C.c_int, [_vp]),
+    "fl_memcpy_to_device": (C.c_int, [_vp, _vp, C.c_uint64]),
+    "fl_memcpy_to_host": (C.c_int, [_vp, _vp, C.c_uint64]),
+    "fl_device_synchronize": (C.c_int, []),
 }
 
 _lib = None

@@ -42,6 +42,33 @@ def _is_device(x) -> bool:
     return hasattr(x, "is_cuda") and bool(x.is_cuda)
 
 
+def _to_host(x) -> np.ndarray:
+    """Copy a DeviceArray or CUDA tensor back to a numpy array."""
+    from .device import DeviceArray
+    if isinstance(x, DeviceArray):
+        return x.numpy()
+    return x.cpu().numpy()
+
+
+def _dev_empty(shape, np_dtype, like):
+    """Output buffer in HBM of the same kind as the input `like`."""
+    from .device import DeviceArray
+    if isinstance(like, DeviceArray):
+        return DeviceArray(shape, np_dtype)
+    import torch
+    tdt = {np.dtype(np.float64): torch.float64, np.dtype(np.int64): torch.int64,
+           np.dtype(np.float32): torch.float32}[np.dtype(np_dtype)]
+    return torch.empty(shape, dtype=tdt, device=like.device)
+
+
+def _dev_from_host(a, like):
+    from .device import DeviceArray, to_device
+    if isinstance(like, DeviceArray):
+        return to_device(a)
+    import torch
+    return torch.as_tensor(a, device=like.device).contiguous()
+
+
 class KMeans:
     """kmeans.rs:55-113.  Defaults from KMeans::new: max_iter 300, tol 1e-4,
     n_init 10, random_state None.  `precision` picks the candidate filter
@@ -96,16 +123,14 @@ class KMeans:
         inertia = C.c_double()
         n_iter = C.c_int64()
         if _is_device(x):
-            import torch
             n, d = x.shape
             dt = _dtype_code(x.dtype)
             k = self.n_clusters
-            centers = torch.empty((max(k, 0), d), dtype=torch.float64, device=x.device)
-            labels = torch.empty((n,), dtype=torch.int64, device=x.device)
+            centers = _dev_empty((max(k, 0), d), np.float64, x)
+            labels = _dev_empty((n,), np.int64, x)
             init_t = None
             if init_centers is not None:
-                init_t = torch.as_tensor(init_centers, dtype=torch.float64,
-                                         device=x.device).contiguous()
+                init_t = _dev_from_host(np.ascontiguousarray(init_centers, dtype=np.float64), x)
             st = L.lib().fl_kmeans_fit_device(
                 C.c_void_p(x.data_ptr()), dt, n, d, C.byref(p),
                 C.c_void_p(init_t.data_ptr()) if init_t is not None else None,
@@ -164,13 +189,13 @@ class FittedKMeans:
     def _centers_host(self):
         c = self.cluster_centers_
         if _is_device(c):
-            c = c.cpu().numpy()
+            c = _to_host(c)
         return np.ascontiguousarray(c, dtype=np.float64)
 
     def predict(self, x):
         """Predict (kmeans.rs:528-553): ShapeMismatch on the feature count."""
         c = self._centers_host()
-        xh = _host2d(x.cpu().numpy() if _is_device(x) else x)
+        xh = _host2d(_to_host(x) if _is_device(x) else x)
         n, d = xh.shape
         out = np.zeros(n, np.int64)
         check(L.lib().fl_kmeans_predict(c.ctypes.data_as(C.c_void_p), c.shape[0], c.shape[1],
@@ -181,7 +206,7 @@ class FittedKMeans:
     def transform(self, x):
         """Transform (kmeans.rs:555-618): n x k Euclidean distances."""
         c = self._centers_host()
-        xh = _host2d(x.cpu().numpy() if _is_device(x) else x)
+        xh = _host2d(_to_host(x) if _is_device(x) else x)
         n, d = xh.shape
         out = np.zeros((n, c.shape[0]), np.float64)
         check(L.lib().fl_kmeans_transform(c.ctypes.data_as(C.c_void_p), c.shape[0], c.shape[1],

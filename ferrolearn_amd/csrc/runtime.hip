@@ -44,6 +44,43 @@ int fl_device_count(void) {
     return n;
 }
 
+fl_status fl_device_malloc(void **ptr, uint64_t bytes) {
+    return guarded([&]() -> fl_status {
+        if (!ptr) return fail(FL_INVALID_PARAMETER, "InvalidParameter: ptr: must not be NULL");
+        *ptr = nullptr;
+        FL_HIP(hipMalloc(ptr, bytes ? bytes : 1));
+        return FL_OK;
+    });
+}
+
+fl_status fl_device_free(void *ptr) {
+    return guarded([&]() -> fl_status {
+        if (ptr) FL_HIP(hipFree(ptr));
+        return FL_OK;
+    });
+}
+
+fl_status fl_memcpy_to_device(void *dst, const void *src, uint64_t bytes) {
+    return guarded([&]() -> fl_status {
+        if (bytes) FL_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+        return FL_OK;
+    });
+}
+
+fl_status fl_memcpy_to_host(void *dst, const void *src, uint64_t bytes) {
+    return guarded([&]() -> fl_status {
+        if (bytes) FL_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+        return FL_OK;
+    });
+}
+
+fl_status fl_device_synchronize(void) {
+    return guarded([&]() -> fl_status {
+        FL_HIP(hipDeviceSynchronize());
+        return FL_OK;
+    });
+}
+
 // Backend::gemm (ferrolearn-core/src/backend.rs:41-44,
 // backend_faer.rs:61-78): ShapeMismatch when the inner dimensions differ.
 fl_status fl_backend_gemm_f64(const double *A, int64_t m, int64_t k, const double *B, int64_t k2,

@@ -17,7 +17,7 @@ import enum
 import numpy as np
 
 from . import _lib as L
-from .cluster import _dtype_code, _host2d, _is_device
+from .cluster import _dev_empty, _dtype_code, _host2d, _is_device, _to_host
 from .errors import check
 
 
@@ -97,9 +97,8 @@ class _Fitted:
         kk = self.n_neighbors if k == 0 else k
         rep = L.fl_report()
         if dev:
-            import torch
-            dist = torch.empty((nq, max(kk, 0)), dtype=torch.float64, device=q.device)
-            idx = torch.empty((nq, max(kk, 0)), dtype=torch.int64, device=q.device)
+            dist = _dev_empty((nq, max(kk, 0)), np.float64, q)
+            idx = _dev_empty((nq, max(kk, 0)), np.int64, q)
             check(L.lib().fl_knn_kneighbors_device(self._h, C.c_void_p(q.data_ptr()),
                                                    _dtype_code(q.dtype), nq, d, k,
                                                    C.c_void_p(dist.data_ptr()),
@@ -130,8 +129,7 @@ class FittedKNeighborsClassifier(_Fitted):
         nq, d = q.shape
         rep = L.fl_report()
         if dev:
-            import torch
-            out = torch.empty((nq,), dtype=torch.int64, device=q.device)
+            out = _dev_empty((nq,), np.int64, q)
             check(L.lib().fl_knn_predict_classify_device(self._h, C.c_void_p(q.data_ptr()),
                                                          _dtype_code(q.dtype), nq, d,
                                                          C.c_void_p(out.data_ptr()),
@@ -145,7 +143,7 @@ class FittedKNeighborsClassifier(_Fitted):
         return out
 
     def predict_proba(self, x):
-        q = _host2d(x.cpu().numpy() if _is_device(x) else x)
+        q = _host2d(_to_host(x) if _is_device(x) else x)
         nq, d = q.shape
         out = np.zeros((nq, self.n_classes()), np.float64)
         rep = L.fl_report()
@@ -163,7 +161,7 @@ class FittedKNeighborsClassifier(_Fitted):
             raise ShapeMismatch("y length must match number of samples in X")
         p = self.predict(x)
         if _is_device(p):
-            p = p.cpu().numpy()
+            p = _to_host(p)
         return float((p == y).sum()) / y.shape[0] if y.shape[0] else 0.0
 
 
@@ -175,8 +173,7 @@ class FittedKNeighborsRegressor(_Fitted):
         nq, d = q.shape
         rep = L.fl_report()
         if dev:
-            import torch
-            out = torch.empty((nq,), dtype=torch.float64, device=q.device)
+            out = _dev_empty((nq,), np.float64, q)
             check(L.lib().fl_knn_predict_regress_device(self._h, C.c_void_p(q.data_ptr()),
                                                         _dtype_code(q.dtype), nq, d,
                                                         C.c_void_p(out.data_ptr()),
@@ -194,7 +191,7 @@ class FittedKNeighborsRegressor(_Fitted):
         y = np.asarray(y, dtype=np.float64)
         p = self.predict(x)
         if _is_device(p):
-            p = p.cpu().numpy()
+            p = _to_host(p)
         n = y.shape[0]
         if n == 0:
             return 0.0
@@ -217,7 +214,7 @@ class KNeighborsClassifier(_Builder):
     """knn.rs:184-254.  Defaults: n_neighbors 5, Auto, Uniform."""
 
     def fit(self, x, y) -> FittedKNeighborsClassifier:
-        xh = _host2d(x.cpu().numpy() if _is_device(x) else x)
+        xh = _host2d(_to_host(x) if _is_device(x) else x)
         yh = np.ascontiguousarray(np.asarray(y, dtype=np.int64))
         n, d = xh.shape
         h = C.c_void_p()
@@ -232,7 +229,7 @@ class KNeighborsRegressor(_Builder):
     """knn.rs:590-658.  Defaults: n_neighbors 5, Auto, Uniform."""
 
     def fit(self, x, y) -> FittedKNeighborsRegressor:
-        xh = _host2d(x.cpu().numpy() if _is_device(x) else x)
+        xh = _host2d(_to_host(x) if _is_device(x) else x)
         yh = np.ascontiguousarray(np.asarray(y, dtype=np.float64))
         n, d = xh.shape
         h = C.c_void_p()

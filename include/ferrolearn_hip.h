@@ -84,6 +84,15 @@ int fl_version(void);
 /* Number of visible HIP devices (0 when none; never fails). */
 int fl_device_count(void);
 
+/* Device memory helpers for callers that hold no GPU allocator of their own
+ * (C / Rust hosts, tests): plain hipMalloc / hipMemcpy / hipFree on the
+ * current device.  No reference counterpart. */
+fl_status fl_device_malloc(void **ptr, uint64_t bytes);
+fl_status fl_device_free(void *ptr);
+fl_status fl_memcpy_to_device(void *dst, const void *src, uint64_t bytes);
+fl_status fl_memcpy_to_host(void *dst, const void *src, uint64_t bytes);
+fl_status fl_device_synchronize(void);
+
 /* ------------------------------------------------------------------ */
 /* KMeans -- ferrolearn-cluster/src/kmeans.rs                          */
 /* ------------------------------------------------------------------ */

@@ -74,18 +74,25 @@ def test_mixture_matches_oracle(precision, k):
                                              Q.astype(np.float64), k)).all()
 
 
-def test_duplicates_force_certificate_fallback():
-    # Many exact duplicates: ties beyond the kept list cannot be certified,
-    # so those queries take the exact scan.  Results must still be exact.
+@pytest.mark.parametrize("layout", ["interleaved", "contiguous"])
+@pytest.mark.parametrize("precision", ["fp64", "bf16x3"])
+def test_duplicate_rows_exact_and_fallback(layout, precision):
+    # Exact duplicates tie.  Spread over the database (interleaved) each split
+    # keeps one copy per point and the certificate still holds; stored next
+    # to each other (contiguous) one split fills its list with equal keys, the
+    # certificate fails and the query takes the exact scan.  Either way the
+    # answer is the (distance, index) order.
     rng = np.random.default_rng(5)
     base = rng.integers(-2, 3, size=(400, 4)).astype(np.float64)
-    X = np.concatenate([base] * 30)
+    X = np.concatenate([base] * 30) if layout == "interleaved" else np.repeat(base, 30, axis=0)
     Q = rng.integers(-2, 3, size=(200, 4)).astype(np.float64)
-    clf = F.KNeighborsClassifier().with_n_neighbors(10).fit(X, np.arange(len(X)) % 3)
+    clf = (F.KNeighborsClassifier().with_n_neighbors(10).with_precision(precision)
+           .fit(X, np.arange(len(X)) % 3))
     dist, idx = clf.kneighbors(Q)
     d_o, i_o = O.kneighbors(X, Q, 10)
     assert (idx == i_o).all() and (bits(dist) == bits(d_o)).all()
-    assert clf.report["n_fallback"] > 0
+    if layout == "contiguous":
+        assert clf.report["n_fallback"] > 0
 
 
 def test_f64_queries_and_proba():
@@ -113,19 +120,55 @@ def test_k_equals_n_and_errors():
         clf.kneighbors(np.ones((2, 3)))
 
 
-def test_device_tensors():
-    torch = pytest.importorskip("torch")
+def test_device_arrays():
     X, comp = mixture(30_000, 96, 32, seed=8)
     Q, _ = mixture(500, 96, 32, seed=9)
-    clf = F.KNeighborsClassifier().with_n_neighbors(10).fit(X, comp % 10)
-    qt = torch.from_numpy(Q).cuda()
-    dist, idx = clf.kneighbors(qt)
-    d_o, i_o = O.kneighbors(X.astype(np.float64), Q.astype(np.float64), 10)
-    assert (idx.cpu().numpy() == i_o).all()
-    assert (bits(dist.cpu().numpy()) == bits(d_o)).all()
-    lab = clf.predict(qt).cpu().numpy()
-    assert (lab == O.knn_classify(X.astype(np.float64), comp % 10,
-                                  Q.astype(np.float64), 10)).all()
+    for prec in ("fp64", "bf16x3"):
+        clf = F.KNeighborsClassifier().with_n_neighbors(10).with_precision(prec).fit(X, comp % 10)
+        qd = F.to_device(Q)
+        dist, idx = clf.kneighbors(qd)
+        d_o, i_o = O.kneighbors(X.astype(np.float64), Q.astype(np.float64), 10)
+        assert (idx.numpy() == i_o).all()
+        assert (bits(dist.numpy()) == bits(d_o)).all()
+        lab = clf.predict(qd).numpy()
+        assert (lab == O.knn_classify(X.astype(np.float64), comp % 10,
+                                      Q.astype(np.float64), 10)).all()
+
+
+TORCH_INTEROP = r"""
+import sys, numpy as np
+import torch
+torch.zeros(1, device="cuda")          # torch's HIP runtime first
+sys.path.insert(0, sys.argv[1])
+import ferrolearn_amd as F
+from oracle import oracle as O
+rng = np.random.default_rng(0)
+X = rng.normal(size=(5000, 24)).astype(np.float32)
+Q = rng.normal(size=(300, 24)).astype(np.float32)
+clf = F.KNeighborsClassifier().with_n_neighbors(7).with_precision("bf16x3").fit(X, np.arange(5000) % 4)
+dist, idx = clf.kneighbors(torch.from_numpy(Q).cuda())
+d_o, i_o = O.kneighbors(X.astype(np.float64), Q.astype(np.float64), 7)
+assert (idx.cpu().numpy() == i_o).all() and (dist.cpu().numpy() == d_o).all()
+f = F.KMeans(5).with_n_init(1).with_max_iter(20).fit(torch.from_numpy(X).cuda(),
+    init_centers=X[:5].astype(np.float64).reshape(1, 5, 24))
+c_o, l_o, _, it_o = O.kmeans_lloyd(X.astype(np.float64), X[:5].astype(np.float64), 20, 1e-4)
+assert (f.labels().cpu().numpy() == l_o).all() and f.n_iter() == it_o
+print("torch interop ok")
+"""
+
+
+def test_torch_tensors_in_fresh_process():
+    # PyTorch ships its own HIP runtime; with torch initialised first the
+    # library shares it and accepts CUDA tensors directly.
+    import os
+    import subprocess
+    import sys
+    pytest.importorskip("torch")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, "-c", TORCH_INTEROP, root], capture_output=True,
+                       text=True, timeout=240)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "torch interop ok" in r.stdout
 
 
 def test_gemm_backend():
