@@ -143,7 +143,8 @@ def run_knn(args):
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12,
                      "peak": PEAK[peak_key] / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / PEAK[peak_key], "traffic": None,
-                     "kernel": "dist_topm_kernel", "algorithmic_flops_per_launch": flops,
+                     "kernel": "dist_bf16_kernel" if peak_key == "bf16" else "dist_topm_kernel",
+                     "algorithmic_flops_per_launch": flops,
                      "kernel_ms": h["ms_distance_kernel"]},
         "forms": {f: {kk: vv for kk, vv in v.items() if kk not in ("labels", "idx")}
                   for f, v in forms.items()},
@@ -165,7 +166,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="knn", choices=["knn"])
-    ap.add_argument("--forms", default="fp64",
+    ap.add_argument("--forms", default="bf16x3,fp64",
                     help="comma list; the first is reported as value (fp64, exact, bf16, bf16x3)")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()

@@ -188,28 +188,53 @@ __global__ __launch_bounds__(256) void dist_bf16_kernel(Bf16Args args) {
         }
     }
 
-    for (int64_t b0 = bbeg; b0 < bend; b0 += BF_T) {
-        __syncthreads();
-        // Stage 64 streamed rows (hi and lo) into LDS, 16 bytes per load.
-        constexpr int CH = DPAD / 8;                 // 16-byte chunks per row
-        for (int t = threadIdx.x; t < 2 * BF_T * CH; t += 256) {
+    // Streamed tiles: thread t owns LOADS 16-byte chunks of each tile; the
+    // next tile's chunks are fetched into registers while the current tile
+    // is multiplied, then written to LDS after a barrier.
+    constexpr int CH = DPAD / 8;                  // 16-byte chunks per row
+    constexpr int CHUNKS = (PASSES == 3 ? 2 : 1) * BF_T * CH;
+    constexpr int LOADS = (CHUNKS + 255) / 256;
+    u16x8_t pf[LOADS];
+    float pf_xn = 0.f, pf_sx = 0.f;
+    auto fetch = [&](int64_t t0) {
+#pragma unroll
+        for (int u = 0; u < LOADS; ++u) {
+            const int t = threadIdx.x + 256 * u;
             const int part = t / (BF_T * CH);
             const int rem = t % (BF_T * CH);
             const int r = rem / CH, ch = rem % CH;
-            const int64_t gb = b0 + r;
-            u16x8_t v = (u16x8_t){0, 0, 0, 0, 0, 0, 0, 0};
-            if (gb < bend && (PASSES == 3 || part == 0)) {
+            const int64_t gb = t0 + r;
+            pf[u] = (u16x8_t){0, 0, 0, 0, 0, 0, 0, 0};
+            if (t < CHUNKS && gb < bend) {
                 const unsigned short *src = part ? args.bl : args.bh;
-                v = *reinterpret_cast<const u16x8_t *>(src + gb * DPAD + ch * 8);
+                pf[u] = *reinterpret_cast<const u16x8_t *>(src + gb * DPAD + ch * 8);
             }
-            *reinterpret_cast<u16x8_t *>(&sh.tile[part][r][ch * 16]) = v;
         }
-        for (int t = threadIdx.x; t < BF_T; t += 256) {
-            const int64_t gb = b0 + t;
-            sh.xnp[t] = gb < bend ? args.xnp[gb] : 0.f;
-            sh.sx[t] = gb < bend ? args.sx[gb] : 0.f;
+        if (threadIdx.x < BF_T) {
+            const int64_t gb = t0 + threadIdx.x;
+            pf_xn = gb < bend ? args.xnp[gb] : 0.f;
+            pf_sx = gb < bend ? args.sx[gb] : 0.f;
+        }
+    };
+    if (bbeg < bend) fetch(bbeg);
+
+    for (int64_t b0 = bbeg; b0 < bend; b0 += BF_T) {
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < LOADS; ++u) {
+            const int t = threadIdx.x + 256 * u;
+            if (t < CHUNKS) {
+                const int part = t / (BF_T * CH);
+                const int rem = t % (BF_T * CH);
+                *reinterpret_cast<u16x8_t *>(&sh.tile[part][rem / CH][(rem % CH) * 16]) = pf[u];
+            }
+        }
+        if (threadIdx.x < BF_T) {
+            sh.xnp[threadIdx.x] = pf_xn;
+            sh.sx[threadIdx.x] = pf_sx;
         }
         __syncthreads();
+        if (b0 + BF_T < bend) fetch(b0 + BF_T);
 
         float4_t acc[4][2];
 #pragma unroll

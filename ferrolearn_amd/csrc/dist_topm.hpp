@@ -155,20 +155,55 @@ __global__ __launch_bounds__(256) void dist_topm_kernel(const TA *__restrict__ A
             a_s[i][v] = a_ok[i][v] ? args.as[r] : 0.0;
         }
 
-    for (int64_t b0 = bbeg; b0 < bend; b0 += TM_BT) {
-        double4_t acc[2][4];
+    // The streamed rows are staged one (64-row tile, 32-feature chunk) step
+    // at a time; each thread fetches its 8 values of the next step into
+    // registers while the current step is multiplied.
+    const int nkc = (d + TM_KC - 1) / TM_KC;
+    const int64_t nsteps = (bend > bbeg ? ceil_div(bend - bbeg, TM_BT) : 0) * nkc;
+    constexpr int PF = TM_BT * TM_KC / 256;
+    constexpr int KQ = TM_KC / 4;   // MFMA k-steps per chunk
+    TB pf[PF];
+    double anx[KQ][2], acur[KQ][2];   // this lane's A fragments, next / current chunk
+    auto fetch = [&](int64_t step) {
+        const int64_t t0 = bbeg + (step / nkc) * TM_BT;
+        const int k0 = (int)(step % nkc) * TM_KC;
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+        for (int u = 0; u < PF; ++u) {
+            const int t = threadIdx.x + 256 * u;
+            const int r = t / TM_KC, c = t % TM_KC;
+            const int64_t gb = t0 + r;
+            const int gk = k0 + c;
+            pf[u] = (gb < bend && gk < d) ? B[gb * d + gk] : (TB)0;
+        }
+        if (MODE == MODE_MFMA64) {
 #pragma unroll
-            for (int j = 0; j < 4; ++j) acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
+            for (int kq = 0; kq < KQ; ++kq)
+#pragma unroll
+                for (int i = 0; i < 2; ++i) {
+                    const int64_t r = row0 + 16 * i + (lane & 15);
+                    const int gk = k0 + 4 * kq + (lane >> 4);
+                    anx[kq][i] = (r < nA && gk < d) ? (double)A[r * d + gk] : 0.0;
+                }
+        }
+    };
+    if (nsteps > 0) fetch(0);
+    double4_t acc[2][4];
 
-        for (int k0 = 0; k0 < d; k0 += TM_KC) {
+    for (int64_t step = 0; step < nsteps; ++step) {
+        const int64_t b0 = bbeg + (step / nkc) * TM_BT;
+        const int k0 = (int)(step % nkc) * TM_KC;
+        if (k0 == 0) {
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
+        }
+        {
             __syncthreads();
-            for (int t = threadIdx.x; t < TM_BT * TM_KC; t += 256) {
-                const int r = t / TM_KC, c = t % TM_KC;
-                const int64_t gb = b0 + r;
-                const int gk = k0 + c;
-                sh.bs[r][c] = (gb < bend && gk < d) ? (double)B[gb * d + gk] : 0.0;
+#pragma unroll
+            for (int u = 0; u < PF; ++u) {
+                const int t = threadIdx.x + 256 * u;
+                sh.bs[t / TM_KC][t % TM_KC] = (double)pf[u];
             }
             if (MODE == MODE_EXACT) {
                 for (int t = threadIdx.x; t < TM_ROWS * TM_KC; t += 256) {
@@ -179,24 +214,30 @@ __global__ __launch_bounds__(256) void dist_topm_kernel(const TA *__restrict__ A
                 }
             }
             __syncthreads();
+            if (MODE == MODE_MFMA64) {
+#pragma unroll
+                for (int kq = 0; kq < KQ; ++kq) {
+                    acur[kq][0] = anx[kq][0];
+                    acur[kq][1] = anx[kq][1];
+                }
+            }
+            if (step + 1 < nsteps) fetch(step + 1);
             const int kc = d - k0 < TM_KC ? d - k0 : TM_KC;
             if (MODE == MODE_MFMA64) {
-                for (int kk = 0; kk < kc; kk += 4) {
-                    const int kl = kk + (lane >> 4);
-                    double af[2], bf[4];
+                // Features past d are zero in both operands, so all KQ steps
+                // can run.
 #pragma unroll
-                    for (int i = 0; i < 2; ++i) {
-                        const int64_t r = row0 + 16 * i + (lane & 15);
-                        af[i] = (r < nA && k0 + kl < d) ? (double)A[r * d + k0 + kl] : 0.0;
-                    }
+                for (int kq = 0; kq < KQ; ++kq) {
+                    const int kl = 4 * kq + (lane >> 4);
+                    double bf[4];
 #pragma unroll
                     for (int j = 0; j < 4; ++j) bf[j] = sh.bs[16 * j + (lane & 15)][kl];
 #pragma unroll
                     for (int i = 0; i < 2; ++i)
 #pragma unroll
                         for (int j = 0; j < 4; ++j)
-                            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j],
-                                                                             acc[i][j], 0, 0, 0);
+                            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                                acur[kq][i], bf[j], acc[i][j], 0, 0, 0);
                 }
             } else {
                 // Exact fold, feature by feature in ascending order.
@@ -218,6 +259,7 @@ __global__ __launch_bounds__(256) void dist_topm_kernel(const TA *__restrict__ A
                 }
             }
         }
+        if (k0 + TM_KC < d) continue;
 
         // ---- epilogue: keys, filter against theta, stage, merge ----
         double th[2][4];
