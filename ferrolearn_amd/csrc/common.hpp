@@ -49,6 +49,13 @@ struct StatusError {
     std::string msg;
 };
 
+// Wait until every outstanding vector-memory load of this wave has landed
+// (s_waitcnt vmcnt(0), other counters untouched).  Used after a kernel's
+// prologue loads so that the compiler does not have to guard their first
+// use inside the main loop with a wait that would also drain the loop's own
+// prefetches.
+__device__ __forceinline__ void wait_vm_loads() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // ------------------------------------------------------------ device math --
 template <typename T>
 __device__ __forceinline__ double ld64(const T *p) {

@@ -197,25 +197,29 @@ __global__ __launch_bounds__(256) void dist_bf16_kernel(Bf16Args args) {
     u16x8_t pf[LOADS];
     float pf_xn = 0.f, pf_sx = 0.f;
     auto fetch = [&](int64_t t0) {
+        // Branch-free: rows past the split end read the last valid row; the
+        // values are zeroed when they are written to LDS, so that nothing
+        // here consumes a load result (which would force a wait on it).
 #pragma unroll
         for (int u = 0; u < LOADS; ++u) {
             const int t = threadIdx.x + 256 * u;
-            const int part = t / (BF_T * CH);
-            const int rem = t % (BF_T * CH);
+            const int tt = t < CHUNKS ? t : 0;
+            const int part = tt / (BF_T * CH);
+            const int rem = tt % (BF_T * CH);
             const int r = rem / CH, ch = rem % CH;
             const int64_t gb = t0 + r;
-            pf[u] = (u16x8_t){0, 0, 0, 0, 0, 0, 0, 0};
-            if (t < CHUNKS && gb < bend) {
-                const unsigned short *src = part ? args.bl : args.bh;
-                pf[u] = *reinterpret_cast<const u16x8_t *>(src + gb * DPAD + ch * 8);
-            }
+            const int64_t gc = gb < bend ? gb : bend - 1;
+            const unsigned short *src = part ? args.bl : args.bh;
+            pf[u] = *reinterpret_cast<const u16x8_t *>(src + gc * DPAD + ch * 8);
         }
         if (threadIdx.x < BF_T) {
             const int64_t gb = t0 + threadIdx.x;
-            pf_xn = gb < bend ? args.xnp[gb] : 0.f;
-            pf_sx = gb < bend ? args.sx[gb] : 0.f;
+            const int64_t gc = gb < bend ? gb : bend - 1;
+            pf_xn = args.xnp[gc];
+            pf_sx = args.sx[gc];
         }
     };
+    wait_vm_loads();   // resident fragments are in registers before the loop
     if (bbeg < bend) fetch(bbeg);
 
     for (int64_t b0 = bbeg; b0 < bend; b0 += BF_T) {
@@ -226,12 +230,15 @@ __global__ __launch_bounds__(256) void dist_bf16_kernel(Bf16Args args) {
             if (t < CHUNKS) {
                 const int part = t / (BF_T * CH);
                 const int rem = t % (BF_T * CH);
-                *reinterpret_cast<u16x8_t *>(&sh.tile[part][rem / CH][(rem % CH) * 16]) = pf[u];
+                const bool ok = b0 + rem / CH < bend;
+                *reinterpret_cast<u16x8_t *>(&sh.tile[part][rem / CH][(rem % CH) * 16]) =
+                    ok ? pf[u] : (u16x8_t){0, 0, 0, 0, 0, 0, 0, 0};
             }
         }
         if (threadIdx.x < BF_T) {
-            sh.xnp[threadIdx.x] = pf_xn;
-            sh.sx[threadIdx.x] = pf_sx;
+            const bool ok = b0 + threadIdx.x < bend;
+            sh.xnp[threadIdx.x] = ok ? pf_xn : 0.f;
+            sh.sx[threadIdx.x] = ok ? pf_sx : 0.f;
         }
         __syncthreads();
         if (b0 + BF_T < bend) fetch(b0 + BF_T);
@@ -241,29 +248,41 @@ __global__ __launch_bounds__(256) void dist_bf16_kernel(Bf16Args args) {
         for (int it = 0; it < 4; ++it)
 #pragma unroll
             for (int jt = 0; jt < 2; ++jt) acc[it][jt] = (float4_t){0.f, 0.f, 0.f, 0.f};
+        // For each feature chunk: read the four 16-row fragments (hi, lo),
+        // then issue the products pass by pass so that eight independent
+        // MFMAs separate two updates of one accumulator.
 #pragma unroll
         for (int kc = 0; kc < NKC; ++kc) {
+            bf16x8_t xh[4], xl[4];
 #pragma unroll
             for (int it = 0; it < 4; ++it) {
                 const int r = 16 * it + (lane & 15);
                 const int boff = kc * 64 + 16 * (lane >> 4);
-                const bf16x8_t xh =
-                    __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u16x8_t *>(&sh.tile[0][r][boff]));
-                bf16x8_t xl;
+                xh[it] = __builtin_bit_cast(bf16x8_t,
+                                            *reinterpret_cast<const u16x8_t *>(&sh.tile[0][r][boff]));
                 if (PASSES == 3)
-                    xl = __builtin_bit_cast(bf16x8_t,
-                                            *reinterpret_cast<const u16x8_t *>(&sh.tile[1][r][boff]));
+                    xl[it] = __builtin_bit_cast(
+                        bf16x8_t, *reinterpret_cast<const u16x8_t *>(&sh.tile[1][r][boff]));
+            }
 #pragma unroll
-                for (int jt = 0; jt < 2; ++jt) {
-                    acc[it][jt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh, qh[jt][kc],
+            for (int it = 0; it < 4; ++it)
+#pragma unroll
+                for (int jt = 0; jt < 2; ++jt)
+                    acc[it][jt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh[it], qh[jt][kc],
                                                                           acc[it][jt], 0, 0, 0);
-                    if (PASSES == 3) {
-                        acc[it][jt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh, ql[jt][kc],
-                                                                              acc[it][jt], 0, 0, 0);
-                        acc[it][jt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xl, qh[jt][kc],
-                                                                              acc[it][jt], 0, 0, 0);
-                    }
-                }
+            if (PASSES == 3) {
+#pragma unroll
+                for (int it = 0; it < 4; ++it)
+#pragma unroll
+                    for (int jt = 0; jt < 2; ++jt)
+                        acc[it][jt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            xh[it], ql[jt][kc], acc[it][jt], 0, 0, 0);
+#pragma unroll
+                for (int it = 0; it < 4; ++it)
+#pragma unroll
+                    for (int jt = 0; jt < 2; ++jt)
+                        acc[it][jt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            xl[it], qh[jt][kc], acc[it][jt], 0, 0, 0);
             }
         }
 

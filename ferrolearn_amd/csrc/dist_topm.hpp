@@ -165,6 +165,9 @@ __global__ __launch_bounds__(256) void dist_topm_kernel(const TA *__restrict__ A
     TB pf[PF];
     double anx[KQ][2], acur[KQ][2];   // this lane's A fragments, next / current chunk
     auto fetch = [&](int64_t step) {
+        // Branch-free: indices past the end read a valid element; they are
+        // zeroed where the values are consumed, so that nothing here waits
+        // on a load.
         const int64_t t0 = bbeg + (step / nkc) * TM_BT;
         const int k0 = (int)(step % nkc) * TM_KC;
 #pragma unroll
@@ -173,7 +176,8 @@ __global__ __launch_bounds__(256) void dist_topm_kernel(const TA *__restrict__ A
             const int r = t / TM_KC, c = t % TM_KC;
             const int64_t gb = t0 + r;
             const int gk = k0 + c;
-            pf[u] = (gb < bend && gk < d) ? B[gb * d + gk] : (TB)0;
+            const bool ok = gb < bend && gk < d;
+            pf[u] = B[(ok ? gb : bbeg) * d + (ok ? gk : 0)];
         }
         if (MODE == MODE_MFMA64) {
 #pragma unroll
@@ -182,10 +186,12 @@ __global__ __launch_bounds__(256) void dist_topm_kernel(const TA *__restrict__ A
                 for (int i = 0; i < 2; ++i) {
                     const int64_t r = row0 + 16 * i + (lane & 15);
                     const int gk = k0 + 4 * kq + (lane >> 4);
-                    anx[kq][i] = (r < nA && gk < d) ? (double)A[r * d + gk] : 0.0;
+                    const bool ok = r < nA && gk < d;
+                    anx[kq][i] = (double)A[(ok ? r : 0) * d + (ok ? gk : 0)];
                 }
         }
     };
+    wait_vm_loads();   // prologue loads are in registers before the loop
     if (nsteps > 0) fetch(0);
     double4_t acc[2][4];
 
@@ -203,7 +209,8 @@ __global__ __launch_bounds__(256) void dist_topm_kernel(const TA *__restrict__ A
 #pragma unroll
             for (int u = 0; u < PF; ++u) {
                 const int t = threadIdx.x + 256 * u;
-                sh.bs[t / TM_KC][t % TM_KC] = (double)pf[u];
+                const bool ok = b0 + t / TM_KC < bend && k0 + t % TM_KC < d;
+                sh.bs[t / TM_KC][t % TM_KC] = ok ? (double)pf[u] : 0.0;
             }
             if (MODE == MODE_EXACT) {
                 for (int t = threadIdx.x; t < TM_ROWS * TM_KC; t += 256) {
@@ -216,10 +223,13 @@ __global__ __launch_bounds__(256) void dist_topm_kernel(const TA *__restrict__ A
             __syncthreads();
             if (MODE == MODE_MFMA64) {
 #pragma unroll
-                for (int kq = 0; kq < KQ; ++kq) {
-                    acur[kq][0] = anx[kq][0];
-                    acur[kq][1] = anx[kq][1];
-                }
+                for (int kq = 0; kq < KQ; ++kq)
+#pragma unroll
+                    for (int i = 0; i < 2; ++i) {
+                        const bool ok = row0 + 16 * i + (lane & 15) < nA &&
+                                        k0 + 4 * kq + (lane >> 4) < d;
+                        acur[kq][i] = ok ? anx[kq][i] : 0.0;
+                    }
             }
             if (step + 1 < nsteps) fetch(step + 1);
             const int kc = d - k0 < TM_KC ? d - k0 : TM_KC;
