@@ -91,7 +91,7 @@ def run_knn(args):
 
     k = 10
     log("generating data ...")
-    X, y, Qx = knn_data()
+    X, y, Qx = knn_data(Q=args.queries)
     N, d = X.shape
     Q = Qx.shape[0]
     qd = F.to_device(Qx)          # queries resident in HBM before timing
@@ -169,6 +169,8 @@ def main():
     ap.add_argument("--forms", default="bf16x3,fp64",
                     help="comma list; the first is reported as value (fp64, exact, bf16, bf16x3)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--queries", type=int, default=100_000,
+                    help="query count (100,000 is the benchmark; fewer only for profiling runs)")
     args = ap.parse_args()
     args.forms = [f for f in args.forms.split(",") if f]
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

@@ -123,20 +123,28 @@ __device__ void bf16_flush(Bf16Shared<M, NKC> &sh, int w, int lane, int scnt) {
     if (lane < BF_QW) {
         const int r = lane;
         int c = sh.lcnt[w][r];
-        for (int e = 0; e < scnt; ++e) {
-            if (sh.srow[w][e] != r) continue;
-            const float k = sh.skey[w][e];
-            const int32_t ix = sh.sidx[w][e];
-            if (c == M && !pair_less_f(k, ix, sh.lkey[w][r][M - 1], sh.lidx[w][r][M - 1]))
-                continue;
-            int p = c < M ? c++ : M - 1;
-            while (p > 0 && pair_less_f(k, ix, sh.lkey[w][r][p - 1], sh.lidx[w][r][p - 1])) {
-                sh.lkey[w][r][p] = sh.lkey[w][r][p - 1];
-                sh.lidx[w][r][p] = sh.lidx[w][r][p - 1];
-                --p;
+        // Scan the staged rows four at a time so that the LDS reads overlap.
+        for (int e0 = 0; e0 < scnt; e0 += 4) {
+            int rows[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) rows[u] = e0 + u < scnt ? sh.srow[w][e0 + u] : -1;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (rows[u] != r) continue;
+                const int e = e0 + u;
+                const float k = sh.skey[w][e];
+                const int32_t ix = sh.sidx[w][e];
+                if (c == M && !pair_less_f(k, ix, sh.lkey[w][r][M - 1], sh.lidx[w][r][M - 1]))
+                    continue;
+                int p = c < M ? c++ : M - 1;
+                while (p > 0 && pair_less_f(k, ix, sh.lkey[w][r][p - 1], sh.lidx[w][r][p - 1])) {
+                    sh.lkey[w][r][p] = sh.lkey[w][r][p - 1];
+                    sh.lidx[w][r][p] = sh.lidx[w][r][p - 1];
+                    --p;
+                }
+                sh.lkey[w][r][p] = k;
+                sh.lidx[w][r][p] = ix;
             }
-            sh.lkey[w][r][p] = k;
-            sh.lidx[w][r][p] = ix;
         }
         sh.lcnt[w][r] = c;
         if (c == M) sh.lth[w][r] = sh.lkey[w][r][M - 1];
@@ -221,6 +229,11 @@ __global__ __launch_bounds__(256) void dist_bf16_kernel(Bf16Args args) {
     };
     wait_vm_loads();   // resident fragments are in registers before the loop
     if (bbeg < bend) fetch(bbeg);
+    // Staged candidates persist across tiles and are merged into the lists
+    // only when the staging area fills (and once at the end): filtering
+    // against a theta that is a little stale keeps a superset, and the merge
+    // re-checks every entry.
+    int scnt = 0;
 
     for (int64_t b0 = bbeg; b0 < bend; b0 += BF_T) {
         __syncthreads();
@@ -310,7 +323,6 @@ __global__ __launch_bounds__(256) void dist_bf16_kernel(Bf16Args args) {
             }
         if (__ballot(any) == 0) continue;
 
-        int scnt = 0;
 #pragma unroll
         for (int it = 0; it < 4; ++it)
 #pragma unroll
@@ -342,8 +354,8 @@ __global__ __launch_bounds__(256) void dist_bf16_kernel(Bf16Args args) {
                     scnt += np;
                 }
             }
-        if (scnt > 0) bf16_flush<M, NKC>(sh, w, lane, scnt);
     }
+    if (scnt > 0) bf16_flush<M, NKC>(sh, w, lane, scnt);
 
     wave_sync_lds();
     if (lane < BF_QW) {

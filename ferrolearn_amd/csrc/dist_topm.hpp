@@ -93,20 +93,28 @@ __device__ void topm_flush(TopmShared<M, MODE> &sh, int w, int lane, int scnt) {
     if (lane < TM_ROWS_W) {
         const int r = lane;
         int c = sh.lcnt[w][r];
-        for (int e = 0; e < scnt; ++e) {
-            if (sh.srow[w][e] != r) continue;
-            const double k = sh.skey[w][e];
-            const int32_t ix = sh.sidx[w][e];
-            if (c == M && !pair_less(k, ix, sh.lkey[w][r][M - 1], sh.lidx[w][r][M - 1]))
-                continue;
-            int p = c < M ? c++ : M - 1;
-            while (p > 0 && pair_less(k, ix, sh.lkey[w][r][p - 1], sh.lidx[w][r][p - 1])) {
-                sh.lkey[w][r][p] = sh.lkey[w][r][p - 1];
-                sh.lidx[w][r][p] = sh.lidx[w][r][p - 1];
-                --p;
+        // Scan the staged rows four at a time so that the LDS reads overlap.
+        for (int e0 = 0; e0 < scnt; e0 += 4) {
+            int rows[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) rows[u] = e0 + u < scnt ? sh.srow[w][e0 + u] : -1;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (rows[u] != r) continue;
+                const int e = e0 + u;
+                const double k = sh.skey[w][e];
+                const int32_t ix = sh.sidx[w][e];
+                if (c == M && !pair_less(k, ix, sh.lkey[w][r][M - 1], sh.lidx[w][r][M - 1]))
+                    continue;
+                int p = c < M ? c++ : M - 1;
+                while (p > 0 && pair_less(k, ix, sh.lkey[w][r][p - 1], sh.lidx[w][r][p - 1])) {
+                    sh.lkey[w][r][p] = sh.lkey[w][r][p - 1];
+                    sh.lidx[w][r][p] = sh.lidx[w][r][p - 1];
+                    --p;
+                }
+                sh.lkey[w][r][p] = k;
+                sh.lidx[w][r][p] = ix;
             }
-            sh.lkey[w][r][p] = k;
-            sh.lidx[w][r][p] = ix;
         }
         sh.lcnt[w][r] = c;
         if (c == M) {
@@ -194,6 +202,8 @@ __global__ __launch_bounds__(256) void dist_topm_kernel(const TA *__restrict__ A
     wait_vm_loads();   // prologue loads are in registers before the loop
     if (nsteps > 0) fetch(0);
     double4_t acc[2][4];
+    // Staged candidates persist across tiles (see dist_bf16.hpp).
+    int scnt = 0;
 
     for (int64_t step = 0; step < nsteps; ++step) {
         const int64_t b0 = bbeg + (step / nkc) * TM_BT;
@@ -281,7 +291,6 @@ __global__ __launch_bounds__(256) void dist_topm_kernel(const TA *__restrict__ A
                 th[i][v] = sh.lth[w][acc_row(i, v, lane)];
                 thi[i][v] = sh.lthi[w][acc_row(i, v, lane)];
             }
-        int scnt = 0;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int64_t col = b0 + 16 * j + (lane & 15);
@@ -334,8 +343,8 @@ __global__ __launch_bounds__(256) void dist_topm_kernel(const TA *__restrict__ A
                     scnt += np;
                 }
         }
-        if (scnt > 0) topm_flush<M, MODE>(sh, w, lane, scnt);
     }
+    if (scnt > 0) topm_flush<M, MODE>(sh, w, lane, scnt);
 
     wave_sync_lds();
     if (lane < TM_ROWS_W) {
