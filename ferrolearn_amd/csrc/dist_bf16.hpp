@@ -13,13 +13,15 @@
 // queries / KMeans samples) are the B operand and stay in VGPRs for the
 // whole kernel, 32 per wave.
 //
-// Key of a pair, in f32:  key' = fma(-2, acc, fma(-csq, sx_r, xn'_r))
-//   xn'_r = (1 - b) |x_r|^2, sx_r = |x_r|, csq = c |q|;
-// the lower bound itself is key' + (1 - b)|q|^2, a per-row constant that
-// does not change the order and is added back (in f64) for the certificate.
-// c and b bound the bf16 representation error of the split, the FP32
-// accumulation (taken as 2^-23 per addition, whatever order MFMA uses), the
-// f32 epilogue and the exact fold's own rounding (host.hpp, bound_bf16).
+// Key of a pair, in f32:  key' = fma(-2, acc, xn'_r),  xn'_r = (1 - b)|x_r|^2.
+// The lower bound of the exact fold is key' + qoff_q with the per-row
+// constant qoff_q = (1 - b)|q|^2 - c |q| Smax (Smax = the largest |x_r| of
+// the streamed rows): a constant per resident row does not change the order,
+// so the epilogue costs one FMA and one compare per pair, and qoff is added
+// back (in f64) for the certificate.  c and b bound the bf16 representation
+// error of the split, the FP32 accumulation (taken as 2^-23 per addition,
+// whatever order MFMA uses), the f32 epilogue and the exact fold's own
+// rounding (host.hpp, bound_bf16); c |q| Smax >= c |q||x_r| for every row.
 #pragma once
 
 #include "common.hpp"
@@ -71,20 +73,32 @@ __global__ void bf16_split_kernel(const T *__restrict__ X, int64_t n, int d, int
     lo[t] = l;
 }
 
-// Per streamed row: xn' = (1-b)|x|^2 and |x| in f32; per resident row:
-// csq = c|q| (f32) and qoff = (1-b)|q|^2 (f64).
-static __global__ __attribute__((unused)) void bf16_row_terms_kernel(
-    const double *__restrict__ n2, const double *__restrict__ n1, int64_t n, double c, double b,
-    float *__restrict__ t0, float *__restrict__ t1, double *__restrict__ off) {
+// Per streamed row: xn' = (1-b)|x|^2 (f32).
+static __global__ __attribute__((unused)) void bf16_stream_terms_kernel(
+    const double *__restrict__ n2, int64_t n, double b, float *__restrict__ xnp) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    if (off) {
-        t0[i] = (float)(c * n1[i]);
-        off[i] = (1.0 - b) * n2[i];
-    } else {
-        t0[i] = (float)((1.0 - b) * n2[i]);
-        t1[i] = (float)n1[i];
-    }
+    if (i < n) xnp[i] = (float)((1.0 - b) * n2[i]);
+}
+
+// Per resident row: qoff = (1-b)|q|^2 - c|q| Smax (f64); *smax is the
+// largest streamed-row norm.
+static __global__ __attribute__((unused)) void bf16_resident_terms_kernel(
+    const double *__restrict__ n2, const double *__restrict__ n1, int64_t n, double c, double b,
+    const double *__restrict__ smax, double *__restrict__ qoff) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) qoff[i] = (1.0 - b) * n2[i] - c * n1[i] * (*smax);
+}
+
+// *out = max(*out, max_i v[i]) for non-negative v; *out must start at 0.
+static __global__ __attribute__((unused)) void max_nonneg_kernel(const double *__restrict__ v,
+                                                                 int64_t n,
+                                                                 unsigned long long *__restrict__ out) {
+    double m = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        m = fmax(m, v[i]);
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
 }
 
 struct Bf16Args {
@@ -92,9 +106,8 @@ struct Bf16Args {
     int64_t b_per_split;
     const unsigned short *ah, *al;   // resident hi/lo [nA][dpad]
     const unsigned short *bh, *bl;   // streamed hi/lo [nB][dpad]
-    const float *csq;                // resident c|q|
-    const double *qoff;              // resident (1-b)|q|^2
-    const float *xnp, *sx;           // streamed (1-b)|x|^2, |x|
+    const double *qoff;              // resident (1-b)|q|^2 - c|q|Smax
+    const float *xnp;                // streamed (1-b)|x|^2
     Cand out;
 };
 
@@ -103,7 +116,7 @@ struct Bf16Shared {
     static constexpr int ROWB = NKC * 64 + 16;   // bytes per staged row (hi or lo) + pad
     unsigned char tile[2][BF_T][ROWB];           // [hi/lo][row][bytes]
     float xnp[BF_T];
-    float sx[BF_T];
+    float kbuf[BF_WAVES][64][32];                // a lane's 32 keys on the slow path
     float lkey[BF_WAVES][BF_QW][M];
     int32_t lidx[BF_WAVES][BF_QW][M];
     int32_t lcnt[BF_WAVES][BF_QW];
@@ -176,13 +189,11 @@ __global__ __launch_bounds__(256) void dist_bf16_kernel(Bf16Args args) {
     // Resident operand fragments: for column tile jt, lane l holds row
     // q0 + 16*jt + l%16, features kc*32 + 8*(l/16) .. +7.
     bf16x8_t qh[2][NKC], ql[2][NKC];
-    float csq[2];
     bool qok[2];
 #pragma unroll
     for (int jt = 0; jt < 2; ++jt) {
         const int64_t q = q0 + 16 * jt + (lane & 15);
         qok[jt] = q < nA;
-        csq[jt] = qok[jt] ? args.csq[q] : 0.f;
 #pragma unroll
         for (int kc = 0; kc < NKC; ++kc) {
             u16x8_t vh = (u16x8_t){0, 0, 0, 0, 0, 0, 0, 0}, vl = vh;
@@ -203,7 +214,7 @@ __global__ __launch_bounds__(256) void dist_bf16_kernel(Bf16Args args) {
     constexpr int CHUNKS = (PASSES == 3 ? 2 : 1) * BF_T * CH;
     constexpr int LOADS = (CHUNKS + 255) / 256;
     u16x8_t pf[LOADS];
-    float pf_xn = 0.f, pf_sx = 0.f;
+    float pf_xn = 0.f;
     auto fetch = [&](int64_t t0) {
         // Branch-free: rows past the split end read the last valid row; the
         // values are zeroed when they are written to LDS, so that nothing
@@ -224,7 +235,6 @@ __global__ __launch_bounds__(256) void dist_bf16_kernel(Bf16Args args) {
             const int64_t gb = t0 + threadIdx.x;
             const int64_t gc = gb < bend ? gb : bend - 1;
             pf_xn = args.xnp[gc];
-            pf_sx = args.sx[gc];
         }
     };
     wait_vm_loads();   // resident fragments are in registers before the loop
@@ -249,9 +259,9 @@ __global__ __launch_bounds__(256) void dist_bf16_kernel(Bf16Args args) {
             }
         }
         if (threadIdx.x < BF_T) {
+            // Rows past the split end get an infinite key: they never pass.
             const bool ok = b0 + threadIdx.x < bend;
-            sh.xnp[threadIdx.x] = ok ? pf_xn : 0.f;
-            sh.sx[threadIdx.x] = ok ? pf_sx : 0.f;
+            sh.xnp[threadIdx.x] = ok ? pf_xn : __builtin_huge_valf();
         }
         __syncthreads();
         if (b0 + BF_T < bend) fetch(b0 + BF_T);
@@ -300,60 +310,61 @@ __global__ __launch_bounds__(256) void dist_bf16_kernel(Bf16Args args) {
         }
 
         // ---- epilogue ----
+        // Fast path: one FMA and one compare per pair; bit it*8 + v*2 + jt of
+        // pm marks a pair whose key is below its row's theta.  Rows past the
+        // end have xn' = +inf and resident rows past nA have theta = -inf.
         float th[2];
 #pragma unroll
-        for (int jt = 0; jt < 2; ++jt) th[jt] = sh.lth[w][16 * jt + (lane & 15)];
-        float key[4][2][4];
-        bool any = false;
+        for (int jt = 0; jt < 2; ++jt)
+            th[jt] = qok[jt] ? sh.lth[w][16 * jt + (lane & 15)] : -__builtin_huge_valf();
+        float key[4][4][2];
+        uint32_t pm = 0;
 #pragma unroll
-        for (int it = 0; it < 4; ++it)
+        for (int it = 0; it < 4; ++it) {
+            const float4_t xr =
+                *reinterpret_cast<const float4_t *>(&sh.xnp[16 * it + bf_acc_row(lane, 0)]);
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int rloc = 16 * it + bf_acc_row(lane, v);
-                const bool rok = b0 + rloc < bend;
-                const float base_xn = sh.xnp[rloc];
-                const float base_sx = sh.sx[rloc];
+            for (int v = 0; v < 4; ++v)
 #pragma unroll
                 for (int jt = 0; jt < 2; ++jt) {
-                    const float t = __builtin_fmaf(-csq[jt], base_sx, base_xn);
-                    const float kv = __builtin_fmaf(-2.f, acc[it][jt][v], t);
-                    key[it][jt][v] = kv;
-                    any = any || (rok && qok[jt] && kv < th[jt]);
+                    const float kv = __builtin_fmaf(-2.f, acc[it][jt][v], xr[v]);
+                    key[it][v][jt] = kv;
+                    pm |= (kv < th[jt] ? 1u : 0u) << (it * 8 + v * 2 + jt);
                 }
-            }
-        if (__ballot(any) == 0) continue;
+        }
+        if (__ballot(pm != 0) == 0) continue;
 
+        // Slow path: park the keys in LDS, then stage the marked pairs round
+        // by round (one marked pair per lane per round).
 #pragma unroll
         for (int it = 0; it < 4; ++it)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int rloc = 16 * it + bf_acc_row(lane, v);
-                const bool rok = b0 + rloc < bend;
-#pragma unroll
-                for (int jt = 0; jt < 2; ++jt) {
-                    const float kv = key[it][jt][v];
-                    bool pass = rok && qok[jt] && kv < th[jt];
-                    uint64_t mask = __ballot(pass);
-                    if (mask == 0) continue;
-                    int np = __popcll(mask);
-                    if (scnt + np > BF_STAGE) {
-                        bf16_flush<M, NKC>(sh, w, lane, scnt);
-                        scnt = 0;
-                        th[0] = sh.lth[w][lane & 15];
-                        th[1] = sh.lth[w][16 + (lane & 15)];
-                        pass = rok && qok[jt] && kv < th[jt];
-                        mask = __ballot(pass);
-                        np = __popcll(mask);
-                    }
-                    if (pass) {
-                        const int pos = scnt + __popcll(mask & ((1ull << lane) - 1ull));
-                        sh.skey[w][pos] = kv;
-                        sh.sidx[w][pos] = (int32_t)(b0 + rloc);
-                        sh.srow[w][pos] = 16 * jt + (lane & 15);
-                    }
-                    scnt += np;
-                }
+            for (int v = 0; v < 4; ++v)
+                *reinterpret_cast<float2 *>(&sh.kbuf[w][lane][it * 8 + v * 2]) =
+                    make_float2(key[it][v][0], key[it][v][1]);
+        wave_sync_lds();
+        uint32_t rem = pm;
+        const uint64_t lt = (1ull << lane) - 1ull;
+        for (;;) {
+            const bool has = rem != 0;
+            const uint64_t mask = __ballot(has);
+            if (mask == 0) break;
+            const int np = __popcll(mask);
+            if (scnt + np > BF_STAGE) {
+                bf16_flush<M, NKC>(sh, w, lane, scnt);
+                scnt = 0;
             }
+            if (has) {
+                const int b = __builtin_ctz(rem);
+                rem &= rem - 1u;
+                const int pos = scnt + __popcll(mask & lt);
+                const int it = b >> 3, v = (b >> 1) & 3, jt = b & 1;
+                sh.skey[w][pos] = sh.kbuf[w][lane][b];
+                sh.sidx[w][pos] = (int32_t)(b0 + 16 * it + bf_acc_row(lane, v));
+                sh.srow[w][pos] = 16 * jt + (lane & 15);
+            }
+            scnt += np;
+        }
     }
     if (scnt > 0) bf16_flush<M, NKC>(sh, w, lane, scnt);
 

@@ -26,7 +26,8 @@ struct fl_knn {
     DevBuf ypos, classes_dev;      // label position per row, classes
     DevBuf yreg;                   // regression targets
     DevBuf bh, bl;                 // BF16 hi / lo split of X, padded rows
-    DevBuf bxnp, bsx;              // BF16 per-row key terms
+    DevBuf bxnp;                   // BF16 per-row key term (1-b)|x|^2
+    DevBuf bsmax;                  // largest row norm (f64 bits)
     int bf_split_ready = 0;        // hi/lo arrays built
     int bf_terms_passes = 0;       // passes the key terms were built for
     hipStream_t stream = nullptr;
@@ -136,21 +137,22 @@ void knn_search(fl_knn *h, const double *Q, int64_t nq, int k, int mode, double 
         }
         if (h->bf_terms_passes != passes) {
             float *xnp = (float *)h->bxnp.get((size_t)nB * 4);
-            float *sx = (float *)h->bsx.get((size_t)nB * 4);
-            hipLaunchKernelGGL(bf16_row_terms_kernel, dim3(grid1(nB, 256)), dim3(256), 0, st,
-                               (const double *)h->xn.ptr(), (const double *)h->xs.ptr(), nB,
-                               bd.alpha, bd.beta, xnp, sx, (double *)nullptr);
+            hipLaunchKernelGGL(bf16_stream_terms_kernel, dim3(grid1(nB, 256)), dim3(256), 0, st,
+                               (const double *)h->xn.ptr(), nB, bd.beta, xnp);
+            unsigned long long *smax = (unsigned long long *)h->bsmax.get(8);
+            FL_HIP(hipMemsetAsync(smax, 0, 8, st));
+            hipLaunchKernelGGL(max_nonneg_kernel, dim3(256), dim3(256), 0, st,
+                               (const double *)h->xs.ptr(), nB, smax);
             check_launch();
             h->bf_terms_passes = passes;
         }
         unsigned short *qh = ws.get<unsigned short>(p + "qh", nq * dpad);
         unsigned short *ql = ws.get<unsigned short>(p + "ql", nq * dpad);
-        float *csq = ws.get<float>(p + "csq", nq);
         double *qoff = ws.get<double>(p + "qoff", nq);
         hipLaunchKernelGGL(bf16_split_kernel<double>, dim3(grid1(nq * dpad, 256)), dim3(256), 0,
                            st, Q, nq, d, dpad, qh, ql);
-        hipLaunchKernelGGL(bf16_row_terms_kernel, dim3(grid1(nq, 256)), dim3(256), 0, st, qn, qs,
-                           nq, bd.alpha, bd.beta, csq, (float *)nullptr, qoff);
+        hipLaunchKernelGGL(bf16_resident_terms_kernel, dim3(grid1(nq, 256)), dim3(256), 0, st, qn,
+                           qs, nq, bd.alpha, bd.beta, (const double *)h->bsmax.ptr(), qoff);
         check_launch();
         Bf16Args a;
         a.nA = nq;
@@ -160,10 +162,8 @@ void knn_search(fl_knn *h, const double *Q, int64_t nq, int k, int mode, double 
         a.al = ql;
         a.bh = (const unsigned short *)h->bh.ptr();
         a.bl = (const unsigned short *)h->bl.ptr();
-        a.csq = csq;
         a.qoff = qoff;
         a.xnp = (const float *)h->bxnp.ptr();
-        a.sx = (const float *)h->bsx.ptr();
         a.out = cand;
         t0 = tm.begin();
         launch_bf16(passes, M, nkc, a, (int)S, st);
