@@ -13,6 +13,15 @@
 // queries / KMeans samples) are the B operand and stay in VGPRs for the
 // whole kernel, 32 per wave.
 //
+// Candidate lists live in registers: a lane's accumulators hold 2 resident
+// rows x 16 streamed rows per tile, and across the stream the four lanes
+// that share a resident row see disjoint quarters of the streamed rows.  Each
+// lane keeps a private sorted top-MP of its quarter for both of its resident
+// rows.  Any pair in the overall top-MP is in the top-MP of its own quarter,
+// so the four private lists together (4*MP candidates) contain it; every
+// pair not kept has a key >= its quarter's MP-th key, so the smallest of the
+// four MP-th keys bounds every dropped pair from below.
+//
 // Key of a pair, in f32:  key' = fma(-2, acc, xn'_r),  xn'_r = (1 - b)|x_r|^2.
 // The lower bound of the exact fold is key' + qoff_q with the per-row
 // constant qoff_q = (1 - b)|q|^2 - c |q| Smax (Smax = the largest |x_r| of
@@ -111,80 +120,51 @@ struct Bf16Args {
     Cand out;
 };
 
-template <int M, int NKC>
+template <int NKC>
 struct Bf16Shared {
     static constexpr int ROWB = NKC * 64 + 16;   // bytes per staged row (hi or lo) + pad
     unsigned char tile[2][BF_T][ROWB];           // [hi/lo][row][bytes]
     float xnp[BF_T];
-    float kbuf[BF_WAVES][64][32];                // a lane's 32 keys on the slow path
-    float lkey[BF_WAVES][BF_QW][M];
-    int32_t lidx[BF_WAVES][BF_QW][M];
-    int32_t lcnt[BF_WAVES][BF_QW];
-    float lth[BF_WAVES][BF_QW];
-    float skey[BF_WAVES][BF_STAGE];
-    int32_t sidx[BF_WAVES][BF_STAGE];
-    int32_t srow[BF_WAVES][BF_STAGE];
+    float kbuf[BF_WAVES][64][33];                // a lane's 32 keys on the slow path
 };
 
 __device__ __forceinline__ bool pair_less_f(float ka, int32_t ia, float kb, int32_t ib) {
     return ka < kb || (ka == kb && ia < ib);
 }
 
-template <int M, int NKC>
-__device__ void bf16_flush(Bf16Shared<M, NKC> &sh, int w, int lane, int scnt) {
-    wave_sync_lds();
-    if (lane < BF_QW) {
-        const int r = lane;
-        int c = sh.lcnt[w][r];
-        // Scan the staged rows four at a time so that the LDS reads overlap.
-        for (int e0 = 0; e0 < scnt; e0 += 4) {
-            int rows[4];
+// Sorted insert of (k, i) into a register list, dropping the last entry.
+template <int MP>
+__device__ __forceinline__ void plist_insert(float (&lk)[MP], int32_t (&li)[MP], float k,
+                                             int32_t i) {
 #pragma unroll
-            for (int u = 0; u < 4; ++u) rows[u] = e0 + u < scnt ? sh.srow[w][e0 + u] : -1;
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                if (rows[u] != r) continue;
-                const int e = e0 + u;
-                const float k = sh.skey[w][e];
-                const int32_t ix = sh.sidx[w][e];
-                if (c == M && !pair_less_f(k, ix, sh.lkey[w][r][M - 1], sh.lidx[w][r][M - 1]))
-                    continue;
-                int p = c < M ? c++ : M - 1;
-                while (p > 0 && pair_less_f(k, ix, sh.lkey[w][r][p - 1], sh.lidx[w][r][p - 1])) {
-                    sh.lkey[w][r][p] = sh.lkey[w][r][p - 1];
-                    sh.lidx[w][r][p] = sh.lidx[w][r][p - 1];
-                    --p;
-                }
-                sh.lkey[w][r][p] = k;
-                sh.lidx[w][r][p] = ix;
-            }
-        }
-        sh.lcnt[w][r] = c;
-        if (c == M) sh.lth[w][r] = sh.lkey[w][r][M - 1];
+    for (int j = 0; j < MP; ++j) {
+        const bool lt = pair_less_f(k, i, lk[j], li[j]);
+        const float tk = lt ? lk[j] : k;
+        const int32_t ti = lt ? li[j] : i;
+        lk[j] = lt ? k : lk[j];
+        li[j] = lt ? i : li[j];
+        k = tk;
+        i = ti;
     }
-    wave_sync_lds();
 }
 
-template <int M, int NKC, int PASSES>
+template <int MP, int NKC, int PASSES>
 __global__ __launch_bounds__(256) void dist_bf16_kernel(Bf16Args args) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    typedef Bf16Shared<M, NKC> Sh;
+    typedef Bf16Shared<NKC> Sh;
     Sh &sh = *reinterpret_cast<Sh *>(smem_raw);
     constexpr int DPAD = NKC * 32;
+    constexpr int MOUT = 4 * MP;
 
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
+    const int g = lane >> 4;   // quarter of the streamed rows seen by this lane
     const int64_t nA = args.nA;
     const int64_t q0 = (int64_t)blockIdx.x * BF_Q + w * BF_QW;   // this wave's resident rows
     const int split = blockIdx.y;
     const int64_t bbeg = (int64_t)split * args.b_per_split;
     int64_t bend = bbeg + args.b_per_split;
     if (bend > args.nB) bend = args.nB;
-
-    if (lane < BF_QW) {
-        sh.lcnt[w][lane] = 0;
-        sh.lth[w][lane] = __builtin_huge_valf();
-    }
 
     // Resident operand fragments: for column tile jt, lane l holds row
     // q0 + 16*jt + l%16, features kc*32 + 8*(l/16) .. +7.
@@ -198,7 +178,7 @@ __global__ __launch_bounds__(256) void dist_bf16_kernel(Bf16Args args) {
         for (int kc = 0; kc < NKC; ++kc) {
             u16x8_t vh = (u16x8_t){0, 0, 0, 0, 0, 0, 0, 0}, vl = vh;
             if (qok[jt]) {
-                const int64_t off = q * DPAD + kc * 32 + 8 * (lane >> 4);
+                const int64_t off = q * DPAD + kc * 32 + 8 * g;
                 vh = *reinterpret_cast<const u16x8_t *>(args.ah + off);
                 if (PASSES == 3) vl = *reinterpret_cast<const u16x8_t *>(args.al + off);
             }
@@ -206,6 +186,17 @@ __global__ __launch_bounds__(256) void dist_bf16_kernel(Bf16Args args) {
             ql[jt][kc] = __builtin_bit_cast(bf16x8_t, vl);
         }
     }
+
+    // Private candidate lists, ascending (key, index).
+    float pk[2][MP];
+    int32_t pi[2][MP];
+#pragma unroll
+    for (int jt = 0; jt < 2; ++jt)
+#pragma unroll
+        for (int j = 0; j < MP; ++j) {
+            pk[jt][j] = __builtin_huge_valf();
+            pi[jt][j] = 0x7fffffff;
+        }
 
     // Streamed tiles: thread t owns LOADS 16-byte chunks of each tile; the
     // next tile's chunks are fetched into registers while the current tile
@@ -239,11 +230,6 @@ __global__ __launch_bounds__(256) void dist_bf16_kernel(Bf16Args args) {
     };
     wait_vm_loads();   // resident fragments are in registers before the loop
     if (bbeg < bend) fetch(bbeg);
-    // Staged candidates persist across tiles and are merged into the lists
-    // only when the staging area fills (and once at the end): filtering
-    // against a theta that is a little stale keeps a superset, and the merge
-    // re-checks every entry.
-    int scnt = 0;
 
     for (int64_t b0 = bbeg; b0 < bend; b0 += BF_T) {
         __syncthreads();
@@ -280,7 +266,7 @@ __global__ __launch_bounds__(256) void dist_bf16_kernel(Bf16Args args) {
 #pragma unroll
             for (int it = 0; it < 4; ++it) {
                 const int r = 16 * it + (lane & 15);
-                const int boff = kc * 64 + 16 * (lane >> 4);
+                const int boff = kc * 64 + 16 * g;
                 xh[it] = __builtin_bit_cast(bf16x8_t,
                                             *reinterpret_cast<const u16x8_t *>(&sh.tile[0][r][boff]));
                 if (PASSES == 3)
@@ -310,13 +296,12 @@ __global__ __launch_bounds__(256) void dist_bf16_kernel(Bf16Args args) {
         }
 
         // ---- epilogue ----
-        // Fast path: one FMA and one compare per pair; bit it*8 + v*2 + jt of
-        // pm marks a pair whose key is below its row's theta.  Rows past the
-        // end have xn' = +inf and resident rows past nA have theta = -inf.
+        // Fast path: one FMA and one compare per pair against the lane's own
+        // list threshold; bit it*8 + v*2 + jt of pm marks a pair to insert.
+        // Rows past the end have xn' = +inf; resident rows past nA never pass.
         float th[2];
 #pragma unroll
-        for (int jt = 0; jt < 2; ++jt)
-            th[jt] = qok[jt] ? sh.lth[w][16 * jt + (lane & 15)] : -__builtin_huge_valf();
+        for (int jt = 0; jt < 2; ++jt) th[jt] = qok[jt] ? pk[jt][MP - 1] : -__builtin_huge_valf();
         float key[4][4][2];
         uint32_t pm = 0;
 #pragma unroll
@@ -334,60 +319,73 @@ __global__ __launch_bounds__(256) void dist_bf16_kernel(Bf16Args args) {
         }
         if (__ballot(pm != 0) == 0) continue;
 
-        // Slow path: park the keys in LDS, then stage the marked pairs round
-        // by round (one marked pair per lane per round).
+        // Slow path: park the keys in this lane's LDS row, then insert the
+        // marked pairs one at a time (a per-lane loop; the lists are private).
 #pragma unroll
         for (int it = 0; it < 4; ++it)
 #pragma unroll
-            for (int v = 0; v < 4; ++v)
-                *reinterpret_cast<float2 *>(&sh.kbuf[w][lane][it * 8 + v * 2]) =
-                    make_float2(key[it][v][0], key[it][v][1]);
+            for (int v = 0; v < 4; ++v) {
+                sh.kbuf[w][lane][it * 8 + v * 2] = key[it][v][0];
+                sh.kbuf[w][lane][it * 8 + v * 2 + 1] = key[it][v][1];
+            }
         wave_sync_lds();
         uint32_t rem = pm;
-        const uint64_t lt = (1ull << lane) - 1ull;
-        for (;;) {
-            const bool has = rem != 0;
-            const uint64_t mask = __ballot(has);
-            if (mask == 0) break;
-            const int np = __popcll(mask);
-            if (scnt + np > BF_STAGE) {
-                bf16_flush<M, NKC>(sh, w, lane, scnt);
-                scnt = 0;
+        while (rem) {
+            const int b = __builtin_ctz(rem);
+            rem &= rem - 1u;
+            const float kv = sh.kbuf[w][lane][b];
+            const int32_t ix = (int32_t)(b0 + 16 * (b >> 3) + bf_acc_row(lane, (b >> 1) & 3));
+            if (b & 1) {
+                if (pair_less_f(kv, ix, pk[1][MP - 1], pi[1][MP - 1])) plist_insert<MP>(pk[1], pi[1], kv, ix);
+            } else {
+                if (pair_less_f(kv, ix, pk[0][MP - 1], pi[0][MP - 1])) plist_insert<MP>(pk[0], pi[0], kv, ix);
             }
-            if (has) {
-                const int b = __builtin_ctz(rem);
-                rem &= rem - 1u;
-                const int pos = scnt + __popcll(mask & lt);
-                const int it = b >> 3, v = (b >> 1) & 3, jt = b & 1;
-                sh.skey[w][pos] = sh.kbuf[w][lane][b];
-                sh.sidx[w][pos] = (int32_t)(b0 + 16 * it + bf_acc_row(lane, v));
-                sh.srow[w][pos] = 16 * jt + (lane & 15);
-            }
-            scnt += np;
         }
     }
-    if (scnt > 0) bf16_flush<M, NKC>(sh, w, lane, scnt);
 
-    wave_sync_lds();
-    if (lane < BF_QW) {
-        const int64_t q = q0 + lane;
-        if (q < nA) {
-            const int64_t o = (int64_t)split * nA + q;
-            const int c = sh.lcnt[w][lane];
+    // ---- output: the four quarter lists of each resident row ----
+    // Quarter g of split s owns the rows r with ((r - bbeg) & 15) >> 2 == g.
+    // A quarter list with fewer than MP entries must have kept every row of
+    // its quarter; otherwise some row never got a finite key (overflow or
+    // NaN) and theta is -inf, which fails the certificate.
+    const int64_t nrows = bend > bbeg ? bend - bbeg : 0;
+    const int64_t full = nrows >> 4, tail = nrows & 15;
+    const int64_t rows_g = full * 4 + (tail > 4 * g ? (tail - 4 * g < 4 ? tail - 4 * g : 4) : 0);
+#pragma unroll
+    for (int jt = 0; jt < 2; ++jt) {
+        const int64_t q = q0 + 16 * jt + (lane & 15);
+        if (q >= nA) continue;
+        const int64_t o = (int64_t)split * nA + q;
+        int c = 0;
+#pragma unroll
+        for (int j = 0; j < MP; ++j) c += pi[jt][j] != 0x7fffffff ? 1 : 0;
+        float thq;
+        if (c == MP) thq = pk[jt][MP - 1];
+        else thq = (c >= rows_g) ? __builtin_huge_valf() : -__builtin_huge_valf();
+#pragma unroll
+        for (int j = 0; j < MP; ++j) {
+            const bool valid = pi[jt][j] != 0x7fffffff;
+            args.out.key[o * MOUT + g * MP + j] = valid ? (double)pk[jt][j] : __builtin_huge_val();
+            args.out.idx[o * MOUT + g * MP + j] = valid ? pi[jt][j] : -1;
+        }
+        // Combine the four quarters (lanes q, q+16, q+32, q+48 of the wave).
+        float tmin = thq;
+        tmin = fminf(tmin, __shfl_xor(tmin, 16));
+        tmin = fminf(tmin, __shfl_xor(tmin, 32));
+        int ct = c;
+        ct += __shfl_xor(ct, 16);
+        ct += __shfl_xor(ct, 32);
+        if (g == 0) {
             const double qo = args.qoff[q];
-            args.out.cnt[o] = c;
-            if (c == M) {
+            args.out.cnt[o] = ct;
+            args.out.theta_idx[o] = 0x7fffffff;
+            if (tmin == -__builtin_huge_valf() || tmin == __builtin_huge_valf()) {
+                args.out.theta[o] = (double)tmin;
+            } else {
                 // key' + qoff, rounded down by a few f32 ulps of each part so
                 // that theta stays below every key that was not kept.
-                const double kf = (double)sh.lth[w][lane];
+                const double kf = (double)tmin;
                 args.out.theta[o] = (kf + qo) - (fabs(kf) + fabs(qo)) * 0x1.0p-22;
-            } else {
-                args.out.theta[o] = __builtin_huge_val();
-            }
-            args.out.theta_idx[o] = c == M ? sh.lidx[w][lane][M - 1] : 0x7fffffff;
-            for (int m = 0; m < M; ++m) {
-                args.out.key[o * M + m] = m < c ? (double)sh.lkey[w][lane][m] + qo : __builtin_huge_val();
-                args.out.idx[o * M + m] = m < c ? sh.lidx[w][lane][m] : -1;
             }
         }
     }

@@ -353,7 +353,12 @@ __global__ __launch_bounds__(256) void dist_topm_kernel(const TA *__restrict__ A
             const int64_t o = (int64_t)split * nA + r;
             const int c = sh.lcnt[w][lane];
             args.out.cnt[o] = c;
-            args.out.theta[o] = c == M ? sh.lth[w][lane] : __builtin_huge_val();
+            // theta: the M-th key; +inf when the list kept every row of the
+            // split; -inf when rows are missing (a key that never compared
+            // below +inf, i.e. overflow or NaN), which fails the certificate.
+            args.out.theta[o] = c == M ? sh.lth[w][lane]
+                                       : (c >= bend - bbeg ? __builtin_huge_val()
+                                                           : -__builtin_huge_val());
             args.out.theta_idx[o] = c == M ? sh.lthi[w][lane] : 0x7fffffff;
             for (int m = 0; m < M; ++m) {
                 args.out.key[o * M + m] = m < c ? sh.lkey[w][lane][m] : __builtin_huge_val();
@@ -395,8 +400,9 @@ __global__ void to_f64_kernel(const T *__restrict__ X, int64_t count, double *__
 // key <= fold.  With T the exact fold of the k-th answer, a pair outside
 // could only enter if its sqrt(fold) <= sqrt(T); correctly rounded sqrt
 // makes that impossible once fold > T*(1 + 2^-50), so theta_min > T*(1+2^-48)
-// certifies.  A split that kept fewer than M pairs must have kept every row
-// it owns.  Uncertified queries are listed for the exact full scan.
+// certifies.  A split whose lists may miss rows reports theta = -inf.  A NaN
+// fold also leaves the query uncertified.  Uncertified queries are listed for
+// the next, tighter filter.
 constexpr int FIN_MAXC = 512;
 
 template <typename TB>
@@ -410,20 +416,18 @@ __global__ __launch_bounds__(64) void knn_finalize_kernel(
     __shared__ int32_t ci[FIN_MAXC];
     __shared__ double kth_fold;
     __shared__ int kth_found;
+    __shared__ int has_nan;
     const int64_t q = blockIdx.x;
     const int lane = threadIdx.x;
     if (q >= nq) return;
 
+    // theta per split: -inf flags a split whose lists miss rows.
     bool complete = true;
     double thmin = __builtin_huge_val();
     for (int s = 0; s < S; ++s) {
-        const int64_t o = (int64_t)s * nq + q;
-        const int c = cand.cnt[o];
-        int64_t lo = (int64_t)s * b_per_split, hi = lo + b_per_split;
-        if (hi > nB) hi = nB;
-        const int64_t rows = hi > lo ? hi - lo : 0;
-        if (c < M && c < rows) complete = false;
-        if (c == M) thmin = fmin(thmin, cand.theta[o]);
+        const double th = cand.theta[(int64_t)s * nq + q];
+        if (th == -__builtin_huge_val()) complete = false;
+        thmin = fmin(thmin, th);
     }
 
     // compact the valid candidates into LDS
@@ -443,12 +447,17 @@ __global__ __launch_bounds__(64) void knn_finalize_kernel(
     }
     __syncthreads();
     const double *qr = Q + q * d;
+    if (lane == 0) {
+        kth_found = 0;
+        has_nan = 0;
+    }
+    __syncthreads();
     for (int e = lane; e < nc; e += 64) {
         const double f = fold_exact(qr, B + (int64_t)ci[e] * d, d);
         cf[e] = f;
         cd[e] = sqrt(f);
+        if (f != f) has_nan = 1;
     }
-    if (lane == 0) kth_found = 0;
     __syncthreads();
     for (int e = lane; e < nc; e += 64) {
         int rank = 0;
@@ -459,7 +468,7 @@ __global__ __launch_bounds__(64) void knn_finalize_kernel(
         }
     }
     __syncthreads();
-    bool ok = complete && (nc >= k) && kth_found;
+    bool ok = complete && (nc >= k) && kth_found && !has_nan;
     if (ok && certify && thmin != __builtin_huge_val()) ok = thmin > kth_fold + kth_fold * 0x1.0p-48;
     if (!ok) {
         if (lane == 0) flag_list[atomicAdd(flag_count, 1)] = (int32_t)q;

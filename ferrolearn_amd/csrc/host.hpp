@@ -129,10 +129,10 @@ inline Bound bound_bf16(int d, int passes) {
 
 inline void check_launch() { FL_HIP(hipGetLastError()); }
 
-template <int M, int NKC, int PASSES>
+template <int MP, int NKC, int PASSES>
 void launch_bf16_t(const Bf16Args &a, int S, hipStream_t st) {
-    const size_t shmem = sizeof(Bf16Shared<M, NKC>);
-    auto kern = dist_bf16_kernel<M, NKC, PASSES>;
+    const size_t shmem = sizeof(Bf16Shared<NKC>);
+    auto kern = dist_bf16_kernel<MP, NKC, PASSES>;
     FL_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)shmem));
     dim3 grid((unsigned)ceil_div(a.nA, BF_Q), (unsigned)S);
@@ -140,14 +140,14 @@ void launch_bf16_t(const Bf16Args &a, int S, hipStream_t st) {
     check_launch();
 }
 
-template <int M, int PASSES>
+template <int MP, int PASSES>
 void launch_bf16_m(int nkc, const Bf16Args &a, int S, hipStream_t st) {
     switch (nkc) {
-        case 1: return launch_bf16_t<M, 1, PASSES>(a, S, st);
-        case 2: return launch_bf16_t<M, 2, PASSES>(a, S, st);
-        case 3: return launch_bf16_t<M, 3, PASSES>(a, S, st);
-        case 4: return launch_bf16_t<M, 4, PASSES>(a, S, st);
-        case 8: return launch_bf16_t<M, 8, PASSES>(a, S, st);
+        case 1: return launch_bf16_t<MP, 1, PASSES>(a, S, st);
+        case 2: return launch_bf16_t<MP, 2, PASSES>(a, S, st);
+        case 3: return launch_bf16_t<MP, 3, PASSES>(a, S, st);
+        case 4: return launch_bf16_t<MP, 4, PASSES>(a, S, st);
+        case 8: return launch_bf16_t<MP, 8, PASSES>(a, S, st);
     }
     throw DeviceError("internal: unsupported feature chunk count");
 }
@@ -161,20 +161,20 @@ inline int bf16_nkc(int64_t d) {
     return 0;
 }
 
+// MP: private list length per quarter (4*MP candidates per split).
 template <int PASSES>
-void launch_bf16_p(int M, int nkc, const Bf16Args &a, int S, hipStream_t st) {
-    switch (M) {
+void launch_bf16_p(int MP, int nkc, const Bf16Args &a, int S, hipStream_t st) {
+    switch (MP) {
         case 4: return launch_bf16_m<4, PASSES>(nkc, a, S, st);
+        case 8: return launch_bf16_m<8, PASSES>(nkc, a, S, st);
         case 16: return launch_bf16_m<16, PASSES>(nkc, a, S, st);
-        case 32: return launch_bf16_m<32, PASSES>(nkc, a, S, st);
-        case 64: return launch_bf16_m<64, PASSES>(nkc, a, S, st);
     }
     throw DeviceError("internal: unsupported candidate list size");
 }
 
-inline void launch_bf16(int passes, int M, int nkc, const Bf16Args &a, int S, hipStream_t st) {
-    if (passes == 3) return launch_bf16_p<3>(M, nkc, a, S, st);
-    return launch_bf16_p<1>(M, nkc, a, S, st);
+inline void launch_bf16(int passes, int MP, int nkc, const Bf16Args &a, int S, hipStream_t st) {
+    if (passes == 3) return launch_bf16_p<3>(MP, nkc, a, S, st);
+    return launch_bf16_p<1>(MP, nkc, a, S, st);
 }
 
 template <typename TA, typename TB, int M, int MODE>

@@ -92,7 +92,7 @@ void knn_search(fl_knn *h, const double *Q, int64_t nq, int k, int mode, double 
     if (nq == 0) return;
     hipStream_t st = h->stream;
     Workspace &ws = h->ws;
-    const std::string p = depth ? "fb_" : "";
+    const std::string p = depth ? "d" + std::to_string(depth) + "_" : "";
     const TB *B = (const TB *)h->X.ptr();
     const int64_t nB = h->n;
     const int d = (int)h->d;
@@ -103,8 +103,12 @@ void knn_search(fl_knn *h, const double *Q, int64_t nq, int k, int mode, double 
                        d, qn, qs);
     check_launch();
 
-    const int M = mode == MODE_EXACT ? pick_m_exact(k) : pick_m_filter(k);
-    const int rows_per_block = (mode == MODE_BF16 || mode == MODE_BF16X3) ? BF_Q : TM_ROWS;
+    const bool bf = mode == MODE_BF16 || mode == MODE_BF16X3;
+    // BF16 kernels keep 4 private lists of MP per resident row (M = 4*MP
+    // candidates per split); the f64 kernels keep one list of M in LDS.
+    const int MP = k <= 6 ? 8 : 16;
+    const int M = bf ? 4 * MP : (mode == MODE_EXACT ? pick_m_exact(k) : pick_m_filter(k));
+    const int rows_per_block = bf ? BF_Q : TM_ROWS;
     const int64_t nblk = ceil_div(nq, rows_per_block);
     int64_t S = ceil_div(2048, nblk);
     S = std::min<int64_t>(S, FIN_MAXC / M);
@@ -166,7 +170,7 @@ void knn_search(fl_knn *h, const double *Q, int64_t nq, int k, int mode, double 
         a.xnp = (const float *)h->bxnp.ptr();
         a.out = cand;
         t0 = tm.begin();
-        launch_bf16(passes, M, nkc, a, (int)S, st);
+        launch_bf16(passes, MP, nkc, a, (int)S, st);
         tm.end(t0);
     } else {
         TopmArgs a;
@@ -217,22 +221,28 @@ void knn_search(fl_knn *h, const double *Q, int64_t nq, int k, int mode, double 
         throw StatusError{FL_NUMERICAL_INSTABILITY,
                           "NumericalInstability: a distance is NaN (the reference panics "
                           "here, kdtree.rs:386)"};
-    acc.n_fb += nflag;
-    double *Qf = ws.get<double>("fbq", (int64_t)nflag * d);
+    // Uncertified queries go to the next, tighter filter: BF16 -> f64 GEMM
+    // form -> exact folds.
+    const int next_mode = bf ? MODE_MFMA64 : MODE_EXACT;
+    if (depth == 0) acc.n_fb += nflag;
+    const std::string fp = "fb" + std::to_string(depth) + "_";
+    double *Qf = ws.get<double>(fp + "q", (int64_t)nflag * d);
     hipLaunchKernelGGL(gather_rows_kernel, dim3(grid1((int64_t)nflag * d, 256)), dim3(256), 0, st,
                        Q, d, flags, (int64_t)nflag, Qf);
     check_launch();
-    double *df = ws.get<double>("fbd", (int64_t)nflag * k);
-    int64_t *xf = ws.get<int64_t>("fbi", (int64_t)nflag * k);
-    knn_search<TB>(h, Qf, nflag, k, MODE_EXACT, df, xf, acc, depth + 1);
+    double *df = ws.get<double>(fp + "d", (int64_t)nflag * k);
+    int64_t *xf = ws.get<int64_t>(fp + "i", (int64_t)nflag * k);
+    knn_search<TB>(h, Qf, nflag, k, next_mode, df, xf, acc, depth + 1);
     hipLaunchKernelGGL(scatter_results_kernel, dim3(grid1((int64_t)nflag * k, 256)), dim3(256), 0,
                        st, df, xf, flags, (int64_t)nflag, k, dist, idx);
     check_launch();
 }
 
-int filter_mode(int32_t precision, int64_t d) {
+// The BF16 filters keep register lists sized for k <= 24 and d <= 256; other
+// shapes use the f64 GEMM-form filter (same results).
+int filter_mode(int32_t precision, int64_t d, int64_t k) {
     if (precision == FL_PREC_EXACT) return MODE_EXACT;
-    if ((precision == FL_PREC_BF16 || precision == FL_PREC_BF16X3) && bf16_nkc(d) > 0)
+    if ((precision == FL_PREC_BF16 || precision == FL_PREC_BF16X3) && bf16_nkc(d) > 0 && k <= 24)
         return precision == FL_PREC_BF16 ? MODE_BF16 : MODE_BF16X3;
     return MODE_MFMA64;
 }
@@ -243,7 +253,7 @@ void search_device(fl_knn *h, const void *dQ, int32_t dtype, int64_t nq, int k, 
     const double *Q = dtype == FL_F32
                           ? as_f64<float>((const float *)dQ, nq * h->d, h->ws, "q64", h->stream)
                           : (const double *)dQ;
-    const int mode = filter_mode(h->precision, h->d);
+    const int mode = filter_mode(h->precision, h->d, k);
     if (h->dtype == FL_F32)
         knn_search<float>(h, Q, nq, k, mode, dist, idx, acc, 0);
     else
