@@ -17,10 +17,14 @@
 // rows x 16 streamed rows per tile, and across the stream the four lanes
 // that share a resident row see disjoint quarters of the streamed rows.  Each
 // lane keeps a private sorted top-MP of its quarter for both of its resident
-// rows.  Any pair in the overall top-MP is in the top-MP of its own quarter,
-// so the four private lists together (4*MP candidates) contain it; every
-// pair not kept has a key >= its quarter's MP-th key, so the smallest of the
-// four MP-th keys bounds every dropped pair from below.
+// rows, and all four lanes filter against T = the smallest of the four
+// lists' MP-th keys.  Each quarter's MP-th key is >= the overall MP-th key,
+// so T never drops a pair of the overall top-MP, and that pair is in the
+// top-MP of its own quarter: the four lists together contain the overall
+// top-MP.  A pair that is not kept was filtered (key >= T at that time) or
+// evicted (key >= its list's MP-th key >= T), and T only decreases, so the
+// final T bounds every dropped pair from below.  Only pairs with key < T
+// leave the kernel.
 //
 // Key of a pair, in f32:  key' = fma(-2, acc, xn'_r),  xn'_r = (1 - b)|x_r|^2.
 // The lower bound of the exact fold is key' + qoff_q with the per-row
@@ -89,13 +93,31 @@ static __global__ __attribute__((unused)) void bf16_stream_terms_kernel(
     if (i < n) xnp[i] = (float)((1.0 - b) * n2[i]);
 }
 
-// Per resident row: qoff = (1-b)|q|^2 - c|q| Smax (f64); *smax is the
-// largest streamed-row norm.
+// Per resident row: qoff = (1-b)|q|^2 - c|q| Smax - eta (f64); *smax is the
+// largest streamed-row norm; eta = (d+1) 2^-100 (1 + |q| + Smax) covers bf16
+// and f32 flushing of values below 2^-126.
 static __global__ __attribute__((unused)) void bf16_resident_terms_kernel(
-    const double *__restrict__ n2, const double *__restrict__ n1, int64_t n, double c, double b,
-    const double *__restrict__ smax, double *__restrict__ qoff) {
+    const double *__restrict__ n2, const double *__restrict__ n1, int64_t n, int d, double c,
+    double b, const double *__restrict__ smax, double *__restrict__ qoff) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) qoff[i] = (1.0 - b) * n2[i] - c * n1[i] * (*smax);
+    if (i >= n) return;
+    const double sm = *smax;
+    const double eta = (double)(d + 1) * 0x1.0p-100 * (1.0 + n1[i] + sm);
+    qoff[i] = (1.0 - b) * n2[i] - c * n1[i] * sm - eta;
+}
+
+// *flag = 1 if any value is non-finite or larger than 2^56 in magnitude (the
+// BF16 filter's bound assumes neither; such data uses the f64 filter).
+template <typename T>
+__global__ void bf16_unsafe_kernel(const T *__restrict__ X, int64_t count,
+                                   unsigned int *__restrict__ flag) {
+    bool bad = false;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double v = (double)X[i];
+        bad = bad || !(fabs(v) <= 0x1.0p56);
+    }
+    if (__ballot(bad) != 0 && (threadIdx.x & 63) == 0) atomicOr(flag, 1u);
 }
 
 // *out = max(*out, max_i v[i]) for non-negative v; *out must start at 0.
@@ -237,11 +259,11 @@ __global__ __launch_bounds__(256) void dist_bf16_kernel(Bf16Args args) {
         for (int u = 0; u < LOADS; ++u) {
             const int t = threadIdx.x + 256 * u;
             if (t < CHUNKS) {
+                // Rows past the split end hold copies of the last row; their
+                // keys are made +inf through xn' below.
                 const int part = t / (BF_T * CH);
                 const int rem = t % (BF_T * CH);
-                const bool ok = b0 + rem / CH < bend;
-                *reinterpret_cast<u16x8_t *>(&sh.tile[part][rem / CH][(rem % CH) * 16]) =
-                    ok ? pf[u] : (u16x8_t){0, 0, 0, 0, 0, 0, 0, 0};
+                *reinterpret_cast<u16x8_t *>(&sh.tile[part][rem / CH][(rem % CH) * 16]) = pf[u];
             }
         }
         if (threadIdx.x < BF_T) {
@@ -301,7 +323,12 @@ __global__ __launch_bounds__(256) void dist_bf16_kernel(Bf16Args args) {
         // Rows past the end have xn' = +inf; resident rows past nA never pass.
         float th[2];
 #pragma unroll
-        for (int jt = 0; jt < 2; ++jt) th[jt] = qok[jt] ? pk[jt][MP - 1] : -__builtin_huge_valf();
+        for (int jt = 0; jt < 2; ++jt) {
+            float t = pk[jt][MP - 1];
+            t = fminf(t, __shfl_xor(t, 16));
+            t = fminf(t, __shfl_xor(t, 32));
+            th[jt] = qok[jt] ? t : -__builtin_huge_valf();
+        }
         float key[4][4][2];
         uint32_t pm = 0;
 #pragma unroll
@@ -329,28 +356,26 @@ __global__ __launch_bounds__(256) void dist_bf16_kernel(Bf16Args args) {
                 sh.kbuf[w][lane][it * 8 + v * 2 + 1] = key[it][v][1];
             }
         wave_sync_lds();
-        uint32_t rem = pm;
-        while (rem) {
-            const int b = __builtin_ctz(rem);
-            rem &= rem - 1u;
-            const float kv = sh.kbuf[w][lane][b];
-            const int32_t ix = (int32_t)(b0 + 16 * (b >> 3) + bf_acc_row(lane, (b >> 1) & 3));
-            if (b & 1) {
-                if (pair_less_f(kv, ix, pk[1][MP - 1], pi[1][MP - 1])) plist_insert<MP>(pk[1], pi[1], kv, ix);
-            } else {
-                if (pair_less_f(kv, ix, pk[0][MP - 1], pi[0][MP - 1])) plist_insert<MP>(pk[0], pi[0], kv, ix);
+#pragma unroll
+        for (int jt = 0; jt < 2; ++jt) {
+            uint32_t rem = pm & (jt ? 0xAAAAAAAAu : 0x55555555u);
+            while (rem) {
+                const int b = __builtin_ctz(rem);
+                rem &= rem - 1u;
+                const float kv = sh.kbuf[w][lane][b];
+                const int32_t ix =
+                    (int32_t)(b0 + 16 * (b >> 3) + bf_acc_row(lane, (b >> 1) & 3));
+                if (pair_less_f(kv, ix, pk[jt][MP - 1], pi[jt][MP - 1]))
+                    plist_insert<MP>(pk[jt], pi[jt], kv, ix);
             }
         }
     }
 
     // ---- output: the four quarter lists of each resident row ----
-    // Quarter g of split s owns the rows r with ((r - bbeg) & 15) >> 2 == g.
-    // A quarter list with fewer than MP entries must have kept every row of
-    // its quarter; otherwise some row never got a finite key (overflow or
-    // NaN) and theta is -inf, which fails the certificate.
-    const int64_t nrows = bend > bbeg ? bend - bbeg : 0;
-    const int64_t full = nrows >> 4, tail = nrows & 15;
-    const int64_t rows_g = full * 4 + (tail > 4 * g ? (tail - 4 * g < 4 ? tail - 4 * g : 4) : 0);
+    // Inputs are finite and bounded (the host routes other data to the f64
+    // filter), so every streamed row gets a finite key: a list that is not
+    // full lost rows only to the shared filter, and its own threshold is
+    // +inf.
 #pragma unroll
     for (int jt = 0; jt < 2; ++jt) {
         const int64_t q = q0 + 16 * jt + (lane & 15);
@@ -359,28 +384,30 @@ __global__ __launch_bounds__(256) void dist_bf16_kernel(Bf16Args args) {
         int c = 0;
 #pragma unroll
         for (int j = 0; j < MP; ++j) c += pi[jt][j] != 0x7fffffff ? 1 : 0;
-        float thq;
-        if (c == MP) thq = pk[jt][MP - 1];
-        else thq = (c >= rows_g) ? __builtin_huge_valf() : -__builtin_huge_valf();
-#pragma unroll
-        for (int j = 0; j < MP; ++j) {
-            const bool valid = pi[jt][j] != 0x7fffffff;
-            args.out.key[o * MOUT + g * MP + j] = valid ? (double)pk[jt][j] : __builtin_huge_val();
-            args.out.idx[o * MOUT + g * MP + j] = valid ? pi[jt][j] : -1;
-        }
+        const float thq = c == MP ? pk[jt][MP - 1] : __builtin_huge_valf();
         // Combine the four quarters (lanes q, q+16, q+32, q+48 of the wave).
         float tmin = thq;
         tmin = fminf(tmin, __shfl_xor(tmin, 16));
         tmin = fminf(tmin, __shfl_xor(tmin, 32));
-        int ct = c;
+        // Pairs at or above the final T cannot be answers once the
+        // certificate holds (their fold is >= T), so only keys below T leave.
+        int ck = 0;
+#pragma unroll
+        for (int j = 0; j < MP; ++j) {
+            const bool valid = pi[jt][j] != 0x7fffffff && pk[jt][j] < tmin;
+            ck += valid ? 1 : 0;
+            args.out.key[o * MOUT + g * MP + j] = valid ? (double)pk[jt][j] : __builtin_huge_val();
+            args.out.idx[o * MOUT + g * MP + j] = valid ? pi[jt][j] : -1;
+        }
+        int ct = ck;
         ct += __shfl_xor(ct, 16);
         ct += __shfl_xor(ct, 32);
         if (g == 0) {
             const double qo = args.qoff[q];
             args.out.cnt[o] = ct;
             args.out.theta_idx[o] = 0x7fffffff;
-            if (tmin == -__builtin_huge_valf() || tmin == __builtin_huge_valf()) {
-                args.out.theta[o] = (double)tmin;
+            if (tmin == __builtin_huge_valf()) {
+                args.out.theta[o] = __builtin_huge_val();
             } else {
                 // key' + qoff, rounded down by a few f32 ulps of each part so
                 // that theta stays below every key that was not kept.

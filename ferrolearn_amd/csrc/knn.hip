@@ -29,6 +29,7 @@ struct fl_knn {
     DevBuf bxnp;                   // BF16 per-row key term (1-b)|x|^2
     DevBuf bsmax;                  // largest row norm (f64 bits)
     int bf_split_ready = 0;        // hi/lo arrays built
+    int bf_db_unsafe = -1;         // X has non-finite or > 2^56 values (-1: unknown)
     int bf_terms_passes = 0;       // passes the key terms were built for
     hipStream_t stream = nullptr;
     Workspace ws;
@@ -156,7 +157,7 @@ void knn_search(fl_knn *h, const double *Q, int64_t nq, int k, int mode, double 
         hipLaunchKernelGGL(bf16_split_kernel<double>, dim3(grid1(nq * dpad, 256)), dim3(256), 0,
                            st, Q, nq, d, dpad, qh, ql);
         hipLaunchKernelGGL(bf16_resident_terms_kernel, dim3(grid1(nq, 256)), dim3(256), 0, st, qn,
-                           qs, nq, bd.alpha, bd.beta, (const double *)h->bsmax.ptr(), qoff);
+                           qs, nq, d, bd.alpha, bd.beta, (const double *)h->bsmax.ptr(), qoff);
         check_launch();
         Bf16Args a;
         a.nA = nq;
@@ -253,7 +254,35 @@ void search_device(fl_knn *h, const void *dQ, int32_t dtype, int64_t nq, int k, 
     const double *Q = dtype == FL_F32
                           ? as_f64<float>((const float *)dQ, nq * h->d, h->ws, "q64", h->stream)
                           : (const double *)dQ;
-    const int mode = filter_mode(h->precision, h->d, k);
+    int mode = filter_mode(h->precision, h->d, k);
+    if (mode == MODE_BF16 || mode == MODE_BF16X3) {
+        // The BF16 bound needs finite inputs below 2^56 in magnitude.
+        unsigned int *flag = h->ws.get<unsigned int>("unsafe", 1);
+        if (h->bf_db_unsafe < 0) {
+            FL_HIP(hipMemsetAsync(flag, 0, 4, h->stream));
+            if (h->dtype == FL_F32)
+                hipLaunchKernelGGL(bf16_unsafe_kernel<float>, dim3(1024), dim3(256), 0, h->stream,
+                                   (const float *)h->X.ptr(), h->n * h->d, flag);
+            else
+                hipLaunchKernelGGL(bf16_unsafe_kernel<double>, dim3(1024), dim3(256), 0,
+                                   h->stream, (const double *)h->X.ptr(), h->n * h->d, flag);
+            check_launch();
+            unsigned int hf = 0;
+            FL_HIP(hipMemcpyAsync(&hf, flag, 4, hipMemcpyDeviceToHost, h->stream));
+            FL_HIP(hipStreamSynchronize(h->stream));
+            h->bf_db_unsafe = hf ? 1 : 0;
+        }
+        unsigned int hq = 0;
+        if (!h->bf_db_unsafe && nq > 0) {
+            FL_HIP(hipMemsetAsync(flag, 0, 4, h->stream));
+            hipLaunchKernelGGL(bf16_unsafe_kernel<double>, dim3(256), dim3(256), 0, h->stream, Q,
+                               nq * h->d, flag);
+            check_launch();
+            FL_HIP(hipMemcpyAsync(&hq, flag, 4, hipMemcpyDeviceToHost, h->stream));
+            FL_HIP(hipStreamSynchronize(h->stream));
+        }
+        if (h->bf_db_unsafe || hq) mode = MODE_MFMA64;
+    }
     if (h->dtype == FL_F32)
         knn_search<float>(h, Q, nq, k, mode, dist, idx, acc, 0);
     else

@@ -196,3 +196,31 @@ def test_sqrt_and_division_correctly_rounded():
     dist, idx = clf.kneighbors(Q)
     d_o, i_o = O.kneighbors(X, Q, 64)
     assert (idx == i_o).all() and (bits(dist) == bits(d_o)).all()
+
+
+@pytest.mark.parametrize("precision", ["bf16x3", "fp64"])
+def test_unsafe_magnitudes_fall_back_exactly(precision):
+    # Values beyond the BF16 filter's range (and f64 overflow of the squares)
+    # must still give the reference's answer: the library routes such data to
+    # a filter that can take it.
+    rng = np.random.default_rng(21)
+    X = rng.normal(size=(3000, 16))
+    X[7] *= 1e20
+    X[11] *= 1e200                      # squares overflow to inf
+    Q = rng.normal(size=(40, 16))
+    Q[3] = X[7] * 1.0000001
+    clf = (F.KNeighborsClassifier().with_n_neighbors(5).with_precision(precision)
+           .fit(X, np.arange(3000) % 4))
+    dist, idx = clf.kneighbors(Q)
+    d_o, i_o = O.kneighbors(X, Q, 5)
+    assert (idx == i_o).all() and (bits(dist) == bits(d_o)).all()
+
+
+def test_nan_input_reports_numerical_instability():
+    # The reference panics on NaN distances (kdtree.rs:386); the library
+    # reports NumericalInstability instead of returning a wrong order.
+    X = np.random.default_rng(0).normal(size=(500, 8))
+    X[10, 3] = np.nan
+    clf = F.KNeighborsClassifier().with_n_neighbors(3).fit(X, np.zeros(500, np.int64))
+    with pytest.raises(F.NumericalInstability):
+        clf.kneighbors(X[:20])
