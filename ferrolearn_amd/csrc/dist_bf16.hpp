@@ -16,15 +16,14 @@
 // Candidate lists live in registers: a lane's accumulators hold 2 resident
 // rows x 16 streamed rows per tile, and across the stream the four lanes
 // that share a resident row see disjoint quarters of the streamed rows.  Each
-// lane keeps a private sorted top-MP of its quarter for both of its resident
-// rows, and all four lanes filter against T = the smallest of the four
-// lists' MP-th keys.  Each quarter's MP-th key is >= the overall MP-th key,
-// so T never drops a pair of the overall top-MP, and that pair is in the
-// top-MP of its own quarter: the four lists together contain the overall
-// top-MP.  A pair that is not kept was filtered (key >= T at that time) or
-// evicted (key >= its list's MP-th key >= T), and T only decreases, so the
-// final T bounds every dropped pair from below.  Only pairs with key < T
-// leave the kernel.
+// lane keeps a private sorted list of up to MP pairs of its quarter for both
+// of its resident rows, and all four lanes filter against a shared threshold
+// T (shared_threshold below) that is an upper bound of the MP-th smallest
+// key kept so far.  A pair that is not kept was filtered (key >= T at that
+// time) or evicted from a full list (key >= that list's last key >= T), and
+// T only decreases, so the final T bounds every dropped pair from below; the
+// lists hold at least MP pairs with key <= T.  The output is the MP smallest
+// pairs of the four lists, and its bound is their MP-th key.
 //
 // Key of a pair, in f32:  key' = fma(-2, acc, xn'_r),  xn'_r = (1 - b)|x_r|^2.
 // The lower bound of the exact fold is key' + qoff_q with the per-row
@@ -170,13 +169,28 @@ __device__ __forceinline__ void plist_insert(float (&lk)[MP], int32_t (&li)[MP],
     }
 }
 
+// Shared filter threshold of the four quarter lists of one resident row
+// (lanes l, l^16, l^32, l^48):  T = min(max_i list_i[MP/4 - 1],
+// min_i list_i[MP - 1]).  The four lists hold MP keys <= the first term, so
+// it bounds the union's MP-th key from above; each quarter's MP-th key does
+// too.  Both terms only decrease as lists fill, so T is monotone.
+template <int MP>
+__device__ __forceinline__ float shared_threshold(const float (&lk)[MP]) {
+    float a = lk[MP / 4 - 1];
+    a = fmaxf(a, __shfl_xor(a, 16));
+    a = fmaxf(a, __shfl_xor(a, 32));
+    float b = lk[MP - 1];
+    b = fminf(b, __shfl_xor(b, 16));
+    b = fminf(b, __shfl_xor(b, 32));
+    return fminf(a, b);
+}
+
 template <int MP, int NKC, int PASSES>
 __global__ __launch_bounds__(256) void dist_bf16_kernel(Bf16Args args) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     typedef Bf16Shared<NKC> Sh;
     Sh &sh = *reinterpret_cast<Sh *>(smem_raw);
     constexpr int DPAD = NKC * 32;
-    constexpr int MOUT = 4 * MP;
 
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
@@ -323,12 +337,7 @@ __global__ __launch_bounds__(256) void dist_bf16_kernel(Bf16Args args) {
         // Rows past the end have xn' = +inf; resident rows past nA never pass.
         float th[2];
 #pragma unroll
-        for (int jt = 0; jt < 2; ++jt) {
-            float t = pk[jt][MP - 1];
-            t = fminf(t, __shfl_xor(t, 16));
-            t = fminf(t, __shfl_xor(t, 32));
-            th[jt] = qok[jt] ? t : -__builtin_huge_valf();
-        }
+        for (int jt = 0; jt < 2; ++jt) th[jt] = qok[jt] ? shared_threshold<MP>(pk[jt]) : -__builtin_huge_valf();
         float key[4][4][2];
         uint32_t pm = 0;
 #pragma unroll
@@ -371,48 +380,59 @@ __global__ __launch_bounds__(256) void dist_bf16_kernel(Bf16Args args) {
         }
     }
 
-    // ---- output: the four quarter lists of each resident row ----
+    // ---- output: the MP smallest pairs of the four quarter lists ----
+    // The bound of every pair not output is the MP-th key of the union (the
+    // lists hold at least MP keys <= T, so it is <= T); with fewer than MP
+    // pairs in the union nothing was ever dropped and the bound is +inf.
     // Inputs are finite and bounded (the host routes other data to the f64
-    // filter), so every streamed row gets a finite key: a list that is not
-    // full lost rows only to the shared filter, and its own threshold is
-    // +inf.
+    // filter), so every streamed row got a finite key.
+    __syncthreads();
+    float *okey = &sh.kbuf[w][0][0];                              // [16][4*MP]
+    int32_t *oidx = reinterpret_cast<int32_t *>(okey + 16 * 4 * MP);
+    const int c16 = lane & 15;
 #pragma unroll
     for (int jt = 0; jt < 2; ++jt) {
-        const int64_t q = q0 + 16 * jt + (lane & 15);
-        if (q >= nA) continue;
-        const int64_t o = (int64_t)split * nA + q;
-        int c = 0;
-#pragma unroll
-        for (int j = 0; j < MP; ++j) c += pi[jt][j] != 0x7fffffff ? 1 : 0;
-        const float thq = c == MP ? pk[jt][MP - 1] : __builtin_huge_valf();
-        // Combine the four quarters (lanes q, q+16, q+32, q+48 of the wave).
-        float tmin = thq;
-        tmin = fminf(tmin, __shfl_xor(tmin, 16));
-        tmin = fminf(tmin, __shfl_xor(tmin, 32));
-        // Pairs at or above the final T cannot be answers once the
-        // certificate holds (their fold is >= T), so only keys below T leave.
-        int ck = 0;
+        wave_sync_lds();
 #pragma unroll
         for (int j = 0; j < MP; ++j) {
-            const bool valid = pi[jt][j] != 0x7fffffff && pk[jt][j] < tmin;
-            ck += valid ? 1 : 0;
-            args.out.key[o * MOUT + g * MP + j] = valid ? (double)pk[jt][j] : __builtin_huge_val();
-            args.out.idx[o * MOUT + g * MP + j] = valid ? pi[jt][j] : -1;
+            okey[c16 * 4 * MP + g * MP + j] = pk[jt][j];
+            oidx[c16 * 4 * MP + g * MP + j] = pi[jt][j];
         }
-        int ct = ck;
-        ct += __shfl_xor(ct, 16);
-        ct += __shfl_xor(ct, 32);
-        if (g == 0) {
-            const double qo = args.qoff[q];
-            args.out.cnt[o] = ct;
-            args.out.theta_idx[o] = 0x7fffffff;
-            if (tmin == __builtin_huge_valf()) {
-                args.out.theta[o] = __builtin_huge_val();
-            } else {
+        wave_sync_lds();
+        const int64_t q = q0 + 16 * jt + c16;
+        int nv = 0;
+#pragma unroll
+        for (int j = 0; j < MP; ++j) nv += pi[jt][j] != 0x7fffffff ? 1 : 0;
+        nv += __shfl_xor(nv, 16);
+        nv += __shfl_xor(nv, 32);
+        if (q >= nA) continue;
+        const int64_t o = (int64_t)split * nA + q;
+        const double qo = args.qoff[q];
+#pragma unroll
+        for (int j = 0; j < MP; ++j) {
+            if (pi[jt][j] == 0x7fffffff) continue;
+            int r = 0;
+            for (int e = 0; e < 4 * MP; ++e)
+                r += pair_less_f(okey[c16 * 4 * MP + e], oidx[c16 * 4 * MP + e], pk[jt][j],
+                                 pi[jt][j]) ? 1 : 0;
+            if (r < MP) {
+                args.out.key[o * MP + r] = (double)pk[jt][j] + qo;
+                args.out.idx[o * MP + r] = pi[jt][j];
+            }
+            if (r == MP - 1) {
                 // key' + qoff, rounded down by a few f32 ulps of each part so
-                // that theta stays below every key that was not kept.
-                const double kf = (double)tmin;
+                // that the bound stays below every key that was not output.
+                const double kf = (double)pk[jt][j];
                 args.out.theta[o] = (kf + qo) - (fabs(kf) + fabs(qo)) * 0x1.0p-22;
+            }
+        }
+        if (g == 0) {
+            args.out.cnt[o] = nv < MP ? nv : MP;
+            args.out.theta_idx[o] = 0x7fffffff;
+            if (nv < MP) args.out.theta[o] = __builtin_huge_val();
+            for (int r = nv; r < MP; ++r) {
+                args.out.key[o * MP + r] = __builtin_huge_val();
+                args.out.idx[o * MP + r] = -1;
             }
         }
     }

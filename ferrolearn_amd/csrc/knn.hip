@@ -105,10 +105,10 @@ void knn_search(fl_knn *h, const double *Q, int64_t nq, int k, int mode, double 
     check_launch();
 
     const bool bf = mode == MODE_BF16 || mode == MODE_BF16X3;
-    // BF16 kernels keep 4 private lists of MP per resident row (M = 4*MP
-    // candidates per split); the f64 kernels keep one list of M in LDS.
+    // BF16 kernels output MP candidates per split (merged from four register
+    // lists per resident row); the f64 kernels keep one list of M in LDS.
     const int MP = k <= 6 ? 8 : 16;
-    const int M = bf ? 4 * MP : (mode == MODE_EXACT ? pick_m_exact(k) : pick_m_filter(k));
+    const int M = bf ? MP : (mode == MODE_EXACT ? pick_m_exact(k) : pick_m_filter(k));
     const int rows_per_block = bf ? BF_Q : TM_ROWS;
     const int64_t nblk = ceil_div(nq, rows_per_block);
     int64_t S = ceil_div(2048, nblk);
