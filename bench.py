@@ -160,12 +160,27 @@ def run_knn(args):
     return line
 
 
+def copy_calibration(mib=512):
+    """A plain device-to-device copy of known size: its FETCH_SIZE / WRITE_SIZE
+    counters give the scale that turns those counters into HBM bytes."""
+    import ferrolearn_amd as F
+    from ferrolearn_amd import _lib as L
+    import ctypes as C
+    n = mib * (1 << 20) // 4
+    a = F.to_device(np.ones(n, np.float32))
+    b = F.device.empty((n,), np.float32)
+    for _ in range(3):
+        L.lib().fl_memcpy_device(C.c_void_p(b.data_ptr()), C.c_void_p(a.data_ptr()), n * 4)
+    F.device.synchronize()
+    log(f"copied {mib} MiB three times")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="knn", choices=["knn"])
+    ap.add_argument("--workload", default="knn", choices=["knn", "copy"])
     ap.add_argument("--forms", default="bf16x3,fp64",
                     help="comma list; the first is reported as value (fp64, exact, bf16, bf16x3)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -177,6 +192,9 @@ def main():
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(--gpus must be 1)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "copy":
+        copy_calibration()
+        return
     line = run_knn(args)
     print(json.dumps(line), flush=True)
 

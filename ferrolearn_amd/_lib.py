@@ -104,6 +104,7 @@ SIGNATURES = {
     "fl_device_free": (C.c_int, [_vp]),
     "fl_memcpy_to_device": (C.c_int, [_vp, _vp, C.c_uint64]),
     "fl_memcpy_to_host": (C.c_int, [_vp, _vp, C.c_uint64]),
+    "fl_memcpy_device": (C.c_int, [_vp, _vp, C.c_uint64]),
     "fl_device_synchronize": (C.c_int, []),
 }
 

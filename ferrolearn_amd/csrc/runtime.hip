@@ -74,6 +74,13 @@ fl_status fl_memcpy_to_host(void *dst, const void *src, uint64_t bytes) {
     });
 }
 
+fl_status fl_memcpy_device(void *dst, const void *src, uint64_t bytes) {
+    return guarded([&]() -> fl_status {
+        if (bytes) FL_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+        return FL_OK;
+    });
+}
+
 fl_status fl_device_synchronize(void) {
     return guarded([&]() -> fl_status {
         FL_HIP(hipDeviceSynchronize());

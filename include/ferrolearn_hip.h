@@ -91,6 +91,7 @@ fl_status fl_device_malloc(void **ptr, uint64_t bytes);
 fl_status fl_device_free(void *ptr);
 fl_status fl_memcpy_to_device(void *dst, const void *src, uint64_t bytes);
 fl_status fl_memcpy_to_host(void *dst, const void *src, uint64_t bytes);
+fl_status fl_memcpy_device(void *dst, const void *src, uint64_t bytes);
 fl_status fl_device_synchronize(void);
 
 /* ------------------------------------------------------------------ */
