@@ -9,21 +9,16 @@
 // Operands are pre-split into hi / lo bf16 arrays padded to NKC*32 features.
 //
 // Roles: the streamed rows (kNN database / KMeans centres) are the MFMA A
-// operand, staged through LDS 64 rows at a time; the resident rows (kNN
-// queries / KMeans samples) are the B operand and stay in VGPRs for the
-// whole kernel, 32 per wave.
+// operand, staged through LDS 64 rows at a time by direct-to-LDS buffer
+// loads (double buffered, shared by the 8 waves of a block); the resident
+// rows (kNN queries / KMeans samples) are the B operand and stay in VGPRs for
+// the whole kernel, 32 per wave.
 //
-// Candidate lists live in registers: a lane's accumulators hold 2 resident
-// rows x 16 streamed rows per tile, and across the stream the four lanes
-// that share a resident row see disjoint quarters of the streamed rows.  Each
-// lane keeps a private sorted list of up to MP pairs of its quarter for both
-// of its resident rows, and all four lanes filter against a shared threshold
-// T (shared_threshold below) that is an upper bound of the MP-th smallest
-// key kept so far.  A pair that is not kept was filtered (key >= T at that
-// time) or evicted from a full list (key >= that list's last key >= T), and
-// T only decreases, so the final T bounds every dropped pair from below; the
-// lists hold at least MP pairs with key <= T.  The output is the MP smallest
-// pairs of the four lists, and its bound is their MP-th key.
+// Candidate lists live in registers: each of the four lanes that share a
+// resident row keeps a sorted list of MP/2 pairs of its quarter of the
+// streamed rows, filtered against a threshold shared by the four lanes; the
+// output is the MP smallest pairs of their union with a bound theta on every
+// pair that was dropped (see dist_bf16_kernel).
 //
 // Key of a pair, in f32:  key' = fma(-2, acc, xn'_r),  xn'_r = (1 - b)|x_r|^2.
 // The lower bound of the exact fold is key' + qoff_q with the per-row
@@ -44,11 +39,10 @@ namespace fl {
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef unsigned short u16x8_t __attribute__((ext_vector_type(8)));
 
-constexpr int BF_WAVES = 4;
+constexpr int BF_WAVES = 8;
 constexpr int BF_QW = 32;                 // resident rows per wave (two 16-col tiles)
 constexpr int BF_Q = BF_WAVES * BF_QW;    // resident rows per block
 constexpr int BF_T = 64;                  // streamed rows per tile (four 16-row tiles)
-constexpr int BF_STAGE = 256;
 
 // Output layout of v_mfma_f32_16x16x32_bf16: lane l holds D[i][l%16] with
 // i = bf_acc_row(l, v) for accumulator register v = 0..3.
@@ -136,17 +130,34 @@ struct Bf16Args {
     int64_t b_per_split;
     const unsigned short *ah, *al;   // resident hi/lo [nA][dpad]
     const unsigned short *bh, *bl;   // streamed hi/lo [nB][dpad]
-    const double *qoff;              // resident (1-b)|q|^2 - c|q|Smax
+    const double *qoff;              // resident (1-b)|q|^2 - c|q|Smax - eta
     const float *xnp;                // streamed (1-b)|x|^2
     Cand out;
 };
 
-template <int NKC>
-struct Bf16Shared {
-    static constexpr int ROWB = NKC * 64 + 16;   // bytes per staged row (hi or lo) + pad
-    unsigned char tile[2][BF_T][ROWB];           // [hi/lo][row][bytes]
-    float xnp[BF_T];
-    float kbuf[BF_WAVES][64][33];                // a lane's 32 keys on the slow path
+// Raw buffer resource over [base, base + bytes): loads at offsets past the
+// end return zero (the hardware range check), so a tile that runs past the
+// end of its split needs no clamping.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t bf_rsrc(const void *base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes,
+                                             0x00020000);
+}
+
+// LDS layout of one staged tile: [part (hi, lo)][row 0..63][16-byte chunk],
+// rows of NKC*64 bytes with the chunk order XOR-swizzled by the row
+// (position = chunk ^ (row & SWZ)), so that the 16 rows read by one MFMA
+// fragment load fall in distinct banks.  Two tiles are resident: one being
+// multiplied while the next one is copied in by direct-to-LDS buffer loads.
+template <int NKC, int PASSES>
+struct Bf16Tile {
+    static constexpr int CH = NKC * 4;                    // 16-byte chunks per row
+    static constexpr int ROWB = CH * 16;
+    static constexpr int SWZ = (CH < 16 ? CH : 16) - 1;
+    static constexpr int PARTS = PASSES == 3 ? 2 : 1;
+    static constexpr int BYTES = PARTS * BF_T * ROWB;
+    static constexpr int NBLK = BYTES / 1024;             // 1 KiB copy blocks per tile
+    static constexpr int BPW = (NBLK + BF_WAVES - 1) / BF_WAVES;
+    static constexpr size_t SHMEM = 2 * (size_t)BYTES + 2 * BF_T * sizeof(float);
 };
 
 __device__ __forceinline__ bool pair_less_f(float ka, int32_t ia, float kb, int32_t ib) {
@@ -154,12 +165,14 @@ __device__ __forceinline__ bool pair_less_f(float ka, int32_t ia, float kb, int3
 }
 
 // Sorted insert of (k, i) into a register list, dropping the last entry.
-template <int MP>
-__device__ __forceinline__ void plist_insert(float (&lk)[MP], int32_t (&li)[MP], float k,
+// A lane sees its pairs in increasing index order, so an equal key goes
+// after the entries already there and the list stays sorted by (key, index).
+template <int L>
+__device__ __forceinline__ void plist_insert(float (&lk)[L], int32_t (&li)[L], float k,
                                              int32_t i) {
 #pragma unroll
-    for (int j = 0; j < MP; ++j) {
-        const bool lt = pair_less_f(k, i, lk[j], li[j]);
+    for (int j = 0; j < L; ++j) {
+        const bool lt = k < lk[j];
         const float tk = lt ? lk[j] : k;
         const int32_t ti = lt ? li[j] : i;
         lk[j] = lt ? k : lk[j];
@@ -169,34 +182,56 @@ __device__ __forceinline__ void plist_insert(float (&lk)[MP], int32_t (&li)[MP],
     }
 }
 
-// Shared filter threshold of the four quarter lists of one resident row
-// (lanes l, l^16, l^32, l^48):  T = min(max_i list_i[MP/4 - 1],
-// min_i list_i[MP - 1]).  The four lists hold MP keys <= the first term, so
-// it bounds the union's MP-th key from above; each quarter's MP-th key does
-// too.  Both terms only decrease as lists fill, so T is monotone.
-template <int MP>
-__device__ __forceinline__ float shared_threshold(const float (&lk)[MP]) {
+// Filter threshold shared by the four lists of one resident row (lanes l,
+// l^16, l^32, l^48), each of length L = MP/2:
+//   T = min(max_i list_i[MP/4 - 1], min_i list_i[L - 1]).
+// When the first term is finite the four lists hold at least MP keys <= it;
+// the second is the smallest key any list could still evict.  Both terms
+// only decrease.
+template <int MP, int L>
+__device__ __forceinline__ float shared_threshold(const float (&lk)[L]) {
     float a = lk[MP / 4 - 1];
     a = fmaxf(a, __shfl_xor(a, 16));
     a = fmaxf(a, __shfl_xor(a, 32));
-    float b = lk[MP - 1];
+    float b = lk[L - 1];
     b = fminf(b, __shfl_xor(b, 16));
     b = fminf(b, __shfl_xor(b, 32));
     return fminf(a, b);
 }
 
+// One block = 8 waves x 32 resident rows, all sharing each staged tile of
+// 64 streamed rows.  Grid: (ceil(nA / 256), S splits of the streamed rows).
+//
+// Candidate lists: lane l's accumulators hold resident rows 16*jt + l%16
+// (jt = 0, 1) against streamed rows 16*it + 4*(l/16) + v of each tile, so
+// the four lanes of a resident row see disjoint quarters of the streamed
+// rows.  Each lane keeps a sorted list of L = MP/2 pairs per resident row.
+// A pair is dropped only when (a) its key is >= T at the start of its tile,
+// (b) it is >= its lane's last entry, or (c) it is evicted (then it is >=
+// the new last entry).  T and the last entries only decrease, so every
+// dropped pair is >= min(T_end, min_i last_i) >= theta, where
+//   theta = min(MP-th key of the union of the four lists, min_i last_i)
+// (T_end >= the MP-th key of the union whenever T_end is finite).  The MP
+// smallest pairs of the union are output with that theta; pairs of the
+// union that are not output are >= its MP-th key.
 template <int MP, int NKC, int PASSES>
-__global__ __launch_bounds__(256) void dist_bf16_kernel(Bf16Args args) {
+__global__ __launch_bounds__(512) void dist_bf16_kernel(Bf16Args args) {
+    // The body is compiled for the device only: the host pass needs just the
+    // launch stub, and it does not accept the buffer-to-LDS builtin with
+    // per-lane offsets.
+#if defined(__HIP_DEVICE_COMPILE__)
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    typedef Bf16Shared<NKC> Sh;
-    Sh &sh = *reinterpret_cast<Sh *>(smem_raw);
+    typedef Bf16Tile<NKC, PASSES> TL;
+    constexpr int L = MP / 2;
     constexpr int DPAD = NKC * 32;
+    float *xs = reinterpret_cast<float *>(smem_raw + 2 * TL::BYTES);   // [2][BF_T]
 
     const int lane = threadIdx.x & 63;
-    const int w = threadIdx.x >> 6;
-    const int g = lane >> 4;   // quarter of the streamed rows seen by this lane
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave-uniform
+    const int g = lane >> 4;
+    const int c16 = lane & 15;
     const int64_t nA = args.nA;
-    const int64_t q0 = (int64_t)blockIdx.x * BF_Q + w * BF_QW;   // this wave's resident rows
+    const int64_t q0 = (int64_t)blockIdx.x * BF_Q + w * BF_QW;
     const int split = blockIdx.y;
     const int64_t bbeg = (int64_t)split * args.b_per_split;
     int64_t bend = bbeg + args.b_per_split;
@@ -208,7 +243,7 @@ __global__ __launch_bounds__(256) void dist_bf16_kernel(Bf16Args args) {
     bool qok[2];
 #pragma unroll
     for (int jt = 0; jt < 2; ++jt) {
-        const int64_t q = q0 + 16 * jt + (lane & 15);
+        const int64_t q = q0 + 16 * jt + c16;
         qok[jt] = q < nA;
 #pragma unroll
         for (int kc = 0; kc < NKC; ++kc) {
@@ -223,91 +258,84 @@ __global__ __launch_bounds__(256) void dist_bf16_kernel(Bf16Args args) {
         }
     }
 
-    // Private candidate lists, ascending (key, index).
-    float pk[2][MP];
-    int32_t pi[2][MP];
+    float lk[2][L];
+    int32_t li[2][L];
 #pragma unroll
     for (int jt = 0; jt < 2; ++jt)
 #pragma unroll
-        for (int j = 0; j < MP; ++j) {
-            pk[jt][j] = __builtin_huge_valf();
-            pi[jt][j] = 0x7fffffff;
+        for (int j = 0; j < L; ++j) {
+            lk[jt][j] = __builtin_huge_valf();
+            li[jt][j] = 0x7fffffff;
         }
+    float th[2];
+#pragma unroll
+    for (int jt = 0; jt < 2; ++jt) th[jt] = qok[jt] ? __builtin_huge_valf() : -__builtin_huge_valf();
 
-    // Streamed tiles: thread t owns LOADS 16-byte chunks of each tile; the
-    // next tile's chunks are fetched into registers while the current tile
-    // is multiplied, then written to LDS after a barrier.
-    constexpr int CH = DPAD / 8;                  // 16-byte chunks per row
-    constexpr int CHUNKS = (PASSES == 3 ? 2 : 1) * BF_T * CH;
-    constexpr int LOADS = (CHUNKS + 255) / 256;
-    u16x8_t pf[LOADS];
+    // Copy plan: 1 KiB block b of a tile is written by wave b % 8; lane l of
+    // the block fills LDS slot 64*b + l, i.e. part, row and swizzled position
+    // (row, pos), with chunk pos ^ (row & SWZ) of that streamed row.
+    const uint32_t split_bytes = (uint32_t)((bend - bbeg) * TL::ROWB);
+    const __amdgpu_buffer_rsrc_t rh = bf_rsrc(args.bh + bbeg * DPAD, split_bytes);
+    const __amdgpu_buffer_rsrc_t rl = bf_rsrc(args.bl + bbeg * DPAD, split_bytes);
+    uint32_t voff[TL::BPW];
+#pragma unroll
+    for (int j = 0; j < TL::BPW; ++j) {
+        const int s = 64 * (j * BF_WAVES + w) + lane;
+        const int rem = s % (BF_T * TL::CH);
+        const int row = rem / TL::CH, pos = rem % TL::CH;
+        voff[j] = (uint32_t)(row * TL::ROWB + ((pos ^ (row & TL::SWZ)) * 16));
+    }
     float pf_xn = 0.f;
-    auto fetch = [&](int64_t t0) {
-        // Branch-free: rows past the split end read the last valid row; the
-        // values are zeroed when they are written to LDS, so that nothing
-        // here consumes a load result (which would force a wait on it).
+    auto issue = [&](int buf, int64_t t0) {
+        const uint32_t soff = (uint32_t)((t0 - bbeg) * TL::ROWB);
 #pragma unroll
-        for (int u = 0; u < LOADS; ++u) {
-            const int t = threadIdx.x + 256 * u;
-            const int tt = t < CHUNKS ? t : 0;
-            const int part = tt / (BF_T * CH);
-            const int rem = tt % (BF_T * CH);
-            const int r = rem / CH, ch = rem % CH;
-            const int64_t gb = t0 + r;
-            const int64_t gc = gb < bend ? gb : bend - 1;
-            const unsigned short *src = part ? args.bl : args.bh;
-            pf[u] = *reinterpret_cast<const u16x8_t *>(src + gc * DPAD + ch * 8);
-        }
-        if (threadIdx.x < BF_T) {
-            const int64_t gb = t0 + threadIdx.x;
-            const int64_t gc = gb < bend ? gb : bend - 1;
-            pf_xn = args.xnp[gc];
-        }
-    };
-    wait_vm_loads();   // resident fragments are in registers before the loop
-    if (bbeg < bend) fetch(bbeg);
-
-    for (int64_t b0 = bbeg; b0 < bend; b0 += BF_T) {
-        __syncthreads();
-#pragma unroll
-        for (int u = 0; u < LOADS; ++u) {
-            const int t = threadIdx.x + 256 * u;
-            if (t < CHUNKS) {
-                // Rows past the split end hold copies of the last row; their
-                // keys are made +inf through xn' below.
-                const int part = t / (BF_T * CH);
-                const int rem = t % (BF_T * CH);
-                *reinterpret_cast<u16x8_t *>(&sh.tile[part][rem / CH][(rem % CH) * 16]) = pf[u];
+        for (int j = 0; j < TL::BPW; ++j) {
+            const int b = j * BF_WAVES + w;   // a block never straddles the hi/lo parts
+            if (TL::NBLK % BF_WAVES == 0 || b < TL::NBLK) {
+                __attribute__((address_space(3))) void *dst =
+                    (__attribute__((address_space(3))) void *)(smem_raw + buf * TL::BYTES +
+                                                                1024 * b);
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(b >= TL::NBLK / TL::PARTS ? rl : rh, dst,
+                                                         16, voff[j], soff, 0, 0);
             }
         }
-        if (threadIdx.x < BF_T) {
-            // Rows past the split end get an infinite key: they never pass.
-            const bool ok = b0 + threadIdx.x < bend;
-            sh.xnp[threadIdx.x] = ok ? pf_xn : __builtin_huge_valf();
+        if (w == 0) {
+            const int64_t r = t0 + lane;
+            pf_xn = r < bend ? args.xnp[r] : __builtin_huge_valf();
         }
-        __syncthreads();
-        if (b0 + BF_T < bend) fetch(b0 + BF_T);
+    };
+
+    int cur = 0;
+    if (bbeg < bend) {
+        issue(0, bbeg);
+        if (w == 0) xs[lane] = pf_xn;
+    }
+    wait_vm_loads();
+    __syncthreads();
+
+    for (int64_t b0 = bbeg; b0 < bend; b0 += BF_T, cur ^= 1) {
+        const bool more = b0 + BF_T < bend;
+        if (more) issue(cur ^ 1, b0 + BF_T);
+        const unsigned char *tb = smem_raw + cur * TL::BYTES;
 
         float4_t acc[4][2];
 #pragma unroll
         for (int it = 0; it < 4; ++it)
 #pragma unroll
             for (int jt = 0; jt < 2; ++jt) acc[it][jt] = (float4_t){0.f, 0.f, 0.f, 0.f};
-        // For each feature chunk: read the four 16-row fragments (hi, lo),
-        // then issue the products pass by pass so that eight independent
-        // MFMAs separate two updates of one accumulator.
 #pragma unroll
         for (int kc = 0; kc < NKC; ++kc) {
+            const int pos = (kc * 4 + g) ^ (c16 & TL::SWZ);
+            const unsigned char *fr = tb + c16 * TL::ROWB + pos * 16;
             bf16x8_t xh[4], xl[4];
 #pragma unroll
             for (int it = 0; it < 4; ++it) {
-                const int r = 16 * it + (lane & 15);
-                const int boff = kc * 64 + 16 * g;
-                xh[it] = __builtin_bit_cast(bf16x8_t,
-                                            *reinterpret_cast<const u16x8_t *>(&sh.tile[0][r][boff]));
+                xh[it] = __builtin_bit_cast(
+                    bf16x8_t, *reinterpret_cast<const u16x8_t *>(fr + it * 16 * TL::ROWB));
                 if (PASSES == 3)
                     xl[it] = __builtin_bit_cast(
-                        bf16x8_t, *reinterpret_cast<const u16x8_t *>(&sh.tile[1][r][boff]));
+                        bf16x8_t, *reinterpret_cast<const u16x8_t *>(
+                                      fr + BF_T * TL::ROWB + it * 16 * TL::ROWB));
             }
 #pragma unroll
             for (int it = 0; it < 4; ++it)
@@ -331,111 +359,106 @@ __global__ __launch_bounds__(256) void dist_bf16_kernel(Bf16Args args) {
             }
         }
 
-        // ---- epilogue ----
-        // Fast path: one FMA and one compare per pair against the lane's own
-        // list threshold; bit it*8 + v*2 + jt of pm marks a pair to insert.
-        // Rows past the end have xn' = +inf; resident rows past nA never pass.
-        float th[2];
-#pragma unroll
-        for (int jt = 0; jt < 2; ++jt) th[jt] = qok[jt] ? shared_threshold<MP>(pk[jt]) : -__builtin_huge_valf();
+        // ---- epilogue: one FMA and one compare per pair ----
+        // Streamed rows past the split end have xn' = +inf and zero data, so
+        // their key is +inf; resident rows past nA have T = -inf.
+        const float *xt = xs + cur * BF_T;
         float key[4][4][2];
-        uint32_t pm = 0;
+        bool any = false;
 #pragma unroll
         for (int it = 0; it < 4; ++it) {
-            const float4_t xr =
-                *reinterpret_cast<const float4_t *>(&sh.xnp[16 * it + bf_acc_row(lane, 0)]);
+            const float4_t xr = *reinterpret_cast<const float4_t *>(&xt[16 * it + 4 * g]);
 #pragma unroll
             for (int v = 0; v < 4; ++v)
 #pragma unroll
                 for (int jt = 0; jt < 2; ++jt) {
                     const float kv = __builtin_fmaf(-2.f, acc[it][jt][v], xr[v]);
                     key[it][v][jt] = kv;
-                    pm |= (kv < th[jt] ? 1u : 0u) << (it * 8 + v * 2 + jt);
+                    any = any || kv < th[jt];
                 }
         }
-        if (__ballot(pm != 0) == 0) continue;
-
-        // Slow path: park the keys in this lane's LDS row, then insert the
-        // marked pairs one at a time (a per-lane loop; the lists are private).
+        if (__ballot(any) != 0) {
+            const int32_t rb = (int32_t)b0 + 4 * g;
 #pragma unroll
-        for (int it = 0; it < 4; ++it)
+            for (int jt = 0; jt < 2; ++jt)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                sh.kbuf[w][lane][it * 8 + v * 2] = key[it][v][0];
-                sh.kbuf[w][lane][it * 8 + v * 2 + 1] = key[it][v][1];
-            }
-        wave_sync_lds();
+                for (int it = 0; it < 4; ++it)
 #pragma unroll
-        for (int jt = 0; jt < 2; ++jt) {
-            uint32_t rem = pm & (jt ? 0xAAAAAAAAu : 0x55555555u);
-            while (rem) {
-                const int b = __builtin_ctz(rem);
-                rem &= rem - 1u;
-                const float kv = sh.kbuf[w][lane][b];
-                const int32_t ix =
-                    (int32_t)(b0 + 16 * (b >> 3) + bf_acc_row(lane, (b >> 1) & 3));
-                if (pair_less_f(kv, ix, pk[jt][MP - 1], pi[jt][MP - 1]))
-                    plist_insert<MP>(pk[jt], pi[jt], kv, ix);
-            }
+                    for (int v = 0; v < 4; ++v) {
+                        const float kv = key[it][v][jt];
+                        if (kv < th[jt] && kv < lk[jt][L - 1])
+                            plist_insert<L>(lk[jt], li[jt], kv, rb + 16 * it + v);
+                    }
+#pragma unroll
+            for (int jt = 0; jt < 2; ++jt)
+                th[jt] = qok[jt] ? shared_threshold<MP, L>(lk[jt]) : -__builtin_huge_valf();
         }
+
+        if (more && w == 0) xs[(cur ^ 1) * BF_T + lane] = pf_xn;
+        wait_vm_loads();   // this wave's copies of the next tile have landed
+        __syncthreads();
     }
 
-    // ---- output: the MP smallest pairs of the four quarter lists ----
-    // The bound of every pair not output is the MP-th key of the union (the
-    // lists hold at least MP keys <= T, so it is <= T); with fewer than MP
-    // pairs in the union nothing was ever dropped and the bound is +inf.
-    // Inputs are finite and bounded (the host routes other data to the f64
-    // filter), so every streamed row got a finite key.
-    __syncthreads();
-    float *okey = &sh.kbuf[w][0][0];                              // [16][4*MP]
-    int32_t *oidx = reinterpret_cast<int32_t *>(okey + 16 * 4 * MP);
-    const int c16 = lane & 15;
+    // ---- output: the MP smallest pairs of the union of the four lists ----
+    float *okey = reinterpret_cast<float *>(smem_raw) + w * (16 * 4 * L * 2);   // [16][4L]
+    int32_t *oidx = reinterpret_cast<int32_t *>(okey + 16 * 4 * L);
 #pragma unroll
     for (int jt = 0; jt < 2; ++jt) {
         wave_sync_lds();
 #pragma unroll
-        for (int j = 0; j < MP; ++j) {
-            okey[c16 * 4 * MP + g * MP + j] = pk[jt][j];
-            oidx[c16 * 4 * MP + g * MP + j] = pi[jt][j];
+        for (int j = 0; j < L; ++j) {
+            okey[c16 * 4 * L + g * L + j] = lk[jt][j];
+            oidx[c16 * 4 * L + g * L + j] = li[jt][j];
         }
         wave_sync_lds();
-        const int64_t q = q0 + 16 * jt + c16;
         int nv = 0;
 #pragma unroll
-        for (int j = 0; j < MP; ++j) nv += pi[jt][j] != 0x7fffffff ? 1 : 0;
+        for (int j = 0; j < L; ++j) nv += li[jt][j] != 0x7fffffff ? 1 : 0;
         nv += __shfl_xor(nv, 16);
         nv += __shfl_xor(nv, 32);
-        if (q >= nA) continue;
+        float lastmin = lk[jt][L - 1];
+        lastmin = fminf(lastmin, __shfl_xor(lastmin, 16));
+        lastmin = fminf(lastmin, __shfl_xor(lastmin, 32));
+        const int64_t q = q0 + 16 * jt + c16;
+        const bool live = q < nA;
         const int64_t o = (int64_t)split * nA + q;
-        const double qo = args.qoff[q];
+        const double qo = live ? args.qoff[q] : 0.0;
+        float ump = __builtin_huge_valf();
 #pragma unroll
-        for (int j = 0; j < MP; ++j) {
-            if (pi[jt][j] == 0x7fffffff) continue;
+        for (int j = 0; j < L; ++j) {
+            if (li[jt][j] == 0x7fffffff) continue;
             int r = 0;
-            for (int e = 0; e < 4 * MP; ++e)
-                r += pair_less_f(okey[c16 * 4 * MP + e], oidx[c16 * 4 * MP + e], pk[jt][j],
-                                 pi[jt][j]) ? 1 : 0;
-            if (r < MP) {
-                args.out.key[o * MP + r] = (double)pk[jt][j] + qo;
-                args.out.idx[o * MP + r] = pi[jt][j];
+            for (int e = 0; e < 4 * L; ++e)
+                r += pair_less_f(okey[c16 * 4 * L + e], oidx[c16 * 4 * L + e], lk[jt][j],
+                                 li[jt][j]) ? 1 : 0;
+            if (live && r < MP) {
+                args.out.key[o * MP + r] = (double)lk[jt][j] + qo;
+                args.out.idx[o * MP + r] = li[jt][j];
             }
-            if (r == MP - 1) {
+            if (r == MP - 1) ump = lk[jt][j];
+        }
+        ump = fminf(ump, __shfl_xor(ump, 16));
+        ump = fminf(ump, __shfl_xor(ump, 32));
+        const float thk = fminf(ump, lastmin);
+        if (live && g == 0) {
+            const int nout = nv < MP ? nv : MP;
+            args.out.cnt[o] = nout;
+            args.out.theta_idx[o] = 0x7fffffff;
+            if (thk == __builtin_huge_valf()) {
+                args.out.theta[o] = __builtin_huge_val();
+            } else {
                 // key' + qoff, rounded down by a few f32 ulps of each part so
                 // that the bound stays below every key that was not output.
-                const double kf = (double)pk[jt][j];
+                const double kf = (double)thk;
                 args.out.theta[o] = (kf + qo) - (fabs(kf) + fabs(qo)) * 0x1.0p-22;
             }
-        }
-        if (g == 0) {
-            args.out.cnt[o] = nv < MP ? nv : MP;
-            args.out.theta_idx[o] = 0x7fffffff;
-            if (nv < MP) args.out.theta[o] = __builtin_huge_val();
-            for (int r = nv; r < MP; ++r) {
+            for (int r = nout; r < MP; ++r) {
                 args.out.key[o * MP + r] = __builtin_huge_val();
                 args.out.idx[o * MP + r] = -1;
             }
         }
     }
+#endif
 }
 
 }  // namespace fl

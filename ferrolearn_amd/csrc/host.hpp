@@ -3,6 +3,7 @@
 // error-bound constants and the filter-kernel launcher.
 #pragma once
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <map>
@@ -131,12 +132,15 @@ inline void check_launch() { FL_HIP(hipGetLastError()); }
 
 template <int MP, int NKC, int PASSES>
 void launch_bf16_t(const Bf16Args &a, int S, hipStream_t st) {
-    const size_t shmem = sizeof(Bf16Shared<NKC>);
+    // Staged tiles during the stream; the output merge reuses the same LDS
+    // (per wave 16 rows x 4 lists x MP/2 pairs of 8 bytes).
+    const size_t merge = (size_t)BF_WAVES * 16 * 4 * (MP / 2) * 8;
+    const size_t shmem = std::max(Bf16Tile<NKC, PASSES>::SHMEM, merge);
     auto kern = dist_bf16_kernel<MP, NKC, PASSES>;
     FL_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)shmem));
     dim3 grid((unsigned)ceil_div(a.nA, BF_Q), (unsigned)S);
-    hipLaunchKernelGGL(kern, grid, dim3(256), shmem, st, a);
+    hipLaunchKernelGGL(kern, grid, dim3(BF_WAVES * 64), shmem, st, a);
     check_launch();
 }
 
@@ -145,27 +149,27 @@ void launch_bf16_m(int nkc, const Bf16Args &a, int S, hipStream_t st) {
     switch (nkc) {
         case 1: return launch_bf16_t<MP, 1, PASSES>(a, S, st);
         case 2: return launch_bf16_t<MP, 2, PASSES>(a, S, st);
-        case 3: return launch_bf16_t<MP, 3, PASSES>(a, S, st);
         case 4: return launch_bf16_t<MP, 4, PASSES>(a, S, st);
         case 8: return launch_bf16_t<MP, 8, PASSES>(a, S, st);
     }
     throw DeviceError("internal: unsupported feature chunk count");
 }
 
-// NKC (32-feature chunks) used for d features by the BF16 kernels; 0 when
-// d is too large for the register-resident layout (d > 256).
+// NKC (32-feature chunks, a power of two for the LDS swizzle) used for d
+// features by the BF16 kernels; 0 when d is too large for the
+// register-resident layout (d > 256).
 inline int bf16_nkc(int64_t d) {
     const int64_t c = (d + 31) / 32;
-    if (c <= 4) return (int)c;
+    if (c <= 2) return (int)c;
+    if (c <= 4) return 4;
     if (c <= 8) return 8;
     return 0;
 }
 
-// MP: private list length per quarter (4*MP candidates per split).
+// MP: candidates output per resident row and split (lists of MP/2 per lane).
 template <int PASSES>
 void launch_bf16_p(int MP, int nkc, const Bf16Args &a, int S, hipStream_t st) {
     switch (MP) {
-        case 4: return launch_bf16_m<4, PASSES>(nkc, a, S, st);
         case 8: return launch_bf16_m<8, PASSES>(nkc, a, S, st);
         case 16: return launch_bf16_m<16, PASSES>(nkc, a, S, st);
     }
@@ -206,6 +210,42 @@ void launch_topm(int M, int mode, const TA *A, const TB *B, const TopmArgs &a, i
         }
     }
     throw DeviceError("internal: unsupported candidate list size");
+}
+
+// Compute units of the current device (cached per device).
+inline int device_cu_count() {
+    static int cache[64] = {0};
+    int dev = 0;
+    FL_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return 256;
+    if (cache[dev] == 0) {
+        int n = 0;
+        FL_HIP(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
+        cache[dev] = n > 0 ? n : 256;
+    }
+    return cache[dev];
+}
+
+// Number of splits of the streamed rows for the BF16 filter.  A block holds
+// one compute unit (8 waves); time ~ rounds of blocks x tiles per split x
+// (1 + 0.3 x the share of tiles whose epilogue inserts), where that share
+// follows the list-filling transient: about 512 (1 + ln(T / 512)) of the T
+// tiles of a split.
+inline int64_t bf16_splits(int64_t nblk, int64_t nB, int64_t smax) {
+    const double cus = device_cu_count();
+    int64_t best = 1;
+    double best_cost = 0.0;
+    for (int64_t S = 1; S <= std::max<int64_t>(1, smax); ++S) {
+        const double tiles = std::ceil((double)nB / (double)S / 64.0);
+        const double rounds = std::ceil((double)(nblk * S) / cus);
+        const double slow = tiles <= 512.0 ? 1.0 : 512.0 * (1.0 + std::log(tiles / 512.0)) / tiles;
+        const double cost = rounds * tiles * (1.0 + 0.3 * slow);
+        if (S == 1 || cost < best_cost) {
+            best = S;
+            best_cost = cost;
+        }
+    }
+    return best;
 }
 
 inline unsigned grid1(int64_t n, int block) { return (unsigned)ceil_div(n, block); }

@@ -111,10 +111,20 @@ void knn_search(fl_knn *h, const double *Q, int64_t nq, int k, int mode, double 
     const int M = bf ? MP : (mode == MODE_EXACT ? pick_m_exact(k) : pick_m_filter(k));
     const int rows_per_block = bf ? BF_Q : TM_ROWS;
     const int64_t nblk = ceil_div(nq, rows_per_block);
-    int64_t S = ceil_div(2048, nblk);
-    S = std::min<int64_t>(S, FIN_MAXC / M);
-    S = std::min<int64_t>(S, std::max<int64_t>(1, nB / 256));
-    S = std::max<int64_t>(S, 1);
+    const int64_t smax = std::max<int64_t>(
+        1, std::min<int64_t>(FIN_MAXC / M, std::max<int64_t>(1, nB / 256)));
+    int64_t S;
+    if (bf) {
+        // Each split's hi/lo rows must stay within 2^31 bytes (32-bit buffer
+        // offsets of the staged copies).
+        const int64_t row_bytes = (int64_t)bf16_nkc(d) * 64;
+        const int64_t smin = ceil_div(nB * row_bytes, (int64_t)1 << 31);
+        S = std::max<int64_t>(smin, bf16_splits(nblk, nB, smax));
+    } else {
+        S = ceil_div(2048, nblk);
+        S = std::min<int64_t>(S, smax);
+        S = std::max<int64_t>(S, 1);
+    }
     const int64_t bps = ceil_div(nB, S);
     S = ceil_div(nB, bps);
 
@@ -241,9 +251,11 @@ void knn_search(fl_knn *h, const double *Q, int64_t nq, int k, int mode, double 
 
 // The BF16 filters keep register lists sized for k <= 24 and d <= 256; other
 // shapes use the f64 GEMM-form filter (same results).
-int filter_mode(int32_t precision, int64_t d, int64_t k) {
+int filter_mode(int32_t precision, int64_t d, int64_t k, int64_t n) {
     if (precision == FL_PREC_EXACT) return MODE_EXACT;
-    if ((precision == FL_PREC_BF16 || precision == FL_PREC_BF16X3) && bf16_nkc(d) > 0 && k <= 24)
+    // BF16 kernels index streamed rows with int32.
+    if ((precision == FL_PREC_BF16 || precision == FL_PREC_BF16X3) && bf16_nkc(d) > 0 && k <= 24 &&
+        n < ((int64_t)1 << 31) - 64)
         return precision == FL_PREC_BF16 ? MODE_BF16 : MODE_BF16X3;
     return MODE_MFMA64;
 }
@@ -254,7 +266,7 @@ void search_device(fl_knn *h, const void *dQ, int32_t dtype, int64_t nq, int k, 
     const double *Q = dtype == FL_F32
                           ? as_f64<float>((const float *)dQ, nq * h->d, h->ws, "q64", h->stream)
                           : (const double *)dQ;
-    int mode = filter_mode(h->precision, h->d, k);
+    int mode = filter_mode(h->precision, h->d, k, h->n);
     if (mode == MODE_BF16 || mode == MODE_BF16X3) {
         // The BF16 bound needs finite inputs below 2^56 in magnitude.
         unsigned int *flag = h->ws.get<unsigned int>("unsafe", 1);
