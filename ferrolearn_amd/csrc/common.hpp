@@ -65,9 +65,25 @@ __device__ __forceinline__ double ld64(const T *p) {
 // Exact reference fold acc = acc + (a_j - b_j)^2, left to right from +0.
 template <typename TA, typename TB>
 __device__ __forceinline__ double fold_exact(const TA *a, const TB *b, int d) {
+    // Loads are issued 16 at a time so that their latencies overlap; the sum
+    // is still the left fold in feature order.
     double acc = 0.0;
-    for (int j = 0; j < d; ++j) {
-        double t = (double)a[j] - (double)b[j];
+    int j = 0;
+    for (; j + 16 <= d; j += 16) {
+        double av[16], bv[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            av[u] = (double)a[j + u];
+            bv[u] = (double)b[j + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const double t = av[u] - bv[u];
+            acc = acc + t * t;
+        }
+    }
+    for (; j < d; ++j) {
+        const double t = (double)a[j] - (double)b[j];
         acc = acc + t * t;
     }
     return acc;

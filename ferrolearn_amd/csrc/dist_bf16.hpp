@@ -146,8 +146,10 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t bf_rsrc(const void *base, uint
 // LDS layout of one staged tile: [part (hi, lo)][row 0..63][16-byte chunk],
 // rows of NKC*64 bytes with the chunk order XOR-swizzled by the row
 // (position = chunk ^ (row & SWZ)), so that the 16 rows read by one MFMA
-// fragment load fall in distinct banks.  Two tiles are resident: one being
-// multiplied while the next one is copied in by direct-to-LDS buffer loads.
+// fragment load fall in distinct banks.  NST tiles are resident: one being
+// multiplied while the next NST-1 are copied in by direct-to-LDS buffer
+// loads.  Each wave also copies the tile's 64 streamed-row terms (xn') into
+// a slot of its own.
 template <int NKC, int PASSES>
 struct Bf16Tile {
     static constexpr int CH = NKC * 4;                    // 16-byte chunks per row
@@ -157,7 +159,10 @@ struct Bf16Tile {
     static constexpr int BYTES = PARTS * BF_T * ROWB;
     static constexpr int NBLK = BYTES / 1024;             // 1 KiB copy blocks per tile
     static constexpr int BPW = (NBLK + BF_WAVES - 1) / BF_WAVES;
-    static constexpr size_t SHMEM = 2 * (size_t)BYTES + 2 * BF_T * sizeof(float);
+    static constexpr int XS = BF_WAVES * BF_T * 4;        // xn' slots per tile
+    static constexpr int NST = 3 * (BYTES + XS) + 1024 <= 160 * 1024 ? 3 : 2;
+    static constexpr int OPS = BPW + 1;                   // copies per wave per tile
+    static constexpr size_t SHMEM = (size_t)NST * (BYTES + XS) + (NBLK % BF_WAVES ? 1024 : 0);
 };
 
 __device__ __forceinline__ bool pair_less_f(float ka, int32_t ia, float kb, int32_t ib) {
@@ -167,19 +172,56 @@ __device__ __forceinline__ bool pair_less_f(float ka, int32_t ia, float kb, int3
 // Sorted insert of (k, i) into a register list, dropping the last entry.
 // A lane sees its pairs in increasing index order, so an equal key goes
 // after the entries already there and the list stays sorted by (key, index).
+// Every position is computed from the old list (no dependency chain).
 template <int L>
 __device__ __forceinline__ void plist_insert(float (&lk)[L], int32_t (&li)[L], float k,
                                              int32_t i) {
+    bool lt[L];
 #pragma unroll
-    for (int j = 0; j < L; ++j) {
-        const bool lt = k < lk[j];
-        const float tk = lt ? lk[j] : k;
-        const int32_t ti = lt ? li[j] : i;
-        lk[j] = lt ? k : lk[j];
-        li[j] = lt ? i : li[j];
-        k = tk;
-        i = ti;
+    for (int j = 0; j < L; ++j) lt[j] = k < lk[j];
+#pragma unroll
+    for (int j = L - 1; j > 0; --j) {
+        const float sk = lt[j - 1] ? lk[j - 1] : k;
+        const int32_t si = lt[j - 1] ? li[j - 1] : i;
+        lk[j] = lt[j] ? sk : lk[j];
+        li[j] = lt[j] ? si : li[j];
     }
+    lk[0] = lt[0] ? k : lk[0];
+    li[0] = lt[0] ? i : li[0];
+}
+
+// Block barrier that orders LDS accesses only.  Copies into LDS that are
+// still in flight (vmcnt) are waited for explicitly where needed, so that
+// the copies of later tiles can stay in flight across it.
+__device__ __forceinline__ void block_sync_lds() {
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// Max / min over lanes l and l^16, then l and l^32, with the row swaps of
+// v_permlane16_swap / v_permlane32_swap (no LDS round trip).  With both
+// operands x, the two results are x with its rows exchanged pairwise; the
+// second operand is passed through an opaque copy so that the compiler does
+// not treat the two results as equal.
+__device__ __forceinline__ unsigned opaque_copy(unsigned x) {
+    unsigned y;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(y) : "v"(x));
+    return y;
+}
+template <bool MAX, int ROWS>
+__device__ __forceinline__ float xred(float x) {
+    const unsigned u = __builtin_bit_cast(unsigned, x);
+    unsigned a, b;
+    if (ROWS == 16) {
+        const auto p = __builtin_amdgcn_permlane16_swap(u, opaque_copy(u), false, false);
+        a = p[0];
+        b = p[1];
+    } else {
+        const auto p = __builtin_amdgcn_permlane32_swap(u, opaque_copy(u), false, false);
+        a = p[0];
+        b = p[1];
+    }
+    const float fa = __builtin_bit_cast(float, a), fb = __builtin_bit_cast(float, b);
+    return MAX ? fmaxf(fa, fb) : fminf(fa, fb);
 }
 
 // Filter threshold shared by the four lists of one resident row (lanes l,
@@ -190,13 +232,8 @@ __device__ __forceinline__ void plist_insert(float (&lk)[L], int32_t (&li)[L], f
 // only decrease.
 template <int MP, int L>
 __device__ __forceinline__ float shared_threshold(const float (&lk)[L]) {
-    float a = lk[MP / 4 - 1];
-    a = fmaxf(a, __shfl_xor(a, 16));
-    a = fmaxf(a, __shfl_xor(a, 32));
-    float b = lk[L - 1];
-    b = fminf(b, __shfl_xor(b, 16));
-    b = fminf(b, __shfl_xor(b, 32));
-    return fminf(a, b);
+    return fminf(xred<true, 32>(xred<true, 16>(lk[MP / 4 - 1])),
+                 xred<false, 32>(xred<false, 16>(lk[L - 1])));
 }
 
 // One block = 8 waves x 32 resident rows, all sharing each staged tile of
@@ -224,7 +261,7 @@ __global__ __launch_bounds__(512) void dist_bf16_kernel(Bf16Args args) {
     typedef Bf16Tile<NKC, PASSES> TL;
     constexpr int L = MP / 2;
     constexpr int DPAD = NKC * 32;
-    float *xs = reinterpret_cast<float *>(smem_raw + 2 * TL::BYTES);   // [2][BF_T]
+    float *xs = reinterpret_cast<float *>(smem_raw + TL::NST * TL::BYTES);   // [NST][waves][BF_T]
 
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave-uniform
@@ -271,54 +308,67 @@ __global__ __launch_bounds__(512) void dist_bf16_kernel(Bf16Args args) {
 #pragma unroll
     for (int jt = 0; jt < 2; ++jt) th[jt] = qok[jt] ? __builtin_huge_valf() : -__builtin_huge_valf();
 
-    // Copy plan: 1 KiB block b of a tile is written by wave b % 8; lane l of
+    // Copy plan: 1 KiB block b of a tile is written by wave b % 8 (every
+    // wave issues BPW block copies; a wave with no block of its own when the
+    // tile has fewer than 8 blocks copies block b % NBLK into a scratch KiB,
+    // so that every wave has the same count of copies in flight); lane l of
     // the block fills LDS slot 64*b + l, i.e. part, row and swizzled position
-    // (row, pos), with chunk pos ^ (row & SWZ) of that streamed row.
+    // (row, pos), with chunk pos ^ (row & SWZ) of that streamed row.  Each
+    // wave then copies the tile's 64 xn' terms into its own slot.
     const uint32_t split_bytes = (uint32_t)((bend - bbeg) * TL::ROWB);
     const __amdgpu_buffer_rsrc_t rh = bf_rsrc(args.bh + bbeg * DPAD, split_bytes);
     const __amdgpu_buffer_rsrc_t rl = bf_rsrc(args.bl + bbeg * DPAD, split_bytes);
+    const __amdgpu_buffer_rsrc_t rx = bf_rsrc(args.xnp + bbeg, (uint32_t)((bend - bbeg) * 4));
     uint32_t voff[TL::BPW];
 #pragma unroll
     for (int j = 0; j < TL::BPW; ++j) {
-        const int s = 64 * (j * BF_WAVES + w) + lane;
+        const int s = 64 * ((j * BF_WAVES + w) % TL::NBLK) + lane;
         const int rem = s % (BF_T * TL::CH);
         const int row = rem / TL::CH, pos = rem % TL::CH;
         voff[j] = (uint32_t)(row * TL::ROWB + ((pos ^ (row & TL::SWZ)) * 16));
     }
-    float pf_xn = 0.f;
-    auto issue = [&](int buf, int64_t t0) {
+    auto issue_data = [&](int buf, int64_t t0) {
         const uint32_t soff = (uint32_t)((t0 - bbeg) * TL::ROWB);
 #pragma unroll
         for (int j = 0; j < TL::BPW; ++j) {
             const int b = j * BF_WAVES + w;   // a block never straddles the hi/lo parts
-            if (TL::NBLK % BF_WAVES == 0 || b < TL::NBLK) {
-                __attribute__((address_space(3))) void *dst =
-                    (__attribute__((address_space(3))) void *)(smem_raw + buf * TL::BYTES +
-                                                                1024 * b);
-                __builtin_amdgcn_raw_ptr_buffer_load_lds(b >= TL::NBLK / TL::PARTS ? rl : rh, dst,
-                                                         16, voff[j], soff, 0, 0);
-            }
-        }
-        if (w == 0) {
-            const int64_t r = t0 + lane;
-            pf_xn = r < bend ? args.xnp[r] : __builtin_huge_valf();
+            const int bs = b % TL::NBLK;
+            unsigned char *dst = b < TL::NBLK ? smem_raw + buf * TL::BYTES + 1024 * b
+                                              : smem_raw + TL::NST * (TL::BYTES + TL::XS);
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                bs >= TL::NBLK / TL::PARTS ? rl : rh, (__attribute__((address_space(3))) void *)dst,
+                16, voff[j], soff, 0, 0);
         }
     };
+    auto issue_xn = [&](int buf, int64_t t0) {
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(
+            rx, (__attribute__((address_space(3))) void *)(xs + (buf * BF_WAVES + w) * BF_T), 4,
+            (uint32_t)lane * 4, (uint32_t)((t0 - bbeg) * 4), 0, 0);
+    };
 
-    int cur = 0;
-    if (bbeg < bend) {
-        issue(0, bbeg);
-        if (w == 0) xs[lane] = pf_xn;
-    }
-    wait_vm_loads();
-    __syncthreads();
+    // Tile t lives in buffer t % NST and is issued NST-1 tiles ahead.  The
+    // epilogue is software-pipelined: while the products of tile t are
+    // issued, the keys of tile t-1 (kept in the other accumulator set) are
+    // filtered, so that the compares overlap the matrix cores; the inserts of
+    // tile t-1 follow.  At the end of tile t each wave waits for its own
+    // copies of tile t+1 (those of tile t+2 may stay in flight), then the
+    // block synchronises.
+    const int64_t ntiles = (bend - bbeg + BF_T - 1) / BF_T;
+#pragma unroll
+    for (int t = 0; t < TL::NST - 1; ++t)
+        if (t < ntiles) {
+            issue_data(t, bbeg + (int64_t)t * BF_T);
+            issue_xn(t, bbeg + (int64_t)t * BF_T);
+        }
+    if (TL::NST == 3 && ntiles > 1)
+        __builtin_amdgcn_s_waitcnt(0x0F70 | (TL::OPS & 15) | ((TL::OPS >> 4) << 14));
+    else
+        wait_vm_loads();
+    block_sync_lds();
 
-    for (int64_t b0 = bbeg; b0 < bend; b0 += BF_T, cur ^= 1) {
-        const bool more = b0 + BF_T < bend;
-        if (more) issue(cur ^ 1, b0 + BF_T);
-        const unsigned char *tb = smem_raw + cur * TL::BYTES;
-
-        float4_t acc[4][2];
+    // Products of one tile into acc.
+    auto products = [&](float4_t (&acc)[4][2], int buf) {
+        const unsigned char *tb = smem_raw + buf * TL::BYTES;
 #pragma unroll
         for (int it = 0; it < 4; ++it)
 #pragma unroll
@@ -358,45 +408,126 @@ __global__ __launch_bounds__(512) void dist_bf16_kernel(Bf16Args args) {
                             xl[it], qh[jt][kc], acc[it][jt], 0, 0, 0);
             }
         }
+    };
 
-        // ---- epilogue: one FMA and one compare per pair ----
-        // Streamed rows past the split end have xn' = +inf and zero data, so
-        // their key is +inf; resident rows past nA have T = -inf.
-        const float *xt = xs + cur * BF_T;
-        float key[4][4][2];
+    // Key of a pair: one FMA, with xn' read from this wave's slot of the
+    // tile's buffer.  Streamed rows past the split end (only in the last
+    // tile; copied as zeros) get +inf.  Resident rows past nA have T = -inf.
+    auto load_xr = [&](float4_t (&xr)[4], int b) {
+        const float *xt = xs + (b * BF_WAVES + w) * BF_T;
+#pragma unroll
+        for (int it = 0; it < 4; ++it)
+            xr[it] = *reinterpret_cast<const float4_t *>(&xt[16 * it + 4 * g]);
+    };
+    auto keyof = [&](const float4_t (&acc)[4][2], const float4_t (&xr)[4], int it, int v, int jt,
+                     int nvalid) {
+        const float k = __builtin_fmaf(-2.f, acc[it][jt][v], xr[it][v]);
+        return (nvalid < BF_T && 16 * it + 4 * g + v >= nvalid) ? __builtin_huge_valf() : k;
+    };
+    // Filter of one tile: gany[jt][it] marks a pair of that group below T.
+    auto filter = [&](const float4_t (&acc)[4][2], int b, int nvalid, bool (&gany)[2][4]) {
+        float4_t xr[4];
+        load_xr(xr, b);
         bool any = false;
 #pragma unroll
-        for (int it = 0; it < 4; ++it) {
-            const float4_t xr = *reinterpret_cast<const float4_t *>(&xt[16 * it + 4 * g]);
+        for (int jt = 0; jt < 2; ++jt)
 #pragma unroll
-            for (int v = 0; v < 4; ++v)
+            for (int it = 0; it < 4; ++it) {
+                bool a = false;
 #pragma unroll
-                for (int jt = 0; jt < 2; ++jt) {
-                    const float kv = __builtin_fmaf(-2.f, acc[it][jt][v], xr[v]);
-                    key[it][v][jt] = kv;
-                    any = any || kv < th[jt];
+                for (int v = 0; v < 4; ++v) a = a || keyof(acc, xr, it, v, jt, nvalid) < th[jt];
+                gany[jt][it] = a;
+                any = any || a;
+            }
+        return any;
+    };
+    // Inserts of one tile: a group of four pairs (one it, one jt) is entered
+    // only when some lane has a pair of it below T.
+    auto inserts = [&](const float4_t (&acc)[4][2], int b, int nvalid, const bool (&gany)[2][4],
+                       int64_t b0) {
+        float4_t xr[4];
+        load_xr(xr, b);
+        const int32_t rb = (int32_t)b0 + 4 * g;
+#pragma unroll
+        for (int jt = 0; jt < 2; ++jt) {
+            if (__ballot(gany[jt][0] || gany[jt][1] || gany[jt][2] || gany[jt][3]) == 0) continue;
+#pragma unroll
+            for (int it = 0; it < 4; ++it) {
+                if (!gany[jt][it]) continue;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const float kv = keyof(acc, xr, it, v, jt, nvalid);
+                    if (kv < th[jt] && kv < lk[jt][L - 1])
+                        plist_insert<L>(lk[jt], li[jt], kv, rb + 16 * it + v);
                 }
+            }
+            th[jt] = qok[jt] ? shared_threshold<MP, L>(lk[jt]) : -__builtin_huge_valf();
         }
-        if (__ballot(any) != 0) {
-            const int32_t rb = (int32_t)b0 + 4 * g;
-#pragma unroll
-            for (int jt = 0; jt < 2; ++jt)
-#pragma unroll
-                for (int it = 0; it < 4; ++it)
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        const float kv = key[it][v][jt];
-                        if (kv < th[jt] && kv < lk[jt][L - 1])
-                            plist_insert<L>(lk[jt], li[jt], kv, rb + 16 * it + v);
-                    }
-#pragma unroll
-            for (int jt = 0; jt < 2; ++jt)
-                th[jt] = qok[jt] ? shared_threshold<MP, L>(lk[jt]) : -__builtin_huge_valf();
-        }
+    };
 
-        if (more && w == 0) xs[(cur ^ 1) * BF_T + lane] = pf_xn;
-        wait_vm_loads();   // this wave's copies of the next tile have landed
-        __syncthreads();
+    float4_t accA[4][2], accB[4][2];
+    // One step: products of tile t into cur_acc while tile t-1 (prev_acc,
+    // buffer pbuf) is filtered, then the inserts of tile t-1.  The tile data
+    // of tile t+NST-1 is copied into the buffer of tile t-1 (read by no one
+    // since the last block barrier), its xn' terms only after this wave's
+    // own reads of that slot.
+    auto step = [&](int64_t t, int buf, float4_t (&cur_acc)[4][2],
+                    const float4_t (&prev_acc)[4][2]) {
+        const int64_t b0 = bbeg + t * BF_T;
+        const int pbuf = buf == 0 ? TL::NST - 1 : buf - 1;
+        const bool more = t + TL::NST - 1 < ntiles;
+        const int64_t tn = b0 + (int64_t)(TL::NST - 1) * BF_T;
+        if (more) issue_data(TL::NST == 3 ? pbuf : (buf ^ 1), tn);
+        // Tile t-1 is never the last tile here.  At t = 0 there is no tile
+        // t-1: the filter runs on unset sums and its result is discarded (no
+        // branch, so that filter and products form one scheduling region).
+        bool gany[2][4];
+        const bool any = filter(prev_acc, pbuf, BF_T, gany) && t > 0;
+        products(cur_acc, buf);
+        // Interleave the filter's compares with the matrix products: three
+        // VALU instructions after every three MFMAs.
+#pragma unroll
+        for (int i = 0; i < 32; ++i) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 3, 0);
+            __builtin_amdgcn_sched_group_barrier(0x002, 3, 0);
+        }
+        if (__ballot(any) != 0) inserts(prev_acc, pbuf, BF_T, gany, b0 - BF_T);
+        if (more) issue_xn(TL::NST == 3 ? pbuf : (buf ^ 1), tn);
+        if (t + 1 < ntiles) {
+            if (TL::NST == 3 && t + 2 < ntiles)
+                __builtin_amdgcn_s_waitcnt(0x0F70 | (TL::OPS & 15) | ((TL::OPS >> 4) << 14));
+            else
+                wait_vm_loads();
+        }
+        block_sync_lds();
+    };
+
+    int buf = 0;
+    int64_t t = 0;
+    for (; t + 1 < ntiles; t += 2) {
+        step(t, buf, accA, accB);
+        buf = buf + 1 == TL::NST ? 0 : buf + 1;
+        step(t + 1, buf, accB, accA);
+        buf = buf + 1 == TL::NST ? 0 : buf + 1;
+    }
+    if (t < ntiles) {
+        step(t, buf, accA, accB);
+        buf = buf + 1 == TL::NST ? 0 : buf + 1;
+        t += 1;
+    }
+    // The last tile's sums are in accA when ntiles is odd, else in accB.
+    if (ntiles > 0) {
+        const int64_t bl0 = bbeg + (ntiles - 1) * BF_T;
+        const int nvalid = (int)(bend - bl0);
+        const int lbuf = buf == 0 ? TL::NST - 1 : buf - 1;   // buffer of the last tile
+        bool gany[2][4];
+        if (ntiles & 1) {
+            if (__ballot(filter(accA, lbuf, nvalid, gany)) != 0)
+                inserts(accA, lbuf, nvalid, gany, bl0);
+        } else {
+            if (__ballot(filter(accB, lbuf, nvalid, gany)) != 0)
+                inserts(accB, lbuf, nvalid, gany, bl0);
+        }
     }
 
     // ---- output: the MP smallest pairs of the union of the four lists ----

@@ -392,8 +392,10 @@ __global__ void to_f64_kernel(const T *__restrict__ X, int64_t count, double *__
 }
 
 // ------------------------------------------------------------ kNN finalize --
-// One wave per query: gather the S*M candidates, fold each exactly, order by
-// (sqrt(fold), index) -- the reference distance (kdtree.rs:342-348,
+// One wave per query: gather the S*M candidates, fold them exactly in
+// increasing key order until the smallest key left exceeds the k-th fold
+// (keys are lower bounds of the folds, so no candidate left can enter), order
+// by (sqrt(fold), index) -- the reference distance (kdtree.rs:342-348,
 // :386-388) with index as the tie rule -- take the first k, and certify.
 //
 // Certificate: every pair not kept has key >= theta_s for its split s, and
@@ -411,10 +413,13 @@ __global__ __launch_bounds__(64) void knn_finalize_kernel(
     int S, int M, int64_t b_per_split, Cand cand, int k, int certify,
     double *__restrict__ dist_out, int64_t *__restrict__ idx_out, int32_t *__restrict__ flag_list,
     int32_t *__restrict__ flag_count, unsigned long long *__restrict__ n_cand) {
+    __shared__ double ck[FIN_MAXC];
     __shared__ double cf[FIN_MAXC];
     __shared__ double cd[FIN_MAXC];
     __shared__ int32_t ci[FIN_MAXC];
+    __shared__ int32_t rk[FIN_MAXC];
     __shared__ double kth_fold;
+    __shared__ double next_key;
     __shared__ int kth_found;
     __shared__ int has_nan;
     const int64_t q = blockIdx.x;
@@ -436,38 +441,77 @@ __global__ __launch_bounds__(64) void knn_finalize_kernel(
     for (int base = 0; base < total; base += 64) {
         const int e = base + lane;
         int32_t ix = -1;
+        double kv = 0.0;
         if (e < total) {
             const int s = e / M, m = e % M;
-            ix = cand.idx[((int64_t)s * nq + q) * M + m];
+            const int64_t o = ((int64_t)s * nq + q) * M + m;
+            ix = cand.idx[o];
+            kv = cand.key[o];
         }
         const bool v = ix >= 0;
         const uint64_t mask = __ballot(v);
-        if (v) ci[nc + __popcll(mask & ((1ull << lane) - 1ull))] = ix;
+        if (v) {
+            const int at = nc + __popcll(mask & ((1ull << lane) - 1ull));
+            ci[at] = ix;
+            ck[at] = kv;
+        }
         nc += __popcll(mask);
     }
-    __syncthreads();
-    const double *qr = Q + q * d;
     if (lane == 0) {
         kth_found = 0;
         has_nan = 0;
+        next_key = __builtin_huge_val();
     }
     __syncthreads();
+
+    // Filter keys are lower bounds of the exact folds, so the candidates are
+    // re-scored in increasing key order: first the k+6 smallest; the rest
+    // only when the smallest key left could still reach the k-th fold.  (The
+    // exact form's keys are distances: everything is re-scored.)
     for (int e = lane; e < nc; e += 64) {
-        const double f = fold_exact(qr, B + (int64_t)ci[e] * d, d);
-        cf[e] = f;
-        cd[e] = sqrt(f);
-        if (f != f) has_nan = 1;
+        int r = 0;
+        if (certify)
+            for (int e2 = 0; e2 < nc; ++e2) r += pair_less(ck[e2], ci[e2], ck[e], ci[e]) ? 1 : 0;
+        rk[e] = r;
     }
     __syncthreads();
-    for (int e = lane; e < nc; e += 64) {
-        int rank = 0;
-        for (int e2 = 0; e2 < nc; ++e2) rank += pair_less(cd[e2], ci[e2], cd[e], ci[e]) ? 1 : 0;
-        if (rank == k - 1) {
-            kth_fold = cf[e];
-            kth_found = 1;
+    const double *qr = Q + q * d;
+    auto rescore = [&](int lo, int hi) {
+        for (int e = lane; e < nc; e += 64) {
+            if (rk[e] < lo || rk[e] >= hi) continue;
+            const double f = fold_exact(qr, B + (int64_t)ci[e] * d, d);
+            cf[e] = f;
+            cd[e] = sqrt(f);
+            if (f != f) has_nan = 1;
         }
+        __syncthreads();
+    };
+    // k-th of the re-scored set (rank < R) by (distance, index).
+    auto kth = [&](int R) {
+        for (int e = lane; e < nc; e += 64) {
+            if (rk[e] >= R) continue;
+            int rank = 0;
+            for (int e2 = 0; e2 < nc; ++e2)
+                rank += (rk[e2] < R && pair_less(cd[e2], ci[e2], cd[e], ci[e])) ? 1 : 0;
+            if (rank == k - 1) {
+                kth_fold = cf[e];
+                kth_found = 1;
+            }
+        }
+        for (int e = lane; e < nc; e += 64)
+            if (rk[e] == R) next_key = ck[e];
+        __syncthreads();
+    };
+    int R = certify ? (nc < k + 6 ? nc : k + 6) : nc;
+    rescore(0, R);
+    kth(R);
+    if (R < nc && !(kth_found && !has_nan && next_key > kth_fold + kth_fold * 0x1.0p-48)) {
+        rescore(R, nc);
+        if (lane == 0) kth_found = 0;
+        __syncthreads();
+        R = nc;
+        kth(R);
     }
-    __syncthreads();
     bool ok = complete && (nc >= k) && kth_found && !has_nan;
     if (ok && certify && thmin != __builtin_huge_val()) ok = thmin > kth_fold + kth_fold * 0x1.0p-48;
     if (!ok) {
@@ -475,14 +519,16 @@ __global__ __launch_bounds__(64) void knn_finalize_kernel(
         return;
     }
     for (int e = lane; e < nc; e += 64) {
+        if (rk[e] >= R) continue;
         int rank = 0;
-        for (int e2 = 0; e2 < nc; ++e2) rank += pair_less(cd[e2], ci[e2], cd[e], ci[e]) ? 1 : 0;
+        for (int e2 = 0; e2 < nc; ++e2)
+            rank += (rk[e2] < R && pair_less(cd[e2], ci[e2], cd[e], ci[e])) ? 1 : 0;
         if (rank < k) {
             dist_out[q * k + rank] = cd[e];
             idx_out[q * k + rank] = ci[e];
         }
     }
-    if (lane == 0) atomicAdd(n_cand, (unsigned long long)nc);
+    if (lane == 0) atomicAdd(n_cand, (unsigned long long)R);
 }
 
 // -------------------------------------------------------- kNN epilogues --

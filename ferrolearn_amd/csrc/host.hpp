@@ -228,7 +228,7 @@ inline int device_cu_count() {
 
 // Number of splits of the streamed rows for the BF16 filter.  A block holds
 // one compute unit (8 waves); time ~ rounds of blocks x tiles per split x
-// (1 + 0.3 x the share of tiles whose epilogue inserts), where that share
+// (1 + 0.5 x the share of tiles whose epilogue inserts), where that share
 // follows the list-filling transient: about 512 (1 + ln(T / 512)) of the T
 // tiles of a split.
 inline int64_t bf16_splits(int64_t nblk, int64_t nB, int64_t smax) {
@@ -239,7 +239,7 @@ inline int64_t bf16_splits(int64_t nblk, int64_t nB, int64_t smax) {
         const double tiles = std::ceil((double)nB / (double)S / 64.0);
         const double rounds = std::ceil((double)(nblk * S) / cus);
         const double slow = tiles <= 512.0 ? 1.0 : 512.0 * (1.0 + std::log(tiles / 512.0)) / tiles;
-        const double cost = rounds * tiles * (1.0 + 0.3 * slow);
+        const double cost = rounds * tiles * (1.0 + 0.5 * slow);
         if (S == 1 || cost < best_cost) {
             best = S;
             best_cost = cost;
