@@ -1,6 +1,6 @@
 """Benchmark of the gfx950 squared-Euclidean path.
 
-Default workload (one GPU): brute-force KNeighborsClassifier predict.
+Headline workload (one GPU): brute-force KNeighborsClassifier predict.
 Database 1,000,000 x 128 float32 from a mixture of 64 Gaussians (means
 U[-5,5], per-component std U[0.5,2]), class = component mod 10; 100,000
 queries from the same mixture; k = 10, uniform weights.  A step is one
@@ -10,7 +10,13 @@ beside it; both must give the same neighbour indices and labels, and a
 sample of queries is checked against the CPU oracle, which is also timed as
 the CPU baseline on the host's cores.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload knn|kmeans]
+Second workload, reported in the same line under "kmeans": KMeans Lloyd on
+10,000,000 x 64 float32 samples from 1,024 Gaussian blobs (means U[-20,20],
+std U[0.5,3]), k = 1,024, k-means++ initial centres (the library's, seed 0),
+50 iterations (tol 0), samples resident in HBM; samples*iterations/s of the
+Lloyd phase for the BF16x3 and plain f64 forms, which must agree bit for bit.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload all|knn|kmeans|copy]
 """
 from __future__ import annotations
 
@@ -160,6 +166,107 @@ def run_knn(args):
     return line
 
 
+def kmeans_data(n=10_000_000, d=64, comps=1024, seed=0):
+    rng = np.random.default_rng(seed)
+    means = rng.uniform(-20, 20, size=(comps, d)).astype(np.float32)
+    sd = rng.uniform(0.5, 3.0, size=comps).astype(np.float32)
+    c = rng.integers(0, comps, size=n)
+    X = rng.standard_normal((n, d), dtype=np.float32)
+    X *= sd[c, None]
+    X += means[c]
+    return X
+
+
+def cpu_baseline_kmeans(X, init, gpu_labels_first):
+    """One Lloyd iteration of the oracle on a bounded sample of rows against
+    the same initial centres (about 10 s of CPU work); also checks the
+    sample's first-iteration labels against the GPU's."""
+    from oracle import oracle as O
+    threads = O.max_threads()
+    m = 2000
+    t0 = time.perf_counter()
+    O.assign(X[:m].astype(np.float64), init, nthreads=threads)
+    rate = m / (time.perf_counter() - t0)
+    m = int(min(len(X), max(m, rate * 10.0)))
+    Xs = X[:m].astype(np.float64)
+    t0 = time.perf_counter()
+    lab, _ = O.assign(Xs, init, nthreads=threads)
+    dt = time.perf_counter() - t0
+    ok = bool((lab[:len(gpu_labels_first)] == gpu_labels_first[:m]).all())
+    return {"value": m / dt, "unit": "samples*iter/s", "cores": threads, "kind": "port",
+            "sample": f"assignment step (the Lloyd iteration's dominant part) of the first {m} "
+                      f"of {len(X)} samples against the 1,024 initial centres, oracle/ref_cpu.c "
+                      f"with {threads} OpenMP threads, {dt:.1f} s"}, ok
+
+
+def run_kmeans(args):
+    import ferrolearn_amd as F
+    k, iters = 1024, args.km_iters
+    log("kmeans: generating data ...")
+    X = kmeans_data(n=args.km_rows)
+    n, d = X.shape
+    t0 = time.perf_counter()
+    init = F.kmeans_plusplus(X, k, 0)
+    init_s = time.perf_counter() - t0
+    log(f"kmeans: k-means++ {init_s:.2f} s")
+    xd = F.to_device(X)
+    forms = {}
+    for form in args.km_forms:
+        km = F.KMeans(k).with_n_init(1).with_tol(0.0).with_precision(form)
+        km.with_max_iter(2).fit(xd, init_centers=init)          # warm-up
+        F.device.synchronize()
+        t0 = time.perf_counter()
+        f = km.with_max_iter(iters).fit(xd, init_centers=init)
+        F.device.synchronize()
+        dt = time.perf_counter() - t0
+        r = f.report
+        forms[form] = {
+            "samples_iter_per_s": n * f.n_iter() / dt, "s": dt, "n_iter": f.n_iter(),
+            "ms_per_iter": dt * 1e3 / max(1, f.n_iter()),
+            "ms_distance_kernel": r["ms_distance"] / max(1, r["n_dist_launches"]),
+            "ms_finalize_per_iter": r["ms_finalize"] / max(1, f.n_iter()),
+            "ms_update_per_iter": r["ms_update"] / max(1, f.n_iter()),
+            "fallback_rows": int(r["n_fallback"]),
+            "labels": f.labels().numpy(), "centers": f.cluster_centers().numpy(),
+            "inertia": f.inertia(),
+        }
+        log(f"kmeans {form}: {forms[form]['samples_iter_per_s']:.3g} samples*iter/s, "
+            f"{forms[form]['ms_per_iter']:.2f} ms/iter")
+    head = args.km_forms[0]
+    h = forms[head]
+    out = {
+        "workload": f"KMeans Lloyd, {n} x {d} float32, 1024 blobs, k={k}, {iters} iterations",
+        "metric": "samples*iter/s", "value": h["samples_iter_per_s"], "form": head,
+        "init_kmeanspp_s": init_s,
+        "forms": {f: {kk: vv for kk, vv in v.items() if kk not in ("labels", "centers")}
+                  for f, v in forms.items()},
+    }
+    if "fp64" in forms and head != "fp64":
+        ref = forms["fp64"]
+        out["speedup_vs_fp64_form"] = h["samples_iter_per_s"] / ref["samples_iter_per_s"]
+        out["exact_vs_fp64_form"] = bool(
+            (h["labels"] == ref["labels"]).all() and h["n_iter"] == ref["n_iter"]
+            and (h["centers"].view(np.int64) == ref["centers"].view(np.int64)).all()
+            and h["inertia"] == ref["inertia"])
+    flops = 2.0 * n * k * d
+    peak_key = "f64" if head in ("fp64", "exact") else "bf16"
+    out["roofline"] = {"bound": "mfma", "achieved": flops / (h["ms_distance_kernel"] * 1e-3) / 1e12,
+                       "peak": PEAK[peak_key] / 1e12, "unit": "TFLOP/s",
+                       "frac": flops / (h["ms_distance_kernel"] * 1e-3) / PEAK[peak_key],
+                       "kernel": "dist_bf16_kernel<0,...> (argmin form)" if peak_key == "bf16"
+                       else "dist_topm_kernel", "kernel_ms": h["ms_distance_kernel"],
+                       "algorithmic_flops_per_launch": flops}
+    if not args.no_cpu:
+        log("kmeans: cpu baseline ...")
+        # labels of the first iteration: one GPU iteration from the same centres
+        f1 = F.KMeans(k).with_n_init(1).with_tol(0.0).with_max_iter(1).with_precision(head) \
+            .fit(xd, init_centers=init)
+        cb, ok = cpu_baseline_kmeans(X, init, f1.labels().numpy())
+        out["cpu_baseline"] = cb
+        out["exact_vs_oracle_sample"] = ok
+    return out
+
+
 def copy_calibration(mib=512):
     """A plain device-to-device copy of known size: its FETCH_SIZE / WRITE_SIZE
     counters give the scale that turns those counters into HBM bytes."""
@@ -180,7 +287,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="knn", choices=["knn", "copy"])
+    ap.add_argument("--workload", default="all", choices=["all", "knn", "kmeans", "copy"])
+    ap.add_argument("--km-forms", default="bf16x3,fp64")
+    ap.add_argument("--km-rows", type=int, default=10_000_000)
+    ap.add_argument("--km-iters", type=int, default=50)
     ap.add_argument("--forms", default="bf16x3,fp64",
                     help="comma list; the first is reported as value (fp64, exact, bf16, bf16x3)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -188,6 +298,7 @@ def main():
                     help="query count (100,000 is the benchmark; fewer only for profiling runs)")
     args = ap.parse_args()
     args.forms = [f for f in args.forms.split(",") if f]
+    args.km_forms = [f for f in args.km_forms.split(",") if f]
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(--gpus must be 1)", file=sys.stderr)
@@ -195,7 +306,12 @@ def main():
     if args.workload == "copy":
         copy_calibration()
         return
+    if args.workload == "kmeans":
+        print(json.dumps(run_kmeans(args)), flush=True)
+        return
     line = run_knn(args)
+    if args.workload == "all":
+        line["kmeans"] = run_kmeans(args)
     print(json.dumps(line), flush=True)
 
 

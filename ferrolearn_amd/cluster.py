@@ -72,7 +72,8 @@ def _dev_from_host(a, like):
 class KMeans:
     """kmeans.rs:55-113.  Defaults from KMeans::new: max_iter 300, tol 1e-4,
     n_init 10, random_state None.  `precision` picks the candidate filter
-    ('fp64' GEMM form, 'exact' direct folds); labels are exact either way."""
+    ('fp64' GEMM form, 'bf16x3' / 'bf16' BF16 MFMA filters, 'exact' direct folds);
+    labels are exact either way."""
 
     def __init__(self, n_clusters: int):
         self.n_clusters = int(n_clusters)

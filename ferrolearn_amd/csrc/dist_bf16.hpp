@@ -232,7 +232,7 @@ __device__ __forceinline__ float xred(float x) {
 // only decrease.
 template <int MP, int L>
 __device__ __forceinline__ float shared_threshold(const float (&lk)[L]) {
-    return fminf(xred<true, 32>(xred<true, 16>(lk[MP / 4 - 1])),
+    return fminf(xred<true, 32>(xred<true, 16>(lk[MP >= 4 ? MP / 4 - 1 : 0])),
                  xred<false, 32>(xred<false, 16>(lk[L - 1])));
 }
 
@@ -259,7 +259,11 @@ __global__ __launch_bounds__(512) void dist_bf16_kernel(Bf16Args args) {
 #if defined(__HIP_DEVICE_COMPILE__)
     extern __shared__ __align__(16) unsigned char smem_raw[];
     typedef Bf16Tile<NKC, PASSES> TL;
-    constexpr int L = MP / 2;
+    // MP == 0: argmin form (KMeans assignment): each lane keeps its best pair
+    // per resident row and the smallest key it dropped; the four lanes of a
+    // row output their four bests and theta = the smallest dropped key.
+    constexpr bool ARGMIN = MP == 0;
+    constexpr int L = ARGMIN ? 1 : MP / 2;
     constexpr int DPAD = NKC * 32;
     float *xs = reinterpret_cast<float *>(smem_raw + TL::NST * TL::BYTES);   // [NST][waves][BF_T]
 
@@ -307,6 +311,9 @@ __global__ __launch_bounds__(512) void dist_bf16_kernel(Bf16Args args) {
     float th[2];
 #pragma unroll
     for (int jt = 0; jt < 2; ++jt) th[jt] = qok[jt] ? __builtin_huge_valf() : -__builtin_huge_valf();
+    float bk[2] = {__builtin_huge_valf(), __builtin_huge_valf()};   // argmin form
+    float sec[2] = {__builtin_huge_valf(), __builtin_huge_valf()};
+    int32_t bi[2] = {0x7fffffff, 0x7fffffff};
 
     // Copy plan: 1 KiB block b of a tile is written by wave b % 8 (every
     // wave issues BPW block copies; a wave with no block of its own when the
@@ -465,6 +472,28 @@ __global__ __launch_bounds__(512) void dist_bf16_kernel(Bf16Args args) {
         }
     };
 
+    // Argmin form: best pair and smallest dropped key per lane and row.  A
+    // lane sees its pairs in increasing index order, so strict < keeps the
+    // lowest index among equal keys.  nvalid = 0 updates nothing.
+    auto argmin_update = [&](const float4_t (&acc)[4][2], int b, int nvalid, int64_t b0) {
+        float4_t xr[4];
+        load_xr(xr, b);
+        const int32_t rb = (int32_t)b0 + 4 * g;
+#pragma unroll
+        for (int jt = 0; jt < 2; ++jt)
+#pragma unroll
+            for (int it = 0; it < 4; ++it)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    float kv = __builtin_fmaf(-2.f, acc[it][jt][v], xr[it][v]);
+                    kv = 16 * it + 4 * g + v < nvalid ? kv : __builtin_huge_valf();
+                    const bool lt = kv < bk[jt];
+                    sec[jt] = fminf(sec[jt], lt ? bk[jt] : kv);
+                    bk[jt] = lt ? kv : bk[jt];
+                    bi[jt] = lt ? rb + 16 * it + v : bi[jt];
+                }
+    };
+
     float4_t accA[4][2], accB[4][2];
     // One step: products of tile t into cur_acc while tile t-1 (prev_acc,
     // buffer pbuf) is filtered, then the inserts of tile t-1.  The tile data
@@ -481,17 +510,29 @@ __global__ __launch_bounds__(512) void dist_bf16_kernel(Bf16Args args) {
         // Tile t-1 is never the last tile here.  At t = 0 there is no tile
         // t-1: the filter runs on unset sums and its result is discarded (no
         // branch, so that filter and products form one scheduling region).
-        bool gany[2][4];
-        const bool any = filter(prev_acc, pbuf, BF_T, gany) && t > 0;
-        products(cur_acc, buf);
-        // Interleave the filter's compares with the matrix products: three
-        // VALU instructions after every three MFMAs.
+        if constexpr (ARGMIN) {
+            argmin_update(prev_acc, pbuf, t > 0 ? BF_T : 0, b0 - BF_T);
+            products(cur_acc, buf);
+            // Interleave the updates (about seven VALU per pair) with the
+            // matrix products.
 #pragma unroll
-        for (int i = 0; i < 32; ++i) {
-            __builtin_amdgcn_sched_group_barrier(0x008, 3, 0);
-            __builtin_amdgcn_sched_group_barrier(0x002, 3, 0);
+            for (int i = 0; i < 96; ++i) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);
+            }
+        } else {
+            bool gany[2][4];
+            const bool any = filter(prev_acc, pbuf, BF_T, gany) && t > 0;
+            products(cur_acc, buf);
+            // Interleave the filter's compares with the matrix products:
+            // three VALU instructions after every three MFMAs.
+#pragma unroll
+            for (int i = 0; i < 32; ++i) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 3, 0);
+                __builtin_amdgcn_sched_group_barrier(0x002, 3, 0);
+            }
+            if (__ballot(any) != 0) inserts(prev_acc, pbuf, BF_T, gany, b0 - BF_T);
         }
-        if (__ballot(any) != 0) inserts(prev_acc, pbuf, BF_T, gany, b0 - BF_T);
         if (more) issue_xn(TL::NST == 3 ? pbuf : (buf ^ 1), tn);
         if (t + 1 < ntiles) {
             if (TL::NST == 3 && t + 2 < ntiles)
@@ -520,16 +561,57 @@ __global__ __launch_bounds__(512) void dist_bf16_kernel(Bf16Args args) {
         const int64_t bl0 = bbeg + (ntiles - 1) * BF_T;
         const int nvalid = (int)(bend - bl0);
         const int lbuf = buf == 0 ? TL::NST - 1 : buf - 1;   // buffer of the last tile
-        bool gany[2][4];
-        if (ntiles & 1) {
-            if (__ballot(filter(accA, lbuf, nvalid, gany)) != 0)
-                inserts(accA, lbuf, nvalid, gany, bl0);
+        if constexpr (ARGMIN) {
+            if (ntiles & 1)
+                argmin_update(accA, lbuf, nvalid, bl0);
+            else
+                argmin_update(accB, lbuf, nvalid, bl0);
         } else {
-            if (__ballot(filter(accB, lbuf, nvalid, gany)) != 0)
-                inserts(accB, lbuf, nvalid, gany, bl0);
+            bool gany[2][4];
+            if (ntiles & 1) {
+                if (__ballot(filter(accA, lbuf, nvalid, gany)) != 0)
+                    inserts(accA, lbuf, nvalid, gany, bl0);
+            } else {
+                if (__ballot(filter(accB, lbuf, nvalid, gany)) != 0)
+                    inserts(accB, lbuf, nvalid, gany, bl0);
+            }
         }
     }
 
+    if constexpr (ARGMIN) {
+        // Four candidates per resident row (one per lane, valid ones first)
+        // and theta = the smallest key any lane dropped.
+#pragma unroll
+        for (int jt = 0; jt < 2; ++jt) {
+            const int64_t q = q0 + 16 * jt + c16;
+            const bool valid = bi[jt] != 0x7fffffff;
+            int pos = 0, cnt = 0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int vj = __shfl(valid ? 1 : 0, c16 + 16 * j);
+                pos += j < g ? vj : 0;
+                cnt += vj;
+            }
+            const float thk = xred<false, 32>(xred<false, 16>(sec[jt]));
+            if (q >= nA) continue;
+            const double qo = args.qoff[q];
+            if (valid) {
+                args.out.key[q * 4 + pos] = (double)bk[jt] + qo;
+                args.out.idx[q * 4 + pos] = bi[jt];
+            }
+            if (g == 0) {
+                args.out.cnt[q] = cnt;
+                args.out.theta_idx[q] = 0x7fffffff;
+                if (thk == __builtin_huge_valf()) {
+                    args.out.theta[q] = __builtin_huge_val();
+                } else {
+                    const double kf = (double)thk;
+                    args.out.theta[q] = (kf + qo) - (fabs(kf) + fabs(qo)) * 0x1.0p-22;
+                }
+            }
+        }
+        return;
+    }
     // ---- output: the MP smallest pairs of the union of the four lists ----
     float *okey = reinterpret_cast<float *>(smem_raw) + w * (16 * 4 * L * 2);   // [16][4L]
     int32_t *oidx = reinterpret_cast<int32_t *>(okey + 16 * 4 * L);

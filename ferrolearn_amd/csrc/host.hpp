@@ -134,7 +134,7 @@ template <int MP, int NKC, int PASSES>
 void launch_bf16_t(const Bf16Args &a, int S, hipStream_t st) {
     // Staged tiles during the stream; the output merge reuses the same LDS
     // (per wave 16 rows x 4 lists x MP/2 pairs of 8 bytes).
-    const size_t merge = (size_t)BF_WAVES * 16 * 4 * (MP / 2) * 8;
+    const size_t merge = MP == 0 ? 0 : (size_t)BF_WAVES * 16 * 4 * (MP / 2) * 8;
     const size_t shmem = std::max(Bf16Tile<NKC, PASSES>::SHMEM, merge);
     auto kern = dist_bf16_kernel<MP, NKC, PASSES>;
     FL_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -170,6 +170,7 @@ inline int bf16_nkc(int64_t d) {
 template <int PASSES>
 void launch_bf16_p(int MP, int nkc, const Bf16Args &a, int S, hipStream_t st) {
     switch (MP) {
+        case 0: return launch_bf16_m<0, PASSES>(nkc, a, S, st);   // argmin form (4 per row)
         case 8: return launch_bf16_m<8, PASSES>(nkc, a, S, st);
         case 16: return launch_bf16_m<16, PASSES>(nkc, a, S, st);
     }

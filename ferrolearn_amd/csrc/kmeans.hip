@@ -86,7 +86,33 @@ class Lloyd {
                                d, xn_, xs_);
             check_launch();
         }
+        // BF16 filter: samples are split into hi / lo bf16 once per fit.  Its
+        // bound needs finite values below 2^56 in magnitude (the centres,
+        // means of samples, then are too); other data uses the f64 filter.
+        if (!direct_ && (precision == FL_PREC_BF16 || precision == FL_PREC_BF16X3) &&
+            bf16_nkc(d) > 0 && k < ((int64_t)1 << 31) - 64) {
+            unsigned int *flag = ws_.get<unsigned int>("unsafe", 1);
+            FL_HIP(hipMemsetAsync(flag, 0, 4, st));
+            hipLaunchKernelGGL(bf16_unsafe_kernel<TX>, dim3(1024), dim3(256), 0, st, X, n * d,
+                               flag);
+            check_launch();
+            unsigned int hflag = 0;
+            FL_HIP(hipMemcpyAsync(&hflag, flag, 4, hipMemcpyDeviceToHost, st));
+            FL_HIP(hipStreamSynchronize(st));
+            if (hflag == 0) {
+                passes_ = precision == FL_PREC_BF16X3 ? 3 : 1;
+                nkc_ = bf16_nkc(d);
+                const int dpad = nkc_ * 32;
+                xh_ = ws_.get<unsigned short>("xh", n * dpad);
+                xl_ = ws_.get<unsigned short>("xl", n * dpad);
+                hipLaunchKernelGGL(bf16_split_kernel<TX>, dim3(grid1(n * dpad, 256)), dim3(256), 0,
+                                   st, X, n, d, dpad, xh_, xl_);
+                check_launch();
+            }
+        }
     }
+
+    bool bf16() const { return passes_ > 0; }
 
     // labels/best for every row against centres C (k*d f64, device).
     void assign(const double *C, int32_t *labels, double *best, KmAcc &acc) {
@@ -113,23 +139,59 @@ class Lloyd {
         cand.cnt = ws_.get<int32_t>("ccnt", n_);
         cand.theta = ws_.get<double>("cth", n_);
         cand.theta_idx = ws_.get<int32_t>("cthi", n_);
-        TopmArgs a;
-        a.nA = n_;
-        a.nB = k_;
-        a.d = d_;
-        a.b_per_split = k_;
-        a.an = xn_;
-        a.as = xs_;
-        a.bn = cn;
-        a.bs = cs;
-        const Bound bd = bound_fp64(d_);
-        a.alpha = bd.alpha;
-        a.beta = bd.beta;
-        a.sqrt_key = 0;
-        a.out = cand;
-        const int t0 = tm.begin();
-        launch_topm<TX, double>(kKmM, MODE_MFMA64, X_, C, a, 1, st_);
-        tm.end(t0);
+        int t0;
+        if (passes_ > 0) {
+            // Centre split, centre terms and the largest centre norm change
+            // every iteration; the samples' split was made once.
+            const int dpad = nkc_ * 32;
+            const Bound bd = bound_bf16(d_, passes_);
+            unsigned short *ch = ws_.get<unsigned short>("ch", k_ * dpad);
+            unsigned short *cl = ws_.get<unsigned short>("cl", k_ * dpad);
+            float *cxnp = ws_.get<float>("cxnp", k_);
+            unsigned long long *smax = ws_.get<unsigned long long>("smax", 1);
+            double *qoff = ws_.get<double>("qoff", n_);
+            hipLaunchKernelGGL(bf16_split_kernel<double>, dim3(grid1(k_ * dpad, 256)), dim3(256), 0,
+                               st_, C, k_, d_, dpad, ch, cl);
+            hipLaunchKernelGGL(bf16_stream_terms_kernel, dim3(grid1(k_, 256)), dim3(256), 0, st_,
+                               cn, k_, bd.beta, cxnp);
+            FL_HIP(hipMemsetAsync(smax, 0, 8, st_));
+            hipLaunchKernelGGL(max_nonneg_kernel, dim3(16), dim3(256), 0, st_, cs, k_, smax);
+            hipLaunchKernelGGL(bf16_resident_terms_kernel, dim3(grid1(n_, 256)), dim3(256), 0, st_,
+                               xn_, xs_, n_, d_, bd.alpha, bd.beta, (const double *)smax, qoff);
+            check_launch();
+            Bf16Args a;
+            a.nA = n_;
+            a.nB = k_;
+            a.b_per_split = k_;
+            a.ah = xh_;
+            a.al = xl_;
+            a.bh = ch;
+            a.bl = cl;
+            a.qoff = qoff;
+            a.xnp = cxnp;
+            a.out = cand;
+            t0 = tm.begin();
+            launch_bf16(passes_, 0, nkc_, a, 1, st_);   // argmin form: 4 candidates per row
+            tm.end(t0);
+        } else {
+            TopmArgs a;
+            a.nA = n_;
+            a.nB = k_;
+            a.d = d_;
+            a.b_per_split = k_;
+            a.an = xn_;
+            a.as = xs_;
+            a.bn = cn;
+            a.bs = cs;
+            const Bound bd = bound_fp64(d_);
+            a.alpha = bd.alpha;
+            a.beta = bd.beta;
+            a.sqrt_key = 0;
+            a.out = cand;
+            t0 = tm.begin();
+            launch_topm<TX, double>(kKmM, MODE_MFMA64, X_, C, a, 1, st_);
+            tm.end(t0);
+        }
         acc.n_launch += 1;
         int32_t *flags = ws_.get<int32_t>("flags", n_);
         int32_t *fcnt = ws_.get<int32_t>("fcnt", 1);
@@ -284,7 +346,9 @@ class Lloyd {
     int64_t k_;
     hipStream_t st_;
     bool direct_ = true;
+    int passes_ = 0, nkc_ = 0;   // BF16 filter (passes_ > 0)
     double *xn_ = nullptr, *xs_ = nullptr;
+    unsigned short *xh_ = nullptr, *xl_ = nullptr;
     Workspace ws_;
 };
 

@@ -64,6 +64,25 @@ def test_filter_path_ties_lowest_index(k):
     assert (got == want).all()
 
 
+@pytest.mark.parametrize("precision", ["bf16x3", "bf16"])
+def test_bf16_filter_ties_lowest_index(precision):
+    # Integer data and duplicated initial centres: exact ties everywhere; the
+    # argmin filter keeps the lowest index of equal keys per lane and the
+    # re-score decides on exact folds.
+    rng = np.random.default_rng(9)
+    k = 70
+    C = rng.integers(-3, 4, size=(k, 6)).astype(np.float64)
+    C[k - 1] = C[3]
+    C[k - 2] = C[0]
+    C[40] = C[5]
+    X = np.concatenate([rng.integers(-4, 5, size=(5000, 6)).astype(np.float64), C])
+    c_o, l_o, i_o, it_o = O.kmeans_lloyd(X, C, 3, 0.0)
+    f = fit_lib(X, C, 3, 0.0, precision)
+    assert f.n_iter() == it_o
+    assert (f.labels() == l_o).all()
+    assert (bits(f.cluster_centers()) == bits(c_o)).all()
+
+
 def test_sklearn_fixture_assign():
     fx = ref_fixture("kmeans.json")
     X = np.array(fx["input"]["X"])
@@ -80,6 +99,10 @@ def test_sklearn_fixture_assign():
     (40_000, 32, 64, np.float64, "fp64"),       # MFMA filter path
     (30_000, 64, 200, np.float32, "fp64"),      # f32 storage, MFMA filter path
     (20_000, 16, 48, np.float64, "exact"),
+    (40_000, 32, 64, np.float64, "bf16x3"),     # BF16 MFMA argmin filter
+    (30_000, 64, 200, np.float32, "bf16x3"),
+    (30_000, 64, 200, np.float32, "bf16"),      # single product: more fallbacks
+    (9_000, 128, 300, np.float32, "bf16x3"),
 ])
 def test_lloyd_matches_oracle(n, d, k, dtype, precision):
     X = blobs(n, d, k, seed=n + k, spread=1.0, dtype=dtype)
@@ -97,13 +120,14 @@ def test_lloyd_matches_oracle(n, d, k, dtype, precision):
     assert abs(math.fsum(dist) - f.inertia()) <= 1e-12 * f.inertia()
 
 
-def test_overlapping_blobs_many_iterations():
+@pytest.mark.parametrize("precision", ["fp64", "bf16x3"])
+def test_overlapping_blobs_many_iterations(precision):
     # Overlapping clusters move for many iterations: one label flip anywhere
     # would change later centres and n_iter.
     X = blobs(20_000, 8, 40, seed=3, spread=6.0, box=5.0)
     init = O.kmeanspp(X, 40, 11)
     c_o, l_o, i_o, it_o = O.kmeans_lloyd(X, init, 60, 0.0)
-    f = fit_lib(X, init, 60, 0.0)
+    f = fit_lib(X, init, 60, 0.0, precision)
     assert it_o == 60 and f.n_iter() == 60
     assert (f.labels() == l_o).all()
     assert (bits(f.cluster_centers()) == bits(c_o)).all()
