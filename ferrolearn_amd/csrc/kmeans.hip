@@ -210,7 +210,7 @@ class Lloyd {
         if (nflag > 0) {
             acc.n_fb += nflag;
             const int t2 = tm.begin();
-            hipLaunchKernelGGL(km_fallback_kernel<TX>, dim3(grid1(nflag, 256)), dim3(256), 0, st_,
+            hipLaunchKernelGGL(km_fallback_kernel<TX>, dim3(grid1(nflag, 4)), dim3(256), 0, st_,
                                X_, C, d_, k_, flags, fcnt, labels, best);
             check_launch();
             tm.end(t2);
@@ -245,7 +245,7 @@ class Lloyd {
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)ctr_b));
         hipLaunchKernelGGL(km_block_hist_kernel, dim3((unsigned)nblk), dim3(256), hist_b, st_,
                            labels, n_, k_, blockcnt);
-        hipLaunchKernelGGL(km_col_scan_kernel, dim3(grid1(k_, 256)), dim3(256), 0, st_, blockcnt,
+        hipLaunchKernelGGL(km_col_scan_kernel, dim3((unsigned)k_), dim3(256), 0, st_, blockcnt,
                            nblk, k_, blockbase, count);
         hipLaunchKernelGGL(km_offsets_kernel, dim3(1), dim3(64), 0, st_, count, k_, offset);
         hipLaunchKernelGGL(km_scatter_kernel, dim3((unsigned)nblk), dim3(64), ctr_b, st_, labels,

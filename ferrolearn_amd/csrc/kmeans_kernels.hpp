@@ -26,10 +26,19 @@ __global__ void km_finalize_kernel(const TX *__restrict__ X, const double *__res
     if (i >= n) return;
     const int c = cand.cnt[i];
     const TX *x = X + i * d;
+    // Fold the candidate with the smallest key first, then only candidates
+    // whose key (a lower bound of their fold) does not exceed the best fold.
+    int first = 0;
+    for (int m = 1; m < c; ++m)
+        if (pair_less(cand.key[i * M + m], cand.idx[i * M + m], cand.key[i * M + first],
+                      cand.idx[i * M + first]))
+            first = m;
     double bf = 0.0;
     int32_t bl = -1;
-    for (int m = 0; m < c; ++m) {
+    for (int r = 0; r < c; ++r) {
+        const int m = r == 0 ? first : (r <= first ? r - 1 : r);
         const int32_t ix = cand.idx[i * M + m];
+        if (bl >= 0 && !(cand.key[i * M + m] <= bf)) continue;
         const double f = fold_exact(x, C + (int64_t)ix * d, d);
         if (bl < 0 || pair_less(f, ix, bf, bl)) {
             bf = f;
@@ -80,20 +89,42 @@ __global__ void km_assign_exact_kernel(const TX *__restrict__ X, const double *_
     best[i] = v;
 }
 
-// Exact scan for the rows the certificate could not settle.
+// Exact scan for the rows the certificate could not settle: one wave per
+// row, lane l folds centres l, l+64, ...; the wave then takes the smallest
+// (fold, index), which is the result of kmeans.rs:295-309's sequential scan
+// (init (0, F::MAX), update iff fold < best): the lowest index among the
+// smallest folds below F::MAX, else (0, F::MAX).  Four rows per block.
 template <typename TX>
-__global__ void km_fallback_kernel(const TX *__restrict__ X, const double *__restrict__ C,
-                                   int d, int64_t k, const int32_t *__restrict__ list,
-                                   const int32_t *__restrict__ count,
-                                   int32_t *__restrict__ labels, double *__restrict__ best) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void km_fallback_kernel(
+    const TX *__restrict__ X, const double *__restrict__ C, int d, int64_t k,
+    const int32_t *__restrict__ list, const int32_t *__restrict__ count,
+    int32_t *__restrict__ labels, double *__restrict__ best) {
+    const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
     if (t >= *count) return;
     const int64_t i = list[t];
-    int32_t l;
-    double v;
-    km_exact_row(X + i * d, C, d, k, l, v);
-    labels[i] = l;
-    best[i] = v;
+    const TX *x = X + i * d;
+    double bd = 1.7976931348623157e308;
+    int64_t bl = -1;
+    for (int64_t c = lane; c < k; c += 64) {
+        const double f = fold_exact(x, C + c * d, d);
+        if (f < bd) {
+            bd = f;
+            bl = c;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const double od = __shfl_xor(bd, o);
+        const int64_t ol = __shfl_xor(bl, o);
+        if (ol >= 0 && (bl < 0 || od < bd || (od == bd && ol < bl))) {
+            bd = od;
+            bl = ol;
+        }
+    }
+    if (lane == 0) {
+        labels[i] = bl < 0 ? 0 : (int32_t)bl;
+        best[i] = bd;
+    }
 }
 
 // ---------------------------------------------------------------- inertia --
@@ -153,17 +184,33 @@ static __global__ __attribute__((unused)) __launch_bounds__(256) void km_block_h
 }
 
 // Per cluster: running position of each block's first member, and the
-// cluster's total count.
-static __global__ __attribute__((unused)) void km_col_scan_kernel(const int32_t *__restrict__ blockcnt, int64_t nblk, int64_t k,
-                                   int32_t *__restrict__ blockbase, int32_t *__restrict__ count) {
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= k) return;
-    int32_t run = 0;
-    for (int64_t b = 0; b < nblk; ++b) {
+// cluster's total count.  One 256-thread block per cluster: each thread sums
+// a contiguous run of counting blocks, an LDS scan orders the runs.
+static __global__ __attribute__((unused)) __launch_bounds__(256) void km_col_scan_kernel(
+    const int32_t *__restrict__ blockcnt, int64_t nblk, int64_t k, int32_t *__restrict__ blockbase,
+    int32_t *__restrict__ count) {
+    __shared__ int32_t part[256];
+    const int64_t c = blockIdx.x;
+    const int t = threadIdx.x;
+    const int64_t per = (nblk + 255) / 256;
+    const int64_t lo = t * per, hi = lo + per < nblk ? lo + per : nblk;
+    int32_t s = 0;
+    for (int64_t b = lo; b < hi; ++b) s += blockcnt[b * k + c];
+    part[t] = s;
+    __syncthreads();
+    // exclusive scan of part[] (Hillis-Steele in LDS)
+    for (int o = 1; o < 256; o <<= 1) {
+        const int32_t v = t >= o ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int32_t run = t > 0 ? part[t - 1] : 0;
+    for (int64_t b = lo; b < hi; ++b) {
         blockbase[b * k + c] = run;
         run += blockcnt[b * k + c];
     }
-    count[c] = run;
+    if (t == 255) count[c] = part[255];
 }
 
 static __global__ __attribute__((unused)) void km_offsets_kernel(const int32_t *__restrict__ count, int64_t k,
@@ -177,13 +224,12 @@ static __global__ __attribute__((unused)) void km_offsets_kernel(const int32_t *
 }
 
 // One wave per counting block: lanes that share a label in a 64-row chunk
-// are grouped with ballots; ranks follow lane (= row) order, so the scatter
-// is stable.
-static __global__ __attribute__((unused)) __launch_bounds__(64) void km_scatter_kernel(const int32_t *__restrict__ labels,
-                                                        int64_t n, int64_t k,
-                                                        const int32_t *__restrict__ blockbase,
-                                                        const int64_t *__restrict__ offset,
-                                                        int32_t *__restrict__ order) {
+// are grouped with ballots (rank = lanes before with that label, so the
+// scatter is stable); the first lane of each group reserves the group's
+// slots with one LDS atomic, and the chunks are taken in row order.
+static __global__ __attribute__((unused)) __launch_bounds__(64) void km_scatter_kernel(
+    const int32_t *__restrict__ labels, int64_t n, int64_t k, const int32_t *__restrict__ blockbase,
+    const int64_t *__restrict__ offset, int32_t *__restrict__ order) {
     extern __shared__ int64_t ctr[];
     const int lane = threadIdx.x;
     const int64_t b = blockIdx.x;
@@ -195,25 +241,30 @@ static __global__ __attribute__((unused)) __launch_bounds__(64) void km_scatter_
         const bool valid = i < n;
         const int32_t lab = valid ? labels[i] : -1;
         uint64_t todo = __ballot(valid);
+        int rank = 0, cnt = 0, myleader = 0;
         while (todo) {
             const int leader = __ffsll((unsigned long long)todo) - 1;
             const int32_t L = __shfl(lab, leader);
             const uint64_t m = __ballot(valid && lab == L);
-            const int64_t base = ctr[L];
             if (valid && lab == L) {
-                const int rank = __popcll(m & ((1ull << lane) - 1ull));
-                order[base + rank] = (int32_t)i;
+                rank = __popcll(m & ((1ull << lane) - 1ull));
+                cnt = __popcll(m);
+                myleader = leader;
             }
-            wave_sync_lds();
-            if (lane == leader) ctr[L] = base + __popcll(m);
-            wave_sync_lds();
             todo &= ~m;
         }
+        unsigned long long base = 0;
+        if (valid && myleader == lane)
+            base = atomicAdd((unsigned long long *)&ctr[lab], (unsigned long long)cnt);
+        const unsigned long long gb = __shfl(base, myleader);
+        if (valid) order[(int64_t)gb + rank] = (int32_t)i;
     }
 }
 
 // sum[c][j] = fold over the cluster's rows in ascending index order; then
-// divide by the count or keep the old centre for an empty cluster.
+// divide by the count or keep the old centre for an empty cluster.  The
+// member indices are staged through LDS 256 at a time and the values loaded
+// 32 at a time, so that the loads of a batch overlap.
 template <typename TX>
 __global__ __launch_bounds__(64) void km_sum_kernel(const TX *__restrict__ X, int d,
                                                     const int32_t *__restrict__ order,
@@ -221,23 +272,31 @@ __global__ __launch_bounds__(64) void km_sum_kernel(const TX *__restrict__ X, in
                                                     const int32_t *__restrict__ count,
                                                     const double *__restrict__ oldc,
                                                     double *__restrict__ newc) {
+    __shared__ int32_t ord[256];
     const int64_t c = blockIdx.x;
     const int j = blockIdx.y * 64 + threadIdx.x;
-    if (j >= d) return;
+    const bool col = j < d;
     const int64_t off = offset[c];
     const int32_t cnt = count[c];
     double acc = 0.0;
-    int32_t m = 0;
-    constexpr int U = 16;
-    for (; m + U <= cnt; m += U) {
-        double v[U];
+    for (int32_t m0 = 0; m0 < cnt; m0 += 256) {
+        const int nb = cnt - m0 < 256 ? cnt - m0 : 256;
+        wave_sync_lds();
+        for (int u = threadIdx.x; u < nb; u += 64) ord[u] = order[off + m0 + u];
+        wave_sync_lds();
+        if (!col) continue;
+        int m = 0;
+        constexpr int U = 32;
+        for (; m + U <= nb; m += U) {
+            double v[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) v[u] = (double)X[(int64_t)order[off + m + u] * d + j];
+            for (int u = 0; u < U; ++u) v[u] = (double)X[(int64_t)ord[m + u] * d + j];
 #pragma unroll
-        for (int u = 0; u < U; ++u) acc = acc + v[u];
+            for (int u = 0; u < U; ++u) acc = acc + v[u];
+        }
+        for (; m < nb; ++m) acc = acc + (double)X[(int64_t)ord[m] * d + j];
     }
-    for (; m < cnt; ++m) acc = acc + (double)X[(int64_t)order[off + m] * d + j];
-    newc[c * d + j] = cnt > 0 ? acc / (double)cnt : oldc[c * d + j];
+    if (col) newc[c * d + j] = cnt > 0 ? acc / (double)cnt : oldc[c * d + j];
 }
 
 // max_shift = sqrt(max_c fold(new_c, old_c)) (kmeans.rs:389-400); the max
