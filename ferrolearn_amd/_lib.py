@@ -63,6 +63,19 @@ class fl_kmeans_params(C.Structure):
     ]
 
 
+class fl_minibatch_params(C.Structure):
+    _fields_ = [
+        ("n_clusters", C.c_int64),
+        ("batch_size", C.c_int64),
+        ("max_iter", C.c_int64),
+        ("tol", C.c_double),
+        ("n_init", C.c_int64),
+        ("random_state", C.c_uint64),
+        ("has_random_state", C.c_int32),
+        ("precision", C.c_int32),
+    ]
+
+
 _vp = C.c_void_p
 _i64 = C.c_int64
 _i32 = C.c_int32
@@ -83,6 +96,12 @@ SIGNATURES = {
     "fl_kmeans_predict": (C.c_int, [_vp, _i64, _i64, _vp, _i32, _i64, _i64, _vp]),
     "fl_kmeans_transform": (C.c_int, [_vp, _i64, _i64, _vp, _i32, _i64, _i64, _vp]),
     "fl_kmeans_plusplus": (C.c_int, [_vp, _i32, _i64, _i64, _i64, C.c_uint64, _vp]),
+    "fl_minibatch_params_default": (None, [C.POINTER(fl_minibatch_params), _i64]),
+    "fl_minibatch_kmeans_fit": (C.c_int, [_vp, _i32, _i64, _i64, C.POINTER(fl_minibatch_params),
+                                          _vp, _vp, _vp, _dp, _ip, _rp]),
+    "fl_minibatch_kmeans_fit_device": (C.c_int, [_vp, _i32, _i64, _i64,
+                                                 C.POINTER(fl_minibatch_params), _vp, _vp, _vp,
+                                                 _dp, _ip, _rp]),
     "fl_knn_fit_classifier": (C.c_int, [_vp, _i32, _i64, _i64, _vp, _i64, _i64, _i32, _i32,
                                         C.POINTER(_vp)]),
     "fl_knn_fit_regressor": (C.c_int, [_vp, _i32, _i64, _i64, _vp, _i64, _i64, _i32, _i32,

@@ -119,6 +119,9 @@ class KMeans:
     def fit(self, x, y=None, init_centers=None) -> "FittedKMeans":
         """Fit<Array2<F>, ()>.  init_centers: optional (n_init, k, d) or
         (k, d) array of initial centres (otherwise the library's k-means++)."""
+        return self._fit(x, init_centers, L.lib().fl_kmeans_fit, L.lib().fl_kmeans_fit_device)
+
+    def _fit(self, x, init_centers, fit_host, fit_dev):
         p = self._params()
         rep = L.fl_report()
         inertia = C.c_double()
@@ -132,7 +135,7 @@ class KMeans:
             init_t = None
             if init_centers is not None:
                 init_t = _dev_from_host(np.ascontiguousarray(init_centers, dtype=np.float64), x)
-            st = L.lib().fl_kmeans_fit_device(
+            st = fit_dev(
                 C.c_void_p(x.data_ptr()), dt, n, d, C.byref(p),
                 C.c_void_p(init_t.data_ptr()) if init_t is not None else None,
                 C.c_void_p(centers.data_ptr()), C.c_void_p(labels.data_ptr()),
@@ -150,7 +153,7 @@ class KMeans:
                 raise ValueError("init_centers must hold n_init * n_clusters * n_features values")
         centers = np.zeros((max(k, 0), d), np.float64)
         labels = np.zeros(n, np.int64)
-        st = L.lib().fl_kmeans_fit(
+        st = fit_host(
             xh.ctypes.data_as(C.c_void_p), _dtype_code(xh.dtype), n, d, C.byref(p),
             init.ctypes.data_as(C.c_void_p) if init is not None else None,
             centers.ctypes.data_as(C.c_void_p), labels.ctypes.data_as(C.c_void_p),
@@ -214,6 +217,42 @@ class FittedKMeans:
                                           xh.ctypes.data_as(C.c_void_p), _dtype_code(xh.dtype),
                                           n, d, out.ctypes.data_as(C.c_void_p)))
         return out
+
+
+class MiniBatchKMeans(KMeans):
+    """mini_batch_kmeans.rs:77-161.  Defaults from MiniBatchKMeans::new
+    (:108-118): batch_size 1024, max_iter 100, tol 0, n_init 3, no seed.
+    `precision` picks the filter of the final full assignment."""
+
+    def __init__(self, n_clusters: int):
+        super().__init__(n_clusters)
+        self.batch_size = 1024
+        self.max_iter = 100
+        self.tol = 0.0
+        self.n_init = 3
+
+    def with_batch_size(self, batch_size: int):
+        return self._with(batch_size=int(batch_size))
+
+    def _params(self):
+        p = L.fl_minibatch_params()
+        L.lib().fl_minibatch_params_default(C.byref(p), self.n_clusters)
+        p.batch_size = self.batch_size
+        p.max_iter = self.max_iter
+        p.tol = self.tol
+        p.n_init = self.n_init
+        if self.random_state is not None:
+            p.random_state = self.random_state
+            p.has_random_state = 1
+        p.precision = L.PRECISIONS[self.precision]
+        return p
+
+    def fit(self, x, y=None, init_centers=None) -> "FittedKMeans":
+        """Fit<Array2<F>, ()> (mini_batch_kmeans.rs:478-592).  init_centers:
+        optional (n_init, k, d) or (k, d) initial centres, otherwise the
+        library's k-means++ from each run's generator."""
+        return self._fit(x, init_centers, L.lib().fl_minibatch_kmeans_fit,
+                         L.lib().fl_minibatch_kmeans_fit_device)
 
 
 def kmeans_plusplus(x, n_clusters: int, seed: int) -> np.ndarray:

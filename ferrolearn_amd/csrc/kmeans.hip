@@ -7,6 +7,7 @@
 // assignment, the centres are the post-update ones (:468-495).
 #include <cmath>
 #include <cstring>
+#include <unordered_map>
 #include <string>
 #include <vector>
 
@@ -262,6 +263,11 @@ class Lloyd {
     // The library's k-means++ into out (k*d f64, device).
     void kmeanspp(uint64_t seed, double *out) {
         Rng rng(seed);
+        kmeanspp(rng, out);
+    }
+    // Same, drawing from a generator the caller continues to use afterwards
+    // (MiniBatchKMeans samples its batches from the same stream).
+    void kmeanspp(Rng &rng, double *out) {
         const int64_t first = rng.range(n_);
         hipLaunchKernelGGL(copy_row_f64_kernel<TX>, dim3(grid1(d_, 64)), dim3(64), 0, st_, X_,
                            first, d_, out);
@@ -489,9 +495,240 @@ fl_status check_predict(int64_t k, int64_t d_centers, int64_t n, int64_t d, int3
     return FL_OK;
 }
 
+// ------------------------------------------------------------- mini-batch --
+// mini_batch_kmeans.rs:451-472 (sample_batch_indices) with the library's
+// generator: a full Fisher-Yates shuffle when batch >= n, else the first
+// `batch` steps of one over 0..n (the swapped entries kept in a map, so a
+// step costs O(1) instead of rebuilding the pool).
+std::vector<int32_t> sample_batch(Rng &rng, int64_t n, int64_t batch) {
+    std::vector<int32_t> out;
+    if (batch >= n) {
+        out.resize((size_t)n);
+        for (int64_t i = 0; i < n; ++i) out[(size_t)i] = (int32_t)i;
+        for (int64_t i = 0; i < n; ++i) {
+            const int64_t j = i + rng.range(n - i);
+            std::swap(out[(size_t)i], out[(size_t)j]);
+        }
+        return out;
+    }
+    std::unordered_map<int64_t, int64_t> pool;
+    pool.reserve((size_t)batch * 2);
+    auto at = [&](int64_t i) {
+        auto it = pool.find(i);
+        return it == pool.end() ? i : it->second;
+    };
+    out.resize((size_t)batch);
+    for (int64_t i = 0; i < batch; ++i) {
+        const int64_t j = i + rng.range(n - i);
+        const int64_t vi = at(i), vj = at(j);
+        pool[i] = vj;
+        pool[j] = vi;
+        out[(size_t)i] = (int32_t)vj;
+    }
+    return out;
+}
+
+// Fit<Array2<F>, ()> for MiniBatchKMeans (mini_batch_kmeans.rs:478-592):
+// per run (seed base + run * 1000003) initial centres (given, or the
+// library's k-means++ from that run's generator), counts = 1, max_iter
+// iterations of sample -> assign batch -> online update -> n_iter += 1 ->
+// stop if shift < tol; then every row is assigned (the candidate filter of
+// `precision` with the exact re-score) and the inertia is the left fold of
+// the best folds in row order.  The run with strictly smaller inertia wins.
+template <typename TX>
+fl_status mb_fit_device(const TX *X, int64_t n, int d, const fl_minibatch_params *p,
+                        const double *d_init, double *d_centers, int64_t *d_labels,
+                        double *inertia_out, int64_t *n_iter_out, fl_report *rep) {
+    WallClock wc;
+    StreamGuard sg;
+    hipStream_t st = sg.s;
+    const int64_t k = p->n_clusters;
+    Lloyd<TX> L(X, n, d, k, p->precision, st);
+    Workspace &ws = L.ws();
+    double *cur = ws.get<double>("mb_cur", k * d);
+    double *old = ws.get<double>("mb_old", k * d);
+    int64_t *counts = ws.get<int64_t>("mb_counts", k);
+    int32_t *labels = ws.get<int32_t>("labels", n);
+    double *best = ws.get<double>("best", n);
+    double *dsc = ws.get<double>("scalars", 2);   // [shift, inertia]
+    const int64_t bmax = p->batch_size < n ? p->batch_size : n;
+    int32_t *bidx = ws.get<int32_t>("mb_bidx", bmax);
+    int32_t *blab = ws.get<int32_t>("mb_blab", bmax);
+    const uint64_t base = p->has_random_state ? p->random_state : 0;
+    KmAcc acc;
+    double best_inertia = 0.0;
+    int64_t best_iter = 0;
+    PhaseTimer tm(st);
+    double ms_batch = 0.0;
+    for (int64_t run = 0; run < p->n_init; ++run) {
+        Rng rng(base + (uint64_t)run * 1000003ULL);
+        if (d_init)
+            FL_HIP(hipMemcpyAsync(cur, d_init + run * k * d, (size_t)(k * d) * 8,
+                                  hipMemcpyDeviceToDevice, st));
+        else
+            L.kmeanspp(rng, cur);
+        hipLaunchKernelGGL(fill_i64_kernel, dim3(grid1(k, 256)), dim3(256), 0, st, counts, k,
+                           (int64_t)1);
+        check_launch();
+        int64_t n_iter = 0;
+        for (int64_t it = 0; it < p->max_iter; ++it) {
+            const std::vector<int32_t> bi = sample_batch(rng, n, p->batch_size);
+            const int64_t b = (int64_t)bi.size();
+            const int t0 = tm.begin();
+            FL_HIP(hipMemcpyAsync(bidx, bi.data(), (size_t)b * 4, hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(mb_assign_batch_kernel<TX>, dim3(grid1(b, 4)), dim3(256), 0, st, X,
+                               cur, d, k, bidx, b, blab);
+            FL_HIP(hipMemcpyAsync(old, cur, (size_t)(k * d) * 8, hipMemcpyDeviceToDevice, st));
+            hipLaunchKernelGGL(mb_update_kernel<TX>, dim3((unsigned)k), dim3(64), 0, st, X, d,
+                               bidx, blab, b, cur, counts);
+            hipLaunchKernelGGL(km_shift_kernel, dim3(1), dim3(256), 0, st, cur, old, k, d, dsc);
+            check_launch();
+            tm.end(t0);
+            double shift = 0.0;
+            FL_HIP(hipMemcpyAsync(&shift, dsc, 8, hipMemcpyDeviceToHost, st));
+            FL_HIP(hipStreamSynchronize(st));
+            ms_batch += tm.ms(t0);
+            n_iter = it + 1;
+            if (shift < p->tol) break;
+        }
+        L.assign(cur, labels, best, acc);
+        hipLaunchKernelGGL(seq_fold_kernel, dim3(1), dim3(64), 0, st, best, n, dsc + 1);
+        check_launch();
+        double inertia = 0.0;
+        FL_HIP(hipMemcpyAsync(&inertia, dsc + 1, 8, hipMemcpyDeviceToHost, st));
+        FL_HIP(hipStreamSynchronize(st));
+        if (run == 0 || inertia < best_inertia) {
+            best_inertia = inertia;
+            best_iter = n_iter;
+            FL_HIP(hipMemcpyAsync(d_centers, cur, (size_t)(k * d) * 8, hipMemcpyDeviceToDevice,
+                                  st));
+            hipLaunchKernelGGL(labels_to_i64_kernel, dim3(grid1(n, 256)), dim3(256), 0, st, labels,
+                               n, d_labels);
+            check_launch();
+        }
+    }
+    FL_HIP(hipStreamSynchronize(st));
+    *inertia_out = best_inertia;
+    *n_iter_out = best_iter;
+    if (rep) {
+        std::memset(rep, 0, sizeof(*rep));
+        rep->ms_total = wc.ms();
+        rep->ms_distance = acc.ms_dist;
+        rep->ms_finalize = acc.ms_fin;
+        rep->ms_fallback = acc.ms_fb;
+        rep->ms_update = ms_batch;
+        rep->n_rows = acc.n_rows;
+        rep->n_candidates = acc.n_cand;
+        rep->n_fallback = acc.n_fb;
+        rep->n_iter = best_iter;
+        rep->n_dist_launches = acc.n_launch;
+    }
+    return FL_OK;
+}
+
+// mini_batch_kmeans.rs:495-526: n_clusters, batch_size, n_init, then the
+// sample count, in that order.
+fl_status validate_mb(int64_t n, int64_t d, const fl_minibatch_params *p, int32_t dtype) {
+    if (!p) return fail(FL_INVALID_PARAMETER, "InvalidParameter: params: must not be NULL");
+    if (p->n_clusters == 0)
+        return fail(FL_INVALID_PARAMETER, "InvalidParameter: n_clusters: must be at least 1");
+    if (p->batch_size == 0)
+        return fail(FL_INVALID_PARAMETER, "InvalidParameter: batch_size: must be at least 1");
+    if (p->n_init == 0)
+        return fail(FL_INVALID_PARAMETER, "InvalidParameter: n_init: must be at least 1");
+    if (n == 0)
+        return fail(FL_INSUFFICIENT_SAMPLES,
+                    "InsufficientSamples: required " + std::to_string(p->n_clusters) +
+                        ", actual 0: MiniBatchKMeans requires at least n_clusters samples");
+    if (n < p->n_clusters)
+        return fail(FL_INSUFFICIENT_SAMPLES,
+                    "InsufficientSamples: required " + std::to_string(p->n_clusters) +
+                        ", actual " + std::to_string(n) +
+                        ": MiniBatchKMeans requires at least n_clusters samples");
+    // Limits of this implementation (no reference counterpart).
+    if (p->n_clusters < 0 || p->n_clusters > kMaxClusters)
+        return fail(FL_INVALID_PARAMETER,
+                    "InvalidParameter: n_clusters: the GPU path supports 1..8192 clusters");
+    if (p->batch_size < 0 || p->n_init < 0 || p->max_iter < 0)
+        return fail(FL_INVALID_PARAMETER,
+                    "InvalidParameter: batch_size/n_init/max_iter: negative");
+    if (dtype != FL_F32 && dtype != FL_F64)
+        return fail(FL_INVALID_PARAMETER, "InvalidParameter: dtype: must be FL_F32 or FL_F64");
+    if (d <= 0 || d > (1 << 20))
+        return fail(FL_INVALID_PARAMETER, "InvalidParameter: n_features: must be in 1..2^20");
+    if (n > 0x7fffffffLL)
+        return fail(FL_INVALID_PARAMETER, "InvalidParameter: n_samples: at most 2^31-1 rows");
+    if (p->precision != FL_PREC_FP64 && p->precision != FL_PREC_EXACT &&
+        p->precision != FL_PREC_BF16 && p->precision != FL_PREC_BF16X3)
+        return fail(FL_INVALID_PARAMETER, "InvalidParameter: precision: unknown mode");
+    return FL_OK;
+}
+
+fl_status mb_dispatch(const void *dX, int32_t dtype, int64_t n, int64_t d,
+                      const fl_minibatch_params *p, const double *d_init, double *d_centers,
+                      int64_t *d_labels, double *inertia, int64_t *n_iter, fl_report *rep) {
+    if (dtype == FL_F32)
+        return mb_fit_device<float>((const float *)dX, n, (int)d, p, d_init, d_centers, d_labels,
+                                    inertia, n_iter, rep);
+    return mb_fit_device<double>((const double *)dX, n, (int)d, p, d_init, d_centers, d_labels,
+                                 inertia, n_iter, rep);
+}
+
 }  // namespace
 
 extern "C" {
+
+void fl_minibatch_params_default(fl_minibatch_params *p, int64_t n_clusters) {
+    std::memset(p, 0, sizeof(*p));
+    p->n_clusters = n_clusters;
+    p->batch_size = 1024;
+    p->max_iter = 100;
+    p->tol = 0.0;
+    p->n_init = 3;
+    p->has_random_state = 0;
+    p->precision = FL_PREC_FP64;
+}
+
+fl_status fl_minibatch_kmeans_fit_device(const void *dX, int32_t dtype, int64_t n, int64_t d,
+                                         const fl_minibatch_params *params, const double *d_init,
+                                         double *d_centers_out, int64_t *d_labels_out,
+                                         double *inertia_out, int64_t *n_iter_out,
+                                         fl_report *report) {
+    return guarded([&]() -> fl_status {
+        if (fl_status s = validate_mb(n, d, params, dtype)) return s;
+        return mb_dispatch(dX, dtype, n, d, params, d_init, d_centers_out, d_labels_out,
+                           inertia_out, n_iter_out, report);
+    });
+}
+
+fl_status fl_minibatch_kmeans_fit(const void *X, int32_t dtype, int64_t n, int64_t d,
+                                  const fl_minibatch_params *params, const double *init_centers,
+                                  double *centers_out, int64_t *labels_out, double *inertia_out,
+                                  int64_t *n_iter_out, fl_report *report) {
+    return guarded([&]() -> fl_status {
+        if (fl_status s = validate_mb(n, d, params, dtype)) return s;
+        const int64_t k = params->n_clusters;
+        DevBuf bx, bi, bc, bl;
+        const size_t xb = (size_t)(n * d) * dtype_size(dtype);
+        void *dX = bx.get(xb);
+        FL_HIP(hipMemcpy(dX, X, xb, hipMemcpyHostToDevice));
+        double *dI = nullptr;
+        if (init_centers) {
+            const size_t ib = (size_t)(params->n_init * k * d) * 8;
+            dI = (double *)bi.get(ib);
+            FL_HIP(hipMemcpy(dI, init_centers, ib, hipMemcpyHostToDevice));
+        }
+        double *dC = (double *)bc.get((size_t)(k * d) * 8);
+        int64_t *dL = (int64_t *)bl.get((size_t)n * 8);
+        fl_status s = mb_dispatch(dX, dtype, n, d, params, dI, dC, dL, inertia_out, n_iter_out,
+                                  report);
+        if (s) return s;
+        FL_HIP(hipMemcpy(centers_out, dC, (size_t)(k * d) * 8, hipMemcpyDeviceToHost));
+        FL_HIP(hipMemcpy(labels_out, dL, (size_t)n * 8, hipMemcpyDeviceToHost));
+        return FL_OK;
+    });
+}
+
 
 void fl_kmeans_params_default(fl_kmeans_params *p, int64_t n_clusters) {
     std::memset(p, 0, sizeof(*p));

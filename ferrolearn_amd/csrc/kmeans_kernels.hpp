@@ -89,21 +89,15 @@ __global__ void km_assign_exact_kernel(const TX *__restrict__ X, const double *_
     best[i] = v;
 }
 
-// Exact scan for the rows the certificate could not settle: one wave per
-// row, lane l folds centres l, l+64, ...; the wave then takes the smallest
-// (fold, index), which is the result of kmeans.rs:295-309's sequential scan
-// (init (0, F::MAX), update iff fold < best): the lowest index among the
-// smallest folds below F::MAX, else (0, F::MAX).  Four rows per block.
+// Exact scan of every centre for one row by one wave: lane l folds centres
+// l, l+64, ...; the wave then takes the smallest (fold, index), which is the
+// result of kmeans.rs:295-309's sequential scan (init (0, F::MAX), update iff
+// fold < best): the lowest index among the smallest folds below F::MAX, else
+// (0, F::MAX).  Every lane returns the result.
 template <typename TX>
-__global__ __launch_bounds__(256) void km_fallback_kernel(
-    const TX *__restrict__ X, const double *__restrict__ C, int d, int64_t k,
-    const int32_t *__restrict__ list, const int32_t *__restrict__ count,
-    int32_t *__restrict__ labels, double *__restrict__ best) {
-    const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (t >= *count) return;
-    const int64_t i = list[t];
-    const TX *x = X + i * d;
+__device__ __forceinline__ void wave_exact_argmin(const TX *__restrict__ x,
+                                                  const double *__restrict__ C, int d, int64_t k,
+                                                  int lane, int32_t &lab, double &dist) {
     double bd = 1.7976931348623157e308;
     int64_t bl = -1;
     for (int64_t c = lane; c < k; c += 64) {
@@ -121,10 +115,100 @@ __global__ __launch_bounds__(256) void km_fallback_kernel(
             bl = ol;
         }
     }
+    lab = bl < 0 ? 0 : (int32_t)bl;
+    dist = bd;
+}
+
+// Exact scan for the rows the certificate could not settle (four rows per
+// 256-thread block, one wave each).
+template <typename TX>
+__global__ __launch_bounds__(256) void km_fallback_kernel(
+    const TX *__restrict__ X, const double *__restrict__ C, int d, int64_t k,
+    const int32_t *__restrict__ list, const int32_t *__restrict__ count,
+    int32_t *__restrict__ labels, double *__restrict__ best) {
+    const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (t >= *count) return;
+    const int64_t i = list[t];
+    int32_t l;
+    double v;
+    wave_exact_argmin(X + i * d, C, d, k, lane, l, v);
     if (lane == 0) {
-        labels[i] = bl < 0 ? 0 : (int32_t)bl;
-        best[i] = bd;
+        labels[i] = l;
+        best[i] = v;
     }
+}
+
+// ------------------------------------------------------------- mini-batch --
+// mini_batch_kmeans.rs:381-404 (assign_batch): nearest centre of each batch
+// row, the same sequential scan; one wave per row.
+template <typename TX>
+__global__ __launch_bounds__(256) void mb_assign_batch_kernel(
+    const TX *__restrict__ X, const double *__restrict__ C, int d, int64_t k,
+    const int32_t *__restrict__ bidx, int64_t b, int32_t *__restrict__ blab) {
+    const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (t >= b) return;
+    int32_t l;
+    double v;
+    wave_exact_argmin(X + (int64_t)bidx[t] * d, C, d, k, lane, l, v);
+    if (lane == 0) blab[t] = l;
+}
+
+// mini_batch_kmeans.rs:410-426 (update_centers_mini_batch): for the batch
+// rows in batch order, counts[l] += 1, lr = 1 / counts[l], and
+// c[l][j] = c[l][j] + lr * (x[j] - c[l][j]).  One 64-thread block per centre:
+// it scans the batch labels in order and updates its features in place
+// (thread t owns features t, t+64, ...); different centres are independent.
+template <typename TX>
+__global__ __launch_bounds__(64) void mb_update_kernel(const TX *__restrict__ X, int d,
+                                                       const int32_t *__restrict__ bidx,
+                                                       const int32_t *__restrict__ blab, int64_t b,
+                                                       double *__restrict__ C,
+                                                       int64_t *__restrict__ counts) {
+    __shared__ int32_t labs[1024];
+    const int64_t c = blockIdx.x;
+    const int t = threadIdx.x;
+    double *cr = C + c * d;
+    int64_t cnt = counts[c];
+    for (int64_t s0 = 0; s0 < b; s0 += 1024) {
+        const int nb = b - s0 < 1024 ? (int)(b - s0) : 1024;
+        wave_sync_lds();
+        for (int u = t; u < nb; u += 64) labs[u] = blab[s0 + u];
+        wave_sync_lds();
+        for (int s = 0; s < nb; ++s) {
+            if (labs[s] != (int32_t)c) continue;
+            cnt += 1;
+            const double lr = 1.0 / (double)cnt;
+            const TX *x = X + (int64_t)bidx[s0 + s] * d;
+            for (int j = t; j < d; j += 64) cr[j] = cr[j] + lr * ((double)x[j] - cr[j]);
+        }
+    }
+    if (t == 0) counts[c] = cnt;
+}
+
+static __global__ __attribute__((unused)) void fill_i64_kernel(int64_t *__restrict__ v, int64_t n,
+                                                                int64_t value) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = value;
+}
+
+// mini_batch_kmeans.rs:366-373: inertia as the left fold of the rows' best
+// folds in row order (one lane; loads in batches of 32).
+static __global__ __attribute__((unused)) void seq_fold_kernel(const double *__restrict__ v,
+                                                                int64_t n, double *__restrict__ out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double s = 0.0;
+    int64_t i = 0;
+    for (; i + 32 <= n; i += 32) {
+        double b[32];
+#pragma unroll
+        for (int u = 0; u < 32; ++u) b[u] = v[i + u];
+#pragma unroll
+        for (int u = 0; u < 32; ++u) s = s + b[u];
+    }
+    for (; i < n; ++i) s = s + v[i];
+    *out = s;
 }
 
 // ---------------------------------------------------------------- inertia --

@@ -154,6 +154,51 @@ fl_status fl_kmeans_plusplus(const void *X, int32_t dtype, int64_t n,
                              double *centers_out);
 
 /* ------------------------------------------------------------------ */
+/* MiniBatchKMeans -- ferrolearn-cluster/src/mini_batch_kmeans.rs      */
+/* ------------------------------------------------------------------ */
+
+/* mini_batch_kmeans.rs:77-93 (MiniBatchKMeans builder).  Defaults from
+ * MiniBatchKMeans::new (mini_batch_kmeans.rs:108-118): batch_size 1024,
+ * max_iter 100, tol 0, n_init 3, no seed. */
+typedef struct fl_minibatch_params {
+    int64_t n_clusters;
+    int64_t batch_size;
+    int64_t max_iter;
+    double tol;
+    int64_t n_init;
+    uint64_t random_state;  /* used when has_random_state != 0 */
+    int32_t has_random_state;
+    int32_t precision;      /* fl_precision for the final full assignment */
+} fl_minibatch_params;
+
+/* mini_batch_kmeans.rs:108-118 (MiniBatchKMeans::new) */
+void fl_minibatch_params_default(fl_minibatch_params *p, int64_t n_clusters);
+
+/* Fit<Array2<F>, ()> for MiniBatchKMeans (mini_batch_kmeans.rs:478-592).
+ * Validation order: n_clusters == 0, batch_size == 0, n_init == 0 ->
+ * InvalidParameter; n == 0 or n < n_clusters -> InsufficientSamples.  Run r
+ * draws from the library's generator seeded with random_state + r*1000003:
+ * k-means++ when init_centers is NULL (else n_init*k*d f64 initial centres),
+ * then the batches.  Outputs as fl_kmeans_fit; the inertia is the left fold
+ * of the final assignment's folds in row order (:366-373).  Predict and
+ * transform of the fitted model are fl_kmeans_predict / fl_kmeans_transform
+ * (mini_batch_kmeans.rs:607-682 is the same nearest-centre rule). */
+fl_status fl_minibatch_kmeans_fit(const void *X, int32_t dtype, int64_t n, int64_t d,
+                                  const fl_minibatch_params *params,
+                                  const double *init_centers, double *centers_out,
+                                  int64_t *labels_out, double *inertia_out,
+                                  int64_t *n_iter_out, fl_report *report);
+
+/* Same with X, init_centers and the centre / label outputs in device memory
+ * (inertia_out and n_iter_out stay host pointers). */
+fl_status fl_minibatch_kmeans_fit_device(const void *dX, int32_t dtype, int64_t n,
+                                         int64_t d, const fl_minibatch_params *params,
+                                         const double *d_init_centers,
+                                         double *d_centers_out, int64_t *d_labels_out,
+                                         double *inertia_out, int64_t *n_iter_out,
+                                         fl_report *report);
+
+/* ------------------------------------------------------------------ */
 /* k-nearest neighbours -- ferrolearn-neighbors/src/knn.rs             */
 /* ------------------------------------------------------------------ */
 

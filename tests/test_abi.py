@@ -49,6 +49,28 @@ def test_kmeans_validation_order():
         F.KMeans(0).fit(np.zeros((0, 2)))
 
 
+def test_minibatch_validation_order():
+    # mini_batch_kmeans.rs:495-526: n_clusters, batch_size, n_init, then the
+    # sample count -- each check fires before the later ones.
+    x = np.ones((3, 2))
+    empty = np.zeros((0, 2))
+    with pytest.raises(F.InvalidParameter, match="n_clusters"):
+        F.MiniBatchKMeans(0).with_batch_size(0).with_n_init(0).fit(empty)
+    with pytest.raises(F.InvalidParameter, match="batch_size"):
+        F.MiniBatchKMeans(2).with_batch_size(0).with_n_init(0).fit(empty)
+    with pytest.raises(F.InvalidParameter, match="n_init"):
+        F.MiniBatchKMeans(2).with_n_init(0).fit(empty)
+    with pytest.raises(F.InsufficientSamples, match="actual 0"):
+        F.MiniBatchKMeans(2).fit(empty)
+    with pytest.raises(F.InsufficientSamples, match="required 4, actual 3"):
+        F.MiniBatchKMeans(4).fit(x)
+
+
+def test_minibatch_defaults():
+    m = F.MiniBatchKMeans(5)
+    assert (m.batch_size, m.max_iter, m.tol, m.n_init, m.random_state) == (1024, 100, 0.0, 3, None)
+
+
 def test_kmeans_predict_shape_mismatch():
     fitted = F.FittedKMeans(np.zeros((2, 2)), np.zeros(4, np.int64), 0.0, 1, {})
     with pytest.raises(F.ShapeMismatch, match="expected \\[2\\], actual \\[3\\]"):
