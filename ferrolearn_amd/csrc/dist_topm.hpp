@@ -27,7 +27,7 @@ namespace fl {
 
 // Candidate filters: f64 MFMA GEMM form, exact folds, BF16 one product,
 // BF16 three products (dist_bf16.hpp).
-enum { MODE_MFMA64 = 0, MODE_EXACT = 1, MODE_BF16 = 2, MODE_BF16X3 = 3 };
+enum { MODE_MFMA64 = 0, MODE_EXACT = 1, MODE_BF16 = 2, MODE_BF16X3 = 3, MODE_F64 = 4 };
 
 constexpr int TM_WAVES = 4;
 constexpr int TM_ROWS_W = 32;   // A rows per wave (two 16-row MFMA tiles)

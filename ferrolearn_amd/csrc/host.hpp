@@ -12,6 +12,7 @@
 
 #include "common.hpp"
 #include "dist_bf16.hpp"
+#include "dist_f64.hpp"
 #include "dist_topm.hpp"
 
 namespace fl {
@@ -180,6 +181,52 @@ void launch_bf16_p(int MP, int nkc, const Bf16Args &a, int S, hipStream_t st) {
 inline void launch_bf16(int passes, int MP, int nkc, const Bf16Args &a, int S, hipStream_t st) {
     if (passes == 3) return launch_bf16_p<3>(MP, nkc, a, S, st);
     return launch_bf16_p<1>(MP, nkc, a, S, st);
+}
+
+// f64 filter: D4 = 4-feature steps (a power of two for the LDS swizzle), 0
+// when d > 128 (the resident fragments would not fit in registers).
+inline int f64_d4(int64_t d) {
+    const int64_t q = (d + 3) / 4;
+    for (int v : {4, 8, 16, 32})
+        if (q <= v) return v;
+    return 0;
+}
+
+template <int MP, int D4, typename TA, typename TB>
+void launch_f64_t(const F64Args &a, int S, hipStream_t st) {
+    const size_t merge = MP == 0 ? 0 : (size_t)F64_WAVES * 16 * 4 * (MP / 2) * 16;
+    const size_t shmem = std::max(F64Tile<D4, TB>::SHMEM, merge);
+    auto kern = dist_f64_kernel<MP, D4, TA, TB>;
+    FL_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)shmem));
+    dim3 grid((unsigned)ceil_div(a.nA, F64_Q), (unsigned)S);
+    hipLaunchKernelGGL(kern, grid, dim3(F64_WAVES * 64), shmem, st, a);
+    check_launch();
+}
+
+template <int MP, typename TA, typename TB>
+void launch_f64_m(int d4, const F64Args &a, int S, hipStream_t st) {
+    switch (d4) {
+        case 4: return launch_f64_t<MP, 4, TA, TB>(a, S, st);
+        case 8: return launch_f64_t<MP, 8, TA, TB>(a, S, st);
+        case 16: return launch_f64_t<MP, 16, TA, TB>(a, S, st);
+        case 32: return launch_f64_t<MP, 32, TA, TB>(a, S, st);
+    }
+    throw DeviceError("internal: unsupported f64 feature steps");
+}
+
+// kNN: f64 queries against a database of TB; MP candidates per row and split.
+template <typename TB>
+void launch_f64_knn(int MP, int d4, const F64Args &a, int S, hipStream_t st) {
+    if (MP == 8) return launch_f64_m<8, double, TB>(d4, a, S, st);
+    if (MP == 16) return launch_f64_m<16, double, TB>(d4, a, S, st);
+    throw DeviceError("internal: unsupported candidate list size");
+}
+
+// KMeans: samples of TA against f64 centres; argmin form.
+template <typename TA>
+void launch_f64_km(int d4, const F64Args &a, hipStream_t st) {
+    launch_f64_m<0, TA, double>(d4, a, 1, st);
 }
 
 template <typename TA, typename TB, int M, int MODE>

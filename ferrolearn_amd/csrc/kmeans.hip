@@ -87,20 +87,22 @@ class Lloyd {
                                d, xn_, xs_);
             check_launch();
         }
-        // BF16 filter: samples are split into hi / lo bf16 once per fit.  Its
-        // bound needs finite values below 2^56 in magnitude (the centres,
-        // means of samples, then are too); other data uses the f64 filter.
-        if (!direct_ && (precision == FL_PREC_BF16 || precision == FL_PREC_BF16X3) &&
-            bf16_nkc(d) > 0 && k < ((int64_t)1 << 31) - 64) {
-            unsigned int *flag = ws_.get<unsigned int>("unsafe", 1);
+        // Range of the samples (the centres, means of samples, stay inside
+        // it): the BF16 bound needs finite values below 2^56 in magnitude,
+        // the f64 filter below 2^250; other data uses the f64 GEMM form.
+        if (!direct_) {
+            unsigned int *flag = ws_.get<unsigned int>("range", 1);
             FL_HIP(hipMemsetAsync(flag, 0, 4, st));
-            hipLaunchKernelGGL(bf16_unsafe_kernel<TX>, dim3(1024), dim3(256), 0, st, X, n * d,
+            hipLaunchKernelGGL(range_flags_kernel<TX>, dim3(1024), dim3(256), 0, st, X, n * d,
                                flag);
             check_launch();
             unsigned int hflag = 0;
             FL_HIP(hipMemcpyAsync(&hflag, flag, 4, hipMemcpyDeviceToHost, st));
             FL_HIP(hipStreamSynchronize(st));
-            if (hflag == 0) {
+            const bool fits = k < ((int64_t)1 << 31) - 64;
+            // BF16 filter: samples are split into hi / lo bf16 once per fit.
+            if ((precision == FL_PREC_BF16 || precision == FL_PREC_BF16X3) && bf16_nkc(d) > 0 &&
+                fits && (hflag & 1u) == 0) {
                 passes_ = precision == FL_PREC_BF16X3 ? 3 : 1;
                 nkc_ = bf16_nkc(d);
                 const int dpad = nkc_ * 32;
@@ -109,6 +111,8 @@ class Lloyd {
                 hipLaunchKernelGGL(bf16_split_kernel<TX>, dim3(grid1(n * dpad, 256)), dim3(256), 0,
                                    st, X, n, d, dpad, xh_, xl_);
                 check_launch();
+            } else if (f64_d4(d) > 0 && fits && (hflag & 2u) == 0) {
+                d4_ = f64_d4(d);   // f64 MFMA argmin filter
             }
         }
     }
@@ -173,6 +177,41 @@ class Lloyd {
             a.out = cand;
             t0 = tm.begin();
             launch_bf16(passes_, 0, nkc_, a, 1, st_);   // argmin form: 4 candidates per row
+            tm.end(t0);
+        } else if (d4_ > 0) {
+            // f64 filter: centre terms, padding and the largest centre norm
+            // change every iteration.
+            const int dp = 4 * d4_;
+            const Bound bd = bound_fp64(d_);
+            const double *cB = C;
+            if (dp != d_) {
+                double *cp = ws_.get<double>("cpad", k_ * dp);
+                hipLaunchKernelGGL(pad_rows_kernel<double>, dim3(grid1(k_ * dp, 256)), dim3(256),
+                                   0, st_, C, k_, d_, dp, cp);
+                cB = cp;
+            }
+            double *cxnp = ws_.get<double>("cxnp64", k_);
+            unsigned long long *smax = ws_.get<unsigned long long>("smax", 1);
+            double *qoff = ws_.get<double>("qoff", n_);
+            hipLaunchKernelGGL(f64_stream_terms_kernel, dim3(grid1(k_, 256)), dim3(256), 0, st_,
+                               cn, k_, bd.beta, cxnp);
+            FL_HIP(hipMemsetAsync(smax, 0, 8, st_));
+            hipLaunchKernelGGL(max_nonneg_kernel, dim3(16), dim3(256), 0, st_, cs, k_, smax);
+            hipLaunchKernelGGL(f64_resident_terms_kernel, dim3(grid1(n_, 256)), dim3(256), 0, st_,
+                               xn_, xs_, n_, d_, bd.alpha, bd.beta, (const double *)smax, qoff);
+            check_launch();
+            F64Args a;
+            a.nA = n_;
+            a.nB = k_;
+            a.d = d_;
+            a.b_per_split = k_;
+            a.A = X_;
+            a.B = cB;
+            a.qoff = qoff;
+            a.xnp = cxnp;
+            a.out = cand;
+            t0 = tm.begin();
+            launch_f64_km<TX>(d4_, a, st_);   // argmin form: 4 candidates per row
             tm.end(t0);
         } else {
             TopmArgs a;
@@ -353,6 +392,7 @@ class Lloyd {
     hipStream_t st_;
     bool direct_ = true;
     int passes_ = 0, nkc_ = 0;   // BF16 filter (passes_ > 0)
+    int d4_ = 0;                 // f64 MFMA filter (d4_ > 0)
     double *xn_ = nullptr, *xs_ = nullptr;
     unsigned short *xh_ = nullptr, *xl_ = nullptr;
     Workspace ws_;

@@ -29,8 +29,13 @@ struct fl_knn {
     DevBuf bxnp;                   // BF16 per-row key term (1-b)|x|^2
     DevBuf bsmax;                  // largest row norm (f64 bits)
     int bf_split_ready = 0;        // hi/lo arrays built
-    int bf_db_unsafe = -1;         // X has non-finite or > 2^56 values (-1: unknown)
+    int db_range = -1;             // range_flags_kernel bits of X (-1: unknown)
     int bf_terms_passes = 0;       // passes the key terms were built for
+    DevBuf xpad;                   // f64 filter: X with rows padded to 4*D4 (if d differs)
+    DevBuf fxnp;                   // f64 filter: per-row key term (1-b)|x|^2
+    int f64_ready = 0;             // xpad / fxnp / bsmax built
+    int smax_ready = 0;            // bsmax built
+    int f64_ok = 0;                // this call's data fits the f64 filter's range
     hipStream_t stream = nullptr;
     Workspace ws;
     ~fl_knn() {
@@ -105,16 +110,22 @@ void knn_search(fl_knn *h, const double *Q, int64_t nq, int k, int mode, double 
     check_launch();
 
     const bool bf = mode == MODE_BF16 || mode == MODE_BF16X3;
-    // BF16 kernels output MP candidates per split (merged from four register
-    // lists per resident row); the f64 kernels keep one list of M in LDS.
+    const bool f64 = mode == MODE_F64;
+    // BF16 and f64-filter kernels output MP candidates per split (merged
+    // from four register lists per resident row); the other kernels keep one
+    // list of M in LDS.
     const int MP = k <= 6 ? 8 : 16;
-    const int M = bf ? MP : (mode == MODE_EXACT ? pick_m_exact(k) : pick_m_filter(k));
-    const int rows_per_block = bf ? BF_Q : TM_ROWS;
+    const int M = (bf || f64) ? MP : (mode == MODE_EXACT ? pick_m_exact(k) : pick_m_filter(k));
+    const int rows_per_block = bf ? BF_Q : (f64 ? F64_Q : TM_ROWS);
     const int64_t nblk = ceil_div(nq, rows_per_block);
     const int64_t smax = std::max<int64_t>(
         1, std::min<int64_t>(FIN_MAXC / M, std::max<int64_t>(1, nB / 256)));
     int64_t S;
-    if (bf) {
+    if (f64) {
+        const int64_t row_bytes = (int64_t)f64_d4(d) * 4 * (int64_t)sizeof(TB);
+        const int64_t smin = ceil_div(nB * row_bytes, (int64_t)1 << 31);
+        S = std::max<int64_t>(smin, bf16_splits(nblk, nB, smax));
+    } else if (bf) {
         // Each split's hi/lo rows must stay within 2^31 bytes (32-bit buffer
         // offsets of the staged copies).
         const int64_t row_bytes = (int64_t)bf16_nkc(d) * 64;
@@ -137,7 +148,48 @@ void knn_search(fl_knn *h, const double *Q, int64_t nq, int k, int mode, double 
 
     PhaseTimer tm(st);
     int t0;
-    if (mode == MODE_BF16 || mode == MODE_BF16X3) {
+    if (!h->smax_ready && (bf || f64)) {
+        unsigned long long *smaxp = (unsigned long long *)h->bsmax.get(8);
+        FL_HIP(hipMemsetAsync(smaxp, 0, 8, st));
+        hipLaunchKernelGGL(max_nonneg_kernel, dim3(256), dim3(256), 0, st,
+                           (const double *)h->xs.ptr(), nB, smaxp);
+        check_launch();
+        h->smax_ready = 1;
+    }
+    if (f64) {
+        const int d4 = f64_d4(d);
+        const int dp = 4 * d4;
+        const Bound bd = bound_fp64(d);
+        if (!h->f64_ready) {
+            if (dp != d) {
+                TB *xp = (TB *)h->xpad.get((size_t)(nB * dp) * sizeof(TB));
+                hipLaunchKernelGGL(pad_rows_kernel<TB>, dim3(grid1(nB * dp, 256)), dim3(256), 0,
+                                   st, B, nB, d, dp, xp);
+            }
+            double *xnp = (double *)h->fxnp.get((size_t)nB * 8);
+            hipLaunchKernelGGL(f64_stream_terms_kernel, dim3(grid1(nB, 256)), dim3(256), 0, st,
+                               (const double *)h->xn.ptr(), nB, bd.beta, xnp);
+            check_launch();
+            h->f64_ready = 1;
+        }
+        double *qoff = ws.get<double>(p + "qoff", nq);
+        hipLaunchKernelGGL(f64_resident_terms_kernel, dim3(grid1(nq, 256)), dim3(256), 0, st, qn,
+                           qs, nq, d, bd.alpha, bd.beta, (const double *)h->bsmax.ptr(), qoff);
+        check_launch();
+        F64Args a;
+        a.nA = nq;
+        a.nB = nB;
+        a.d = d;
+        a.b_per_split = bps;
+        a.A = Q;
+        a.B = dp != d ? h->xpad.ptr() : (const void *)B;
+        a.qoff = qoff;
+        a.xnp = (const double *)h->fxnp.ptr();
+        a.out = cand;
+        t0 = tm.begin();
+        launch_f64_knn<TB>(MP, d4, a, (int)S, st);
+        tm.end(t0);
+    } else if (mode == MODE_BF16 || mode == MODE_BF16X3) {
         const int passes = mode == MODE_BF16X3 ? 3 : 1;
         const int nkc = bf16_nkc(d);
         const int dpad = nkc * 32;
@@ -154,10 +206,6 @@ void knn_search(fl_knn *h, const double *Q, int64_t nq, int k, int mode, double 
             float *xnp = (float *)h->bxnp.get((size_t)nB * 4);
             hipLaunchKernelGGL(bf16_stream_terms_kernel, dim3(grid1(nB, 256)), dim3(256), 0, st,
                                (const double *)h->xn.ptr(), nB, bd.beta, xnp);
-            unsigned long long *smax = (unsigned long long *)h->bsmax.get(8);
-            FL_HIP(hipMemsetAsync(smax, 0, 8, st));
-            hipLaunchKernelGGL(max_nonneg_kernel, dim3(256), dim3(256), 0, st,
-                               (const double *)h->xs.ptr(), nB, smax);
             check_launch();
             h->bf_terms_passes = passes;
         }
@@ -234,7 +282,8 @@ void knn_search(fl_knn *h, const double *Q, int64_t nq, int k, int mode, double 
                           "here, kdtree.rs:386)"};
     // Uncertified queries go to the next, tighter filter: BF16 -> f64 GEMM
     // form -> exact folds.
-    const int next_mode = bf ? MODE_MFMA64 : MODE_EXACT;
+    const int next_mode = bf ? (h->f64_ok && k <= 24 && f64_d4(d) > 0 ? MODE_F64 : MODE_MFMA64)
+                             : MODE_EXACT;
     if (depth == 0) acc.n_fb += nflag;
     const std::string fp = "fb" + std::to_string(depth) + "_";
     double *Qf = ws.get<double>(fp + "q", (int64_t)nflag * d);
@@ -267,33 +316,38 @@ void search_device(fl_knn *h, const void *dQ, int32_t dtype, int64_t nq, int k, 
                           ? as_f64<float>((const float *)dQ, nq * h->d, h->ws, "q64", h->stream)
                           : (const double *)dQ;
     int mode = filter_mode(h->precision, h->d, k, h->n);
-    if (mode == MODE_BF16 || mode == MODE_BF16X3) {
-        // The BF16 bound needs finite inputs below 2^56 in magnitude.
-        unsigned int *flag = h->ws.get<unsigned int>("unsafe", 1);
-        if (h->bf_db_unsafe < 0) {
+    if (mode != MODE_EXACT) {
+        // Range of the data: the BF16 bound needs finite values below 2^56 in
+        // magnitude, the f64 filter below 2^250; other data takes the next
+        // form (f64 GEMM form with its own incompleteness flag, or exact).
+        unsigned int *flag = h->ws.get<unsigned int>("range", 1);
+        if (h->db_range < 0) {
             FL_HIP(hipMemsetAsync(flag, 0, 4, h->stream));
             if (h->dtype == FL_F32)
-                hipLaunchKernelGGL(bf16_unsafe_kernel<float>, dim3(1024), dim3(256), 0, h->stream,
+                hipLaunchKernelGGL(range_flags_kernel<float>, dim3(1024), dim3(256), 0, h->stream,
                                    (const float *)h->X.ptr(), h->n * h->d, flag);
             else
-                hipLaunchKernelGGL(bf16_unsafe_kernel<double>, dim3(1024), dim3(256), 0,
+                hipLaunchKernelGGL(range_flags_kernel<double>, dim3(1024), dim3(256), 0,
                                    h->stream, (const double *)h->X.ptr(), h->n * h->d, flag);
             check_launch();
             unsigned int hf = 0;
             FL_HIP(hipMemcpyAsync(&hf, flag, 4, hipMemcpyDeviceToHost, h->stream));
             FL_HIP(hipStreamSynchronize(h->stream));
-            h->bf_db_unsafe = hf ? 1 : 0;
+            h->db_range = (int)hf;
         }
         unsigned int hq = 0;
-        if (!h->bf_db_unsafe && nq > 0) {
+        if (nq > 0) {
             FL_HIP(hipMemsetAsync(flag, 0, 4, h->stream));
-            hipLaunchKernelGGL(bf16_unsafe_kernel<double>, dim3(256), dim3(256), 0, h->stream, Q,
+            hipLaunchKernelGGL(range_flags_kernel<double>, dim3(256), dim3(256), 0, h->stream, Q,
                                nq * h->d, flag);
             check_launch();
             FL_HIP(hipMemcpyAsync(&hq, flag, 4, hipMemcpyDeviceToHost, h->stream));
             FL_HIP(hipStreamSynchronize(h->stream));
         }
-        if (h->bf_db_unsafe || hq) mode = MODE_MFMA64;
+        const unsigned int fl = (unsigned int)h->db_range | hq;
+        h->f64_ok = (fl & 2u) == 0 && h->n < ((int64_t)1 << 31) - 64;
+        if ((mode == MODE_BF16 || mode == MODE_BF16X3) && (fl & 1u)) mode = MODE_MFMA64;
+        if (mode == MODE_MFMA64 && h->f64_ok && k <= 24 && f64_d4(h->d) > 0) mode = MODE_F64;
     }
     if (h->dtype == FL_F32)
         knn_search<float>(h, Q, nq, k, mode, dist, idx, acc, 0);
