@@ -34,6 +34,8 @@
 #include "common.hpp"
 #include "dist_topm.hpp"
 
+#include <type_traits>
+
 namespace fl {
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
@@ -133,6 +135,19 @@ struct Bf16Args {
     const double *qoff;              // resident (1-b)|q|^2 - c|q|Smax - eta
     const float *xnp;                // streamed (1-b)|x|^2
     Cand out;
+    // Argmin form with the decision fused (KMeans assignment, dec_X set):
+    // the kernel folds the smallest-key candidate exactly against the
+    // samples dec_X ([nA][d], f32 if dec_x_f32 else f64) and the f64 centres
+    // dec_C, certifies it, and writes labels / best, or appends the row to
+    // flags (count in *fcount) for the exact scan.
+    const void *dec_X = nullptr;
+    int dec_x_f32 = 0;
+    int d = 0;
+    const double *dec_C = nullptr;
+    int32_t *labels = nullptr;
+    double *best = nullptr;
+    int32_t *flags = nullptr;
+    int32_t *fcount = nullptr;
 };
 
 // Raw buffer resource over [base, base + bytes): loads at offsets past the
@@ -252,7 +267,8 @@ __device__ __forceinline__ float shared_threshold(const float (&lk)[L]) {
 // smallest pairs of the union are output with that theta; pairs of the
 // union that are not output are >= its MP-th key.
 template <int MP, int NKC, int PASSES>
-__global__ __launch_bounds__(512) void dist_bf16_kernel(Bf16Args args) {
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu((MP == 0 && NKC <= 2) ? 4 : 1)))
+void dist_bf16_kernel(Bf16Args args) {
     // The body is compiled for the device only: the host pass needs just the
     // launch stub, and it does not accept the buffer-to-LDS builtin with
     // per-lane offsets.
@@ -472,10 +488,15 @@ __global__ __launch_bounds__(512) void dist_bf16_kernel(Bf16Args args) {
         }
     };
 
-    // Argmin form: best pair and smallest dropped key per lane and row.  A
-    // lane sees its pairs in increasing index order, so strict < keeps the
-    // lowest index among equal keys.  nvalid = 0 updates nothing.
-    auto argmin_update = [&](const float4_t (&acc)[4][2], int b, int nvalid, int64_t b0) {
+    // Argmin form: per lane and row the smallest key bk (with its index bi)
+    // and the second smallest sec, which is the smallest key dropped so far:
+    // bk' = min(bk, k), sec' = med3(bk, sec, k).  A lane sees its pairs in
+    // increasing index order, so strict < keeps the lowest index among equal
+    // keys (and an equal key becomes sec).  MASK: the last tile, whose rows
+    // past the split end get +inf.
+    auto argmin_update = [&](const float4_t (&acc)[4][2], int b, int nvalid, int64_t b0,
+                             auto mask_tag) {
+        constexpr bool MASK = decltype(mask_tag)::value;
         float4_t xr[4];
         load_xr(xr, b);
         const int32_t rb = (int32_t)b0 + 4 * g;
@@ -486,13 +507,15 @@ __global__ __launch_bounds__(512) void dist_bf16_kernel(Bf16Args args) {
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     float kv = __builtin_fmaf(-2.f, acc[it][jt][v], xr[it][v]);
-                    kv = 16 * it + 4 * g + v < nvalid ? kv : __builtin_huge_valf();
+                    if (MASK) kv = 16 * it + 4 * g + v < nvalid ? kv : __builtin_huge_valf();
                     const bool lt = kv < bk[jt];
-                    sec[jt] = fminf(sec[jt], lt ? bk[jt] : kv);
-                    bk[jt] = lt ? kv : bk[jt];
+                    sec[jt] = __builtin_amdgcn_fmed3f(bk[jt], sec[jt], kv);
+                    bk[jt] = fminf(bk[jt], kv);
                     bi[jt] = lt ? rb + 16 * it + v : bi[jt];
                 }
     };
+    using NoMask = std::integral_constant<bool, false>;
+    using Mask = std::integral_constant<bool, true>;
 
     float4_t accA[4][2], accB[4][2];
     // One step: products of tile t into cur_acc while tile t-1 (prev_acc,
@@ -501,7 +524,8 @@ __global__ __launch_bounds__(512) void dist_bf16_kernel(Bf16Args args) {
     // since the last block barrier), its xn' terms only after this wave's
     // own reads of that slot.
     auto step = [&](int64_t t, int buf, float4_t (&cur_acc)[4][2],
-                    const float4_t (&prev_acc)[4][2]) {
+                    const float4_t (&prev_acc)[4][2], auto first_tag) {
+        constexpr bool FIRST = decltype(first_tag)::value;   // t == 0: no tile t-1
         const int64_t b0 = bbeg + t * BF_T;
         const int pbuf = buf == 0 ? TL::NST - 1 : buf - 1;
         const bool more = t + TL::NST - 1 < ntiles;
@@ -510,19 +534,9 @@ __global__ __launch_bounds__(512) void dist_bf16_kernel(Bf16Args args) {
         // Tile t-1 is never the last tile here.  At t = 0 there is no tile
         // t-1: the filter runs on unset sums and its result is discarded (no
         // branch, so that filter and products form one scheduling region).
-        if constexpr (ARGMIN) {
-            argmin_update(prev_acc, pbuf, t > 0 ? BF_T : 0, b0 - BF_T);
-            products(cur_acc, buf);
-            // Interleave the updates (about seven VALU per pair) with the
-            // matrix products.
-#pragma unroll
-            for (int i = 0; i < 96; ++i) {
-                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-                __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);
-            }
-        } else {
+        {
             bool gany[2][4];
-            const bool any = filter(prev_acc, pbuf, BF_T, gany) && t > 0;
+            const bool any = filter(prev_acc, pbuf, BF_T, gany) && !FIRST;
             products(cur_acc, buf);
             // Interleave the filter's compares with the matrix products:
             // three VALU instructions after every three MFMAs.
@@ -545,28 +559,55 @@ __global__ __launch_bounds__(512) void dist_bf16_kernel(Bf16Args args) {
 
     int buf = 0;
     int64_t t = 0;
-    for (; t + 1 < ntiles; t += 2) {
-        step(t, buf, accA, accB);
+    if constexpr (ARGMIN) {
+        // Argmin form: the update is branch-free VALU work that the other
+        // waves' products cover, so tiles are simply taken one at a time.
+        for (; t < ntiles; ++t) {
+            const int64_t b0 = bbeg + t * BF_T;
+            const int nb = TL::NST == 3 ? (buf == 0 ? TL::NST - 1 : buf - 1) : (buf ^ 1);
+            if (t + TL::NST - 1 < ntiles) {
+                issue_data(nb, b0 + (int64_t)(TL::NST - 1) * BF_T);
+                issue_xn(nb, b0 + (int64_t)(TL::NST - 1) * BF_T);
+            }
+            products(accA, buf);
+            const int nvalid = (int)(bend - b0 < BF_T ? bend - b0 : BF_T);
+            if (nvalid < BF_T)
+                argmin_update(accA, buf, nvalid, b0, Mask{});
+            else
+                argmin_update(accA, buf, BF_T, b0, NoMask{});
+            if (t + 1 < ntiles) {
+                if (TL::NST == 3 && t + 2 < ntiles)
+                    __builtin_amdgcn_s_waitcnt(0x0F70 | (TL::OPS & 15) | ((TL::OPS >> 4) << 14));
+                else
+                    wait_vm_loads();
+            }
+            block_sync_lds();
+            buf = buf + 1 == TL::NST ? 0 : buf + 1;
+        }
+    } else if (ntiles > 0) {
+        step(0, buf, accA, accB, Mask{});
         buf = buf + 1 == TL::NST ? 0 : buf + 1;
-        step(t + 1, buf, accB, accA);
-        buf = buf + 1 == TL::NST ? 0 : buf + 1;
+        t = 1;
     }
-    if (t < ntiles) {
-        step(t, buf, accA, accB);
-        buf = buf + 1 == TL::NST ? 0 : buf + 1;
-        t += 1;
+    if constexpr (!ARGMIN) {
+        for (; t + 1 < ntiles; t += 2) {
+            step(t, buf, accB, accA, NoMask{});
+            buf = buf + 1 == TL::NST ? 0 : buf + 1;
+            step(t + 1, buf, accA, accB, NoMask{});
+            buf = buf + 1 == TL::NST ? 0 : buf + 1;
+        }
+        if (t < ntiles) {
+            step(t, buf, accB, accA, NoMask{});
+            buf = buf + 1 == TL::NST ? 0 : buf + 1;
+            t += 1;
+        }
     }
     // The last tile's sums are in accA when ntiles is odd, else in accB.
-    if (ntiles > 0) {
+    if (!ARGMIN && ntiles > 0) {
         const int64_t bl0 = bbeg + (ntiles - 1) * BF_T;
         const int nvalid = (int)(bend - bl0);
         const int lbuf = buf == 0 ? TL::NST - 1 : buf - 1;   // buffer of the last tile
-        if constexpr (ARGMIN) {
-            if (ntiles & 1)
-                argmin_update(accA, lbuf, nvalid, bl0);
-            else
-                argmin_update(accB, lbuf, nvalid, bl0);
-        } else {
+        {
             bool gany[2][4];
             if (ntiles & 1) {
                 if (__ballot(filter(accA, lbuf, nvalid, gany)) != 0)
@@ -579,6 +620,47 @@ __global__ __launch_bounds__(512) void dist_bf16_kernel(Bf16Args args) {
     }
 
     if constexpr (ARGMIN) {
+        if (args.dec_X) {
+            // Decision in place: the smallest (key, index) of the four lanes
+            // is folded exactly; every other centre has a key >= the
+            // smallest of the other lanes' bests and every lane's dropped
+            // keys (theta), so theta > fold certifies the label (an equal
+            // key fails it and the row takes the exact scan).
+#pragma unroll
+            for (int jt = 0; jt < 2; ++jt) {
+                const int64_t q = q0 + 16 * jt + c16;
+                const float kmin = xred<false, 32>(xred<false, 16>(bk[jt]));
+                // indices < 2^24 (k <= 8192) are exact in f32
+                const float icand = bk[jt] == kmin ? (float)bi[jt] : 0x1.0p30f;
+                const float imf = xred<false, 32>(xred<false, 16>(icand));
+                const int32_t imin = imf >= 0x1.0p30f ? -1 : (int32_t)imf;
+                const bool mine = bk[jt] == kmin && bi[jt] == imin;
+                const float other = mine ? sec[jt] : fminf(sec[jt], bk[jt]);
+                const float thk = xred<false, 32>(xred<false, 16>(other));
+                if (q >= nA || g != jt) continue;
+                const int d = args.d;
+                bool ok = imin >= 0;
+                double bf = 0.0;
+                if (ok) {
+                    bf = args.dec_x_f32
+                             ? fold_exact((const float *)args.dec_X + q * d,
+                                          args.dec_C + (int64_t)imin * d, d)
+                             : fold_exact((const double *)args.dec_X + q * d,
+                                          args.dec_C + (int64_t)imin * d, d);
+                    if (thk != __builtin_huge_valf()) {
+                        const double kf = (double)thk, qo = args.qoff[q];
+                        ok = (kf + qo) - (fabs(kf) + fabs(qo)) * 0x1.0p-22 > bf;
+                    }
+                }
+                if (ok) {
+                    args.labels[q] = imin;
+                    args.best[q] = bf;
+                } else {
+                    args.flags[atomicAdd(args.fcount, 1)] = (int32_t)q;
+                }
+            }
+            return;
+        }
         // Four candidates per resident row (one per lane, valid ones first)
         // and theta = the smallest key any lane dropped.
 #pragma unroll

@@ -145,6 +145,7 @@ class Lloyd {
         cand.theta = ws_.get<double>("cth", n_);
         cand.theta_idx = ws_.get<int32_t>("cthi", n_);
         int t0;
+        bool fused = false;   // the filter kernel also took the exact decision
         if (passes_ > 0) {
             // Centre split, centre terms and the largest centre norm change
             // every iteration; the samples' split was made once.
@@ -175,9 +176,19 @@ class Lloyd {
             a.qoff = qoff;
             a.xnp = cxnp;
             a.out = cand;
+            a.dec_X = X_;
+            a.dec_x_f32 = sizeof(TX) == 4;
+            a.d = d_;
+            a.dec_C = C;
+            a.labels = labels;
+            a.best = best;
+            a.flags = ws_.get<int32_t>("flags", n_);
+            a.fcount = ws_.get<int32_t>("fcnt", 1);
+            FL_HIP(hipMemsetAsync(a.fcount, 0, sizeof(int32_t), st_));
             t0 = tm.begin();
-            launch_bf16(passes_, 0, nkc_, a, 1, st_);   // argmin form: 4 candidates per row
+            launch_bf16(passes_, 0, nkc_, a, 1, st_);   // argmin form, decision fused
             tm.end(t0);
+            fused = true;
         } else if (d4_ > 0) {
             // f64 filter: centre terms, padding and the largest centre norm
             // change every iteration.
@@ -235,18 +246,21 @@ class Lloyd {
         acc.n_launch += 1;
         int32_t *flags = ws_.get<int32_t>("flags", n_);
         int32_t *fcnt = ws_.get<int32_t>("fcnt", 1);
-        FL_HIP(hipMemsetAsync(fcnt, 0, sizeof(int32_t), st_));
-        const int t1 = tm.begin();
-        hipLaunchKernelGGL(km_finalize_kernel<TX>, dim3(grid1(n_, 256)), dim3(256), 0, st_, X_, C,
-                           n_, d_, k_, kKmM, cand, labels, best, flags, fcnt);
-        check_launch();
-        tm.end(t1);
+        int t1 = -1;
+        if (!fused) {
+            FL_HIP(hipMemsetAsync(fcnt, 0, sizeof(int32_t), st_));
+            t1 = tm.begin();
+            hipLaunchKernelGGL(km_finalize_kernel<TX>, dim3(grid1(n_, 256)), dim3(256), 0, st_, X_,
+                               C, n_, d_, k_, kKmM, cand, labels, best, flags, fcnt);
+            check_launch();
+            tm.end(t1);
+        }
         int32_t nflag = 0;
         FL_HIP(hipMemcpyAsync(&nflag, fcnt, sizeof(int32_t), hipMemcpyDeviceToHost, st_));
         FL_HIP(hipStreamSynchronize(st_));
         acc.ms_dist += tm.ms(t0);
-        acc.ms_fin += tm.ms(t1);
-        acc.n_cand += n_ * (int64_t)(k_ < kKmM ? k_ : kKmM);
+        if (t1 >= 0) acc.ms_fin += tm.ms(t1);
+        acc.n_cand += n_ * (int64_t)(fused ? 1 : (k_ < kKmM ? k_ : kKmM));
         if (nflag > 0) {
             acc.n_fb += nflag;
             const int t2 = tm.begin();
@@ -263,7 +277,7 @@ class Lloyd {
         const int64_t nb = ceil_div(n_, SUM_BLK);
         double *part = ws_.get<double>("part", nb);
         hipLaunchKernelGGL(block_sum_kernel, dim3((unsigned)nb), dim3(256), 0, st_, best, n_, part);
-        hipLaunchKernelGGL(seq_sum_kernel, dim3(1), dim3(64), 0, st_, part, nb, out);
+        hipLaunchKernelGGL(runs_sum_kernel, dim3(1), dim3(64), 0, st_, part, nb, out);
         check_launch();
     }
 
@@ -287,7 +301,7 @@ class Lloyd {
                            labels, n_, k_, blockcnt);
         hipLaunchKernelGGL(km_col_scan_kernel, dim3((unsigned)k_), dim3(256), 0, st_, blockcnt,
                            nblk, k_, blockbase, count);
-        hipLaunchKernelGGL(km_offsets_kernel, dim3(1), dim3(64), 0, st_, count, k_, offset);
+        hipLaunchKernelGGL(km_offsets_kernel, dim3(1), dim3(256), 0, st_, count, k_, offset);
         hipLaunchKernelGGL(km_scatter_kernel, dim3((unsigned)nblk), dim3(64), ctr_b, st_, labels,
                            n_, k_, blockbase, offset, order);
         hipLaunchKernelGGL(km_sum_kernel<TX>, dim3((unsigned)k_, (unsigned)ceil_div(d_, 64)),

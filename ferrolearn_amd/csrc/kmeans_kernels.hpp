@@ -236,12 +236,30 @@ static __global__ __attribute__((unused)) __launch_bounds__(256) void block_sum_
     if (threadIdx.x == 0) part[blockIdx.x] = s[0];
 }
 
+// Sequential fold of m values (one lane): the second level of k-means++'s
+// two-level sums, which the oracle restates in the same order.
 static __global__ __attribute__((unused)) void seq_sum_kernel(const double *__restrict__ part, int64_t m,
                                double *__restrict__ out) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     double acc = 0.0;
     for (int64_t i = 0; i < m; ++i) acc = acc + part[i];
     *out = acc;
+}
+
+// Second level of the inertia: lane l folds the l-th of 64 contiguous runs
+// of block sums, then lane 0 folds the 64 run sums in order (a fixed order;
+// launched with one wave).
+static __global__ __attribute__((unused)) void runs_sum_kernel(const double *__restrict__ part,
+                                                                int64_t m, double *__restrict__ out) {
+    const int lane = threadIdx.x;
+    if (blockIdx.x != 0 || lane >= 64) return;
+    const int64_t per = (m + 63) / 64;
+    const int64_t lo = lane * per, hi = lo + per < m ? lo + per : m;
+    double acc = 0.0;
+    for (int64_t i = lo; i < hi; ++i) acc = acc + part[i];
+    double tot = 0.0;
+    for (int l = 0; l < 64; ++l) tot = tot + __shfl(acc, l);
+    if (lane == 0) *out = tot;
 }
 
 // ---------------------------------------------------------- centroid update --
@@ -297,11 +315,25 @@ static __global__ __attribute__((unused)) __launch_bounds__(256) void km_col_sca
     if (t == 255) count[c] = part[255];
 }
 
-static __global__ __attribute__((unused)) void km_offsets_kernel(const int32_t *__restrict__ count, int64_t k,
-                                  int64_t *__restrict__ offset) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    int64_t run = 0;
-    for (int64_t c = 0; c < k; ++c) {
+// Exclusive scan of the cluster counts (one 256-thread block).
+static __global__ __attribute__((unused)) __launch_bounds__(256) void km_offsets_kernel(
+    const int32_t *__restrict__ count, int64_t k, int64_t *__restrict__ offset) {
+    __shared__ int64_t part[256];
+    const int t = threadIdx.x;
+    const int64_t per = (k + 255) / 256;
+    const int64_t lo = t * per, hi = lo + per < k ? lo + per : k;
+    int64_t s = 0;
+    for (int64_t c = lo; c < hi; ++c) s += count[c];
+    part[t] = s;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+        const int64_t v = t >= o ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int64_t run = t > 0 ? part[t - 1] : 0;
+    for (int64_t c = lo; c < hi; ++c) {
         offset[c] = run;
         run += count[c];
     }
@@ -347,8 +379,8 @@ static __global__ __attribute__((unused)) __launch_bounds__(64) void km_scatter_
 
 // sum[c][j] = fold over the cluster's rows in ascending index order; then
 // divide by the count or keep the old centre for an empty cluster.  The
-// member indices are staged through LDS 256 at a time and the values loaded
-// 32 at a time, so that the loads of a batch overlap.
+// member indices are staged through LDS 512 at a time and the values loaded
+// 64 at a time, so that the loads of a batch overlap.
 template <typename TX>
 __global__ __launch_bounds__(64) void km_sum_kernel(const TX *__restrict__ X, int d,
                                                     const int32_t *__restrict__ order,
@@ -356,21 +388,21 @@ __global__ __launch_bounds__(64) void km_sum_kernel(const TX *__restrict__ X, in
                                                     const int32_t *__restrict__ count,
                                                     const double *__restrict__ oldc,
                                                     double *__restrict__ newc) {
-    __shared__ int32_t ord[256];
+    __shared__ int32_t ord[512];
     const int64_t c = blockIdx.x;
     const int j = blockIdx.y * 64 + threadIdx.x;
     const bool col = j < d;
     const int64_t off = offset[c];
     const int32_t cnt = count[c];
     double acc = 0.0;
-    for (int32_t m0 = 0; m0 < cnt; m0 += 256) {
-        const int nb = cnt - m0 < 256 ? cnt - m0 : 256;
+    for (int32_t m0 = 0; m0 < cnt; m0 += 512) {
+        const int nb = cnt - m0 < 512 ? cnt - m0 : 512;
         wave_sync_lds();
         for (int u = threadIdx.x; u < nb; u += 64) ord[u] = order[off + m0 + u];
         wave_sync_lds();
         if (!col) continue;
         int m = 0;
-        constexpr int U = 32;
+        constexpr int U = 64;
         for (; m + U <= nb; m += U) {
             double v[U];
 #pragma unroll
