@@ -33,6 +33,17 @@ sys.path.insert(0, ROOT)
 
 PEAK = {"f64": 78.6e12, "bf16": 2.5e15, "hbm": 8.0e12}   # MI355X spec: FLOP/s, B/s
 
+# HBM bytes per launch of the filter kernel at these workloads, from two
+# rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch),
+# each scaled by what the counter reads for a plain 512 MiB device copy
+# (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).
+TRAFFIC = {
+    ("knn", "bf16x3"): (5.19e6 * 2.0 + 6.74e4) * 1024,
+    ("knn", "fp64"): (8.51e6 * 2.0 + 6.17e4) * 1024,
+    ("kmeans", "bf16x3"): (2.87e6 * 2.0 + 5.95e5) * 1024,
+    ("kmeans", "fp64"): (1.29e6 * 2.0 + 6.25e5) * 1024,
+}
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -148,8 +159,11 @@ def run_knn(args):
                    "db_dtype": "float32", "form": head},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12,
                      "peak": PEAK[peak_key] / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK[peak_key], "traffic": None,
-                     "kernel": "dist_bf16_kernel" if peak_key == "bf16" else "dist_topm_kernel",
+                     "frac": achieved / PEAK[peak_key],
+                     "traffic": TRAFFIC.get(("knn", head)) if Q == 100_000 else None,
+                     "traffic_unit": "bytes per launch (HBM, rocprofv3 counters)",
+                     "algorithmic_bytes_per_launch": N * d * 4 + Q * (d * 4 + k * 12),
+                     "kernel": "dist_bf16_kernel<16,4,3>" if peak_key == "bf16" else "dist_f64_kernel<16,32,double,float>",
                      "algorithmic_flops_per_launch": flops,
                      "kernel_ms": h["ms_distance_kernel"]},
         "forms": {f: {kk: vv for kk, vv in v.items() if kk not in ("labels", "idx")}
@@ -253,9 +267,13 @@ def run_kmeans(args):
     out["roofline"] = {"bound": "mfma", "achieved": flops / (h["ms_distance_kernel"] * 1e-3) / 1e12,
                        "peak": PEAK[peak_key] / 1e12, "unit": "TFLOP/s",
                        "frac": flops / (h["ms_distance_kernel"] * 1e-3) / PEAK[peak_key],
-                       "kernel": "dist_bf16_kernel<0,...> (argmin form)" if peak_key == "bf16"
-                       else "dist_topm_kernel", "kernel_ms": h["ms_distance_kernel"],
-                       "algorithmic_flops_per_launch": flops}
+                       "kernel": "dist_bf16_kernel<0,...> (argmin form, decision fused)"
+                       if peak_key == "bf16" else "dist_f64_kernel<0,...> (argmin form)",
+                       "kernel_ms": h["ms_distance_kernel"],
+                       "algorithmic_flops_per_launch": flops,
+                       "traffic": TRAFFIC.get(("kmeans", head)) if n == 10_000_000 else None,
+                       "traffic_unit": "bytes per launch (HBM, rocprofv3 counters)",
+                       "algorithmic_bytes_per_launch": n * (d * 4 + 4) + k * d * 8}
     if not args.no_cpu:
         log("kmeans: cpu baseline ...")
         # labels of the first iteration: one GPU iteration from the same centres

@@ -405,19 +405,20 @@ __global__ void to_f64_kernel(const T *__restrict__ X, int64_t count, double *__
 // certifies.  A split whose lists may miss rows reports theta = -inf.  A NaN
 // fold also leaves the query uncertified.  Uncertified queries are listed for
 // the next, tighter filter.
-constexpr int FIN_MAXC = 512;
+constexpr int FIN_MAXC = 512;        // candidates per query, usual launches
+constexpr int FIN_MAXC_BIG = 4096;   // a few queries over many splits (fallbacks)
 
-template <typename TB>
+template <typename TB, int CAP>
 __global__ __launch_bounds__(64) void knn_finalize_kernel(
     const double *__restrict__ Q, const TB *__restrict__ B, int d, int64_t nq, int64_t nB,
     int S, int M, int64_t b_per_split, Cand cand, int k, int certify,
     double *__restrict__ dist_out, int64_t *__restrict__ idx_out, int32_t *__restrict__ flag_list,
     int32_t *__restrict__ flag_count, unsigned long long *__restrict__ n_cand) {
-    __shared__ double ck[FIN_MAXC];
-    __shared__ double cf[FIN_MAXC];
-    __shared__ double cd[FIN_MAXC];
-    __shared__ int32_t ci[FIN_MAXC];
-    __shared__ int32_t rk[FIN_MAXC];
+    __shared__ double ck[CAP];
+    __shared__ double cf[CAP];
+    __shared__ double cd[CAP];
+    __shared__ int32_t ci[CAP];
+    __shared__ int32_t rk[CAP];
     __shared__ double kth_fold;
     __shared__ double next_key;
     __shared__ int kth_found;

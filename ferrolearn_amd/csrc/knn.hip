@@ -258,9 +258,14 @@ void knn_search(fl_knn *h, const double *Q, int64_t nq, int k, int mode, double 
     FL_HIP(hipMemsetAsync(fcnt, 0, sizeof(int32_t), st));
     FL_HIP(hipMemsetAsync(ncand, 0, sizeof(unsigned long long), st));
     const int t1 = tm.begin();
-    hipLaunchKernelGGL(knn_finalize_kernel<TB>, dim3((unsigned)nq), dim3(64), 0, st, Q, B, d, nq,
-                       nB, (int)S, M, bps, cand, k, mode == MODE_EXACT ? 0 : 1, dist, idx, flags,
-                       fcnt, ncand);
+    if (S * M > FIN_MAXC)
+        hipLaunchKernelGGL((knn_finalize_kernel<TB, FIN_MAXC_BIG>), dim3((unsigned)nq), dim3(64), 0,
+                           st, Q, B, d, nq, nB, (int)S, M, bps, cand, k,
+                           mode == MODE_EXACT ? 0 : 1, dist, idx, flags, fcnt, ncand);
+    else
+        hipLaunchKernelGGL((knn_finalize_kernel<TB, FIN_MAXC>), dim3((unsigned)nq), dim3(64), 0, st,
+                           Q, B, d, nq, nB, (int)S, M, bps, cand, k, mode == MODE_EXACT ? 0 : 1,
+                           dist, idx, flags, fcnt, ncand);
     check_launch();
     tm.end(t1);
     int32_t nflag = 0;
