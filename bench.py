@@ -154,7 +154,7 @@ def run_knn(args):
         "vs_baseline": None,
         "dtype": "f64" if peak_key == "f64" else "bf16",
         "data": "synthetic",
-        "config": {"workload": "knn_classify_bruteforce (configs[1])", "db_rows": N,
+        "config": {"workload": "knn_classify_bruteforce_1Mx128_q100k_k10", "db_rows": N,
                    "features": d, "queries": Q, "k": k, "weights": "uniform",
                    "db_dtype": "float32", "form": head},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12,
