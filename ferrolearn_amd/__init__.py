@@ -12,11 +12,12 @@ from .errors import (DeviceError, FerroError, InsufficientSamples, InvalidParame
                      NumericalInstability, ShapeMismatch)
 from .neighbors import (Algorithm, FittedKNeighborsClassifier, FittedKNeighborsRegressor,
                         KNeighborsClassifier, KNeighborsRegressor, Weights)
+from .router import CellRouter
 
 __all__ = [
     "KMeans", "MiniBatchKMeans", "FittedKMeans", "kmeans_plusplus", "KNeighborsClassifier",
     "KNeighborsRegressor", "FittedKNeighborsClassifier", "FittedKNeighborsRegressor",
     "Algorithm", "Weights", "FerroError", "ShapeMismatch", "InsufficientSamples",
     "InvalidParameter", "NumericalInstability", "DeviceError", "DeviceArray", "to_device",
-    "device",
+    "device", "CellRouter",
 ]

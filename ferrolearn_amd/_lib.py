@@ -76,6 +76,59 @@ class fl_minibatch_params(C.Structure):
     ]
 
 
+FL_ROUTE_NETWORK = 0
+FL_ROUTE_BOUNDS = 1
+ROUTES = {"network": FL_ROUTE_NETWORK, "bounds": FL_ROUTE_BOUNDS}
+
+
+class fl_router_params(C.Structure):
+    _fields_ = [
+        ("n_cells", C.c_int64),
+        ("cell_iter", C.c_int64),
+        ("hidden", C.c_int64),
+        ("seed", C.c_uint64),
+    ]
+
+
+class fl_router_train_params(C.Structure):
+    _fields_ = [
+        ("n_neighbors", C.c_int64),
+        ("epochs", C.c_int64),
+        ("batch_size", C.c_int64),
+        ("learning_rate", C.c_double),
+        ("alpha", C.c_double),
+        ("seed", C.c_uint64),
+        ("target", C.c_int32),
+    ]
+
+
+FL_TARGET_CERTIFY = 0
+FL_TARGET_NEIGHBORS = 1
+TARGETS = {"certify": FL_TARGET_CERTIFY, "neighbors": FL_TARGET_NEIGHBORS}
+
+
+class fl_router_report(C.Structure):
+    _fields_ = [
+        ("ms_total", C.c_double),
+        ("ms_cells", C.c_double),
+        ("ms_targets", C.c_double),
+        ("ms_train", C.c_double),
+        ("ms_route", C.c_double),
+        ("ms_bounds", C.c_double),
+        ("ms_scan", C.c_double),
+        ("ms_merge", C.c_double),
+        ("ms_fallback", C.c_double),
+        ("n_queries", C.c_int64),
+        ("n_fallback", C.c_int64),
+        ("rows_scanned", C.c_int64),
+        ("loss_first", C.c_double),
+        ("loss_last", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 _vp = C.c_void_p
 _i64 = C.c_int64
 _i32 = C.c_int32
@@ -118,6 +171,22 @@ SIGNATURES = {
     "fl_knn_classes": (C.c_int, [_vp, _vp]),
     "fl_knn_n_samples_fit": (_i64, [_vp]),
     "fl_knn_free": (None, [_vp]),
+    "fl_router_params_default": (None, [C.POINTER(fl_router_params), _i64]),
+    "fl_router_train_params_default": (None, [C.POINTER(fl_router_train_params)]),
+    "fl_router_build": (C.c_int, [_vp, C.POINTER(fl_router_params), C.POINTER(_vp),
+                                  C.POINTER(fl_router_report)]),
+    "fl_router_train_device": (C.c_int, [_vp, _vp, _i32, _i64, _i64,
+                                         C.POINTER(fl_router_train_params),
+                                         C.POINTER(fl_router_report)]),
+    "fl_router_kneighbors_device": (C.c_int, [_vp, _vp, _i32, _i64, _i64, _i64, _i64, _i32,
+                                              _vp, _vp, C.POINTER(fl_router_report)]),
+    "fl_router_n_params": (_i64, [_vp]),
+    "fl_router_get_params": (C.c_int, [_vp, _vp]),
+    "fl_router_set_params": (C.c_int, [_vp, _vp]),
+    "fl_router_cells": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "fl_router_mlp_gradients": (C.c_int, [_i64, _i64, _i64, _vp, _vp, _i64, _vp, _i64,
+                                          C.c_double, _dp, _vp]),
+    "fl_router_free": (None, [_vp]),
     "fl_backend_gemm_f64": (C.c_int, [_vp, _i64, _i64, _vp, _i64, _i64, _vp]),
     "fl_device_malloc": (C.c_int, [C.POINTER(_vp), C.c_uint64]),
     "fl_device_free": (C.c_int, [_vp]),

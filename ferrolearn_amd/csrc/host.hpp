@@ -129,6 +129,38 @@ inline Bound bound_bf16(int d, int passes) {
     return {2.0 * (2.0 * cdot + 3.0 * u32 + 2.0 * gf), 2.0 * (2.0 * u32 + 2.0 * gf)};
 }
 
+// The library's generator (SplitMix64 seeding xoshiro256**), identical to
+// the oracle's restatement; kmeans.rs:178,203,213 draw from rand's StdRng,
+// whose stream is not reproduced.
+struct Rng {
+    uint64_t s[4];
+    explicit Rng(uint64_t seed) {
+        uint64_t x = seed;
+        for (int i = 0; i < 4; ++i) {
+            uint64_t z = (x += 0x9E3779B97F4A7C15ULL);
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+            s[i] = z ^ (z >> 31);
+        }
+    }
+    static uint64_t rotl(uint64_t v, int k) { return (v << k) | (v >> (64 - k)); }
+    uint64_t next() {
+        const uint64_t out = rotl(s[1] * 5, 7) * 9;
+        const uint64_t t = s[1] << 17;
+        s[2] ^= s[0];
+        s[3] ^= s[1];
+        s[1] ^= s[2];
+        s[0] ^= s[3];
+        s[2] ^= t;
+        s[3] = rotl(s[3], 45);
+        return out;
+    }
+    double f64() { return (double)(next() >> 11) * 0x1.0p-53; }
+    int64_t range(int64_t n) {
+        return (int64_t)(((unsigned __int128)next() * (uint64_t)n) >> 64);
+    }
+};
+
 inline void check_launch() { FL_HIP(hipGetLastError()); }
 
 template <int MP, int NKC, int PASSES>

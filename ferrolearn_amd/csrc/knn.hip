@@ -730,3 +730,14 @@ int64_t fl_knn_n_samples_fit(fl_knn_handle h) { return h ? h->n : 0; }
 void fl_knn_free(fl_knn_handle h) { delete h; }
 
 }  // extern "C"
+
+namespace fl {
+// The fitted handle's device rows and stream, for the cell router built on it.
+void knn_describe(const fl_knn *h, const void **X, int32_t *dtype, int64_t *n, int64_t *d) {
+    *X = h->X.ptr();
+    *dtype = h->dtype;
+    *n = h->n;
+    *d = h->d;
+}
+int64_t knn_n_neighbors(const fl_knn *h) { return h->n_neighbors; }
+}  // namespace fl

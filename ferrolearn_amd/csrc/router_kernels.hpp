@@ -1,0 +1,634 @@
+// router_kernels.hpp -- kernels of the routed exact kNN search.
+//
+// The database is partitioned into cells (a coarse KMeans); each cell c has
+// a centre mu_c and a radius r_c >= max over its rows of |x - mu_c|.  A query
+// scans the rows of n_probe cells, chosen by the router network or by the
+// cells' triangle-inequality lower bounds, with the exact reference
+// distance sqrt(fold) (kdtree.rs:342-348) and the (distance, index) order of
+// brute_force_knn (kdtree.rs:364-391).  The answer is certified when every
+// unscanned cell's lower bound on the reference distance of its rows exceeds
+// the k-th distance found; other queries go to the brute-force search.
+#pragma once
+
+#include "common.hpp"
+
+namespace fl {
+
+constexpr int RT_Q = 64;   // lists (queries or query-cell pairs) per block
+constexpr int RT_R = 64;   // rows per tile
+constexpr int RT_F = 32;   // features per staged chunk
+constexpr int RT_FPAD = RT_R + 1;   // fold buffer row stride (doubles)
+constexpr int32_t RT_NOID = 0x7fffffff;
+
+// Insert (key, id) into an ascending (key, id) list of K entries held in
+// registers, dropping the last entry.  Caller has checked that the pair is
+// below the last entry.
+template <int K>
+__device__ __forceinline__ void rt_insert(double (&lk)[K], int32_t (&li)[K], double key,
+                                          int32_t id) {
+    bool placed = false;
+#pragma unroll
+    for (int s = K - 1; s > 0; --s) {
+        if (!placed) {
+            if (pair_less(key, id, lk[s - 1], li[s - 1])) {
+                lk[s] = lk[s - 1];
+                li[s] = li[s - 1];
+            } else {
+                lk[s] = key;
+                li[s] = id;
+                placed = true;
+            }
+        }
+    }
+    if (!placed) {
+        lk[0] = key;
+        li[0] = id;
+    }
+}
+
+// Lower bound on the reference distance sqrt(fold(q, x)) of every row x of
+// a cell, from rho = sqrt(fold(q, mu)) and r = computed radius:
+// |q - mu| >= rho (1 - e), |x - mu| <= r (1 + e) and sqrt(fold(q, x)) >=
+// |q - x| (1 - e) with e covering the fold's d+2 roundings and the sqrt's;
+// ep = e plus the roundings of this expression.  NaN gives -inf (no bound).
+__device__ __forceinline__ double rt_lower_bound(double rho, double r, double ep) {
+    const double a = rho * (1.0 - ep);
+    const double b = r * (1.0 + ep);
+    const double lb = (a - b) * (1.0 - ep);
+    return lb == lb ? lb : -__builtin_huge_val();
+}
+
+// Lower bound from the bisector of the query's nearest centre a and cell c
+// (cells are exact nearest-centre cells, so every row x of c has
+// fold(x, mu_c) <= fold(x, mu_a)).  In real arithmetic that puts x in the
+// half-space 2 x.(mu_a - mu_c) <= |mu_a|^2 - |mu_c|^2 + s, with s = 2.02 g
+// (|x - mu_c| + |mu_c - mu_a|)^2 covering the folds' roundings, so
+// |q - x| >= (|q - mu_c|^2 - |q - mu_a|^2 - s) / (2 |mu_c - mu_a|).  fc, fa:
+// computed folds of q with mu_c and mu_a; D: computed |mu_c - mu_a|; r:
+// computed radius of c; g bounds the relative error of a fold (d + 2
+// roundings) or of a distance, plus the roundings of this expression.
+__device__ __forceinline__ double rt_bisector_bound(double fc, double fa, double D, double r,
+                                                    double g) {
+    const double w = (r + D) * (1.0 + g);
+    const double s = 2.02 * g * w * w;
+    const double num = fc * (1.0 - g) - fa * (1.0 + g) - s;
+    const double lb = num / (2.0 * D * (1.0 + g)) * (1.0 - g);
+    return lb == lb ? lb : -__builtin_huge_val();
+}
+
+// Shared core of the two list kernels: 64 lists, each fed by the exact folds
+// of one query row against the rows [r_begin, r_end) of X.  Thread t folds
+// queries (t & 15) + 16i against rows (t >> 4) + 16j of each 64-row tile, in
+// 32-feature chunks staged as f64 in LDS (the fold stays a left fold in
+// feature order).  The tile's folds then go through LDS to the list owners:
+// thread t keeps list (t & 63) for rows 16 (t >> 6) .. +15 of every tile.
+// key(fold, row, &key, &id) maps a fold to its list key; thr(list_last_key)
+// gives a fold above which the key cannot enter the list.
+template <typename TQ, typename TX, int K, typename KeyFn, typename ThrFn>
+__device__ __forceinline__ void rt_fold_lists(const TQ *const *qrow, int d, const TX *__restrict__ X,
+                                              int64_t r_begin, int64_t r_end, double *lds,
+                                              double (&lk)[K], int32_t (&li)[K], KeyFn keyfn,
+                                              ThrFn thrfn) {
+    const int t = threadIdx.x;
+    const int qg = t & 15, rg = t >> 4;
+    const int own = t & 63, sub = t >> 6;
+    double *Qs = lds;                  // [RT_F][RT_Q]
+    double *Xt = lds + RT_F * RT_Q;    // [RT_F][RT_R]
+    double *F = lds;                   // [RT_Q][RT_FPAD], after the folds
+    double thr = thrfn(lk[K - 1]);
+    for (int64_t r0 = r_begin; r0 < r_end; r0 += RT_R) {
+        double acc[4][4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+        for (int f0 = 0; f0 < d; f0 += RT_F) {
+            {
+                const int q = t >> 2, fb = (t & 3) * 8;
+                const TQ *qp = qrow[q];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int f = f0 + fb + u;
+                    Qs[(fb + u) * RT_Q + q] = (qp && f < d) ? (double)qp[f] : 0.0;
+                }
+                const int64_t row = r0 + q;
+                const TX *xp = X + row * d;
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int f = f0 + fb + u;
+                    Xt[(fb + u) * RT_R + q] = (row < r_end && f < d) ? (double)xp[f] : 0.0;
+                }
+            }
+            __syncthreads();
+            const int fc = d - f0 < RT_F ? d - f0 : RT_F;
+            for (int f = 0; f < fc; ++f) {
+                double qv[4], xv[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) qv[i] = Qs[f * RT_Q + qg + 16 * i];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) xv[j] = Xt[f * RT_R + rg + 16 * j];
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const double df = qv[i] - xv[j];
+                        acc[i][j] = acc[i][j] + df * df;
+                    }
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) F[(qg + 16 * i) * RT_FPAD + rg + 16 * j] = acc[i][j];
+        __syncthreads();
+        for (int jj = 0; jj < 16; ++jj) {
+            const int r = sub * 16 + jj;
+            const int64_t row = r0 + r;
+            if (row >= r_end) break;
+            const double fold = F[own * RT_FPAD + r];
+            if (fold > thr) continue;   // NaN goes on to the key function
+            double key;
+            int32_t id;
+            keyfn(fold, row, key, id);
+            if (pair_less(key, id, lk[K - 1], li[K - 1])) {
+                rt_insert<K>(lk, li, key, id);
+                thr = thrfn(lk[K - 1]);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Merge the four sub-lists of each of the 64 lists (one per wave) and hand
+// the first kout entries of list `own` to emit(own, j, key, id), run by the
+// threads of wave 0.
+template <int K, typename Emit>
+__device__ __forceinline__ void rt_merge4(double *lds, const double (&lk)[K],
+                                          const int32_t (&li)[K], int kout, Emit emit) {
+    const int t = threadIdx.x;
+    const int own = t & 63, sub = t >> 6;
+    double *Lk = lds;                                        // [4][64][K]
+    int32_t *Li = reinterpret_cast<int32_t *>(lds + 4 * RT_Q * K);
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+        Lk[(sub * RT_Q + own) * K + s] = lk[s];
+        Li[(sub * RT_Q + own) * K + s] = li[s];
+    }
+    __syncthreads();
+    if (sub == 0) {
+        int pos[4] = {0, 0, 0, 0};
+        for (int j = 0; j < kout; ++j) {
+            int best = 0;
+            double bk = Lk[(0 * RT_Q + own) * K + pos[0]];
+            int32_t bi = Li[(0 * RT_Q + own) * K + pos[0]];
+#pragma unroll
+            for (int s = 1; s < 4; ++s) {
+                const double k2 = Lk[(s * RT_Q + own) * K + pos[s]];
+                const int32_t i2 = Li[(s * RT_Q + own) * K + pos[s]];
+                if (pair_less(k2, i2, bk, bi)) {
+                    best = s;
+                    bk = k2;
+                    bi = i2;
+                }
+            }
+            emit(own, j, bk, bi);
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+                if (s == best) ++pos[s];
+        }
+    }
+}
+
+inline size_t rt_lds_bytes(int K) {
+    const size_t folds = (size_t)RT_Q * RT_FPAD * 8;
+    const size_t merge = (size_t)4 * RT_Q * K * 12;
+    return folds > merge ? folds : merge;
+}
+
+// Scan: one block per work item (a cell and up to 64 of the query-cell pairs
+// routed to it).  Pair p = query * n_probe + slot; its list holds the k
+// smallest (sqrt(fold), global index) over the cell's rows, written to
+// out_key / out_id [p][k] (absent entries: +inf, RT_NOID).
+struct RtScanArgs {
+    const void *Q;                 // queries, nq x d (TQ)
+    const void *Xs;                // cell-sorted rows, n x d (TX)
+    const int32_t *row_id;         // global index of each sorted row
+    const int64_t *cell_first;     // first sorted row of each cell
+    const int32_t *cell_count;     // rows per cell
+    const int32_t *pair_order;     // pairs sorted by cell (stable)
+    const int64_t *pair_first;     // first sorted pair of each cell
+    const int32_t *pair_count;     // pairs per cell
+    const int32_t *work_cell;      // per block: cell
+    const int32_t *work_start;     // per block: offset into the cell's pairs
+    int d, n_probe, k;
+    double *out_key;
+    int32_t *out_id;
+};
+
+template <typename TQ, typename TX, int K>
+__global__ __launch_bounds__(256) void rt_scan_kernel(RtScanArgs a) {
+    extern __shared__ double lds[];
+    __shared__ const TQ *qrow[RT_Q];
+    __shared__ int32_t pid[RT_Q];
+    const int t = threadIdx.x;
+    const int c = a.work_cell[blockIdx.x];
+    const int s0 = a.work_start[blockIdx.x];
+    const int np = min(RT_Q, a.pair_count[c] - s0);
+    if (t < RT_Q) {
+        if (t < np) {
+            const int32_t p = a.pair_order[a.pair_first[c] + s0 + t];
+            pid[t] = p;
+            qrow[t] = (const TQ *)a.Q + (int64_t)(p / a.n_probe) * a.d;
+        } else {
+            pid[t] = -1;
+            qrow[t] = nullptr;
+        }
+    }
+    __syncthreads();
+    double lk[K];
+    int32_t li[K];
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+        lk[s] = __builtin_huge_val();
+        li[s] = RT_NOID;
+    }
+    const int64_t rb = a.cell_first[c], re = rb + a.cell_count[c];
+    const int32_t *rid = a.row_id;
+    rt_fold_lists<TQ, TX, K>(
+        qrow, a.d, (const TX *)a.Xs, rb, re, lds, lk, li,
+        [&](double fold, int64_t row, double &key, int32_t &id) {
+            key = sqrt(fold);
+            id = rid[row];
+        },
+        // A fold above last^2 (1 + 2^-48) has a correctly rounded sqrt above
+        // `last`, so it cannot enter the list.
+        [](double last) { return last * last * (1.0 + 0x1.0p-48); });
+    double *ok = a.out_key;
+    int32_t *oi = a.out_id;
+    const int kk = a.k;
+    rt_merge4<K>(lds, lk, li, kk, [&](int own, int j, double key, int32_t id) {
+        const int32_t p = pid[own];
+        if (p >= 0) {
+            ok[(int64_t)p * kk + j] = key;
+            oi[(int64_t)p * kk + j] = id;
+        }
+    });
+}
+
+// Bounds: one block per 64 queries; list = the kb smallest (lower bound,
+// cell) over all cells (empty cells: +inf), written to out_key / out_id
+// [q][kb].
+struct RtBoundsArgs {
+    const void *Q;
+    int64_t nq;
+    int d;
+    const double *centers;         // C x d
+    const double *radius;          // C (negative: empty cell)
+    const double *cdist;           // C x C centre distances
+    const double *near_fold;       // per query: fold with its nearest centre
+    const int32_t *near_cell;      // per query: its nearest centre
+    int C;
+    double ep;
+    int kb;
+    double *out_key;
+    int32_t *out_id;
+};
+
+// NEAREST: the list is the exact nearest centre (fold, cell) -- kb = 1, the
+// reference's argmin order.  Otherwise the key of a cell is the larger of
+// its ball bound and its bisector bound against the query's nearest centre.
+template <typename TQ, int K, bool NEAREST>
+__global__ __launch_bounds__(256) void rt_bounds_kernel(RtBoundsArgs a) {
+    extern __shared__ double lds[];
+    __shared__ const TQ *qrow[RT_Q];
+    const int t = threadIdx.x;
+    const int64_t q0 = (int64_t)blockIdx.x * RT_Q;
+    if (t < RT_Q) qrow[t] = q0 + t < a.nq ? (const TQ *)a.Q + (q0 + t) * a.d : nullptr;
+    __syncthreads();
+    double lk[K];
+    int32_t li[K];
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+        lk[s] = __builtin_huge_val();
+        li[s] = RT_NOID;
+    }
+    const double *rad = a.radius;
+    const double ep = a.ep;
+    const int64_t qo = q0 + (t & 63);
+    const int32_t near = (!NEAREST && qo < a.nq) ? a.near_cell[qo] : 0;
+    const double fnear = (!NEAREST && qo < a.nq) ? a.near_fold[qo] : 0.0;
+    const double *drow = NEAREST ? nullptr : a.cdist + (int64_t)near * a.C;
+    rt_fold_lists<TQ, double, K>(
+        qrow, a.d, a.centers, 0, a.C, lds, lk, li,
+        [&](double fold, int64_t c, double &key, int32_t &id) {
+            id = (int32_t)c;
+            if (NEAREST) {
+                key = fold;
+                return;
+            }
+            const double r = rad[c];
+            if (r < 0.0) {
+                key = __builtin_huge_val();
+                return;
+            }
+            key = rt_lower_bound(sqrt(fold), r, ep);
+            const double D = drow[c];
+            if (c != near && D > 0.0) {
+                const double v = rt_bisector_bound(fold, fnear, D, r, ep);
+                key = v > key ? v : key;
+            }
+        },
+        [](double) { return __builtin_huge_val(); });
+    double *ok = a.out_key;
+    int32_t *oi = a.out_id;
+    const int kb = a.kb;
+    const int64_t nq = a.nq;
+    rt_merge4<K>(lds, lk, li, kb, [&](int own, int j, double key, int32_t id) {
+        if (q0 + own < nq) {
+            ok[(q0 + own) * kb + j] = key;
+            oi[(q0 + own) * kb + j] = id;
+        }
+    });
+}
+
+// Router choice: the n_probe cells with the largest logits (ties: lower
+// cell), per row, one wave per row.  Each lane keeps its best M in
+// registers; the wave then pops the best head n_probe times.
+template <int M>
+__global__ __launch_bounds__(256) void rt_top_cells_kernel(const float *__restrict__ Z, int64_t ldz,
+                                                           int64_t rows, int C, int n_probe,
+                                                           int32_t *__restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= rows) return;
+    const float *z = Z + row * ldz;
+    // keys: -logit ascending, so the list code orders (key, cell) ascending
+    double lk[M];
+    int32_t li[M];
+#pragma unroll
+    for (int s = 0; s < M; ++s) {
+        lk[s] = __builtin_huge_val();
+        li[s] = RT_NOID;
+    }
+    for (int c = lane; c < C; c += 64) {
+        float v = z[c];
+        const double key = v == v ? -(double)v : __builtin_huge_val();
+        if (pair_less(key, c, lk[M - 1], li[M - 1])) rt_insert<M>(lk, li, key, c);
+    }
+    int head = 0;
+    for (int j = 0; j < n_probe; ++j) {
+        double hk = __builtin_huge_val();
+        int32_t hi = RT_NOID;
+#pragma unroll
+        for (int s = 0; s < M; ++s)
+            if (s == head) {
+                hk = lk[s];
+                hi = li[s];
+            }
+        double bk = hk;
+        int32_t bi = hi;
+        for (int o = 32; o > 0; o >>= 1) {
+            const double k2 = __shfl_xor(bk, o);
+            const int32_t i2 = __shfl_xor(bi, o);
+            if (pair_less(k2, i2, bk, bi)) {
+                bk = k2;
+                bi = i2;
+            }
+        }
+        if (hi == bi && hk == bk) ++head;
+        if (lane == 0) out[row * n_probe + j] = bi == RT_NOID ? 0 : bi;
+    }
+}
+
+// Merge a query's n_probe pair lists, then certify: the query is finite and
+// the smallest lower bound over cells it did not scan exceeds its k-th
+// distance.  Thread per query.  Uncertified queries get flag 1.
+template <typename TQ, int K>
+__global__ void rt_merge_certify_kernel(const double *__restrict__ pkey,
+                                        const int32_t *__restrict__ pid, int64_t nq, int n_probe,
+                                        int k, const int32_t *__restrict__ routes,
+                                        const double *__restrict__ bkey,
+                                        const int32_t *__restrict__ bid, int kb,
+                                        const TQ *__restrict__ Q, int d,
+                                        const int32_t *__restrict__ cell_count,
+                                        double *__restrict__ dist, int64_t *__restrict__ idx,
+                                        uint8_t *__restrict__ flag,
+                                        unsigned long long *__restrict__ rows_scanned) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    double lk[K];
+    int32_t li[K];
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+        lk[s] = __builtin_huge_val();
+        li[s] = RT_NOID;
+    }
+    unsigned long long scanned = 0;
+    for (int s = 0; s < n_probe; ++s) {
+        const int64_t p = q * n_probe + s;
+        scanned += (unsigned long long)cell_count[routes[p]];
+        for (int j = 0; j < k; ++j) {
+            const double key = pkey[p * k + j];
+            const int32_t id = pid[p * k + j];
+            if (!pair_less(key, id, lk[K - 1], li[K - 1])) break;
+            rt_insert<K>(lk, li, key, id);
+        }
+    }
+    double dk = __builtin_huge_val();
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+        if (j < k) {
+            dist[q * k + j] = lk[j];
+            idx[q * k + j] = li[j] == RT_NOID ? (int64_t)-1 : (int64_t)li[j];
+            if (j == k - 1) dk = lk[j];
+        }
+    // smallest lower bound over unscanned cells: the first entry of the
+    // (n_probe + 1)-smallest bound list that is not a routed cell
+    double lbx = __builtin_huge_val();
+    for (int j = 0; j < kb; ++j) {
+        const int32_t c = bid[q * kb + j];
+        bool routed = false;
+        for (int s = 0; s < n_probe; ++s) routed |= routes[q * n_probe + s] == c;
+        if (!routed) {
+            lbx = c == RT_NOID ? __builtin_huge_val() : bkey[q * kb + j];
+            break;
+        }
+    }
+    bool finite = true;
+    for (int j = 0; j < d; ++j) {
+        const double v = (double)Q[q * d + j];
+        finite &= v - v == 0.0;
+    }
+    flag[q] = (finite && lbx > dk) ? 0 : 1;
+    atomicAdd(rows_scanned, scanned);
+}
+
+// Cell radii: max over a cell's rows of fold(x, mu) as ordered f64 bits
+// (non-negative doubles order like their bit patterns).
+template <typename TX>
+__global__ void rt_radius_kernel(const TX *__restrict__ Xs, int64_t n, int d,
+                                 const int32_t *__restrict__ row_cell,
+                                 const double *__restrict__ centers,
+                                 unsigned long long *__restrict__ maxfold) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t c = row_cell[i];
+    const double f = fold_exact(Xs + i * d, centers + (int64_t)c * d, d);
+    atomicMax(maxfold + c, (unsigned long long)__double_as_longlong(f));
+}
+
+// radius[c] = sqrt(max fold), or -1 for an empty cell.
+__global__ void rt_radius_finish_kernel(const unsigned long long *__restrict__ maxfold,
+                                        const int32_t *__restrict__ count, int C,
+                                        double *__restrict__ radius) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    radius[c] = count[c] > 0 ? sqrt(__longlong_as_double((long long)maxfold[c])) : -1.0;
+}
+
+// Rows in cell order: Xs[pos] = X[order[pos]], and the cell of each sorted
+// row.
+template <typename TX>
+__global__ void rt_gather_sorted_kernel(const TX *__restrict__ X, int64_t n, int d,
+                                        const int32_t *__restrict__ order,
+                                        const int32_t *__restrict__ label,
+                                        TX *__restrict__ Xs, int32_t *__restrict__ row_cell) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * d) return;
+    const int64_t pos = t / d;
+    const int j = (int)(t % d);
+    const int32_t r = order[pos];
+    Xs[t] = X[(int64_t)r * d + j];
+    if (j == 0) row_cell[pos] = label[r];
+}
+
+// int64 labels (KMeans output) -> int32.
+__global__ void rt_labels32_kernel(const int64_t *__restrict__ in, int64_t n,
+                                   int32_t *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (int32_t)in[i];
+}
+
+// Training targets: the cell of each of a query's k exact neighbours.
+__global__ void rt_target_cells_kernel(const int64_t *__restrict__ nbr, int64_t count,
+                                       const int32_t *__restrict__ label,
+                                       int32_t *__restrict__ cells) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) cells[i] = nbr[i] >= 0 ? label[nbr[i]] : -1;
+}
+
+// Work items of the scan: per cell, ceil(pairs / 64) blocks.  One 256-thread
+// block scans the per-cell block counts; total[0] = number of items.
+__global__ __launch_bounds__(256) void rt_work_scan_kernel(const int32_t *__restrict__ pair_count,
+                                                           int C, int32_t *__restrict__ first,
+                                                           int32_t *__restrict__ total) {
+    __shared__ int32_t part[256];
+    const int t = threadIdx.x;
+    const int per = (C + 255) / 256;
+    const int lo = t * per, hi = lo + per < C ? lo + per : C;
+    int32_t s = 0;
+    for (int c = lo; c < hi; ++c) s += (pair_count[c] + RT_Q - 1) / RT_Q;
+    part[t] = s;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+        const int32_t v = t >= o ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int32_t run = t > 0 ? part[t - 1] : 0;
+    for (int c = lo; c < hi; ++c) {
+        first[c] = run;
+        run += (pair_count[c] + RT_Q - 1) / RT_Q;
+    }
+    if (t == 255) total[0] = part[255];
+}
+
+__global__ void rt_work_fill_kernel(const int32_t *__restrict__ pair_count,
+                                    const int32_t *__restrict__ first, int C,
+                                    int32_t *__restrict__ work_cell,
+                                    int32_t *__restrict__ work_start) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    const int nb = (pair_count[c] + RT_Q - 1) / RT_Q;
+    for (int j = 0; j < nb; ++j) {
+        work_cell[first[c] + j] = c;
+        work_start[first[c] + j] = j * RT_Q;
+    }
+}
+
+// Rows of the queries that failed their certificate, for the brute force.
+template <typename TQ>
+__global__ void rt_gather_queries_kernel(const TQ *__restrict__ Q, int d,
+                                         const int32_t *__restrict__ which, int64_t m,
+                                         TQ *__restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m * d) return;
+    out[t] = Q[(int64_t)which[t / d] * d + t % d];
+}
+
+__global__ void rt_scatter_results_kernel(const double *__restrict__ fd,
+                                          const int64_t *__restrict__ fi,
+                                          const int32_t *__restrict__ which, int64_t m, int k,
+                                          double *__restrict__ dist, int64_t *__restrict__ idx) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m * k) return;
+    const int64_t q = which[t / k];
+    dist[q * k + t % k] = fd[t];
+    idx[q * k + t % k] = fi[t];
+}
+
+// Selected rows widened to f64 (initial cell centres).
+template <typename TX>
+__global__ void rt_rows_f64_kernel(const TX *__restrict__ X, int d, const int32_t *__restrict__ rows,
+                                   int64_t m, double *__restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m * d) return;
+    out[t] = (double)X[(int64_t)rows[t / d] * d + t % d];
+}
+
+// Bound-ordered routing: the n_probe cells with the smallest lower bounds.
+__global__ void rt_routes_from_bounds_kernel(const int32_t *__restrict__ bid, int kb, int64_t nq,
+                                             int n_probe, int32_t *__restrict__ routes) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nq * n_probe) return;
+    // every cell enters the bound list (NaN bounds become -inf), so with
+    // n_cells >= n_probe the first n_probe entries are cells; the clamp
+    // only keeps the counting sort's labels in range
+    const int32_t c = bid[(t / n_probe) * kb + t % n_probe];
+    routes[t] = c == RT_NOID ? 0 : c;
+}
+
+// Centre distances D[a][c] = sqrt(fold(mu_a, mu_c)).
+__global__ void rt_center_dist_kernel(const double *__restrict__ centers, int C, int d,
+                                      double *__restrict__ D) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)C * C) return;
+    const int64_t a = t / C, c = t % C;
+    D[t] = sqrt(fold_exact(centers + a * d, centers + c * d, d));
+}
+
+// Certification targets: the cells whose lower bound does not exceed the
+// query's exact k-th distance (from the T smallest bounds; -1 pads).  A
+// search that scans exactly these cells finds the exact answer and
+// certifies it; the cells of the k neighbours are among them.
+__global__ void rt_certify_targets_kernel(const double *__restrict__ kth, int64_t nq,
+                                          const double *__restrict__ bkey,
+                                          const int32_t *__restrict__ bid, int T,
+                                          int32_t *__restrict__ cells) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nq * T) return;
+    const int64_t q = t / T;
+    const int32_t c = bid[t];
+    cells[t] = (c != RT_NOID && bkey[t] <= kth[q]) ? c : -1;
+}
+
+// k-th distance of each query from its sorted kNN distances.
+__global__ void rt_kth_kernel(const double *__restrict__ dist, int64_t nq, int k,
+                              double *__restrict__ kth) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < nq) kth[q] = dist[q * k + k - 1];
+}
+
+}  // namespace fl

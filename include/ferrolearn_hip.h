@@ -272,6 +272,118 @@ int64_t fl_knn_n_samples_fit(fl_knn_handle h);
 void fl_knn_free(fl_knn_handle h);
 
 /* ------------------------------------------------------------------ */
+/* Cell router for exact kNN (no reference counterpart)                */
+/* ------------------------------------------------------------------ */
+
+/* A learned index over a fitted brute-force handle.  The database rows are
+ * partitioned into cells by a coarse KMeans (kmeans.rs:403-513 with
+ * initial centres drawn from the rows); each cell keeps its centre and a
+ * radius bounding its rows' distance to the centre.  A perceptron
+ * d -> hidden -> hidden -> n_cells (the layer, activation, loss and Adam
+ * rules of ferrolearn-neural/src/mlp.rs:217-409, 1326-1340) is trained on
+ * the cells that hold each training query's exact k nearest neighbours
+ * (brute_force_knn, kdtree.rs:364-391).  A routed search scans the rows of
+ * n_probe cells with the exact reference distance and certifies each query
+ * with the triangle inequality: every unscanned cell's lower bound must
+ * exceed the k-th distance found.  Other queries run the brute force, so
+ * results equal fl_knn_kneighbors on the same handle, bit for bit. */
+typedef struct fl_router *fl_router_handle;
+
+typedef enum fl_route {
+    FL_ROUTE_NETWORK = 0, /* cells with the largest router outputs          */
+    FL_ROUTE_BOUNDS = 1   /* cells with the smallest lower bounds (no network) */
+} fl_route;
+
+typedef struct fl_router_params {
+    int64_t n_cells;    /* coarse cells, 1..min(n, 8192)                     */
+    int64_t cell_iter;  /* Lloyd iterations of the coarse KMeans (>= 1)      */
+    int64_t hidden;     /* width of both hidden layers, multiple of 64, <= 1024 */
+    uint64_t seed;      /* initial centres and network weights               */
+} fl_router_params;
+
+typedef struct fl_router_train_params {
+    int64_t n_neighbors;  /* k of the exact targets, 1..64                  */
+    int64_t epochs;
+    int64_t batch_size;   /* 1..65536                                        */
+    double learning_rate; /* Adam step (beta1 0.9, beta2 0.999, eps 1e-8)    */
+    double alpha;         /* L2 penalty, grad_w += alpha * W                 */
+    uint64_t seed;        /* per-epoch shuffles                              */
+    int32_t target;       /* fl_router_target                                */
+} fl_router_train_params;
+
+typedef enum fl_router_target {
+    /* all cells whose lower bound does not exceed the exact k-th distance:
+     * the cells a search must scan to certify the exact answer (at most 32,
+     * smallest bounds first) */
+    FL_TARGET_CERTIFY = 0,
+    /* the cells holding the exact k nearest rows */
+    FL_TARGET_NEIGHBORS = 1
+} fl_router_target;
+
+typedef struct fl_router_report {
+    double ms_total;
+    double ms_cells;      /* build: coarse KMeans, sort, radii               */
+    double ms_targets;    /* train: exact kNN of the training queries        */
+    double ms_train;      /* train: epochs                                   */
+    double ms_route;      /* search: network inference and cell choice       */
+    double ms_bounds;     /* search: centre distances and lower bounds       */
+    double ms_scan;       /* search: exact scan of the chosen cells          */
+    double ms_merge;      /* search: merge and certificate                   */
+    double ms_fallback;   /* search: brute force for uncertified queries     */
+    int64_t n_queries;
+    int64_t n_fallback;   /* queries whose certificate failed                */
+    int64_t rows_scanned; /* database rows scanned by the routed search      */
+    double loss_first;    /* train: mean loss of the first and last epochs   */
+    double loss_last;
+} fl_router_report;
+
+void fl_router_params_default(fl_router_params *p, int64_t n_cells);
+void fl_router_train_params_default(fl_router_train_params *p);
+
+/* Build the cells over the rows of a fitted kNN handle (which must outlive
+ * the router).  The data must be finite with magnitudes <= 2^250.  The
+ * network starts from Xavier-uniform weights (mlp.rs:181-192) and zero
+ * biases. */
+fl_status fl_router_build(fl_knn_handle knn, const fl_router_params *params,
+                          fl_router_handle *out, fl_router_report *report);
+
+/* Train the network on nq device queries (dQ: nq*d of dtype). */
+fl_status fl_router_train_device(fl_router_handle r, const void *dQ,
+                                 int32_t dtype, int64_t nq, int64_t d,
+                                 const fl_router_train_params *params,
+                                 fl_router_report *report);
+
+/* Routed kneighbors: same contract and results as fl_knn_kneighbors_device
+ * (k == 0 means the handle's n_neighbors; 1 <= k <= 32 here), scanning
+ * n_probe cells (1..31, <= n_cells) chosen by `route` (fl_route). */
+fl_status fl_router_kneighbors_device(fl_router_handle r, const void *dQ,
+                                      int32_t dtype, int64_t nq, int64_t d,
+                                      int64_t k, int64_t n_probe, int32_t route,
+                                      double *d_dist_out, int64_t *d_idx_out,
+                                      fl_router_report *report);
+
+/* Network parameters in mlp.rs order: W1 (d x hidden), b1, W2 (hidden x
+ * hidden), b2, W3 (hidden x n_cells), b3, as f32. */
+int64_t fl_router_n_params(fl_router_handle r);
+fl_status fl_router_get_params(fl_router_handle r, float *params);
+fl_status fl_router_set_params(fl_router_handle r, const float *params);
+
+/* The cells: centres (n_cells x d), radii (negative: empty cell), row
+ * counts, and the cell of each database row. */
+fl_status fl_router_cells(fl_router_handle r, double *centers, double *radius,
+                          int64_t *counts, int64_t *row_cell);
+
+/* One forward and backward pass of the router network on a host batch
+ * (X: nb x d f32, cells: nb x k target cells), as used in training: mean
+ * loss and the gradients in fl_router_get_params order. */
+fl_status fl_router_mlp_gradients(int64_t d, int64_t hidden, int64_t n_cells,
+                                  const float *params, const float *X,
+                                  int64_t nb, const int32_t *cells, int64_t k,
+                                  double alpha, double *loss, float *grads);
+
+void fl_router_free(fl_router_handle r);
+
+/* ------------------------------------------------------------------ */
 /* Backend -- ferrolearn-core/src/backend.rs:37-123                    */
 /* ------------------------------------------------------------------ */
 

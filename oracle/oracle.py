@@ -293,3 +293,66 @@ def knn_regress(data, y, Q, k, weights=0, nthreads=0):
     pred = np.zeros(nq, np.float64)
     lib().or_knn_regress(_p(data), N, d, _p(y), _p(Q), nq, k, weights, _p(pred), nthreads)
     return pred
+
+
+# ---------------------------------------------------------------- router net --
+# The router network of the cell router, restated in numpy float64 from
+# ferrolearn-neural/src/mlp.rs: forward z = a @ W + b with ReLU hidden layers
+# (:217-261, :69-74), softmax rows (:106-127), loss -sum(y ln max(p, 1e-15)) /
+# batch with delta p - y (:1326-1340), grad_w = a^T delta / batch + alpha W,
+# grad_b = mean(delta), delta_prev = (delta @ W^T) * (z > 0) (:302-345).
+# The soft target y of a row is the fraction of its valid (>= 0) target cells
+# equal to each cell; a row without targets contributes nothing.  With emulate_bf16 the inputs, weights, hidden activations and
+# deltas are rounded to bfloat16 where the GPU stores them in bfloat16, so
+# that the comparison isolates the accumulation order.
+
+def bf16_round(a) -> np.ndarray:
+    """Round to the nearest bfloat16 (ties to even), returned as float64."""
+    a32 = np.ascontiguousarray(a, np.float32)
+    u = a32.view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def mlp_split(p, d, hidden, n_cells):
+    shapes = [(d, hidden), (hidden,), (hidden, hidden), (hidden,), (hidden, n_cells),
+              (n_cells,)]
+    out, o = [], 0
+    for s in shapes:
+        n = int(np.prod(s))
+        out.append(np.asarray(p[o:o + n], np.float64).reshape(s))
+        o += n
+    return out
+
+
+def mlp_loss_grad(params, X, cells, d, hidden, n_cells, alpha=0.0, emulate_bf16=False):
+    """(mean loss, flat gradient) of the router network on one batch."""
+    r = bf16_round if emulate_bf16 else (lambda a: np.asarray(a, np.float64))
+    W1, b1, W2, b2, W3, b3 = mlp_split(params, d, hidden, n_cells)
+    B, k = cells.shape
+    x = r(X)
+    w1, w2, w3 = r(W1), r(W2), r(W3)
+    a1 = r(np.maximum(x @ w1 + b1, 0.0))
+    a2 = r(np.maximum(a1 @ w2 + b2, 0.0))
+    z3 = a2 @ w3 + b3
+    z3 = z3 - z3.max(axis=1, keepdims=True)
+    e = np.exp(z3)
+    p = e / e.sum(axis=1, keepdims=True)
+    y = np.zeros_like(p)
+    for i in range(B):
+        valid = [c for c in cells[i] if c >= 0]
+        for c in valid:
+            y[i, c] += 1.0 / len(valid)
+    has = (y.sum(axis=1) > 0)[:, None]
+    loss = float(-(y * np.log(np.maximum(p, 1e-15))).sum() / B)
+    d3 = r(np.where(has, p - y, 0.0))
+    gW3 = a2.T @ d3 / B + alpha * W3
+    gb3 = d3.sum(axis=0) / B
+    d2 = r((d3 @ w3.T) * (a2 > 0))
+    gW2 = a1.T @ d2 / B + alpha * W2
+    gb2 = d2.sum(axis=0) / B
+    d1 = r((d2 @ w2.T) * (a1 > 0))
+    gW1 = x.T @ d1 / B + alpha * W1
+    gb1 = d1.sum(axis=0) / B
+    grad = np.concatenate([g.ravel() for g in (gW1, gb1, gW2, gb2, gW3, gb3)])
+    return loss, grad

@@ -1,0 +1,172 @@
+"""Cell router parity on the GPU.
+
+Every routed answer must equal the brute force's (distances and indices bit
+for bit, (distance, index) order), whatever the route, n_probe or training
+state; the network's gradients must match the numpy restatement of mlp.rs
+(oracle.mlp_loss_grad with the same bfloat16 roundings); and a trained
+network must certify more queries than the untrained one."""
+import numpy as np
+import pytest
+
+import ferrolearn_amd as F
+from ferrolearn_amd.router import mlp_gradients
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def mixture(N, d, comps, seed, spread=10.0, dtype=np.float32):
+    """N points of one fixed Gaussian mixture (means and spreads from seed
+    0); `seed` only draws the points, so database and queries share it."""
+    g = np.random.default_rng(0)
+    means = g.uniform(-spread, spread, size=(comps, d))
+    sd = g.uniform(0.5, 1.5, size=comps)
+    rng = np.random.default_rng(seed)
+    c = rng.integers(0, comps, size=N)
+    X = means[c] + rng.normal(size=(N, d)) * sd[c, None]
+    return X.astype(dtype), c
+
+
+@pytest.fixture(scope="module")
+def setup():
+    X, comp = mixture(20000, 24, 40, seed=1)
+    # exact duplicates of some rows: distance ties resolved by index
+    X = np.concatenate([X, X[:500]], axis=0)
+    y = np.arange(X.shape[0]) % 7
+    fitted = (F.KNeighborsClassifier().with_n_neighbors(10).with_precision("bf16x3")
+              .fit(X, y))
+    router = F.CellRouter(fitted, n_cells=40, cell_iter=8, hidden=64, seed=1)
+    Qt, _ = mixture(4000, 24, 40, seed=2)
+    untrained = router.params()
+    rep = router.train(Qt, n_neighbors=10, epochs=10, batch_size=512, learning_rate=3e-3, seed=3)
+    Q, _ = mixture(2000, 24, 40, seed=4)
+    return X, fitted, router, untrained, rep, Q
+
+
+def test_cells_cover_rows(setup):
+    X, fitted, router, *_ = setup
+    cen, rad, cnt, lab = router.cells()
+    assert cnt.sum() == X.shape[0]
+    assert (np.bincount(lab, minlength=router.n_cells) == cnt).all()
+    # every row lies within its cell's radius (exact folds on the CPU)
+    sample = np.arange(0, X.shape[0], 37)
+    for i in sample:
+        c = lab[i]
+        assert O.euclid(X[i].astype(np.float64), cen[c]) <= rad[c]
+
+
+def test_training_lowers_loss(setup):
+    rep = setup[4]
+    assert rep["loss_last"] < rep["loss_first"]
+
+
+@pytest.mark.parametrize("route", ["network", "bounds"])
+@pytest.mark.parametrize("n_probe", [1, 4])
+def test_routed_equals_brute_force(setup, route, n_probe):
+    X, fitted, router, _, _, Q = setup
+    bd, bi = fitted.kneighbors(Q)
+    rd, ri = router.kneighbors(Q, n_probe=n_probe, route=route)
+    assert (ri == bi).all()
+    assert (rd.view(np.int64) == bd.view(np.int64)).all()
+    rep = router.report
+    assert 0 <= rep["n_fallback"] <= Q.shape[0]
+    assert rep["rows_scanned"] > 0
+
+
+def test_routed_against_oracle(setup):
+    X, fitted, router, _, _, Q = setup
+    rd, ri = router.kneighbors(Q[:40], n_probe=4)
+    od, oi = O.kneighbors(X, Q[:40], 10)
+    assert (ri == oi).all()
+    assert (rd.view(np.int64) == od.view(np.int64)).all()
+
+
+@pytest.mark.parametrize("k", [1, 17, 32])
+def test_routed_k_and_dtypes(setup, k):
+    X, fitted, router, _, _, Q = setup
+    Q64 = Q[:500].astype(np.float64)
+    bd, bi = fitted.kneighbors(Q64, n_neighbors=k)
+    rd, ri = router.kneighbors(Q64, n_neighbors=k, n_probe=3)
+    assert (ri == bi).all()
+    assert (rd.view(np.int64) == bd.view(np.int64)).all()
+
+
+def test_all_cells_always_certify(setup):
+    X, fitted, router, _, _, Q = setup
+    rd, ri = router.kneighbors(Q[:300], n_probe=min(31, router.n_cells), route="bounds")
+    # 31 of 40 cells scanned, smallest bounds first: every query certifies
+    assert router.report["n_fallback"] == 0
+    bd, bi = fitted.kneighbors(Q[:300])
+    assert (ri == bi).all()
+    assert (rd.view(np.int64) == bd.view(np.int64)).all()
+
+
+def test_network_gain_over_untrained(setup):
+    X, fitted, router, untrained, _, Q = setup
+    router.kneighbors(Q, n_probe=2, route="network")
+    trained_fb = router.report["n_fallback"]
+    trained_rows = router.report["rows_scanned"]
+    trained = router.params()
+    router.set_params(untrained)
+    try:
+        router.kneighbors(Q, n_probe=2, route="network")
+        untrained_fb = router.report["n_fallback"]
+    finally:
+        router.set_params(trained)
+    router.kneighbors(Q, n_probe=2, route="bounds")
+    bounds_fb = router.report["n_fallback"]
+    print(f"fallback at n_probe=2: trained {trained_fb}, untrained {untrained_fb}, "
+          f"bounds {bounds_fb} of {Q.shape[0]}; rows scanned {trained_rows}")
+    assert trained_fb < untrained_fb
+    # the certificate covers most queries once the right cells are scanned
+    assert bounds_fb < 0.5 * Q.shape[0]
+    # the routed search scans a small part of the database
+    assert trained_rows < 0.25 * X.shape[0] * Q.shape[0]
+
+
+@pytest.mark.parametrize("alpha", [0.0, 1e-3])
+def test_gradients_match_oracle(alpha):
+    rng = np.random.default_rng(7)
+    d, h, c, B, k = 20, 64, 45, 130, 5
+    n = d * h + h + h * h + h + h * c + c
+    p = rng.normal(scale=0.2, size=n).astype(np.float32)
+    X = rng.normal(size=(B, d)).astype(np.float32)
+    cells = rng.integers(0, c, size=(B, k)).astype(np.int32)
+    cells[::3, 3:] = -1          # rows with fewer targets
+    cells[7, :] = -1             # a row without targets
+    loss, g = mlp_gradients(p, X, cells, h, c, alpha=alpha)
+    ol, og = O.mlp_loss_grad(p, X, cells, d, h, c, alpha=alpha, emulate_bf16=True)
+    # fp32 accumulation against float64, same bfloat16 roundings: loss to
+    # 1e-5 relative, every parameter block to 1e-2 in norm
+    assert abs(loss - ol) <= 1e-5 * abs(ol)
+    o = 0
+    for s in [(d, h), (h,), (h, h), (h,), (h, c), (c,)]:
+        m = int(np.prod(s))
+        gg, oo = g[o:o + m].astype(np.float64), og[o:o + m]
+        assert np.linalg.norm(gg - oo) <= 1e-2 * np.linalg.norm(oo) + 1e-12, s
+        o += m
+
+
+def test_neighbor_target_training(setup):
+    X, fitted, router, untrained, _, Q = setup
+    r2 = F.CellRouter(fitted, n_cells=40, cell_iter=8, hidden=64, seed=1)
+    Qt, _ = mixture(4000, 24, 40, seed=2)
+    rep = r2.train(Qt, n_neighbors=10, epochs=10, batch_size=512, learning_rate=3e-3, seed=3,
+                   target="neighbors")
+    assert rep["loss_last"] < rep["loss_first"]
+    rd, ri = r2.kneighbors(Q, n_probe=2)
+    bd, bi = fitted.kneighbors(Q)
+    assert (ri == bi).all()
+    print(f"neighbors-target fallback at n_probe=2: {r2.report['n_fallback']}")
+
+
+def test_router_validation(setup):
+    X, fitted, router, *_ = setup
+    with pytest.raises(F.InvalidParameter):
+        router.kneighbors(X[:3], n_probe=0)
+    with pytest.raises(F.InvalidParameter):
+        router.kneighbors(X[:3], n_probe=40)
+    with pytest.raises(F.ShapeMismatch):
+        router.kneighbors(X[:3, :5])
+    with pytest.raises(F.InvalidParameter):
+        F.CellRouter(fitted, n_cells=16, hidden=100)

@@ -387,3 +387,57 @@ def test_kmeanspp_spreads_over_blobs():
     C = O.kmeanspp(X, 6, 7)
     blob = [int(np.argmin(((cen - c) ** 2).sum(1))) for c in C]
     assert sorted(blob) == list(range(6))
+
+
+# ------------------------------------------------------------ router net -----
+# The numpy restatement of the router network (mlp.rs) is pinned by central
+# finite differences of its own loss (a dropped term, a transposed operand or
+# a wrong mask in the analytic gradient fails these), and by the closed form
+# of a zero network.
+def _mlp_case(seed, d=5, h=6, c=7, B=9, k=3):
+    rng = np.random.default_rng(seed)
+    n = d * h + h + h * h + h + h * c + c
+    p = rng.normal(scale=0.5, size=n)
+    X = rng.normal(size=(B, d))
+    cells = rng.integers(0, c, size=(B, k))
+    return p, X, cells, d, h, c
+
+
+@pytest.mark.parametrize("alpha", [0.0, 0.01])
+def test_mlp_oracle_finite_differences(alpha):
+    p, X, cells, d, h, c = _mlp_case(3)
+    loss, g = O.mlp_loss_grad(p, X, cells, d, h, c, alpha=alpha)
+    eps = 1e-6
+    for i in range(p.size):
+        e = np.zeros_like(p)
+        e[i] = eps
+        lp, _ = O.mlp_loss_grad(p + e, X, cells, d, h, c)
+        lm, _ = O.mlp_loss_grad(p - e, X, cells, d, h, c)
+        # mlp.rs adds alpha W to grad_w without a matching loss term
+        w_entries = np.concatenate([np.ones(d * h), np.zeros(h), np.ones(h * h), np.zeros(h),
+                                    np.ones(h * c), np.zeros(c)])
+        fd = (lp - lm) / (2 * eps) + alpha * p[i] * w_entries[i]
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)), (i, fd, g[i])
+
+
+def test_mlp_oracle_zero_network():
+    d, h, c, B, k = 4, 3, 5, 6, 2
+    rng = np.random.default_rng(0)
+    n = d * h + h + h * h + h + h * c + c
+    X = rng.normal(size=(B, d))
+    cells = rng.integers(0, c, size=(B, k))
+    loss, g = O.mlp_loss_grad(np.zeros(n), X, cells, d, h, c)
+    # uniform softmax: loss ln c; grad_b3 = 1/c - mean target
+    assert abs(loss - math.log(c)) < 1e-15
+    y = np.zeros((B, c))
+    for i in range(B):
+        for cc in cells[i]:
+            y[i, cc] += 1.0 / k
+    gb3 = g[-c:]
+    assert np.allclose(gb3, 1.0 / c - y.mean(axis=0), atol=1e-15)
+    assert np.all(g[:-c] == 0.0)
+
+
+def test_bf16_round_ties_to_even():
+    v = np.array([1.0, 1.0 + 2**-8, 1.0 + 3 * 2**-8, -2.0 - 2**-7 - 2**-9, 0.0])
+    assert list(O.bf16_round(v)) == [1.0, 1.0, 1.0 + 2**-6, -2.0 - 2**-6, 0.0]
