@@ -285,6 +285,125 @@ def run_kmeans(args):
     return out
 
 
+def router_data(N, d=96, comps=4096, n_train=2_000_000, n_test=1_000_000, seed=0):
+    """Mixture of `comps` anisotropic Gaussians in d dimensions: random
+    rotations, axis std log-uniform in [0.2, 3], means uniform in [-10, 10].
+    Returns the database (rows grouped by component, sizes multinomial),
+    training queries and test queries, float32."""
+    g = np.random.default_rng(seed)
+    means = g.uniform(-10, 10, size=(comps, d)).astype(np.float32)
+    sd = np.exp(g.uniform(np.log(0.2), np.log(3.0), size=(comps, d))).astype(np.float32)
+    rot = np.linalg.qr(g.standard_normal((comps, d, d)))[0].astype(np.float32)
+
+    def draw(n, rng):
+        counts = rng.multinomial(n, np.full(comps, 1.0 / comps))
+        out = np.empty((n, d), np.float32)
+        o = 0
+        for c in range(comps):
+            m = int(counts[c])
+            if m:
+                z = rng.standard_normal((m, d), dtype=np.float32) * sd[c]
+                out[o:o + m] = z @ rot[c].T + means[c]
+                o += m
+        return out
+
+    X = draw(N, np.random.default_rng(seed + 1))
+    Qt = draw(n_train, np.random.default_rng(seed + 2))
+    Qe = draw(n_test, np.random.default_rng(seed + 3))
+    # queries in random order (the database order does not matter)
+    Qt = Qt[np.random.default_rng(seed + 4).permutation(n_train)]
+    Qe = Qe[np.random.default_rng(seed + 5).permutation(n_test)]
+    return X, Qt, Qe
+
+
+def run_router(args):
+    """Learned-router exact kNN: routed search vs the brute force on the same
+    handle, with the fallback rate; results must equal the brute force's."""
+    import ferrolearn_amd as F
+    sc = args.router_scale
+    N, comps = int(20_000_000 * sc), max(16, int(4096 * sc))
+    n_train, n_test = int(2_000_000 * sc), int(1_000_000 * sc)
+    k, n_probe = 10, args.n_probe
+    log(f"router: generating {N} x 96 rows, {comps} components ...")
+    t0 = time.perf_counter()
+    X, Qt, Qe = router_data(N, comps=comps, n_train=n_train, n_test=n_test)
+    log(f"router: data {time.perf_counter() - t0:.1f} s")
+    y = np.arange(N) % 10
+    fitted = F.KNeighborsClassifier().with_n_neighbors(k).with_precision("bf16x3").fit(X, y)
+    t0 = time.perf_counter()
+    router = F.CellRouter(fitted, n_cells=comps, cell_iter=10, hidden=512, seed=0)
+    build_s = time.perf_counter() - t0
+    log(f"router: cells built in {build_s:.1f} s")
+    qt = F.to_device(Qt)
+    trep = router.train(qt, n_neighbors=k, epochs=args.router_epochs, batch_size=4096,
+                        learning_rate=1e-3, seed=0)
+    log(f"router: trained, loss {trep['loss_first']:.3f} -> {trep['loss_last']:.3f}, "
+        f"targets {trep['ms_targets'] / 1e3:.1f} s, epochs {trep['ms_train'] / 1e3:.1f} s")
+    del qt
+    qe = F.to_device(Qe)
+    router.kneighbors(F.to_device(Qe[:4096]), n_probe=n_probe)    # warm-up
+    F.device.synchronize()
+    t0 = time.perf_counter()
+    rd, ri = router.kneighbors(qe, n_probe=n_probe, route="network")
+    F.device.synchronize()
+    routed_s = time.perf_counter() - t0
+    rrep = dict(router.report)
+    log(f"router: routed {n_test / routed_s:.0f} q/s, fallback {rrep['n_fallback']}")
+    router.kneighbors(qe, n_probe=n_probe, route="bounds")
+    brep = dict(router.report)
+    # brute force on a sample of the same queries, both precisions
+    ns = min(n_test, int(args.router_brute_sample))
+    qs = F.to_device(Qe[:ns])
+    fitted.kneighbors(F.to_device(Qe[:256]))
+    F.device.synchronize()
+    t0 = time.perf_counter()
+    bd, bi = fitted.kneighbors(qs)
+    F.device.synchronize()
+    brute_s = time.perf_counter() - t0
+    exact = bool((bi.numpy() == ri.numpy()[:ns]).all()
+                 and (bd.numpy().view(np.int64) == rd.numpy()[:ns].view(np.int64)).all())
+    n64 = min(ns, 10_000)
+    fitted.set_precision("fp64")
+    t0 = time.perf_counter()
+    fd, fi = fitted.kneighbors(F.to_device(Qe[:n64]))
+    F.device.synchronize()
+    fp64_s = time.perf_counter() - t0
+    exact = exact and bool((fi.numpy() == ri.numpy()[:n64]).all())
+    log(f"router: brute bf16x3 {ns / brute_s:.0f} q/s, fp64 {n64 / fp64_s:.0f} q/s, exact {exact}")
+    out = {
+        "workload": f"learned-router exact kNN, {N} x 96 float32, {comps} anisotropic "
+                    f"Gaussians, {comps} cells, router 96-512-512-{comps} bf16, "
+                    f"{n_train} training / {n_test} test queries, k={k}, n_probe={n_probe}",
+        "metric": "queries/s",
+        "value": n_test / routed_s,
+        "fallback_rate": rrep["n_fallback"] / n_test,
+        "fallback_rate_bounds_route": brep["n_fallback"] / n_test,
+        "rows_scanned_per_query": rrep["rows_scanned"] / n_test,
+        "brute_bf16x3_queries_per_s": ns / brute_s,
+        "brute_fp64_queries_per_s": n64 / fp64_s,
+        "speedup_vs_brute_bf16x3": (n_test / routed_s) / (ns / brute_s),
+        "speedup_vs_fp64_form": (n_test / routed_s) / (n64 / fp64_s),
+        "exact_vs_brute_sample": exact,
+        "brute_sample_queries": ns,
+        "report": {kk: rrep[kk] for kk in ("ms_route", "ms_bounds", "ms_scan", "ms_merge",
+                                           "ms_fallback", "ms_total")},
+        "build_s": build_s,
+        "train": {kk: trep[kk] for kk in ("ms_targets", "ms_train", "loss_first", "loss_last")},
+        "epochs": args.router_epochs,
+    }
+    if not args.no_cpu:
+        from oracle import oracle as O
+        nc = 16
+        t0 = time.perf_counter()
+        od, oi = O.kneighbors(X, Qe[:nc], k)
+        cdt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": nc / cdt, "unit": "queries/s", "cores": O.max_threads(),
+                               "kind": "port",
+                               "sample": f"{nc} queries, oracle brute force, all host threads"}
+        out["exact_vs_oracle_sample"] = bool((oi == ri.numpy()[:nc]).all())
+    return out
+
+
 def copy_calibration(mib=512):
     """A plain device-to-device copy of known size: its FETCH_SIZE / WRITE_SIZE
     counters give the scale that turns those counters into HBM bytes."""
@@ -305,7 +424,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="all", choices=["all", "knn", "kmeans", "copy"])
+    ap.add_argument("--workload", default="all",
+                    choices=["all", "knn", "kmeans", "router", "copy"])
+    ap.add_argument("--router-scale", type=float, default=1.0,
+                    help="fraction of the router workload's rows, cells and queries")
+    ap.add_argument("--router-epochs", type=int, default=3)
+    ap.add_argument("--router-brute-sample", type=int, default=100_000)
+    ap.add_argument("--n-probe", type=int, default=4)
     ap.add_argument("--km-forms", default="bf16x3,fp64")
     ap.add_argument("--km-rows", type=int, default=10_000_000)
     ap.add_argument("--km-iters", type=int, default=50)
@@ -327,9 +452,13 @@ def main():
     if args.workload == "kmeans":
         print(json.dumps(run_kmeans(args)), flush=True)
         return
+    if args.workload == "router":
+        print(json.dumps(run_router(args)), flush=True)
+        return
     line = run_knn(args)
     if args.workload == "all":
         line["kmeans"] = run_kmeans(args)
+        line["router"] = run_router(args)
     print(json.dumps(line), flush=True)
 
 
