@@ -348,7 +348,8 @@ def run_router(args):
     F.device.synchronize()
     routed_s = time.perf_counter() - t0
     rrep = dict(router.report)
-    log(f"router: routed {n_test / routed_s:.0f} q/s, fallback {rrep['n_fallback']}")
+    log(f"router: routed {n_test / routed_s:.0f} q/s, rescans {rrep['n_rescan']}, "
+        f"brute-force fallback {rrep['n_fallback']}")
     router.kneighbors(qe, n_probe=n_probe, route="bounds")
     brep = dict(router.report)
     # brute force on a sample of the same queries, both precisions
@@ -377,7 +378,8 @@ def run_router(args):
         "metric": "queries/s",
         "value": n_test / routed_s,
         "fallback_rate": rrep["n_fallback"] / n_test,
-        "fallback_rate_bounds_route": brep["n_fallback"] / n_test,
+        "rescan_rate": rrep["n_rescan"] / n_test,
+        "rescan_rate_bounds_route": brep["n_rescan"] / n_test,
         "rows_scanned_per_query": rrep["rows_scanned"] / n_test,
         "brute_bf16x3_queries_per_s": ns / brute_s,
         "brute_fp64_queries_per_s": n64 / fp64_s,

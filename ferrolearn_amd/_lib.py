@@ -119,6 +119,7 @@ class fl_router_report(C.Structure):
         ("ms_merge", C.c_double),
         ("ms_fallback", C.c_double),
         ("n_queries", C.c_int64),
+        ("n_rescan", C.c_int64),
         ("n_fallback", C.c_int64),
         ("rows_scanned", C.c_int64),
         ("loss_first", C.c_double),

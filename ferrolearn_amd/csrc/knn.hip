@@ -118,8 +118,11 @@ void knn_search(fl_knn *h, const double *Q, int64_t nq, int k, int mode, double 
     const int M = (bf || f64) ? MP : (mode == MODE_EXACT ? pick_m_exact(k) : pick_m_filter(k));
     const int rows_per_block = bf ? BF_Q : (f64 ? F64_Q : TM_ROWS);
     const int64_t nblk = ceil_div(nq, rows_per_block);
+    // The exact form needs no certificate, so a few queries may use many
+    // splits (parallelism) up to the large finalize capacity.
+    const int64_t cap = mode == MODE_EXACT ? FIN_MAXC_BIG : FIN_MAXC;
     const int64_t smax = std::max<int64_t>(
-        1, std::min<int64_t>(FIN_MAXC / M, std::max<int64_t>(1, nB / 256)));
+        1, std::min<int64_t>(cap / M, std::max<int64_t>(1, nB / 256)));
     int64_t S;
     if (f64) {
         const int64_t row_bytes = (int64_t)f64_d4(d) * 4 * (int64_t)sizeof(TB);

@@ -197,16 +197,6 @@ __global__ __launch_bounds__(256) void mlp_softmax_xent_kernel(
     for (int c = t; c < Cp; c += 256) g[c] = c < C ? bf16_bits_rne(p[c]) : (uint16_t)0;
 }
 
-// grad_b[n] = scale * sum over rows < nb of G[row][n] (mlp.rs:324).
-__global__ void mlp_colsum_kernel(const uint16_t *__restrict__ G, int64_t ldg, int nb, int N,
-                                  float scale, float *__restrict__ out) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
-    float s = 0.f;
-    for (int b = 0; b < nb; ++b) s += bf16_bits_to_f32(G[(int64_t)b * ldg + n]);
-    out[n] = s * scale;
-}
-
 // Adam step over one parameter block (mlp.rs:372-409); optionally refresh
 // the bf16 working copy.
 __global__ void mlp_adam_kernel(float *__restrict__ P, const float *__restrict__ G,

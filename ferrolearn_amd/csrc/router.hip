@@ -152,12 +152,18 @@ void gemm(const MlpGemmArgs &a, hipStream_t st) {
 
 // Buffers of one batch of nbp (padded) rows.
 struct Batch {
-    uint16_t *xb, *a1, *a2, *g3, *d2, *d1;
+    uint16_t *xb, *a1, *a2, *g3, *d2, *d1, *ones;
     float *z;
 };
 
-Batch batch_buffers(Workspace &ws, const Mlp &net, int64_t nbp, bool backward) {
+Batch batch_buffers(Workspace &ws, const Mlp &net, int64_t nbp, bool backward, hipStream_t st) {
     Batch b;
+    b.ones = nullptr;
+    if (backward) {
+        // bf16 1.0 per row: bias gradients are G^T 1 on the GEMM kernel
+        b.ones = ws.get<uint16_t>("mlp_ones", nbp);
+        FL_HIP(hipMemsetD16Async((hipDeviceptr_t)b.ones, 0x3F80, (size_t)nbp, st));
+    }
     b.xb = ws.get<uint16_t>("mlp_xb", nbp * net.dp);
     b.a1 = ws.get<uint16_t>("mlp_a1", nbp * net.H);
     b.a2 = ws.get<uint16_t>("mlp_a2", nbp * net.H);
@@ -187,6 +193,17 @@ void mlp_forward(const Mlp &net, const Batch &b, int nbp, hipStream_t st) {
     gemm<true, false>(a, st);
 }
 
+// grad_b[n] = scale * sum over rows of G[row][n] (rows past the batch are
+// zero), as the GEMM G^T 1.
+void bias_grad(const uint16_t *G, int ldg, int N, int nbp, float scale, float *out,
+               const uint16_t *ones, hipStream_t st) {
+    MlpGemmArgs a = gemm_args();
+    a.A = G, a.lda = ldg, a.B = ones, a.ldb = 1;
+    a.M = N, a.N = 1, a.K = nbp, a.scale = scale;
+    a.Cf = out, a.ldcf = 1;
+    gemm<false, false>(a, st);
+}
+
 // Softmax cross-entropy, then the gradients of mlp.rs:302-345 into net.G.
 void mlp_backward(Mlp &net, const Batch &b, const int32_t *cells, int k, int nb, int nbp,
                   float alpha, double *row_loss, hipStream_t st) {
@@ -206,8 +223,7 @@ void mlp_backward(Mlp &net, const Batch &b, const int32_t *cells, int k, int nb,
     if (alpha != 0.f) a.addf = net.p() + net.oW3, a.ldadd = net.cp, a.addscale = alpha;
     a.Cf = gp + net.oW3, a.ldcf = net.cp;
     gemm<false, false>(a, st);
-    hipLaunchKernelGGL(mlp_colsum_kernel, dim3(grid1(net.cp, 256)), dim3(256), 0, st, b.g3,
-                       (int64_t)net.cp, nb, net.cp, inv, gp + net.ob3);
+    bias_grad(b.g3, net.cp, net.cp, nbp, inv, gp + net.ob3, b.ones, st);
     // delta2 = (g3 W3^T) * (a2 > 0)
     a = gemm_args();
     a.A = b.g3, a.lda = net.cp, a.B = net.w3b(), a.ldb = net.cp;
@@ -221,8 +237,7 @@ void mlp_backward(Mlp &net, const Batch &b, const int32_t *cells, int k, int nb,
     if (alpha != 0.f) a.addf = net.p() + net.oW2, a.ldadd = net.H, a.addscale = alpha;
     a.Cf = gp + net.oW2, a.ldcf = net.H;
     gemm<false, false>(a, st);
-    hipLaunchKernelGGL(mlp_colsum_kernel, dim3(grid1(net.H, 256)), dim3(256), 0, st, b.d2,
-                       (int64_t)net.H, nb, net.H, inv, gp + net.ob2);
+    bias_grad(b.d2, net.H, net.H, nbp, inv, gp + net.ob2, b.ones, st);
     a = gemm_args();
     a.A = b.d2, a.lda = net.H, a.B = net.w2b(), a.ldb = net.H;
     a.M = nbp, a.N = net.H, a.K = net.H;
@@ -235,8 +250,7 @@ void mlp_backward(Mlp &net, const Batch &b, const int32_t *cells, int k, int nb,
     if (alpha != 0.f) a.addf = net.p() + net.oW1, a.ldadd = net.H, a.addscale = alpha;
     a.Cf = gp + net.oW1, a.ldcf = net.H;
     gemm<false, false>(a, st);
-    hipLaunchKernelGGL(mlp_colsum_kernel, dim3(grid1(net.H, 256)), dim3(256), 0, st, b.d1,
-                       (int64_t)net.H, nb, net.H, inv, gp + net.ob1);
+    bias_grad(b.d1, net.H, net.H, nbp, inv, gp + net.ob1, b.ones, st);
     check_launch();
 }
 
@@ -311,7 +325,7 @@ namespace {
 
 struct RtAcc {
     double ms_route = 0, ms_bounds = 0, ms_scan = 0, ms_merge = 0, ms_fb = 0;
-    int64_t n_fb = 0, rows = 0;
+    int64_t n_fb = 0, n_rescan = 0, rows = 0;
 };
 
 template <typename TX>
@@ -454,9 +468,12 @@ void launch_top_cells(const float *Z, int64_t ldz, int64_t rows, int C, int n_pr
 }
 
 // Routed search of nq device queries (one pass; nq * n_probe < 2^31).
+// Queries whose certificate fails are searched again over the
+// min(31, n_cells) cells with the smallest bounds (depth 1), and what still
+// fails goes to the brute force.
 template <typename TQ, typename TX>
 void route_search(fl_router *r, const TQ *Q, int64_t nq, int k, int n_probe, int route,
-                  double *dist, int64_t *idx, RtAcc &acc) {
+                  double *dist, int64_t *idx, RtAcc &acc, int depth = 0) {
     hipStream_t st = r->stream;
     const int d = (int)r->d, C = r->C;
     PhaseTimer tm(st);
@@ -468,7 +485,7 @@ void route_search(fl_router *r, const TQ *Q, int64_t nq, int k, int n_probe, int
         for (int64_t q0 = 0; q0 < nq; q0 += kInferRows) {
             const int nb = (int)std::min<int64_t>(kInferRows, nq - q0);
             const int nbp = (int)ceil_div(nb, 64) * 64;
-            Batch b = batch_buffers(r->ws, net, nbp, false);
+            Batch b = batch_buffers(r->ws, net, nbp, false, st);
             hipLaunchKernelGGL(mlp_gather_rows_kernel<TQ>, dim3(grid1((int64_t)nbp * net.dp, 256)),
                                dim3(256), 0, st, Q, d, (const int32_t *)nullptr, q0, nb, nbp,
                                net.dp, b.xb);
@@ -559,26 +576,36 @@ void route_search(fl_router *r, const TQ *Q, int64_t nq, int k, int n_probe, int
     acc.ms_scan += tm.ms(ts);
     acc.ms_merge += tm.ms(tmg);
     acc.rows += (int64_t)hrows;
-    // 6. brute force for the uncertified queries
+    // 6. uncertified queries: a wider routed scan, then the brute force
     std::vector<int32_t> which;
     for (int64_t q = 0; q < nq; ++q)
         if (hflag[(size_t)q]) which.push_back((int32_t)q);
     const int64_t m = (int64_t)which.size();
-    acc.n_fb += m;
     if (m == 0) return;
     WallClock wc;
-    int32_t *dw = r->ws.get<int32_t>("fb_which", m);
-    TQ *fq = r->ws.get<TQ>("fb_q", m * d);
-    double *fd = r->ws.get<double>("fb_d", m * k);
-    int64_t *fi = r->ws.get<int64_t>("fb_i", m * k);
+    const std::string fp = "fb" + std::to_string(depth) + "_";
+    int32_t *dw = r->ws.get<int32_t>(fp + "which", m);
+    TQ *fq = r->ws.get<TQ>(fp + "q", m * d);
+    double *fd = r->ws.get<double>(fp + "d", m * k);
+    int64_t *fi = r->ws.get<int64_t>(fp + "i", m * k);
     FL_HIP(hipMemcpyAsync(dw, which.data(), (size_t)m * 4, hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(rt_gather_queries_kernel<TQ>, dim3(grid1(m * d, 256)), dim3(256), 0, st, Q,
                        d, dw, m, fq);
     check_launch();
-    FL_HIP(hipStreamSynchronize(st));
-    const int32_t qdt = std::is_same<TQ, float>::value ? FL_F32 : FL_F64;
-    const fl_status s = fl_knn_kneighbors_device(r->knn, fq, qdt, m, d, k, fd, fi, nullptr);
-    if (s != FL_OK) throw StatusError{s, fl_last_error()};
+    const int wide = std::min(kMaxProbe, C);
+    if (depth == 0 && (wide > n_probe || route != FL_ROUTE_BOUNDS)) {
+        acc.n_rescan += m;
+        RtAcc inner;
+        route_search<TQ, TX>(r, fq, m, k, wide, FL_ROUTE_BOUNDS, fd, fi, inner, 1);
+        acc.rows += inner.rows;
+        acc.n_fb += inner.n_fb;
+    } else {
+        FL_HIP(hipStreamSynchronize(st));
+        acc.n_fb += m;
+        const int32_t qdt = std::is_same<TQ, float>::value ? FL_F32 : FL_F64;
+        const fl_status s = fl_knn_kneighbors_device(r->knn, fq, qdt, m, d, k, fd, fi, nullptr);
+        if (s != FL_OK) throw StatusError{s, fl_last_error()};
+    }
     hipLaunchKernelGGL(rt_scatter_results_kernel, dim3(grid1(m * k, 256)), dim3(256), 0, st, fd,
                        fi, dw, m, k, dist, idx);
     check_launch();
@@ -593,7 +620,7 @@ void train_epochs(fl_router *r, const TQ *Q, int64_t nq, const int32_t *cells, i
     Mlp &net = r->net;
     const int64_t B = p->batch_size;
     const int64_t nbp_max = ceil_div(std::min<int64_t>(B, nq), 64) * 64;
-    Batch b = batch_buffers(r->ws, net, nbp_max, true);
+    Batch b = batch_buffers(r->ws, net, nbp_max, true, st);
     int32_t *perm_d = r->ws.get<int32_t>("perm", nq);
     int32_t *cells_b = r->ws.get<int32_t>("cells_b", nbp_max * k);
     double *row_loss = r->ws.get<double>("row_loss", nq + nbp_max);
@@ -762,10 +789,21 @@ fl_status fl_router_train_device(fl_router_handle r, const void *dQ, int32_t dty
         const int k = (int)p->n_neighbors;
         // exact targets: the cells of each query's k nearest rows
         WallClock wt;
+        // The exact neighbours come from this index's own search (bound
+        // route over the widest probe, certified or brute force): the same
+        // answer as the brute force, at a fraction of its cost.
         double *td = r->ws.get<double>("t_dist", nq * k);
         int64_t *ti = r->ws.get<int64_t>("t_idx", nq * k);
-        const fl_status s = fl_knn_kneighbors_device(r->knn, dQ, dtype, nq, d, k, td, ti, nullptr);
-        if (s != FL_OK) return s;
+        if (k <= kMaxRoutedK) {
+            const fl_status s = fl_router_kneighbors_device(
+                r, dQ, dtype, nq, d, k, std::min<int64_t>(8, r->C), FL_ROUTE_BOUNDS, td, ti,
+                nullptr);
+            if (s != FL_OK) return s;
+        } else {
+            const fl_status s =
+                fl_knn_kneighbors_device(r->knn, dQ, dtype, nq, d, k, td, ti, nullptr);
+            if (s != FL_OK) return s;
+        }
         int tw = k;
         int32_t *cells;
         if (p->target == FL_TARGET_NEIGHBORS) {
@@ -867,6 +905,7 @@ fl_status fl_router_kneighbors_device(fl_router_handle r, const void *dQ, int32_
             rep->ms_fallback = acc.ms_fb;
             rep->n_queries = nq;
             rep->n_fallback = acc.n_fb;
+            rep->n_rescan = acc.n_rescan;
             rep->rows_scanned = acc.rows;
         }
         return FL_OK;
@@ -942,7 +981,7 @@ fl_status fl_router_mlp_gradients(int64_t d, int64_t hidden, int64_t n_cells, co
         net.upload(pad, st);
         Workspace ws;
         const int nbp = (int)ceil_div(nb, 64) * 64;
-        Batch b = batch_buffers(ws, net, nbp, true);
+        Batch b = batch_buffers(ws, net, nbp, true, st);
         float *xd = ws.get<float>("x", nb * d);
         int32_t *cd = ws.get<int32_t>("c", nb * k);
         double *rl = ws.get<double>("rl", nbp);

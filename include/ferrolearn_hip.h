@@ -331,7 +331,10 @@ typedef struct fl_router_report {
     double ms_merge;      /* search: merge and certificate                   */
     double ms_fallback;   /* search: brute force for uncertified queries     */
     int64_t n_queries;
-    int64_t n_fallback;   /* queries whose certificate failed                */
+    int64_t n_rescan;     /* queries whose first certificate failed; they are
+                             searched again over the min(31, n_cells) cells
+                             with the smallest bounds                       */
+    int64_t n_fallback;   /* queries that still failed: brute force          */
     int64_t rows_scanned; /* database rows scanned by the routed search      */
     double loss_first;    /* train: mean loss of the first and last epochs   */
     double loss_last;

@@ -95,7 +95,7 @@ def test_all_cells_always_certify(setup):
     X, fitted, router, _, _, Q = setup
     rd, ri = router.kneighbors(Q[:300], n_probe=min(31, router.n_cells), route="bounds")
     # 31 of 40 cells scanned, smallest bounds first: every query certifies
-    assert router.report["n_fallback"] == 0
+    assert router.report["n_rescan"] == 0 and router.report["n_fallback"] == 0
     bd, bi = fitted.kneighbors(Q[:300])
     assert (ri == bi).all()
     assert (rd.view(np.int64) == bd.view(np.int64)).all()
@@ -103,24 +103,27 @@ def test_all_cells_always_certify(setup):
 
 def test_network_gain_over_untrained(setup):
     X, fitted, router, untrained, _, Q = setup
+    # n_rescan: queries whose certificate failed after the n_probe cells the
+    # route chose (they are searched again, then by brute force)
     router.kneighbors(Q, n_probe=2, route="network")
-    trained_fb = router.report["n_fallback"]
+    trained_fb = router.report["n_rescan"]
     trained_rows = router.report["rows_scanned"]
+    assert router.report["n_fallback"] <= trained_fb
     trained = router.params()
     router.set_params(untrained)
     try:
         router.kneighbors(Q, n_probe=2, route="network")
-        untrained_fb = router.report["n_fallback"]
+        untrained_fb = router.report["n_rescan"]
     finally:
         router.set_params(trained)
     router.kneighbors(Q, n_probe=2, route="bounds")
-    bounds_fb = router.report["n_fallback"]
+    bounds_fb = router.report["n_rescan"]
     print(f"fallback at n_probe=2: trained {trained_fb}, untrained {untrained_fb}, "
           f"bounds {bounds_fb} of {Q.shape[0]}; rows scanned {trained_rows}")
     assert trained_fb < untrained_fb
     # the certificate covers most queries once the right cells are scanned
     assert bounds_fb < 0.5 * Q.shape[0]
-    # the routed search scans a small part of the database
+    # the routed search (with its rescans) scans a small part of the database
     assert trained_rows < 0.25 * X.shape[0] * Q.shape[0]
 
 
@@ -157,7 +160,7 @@ def test_neighbor_target_training(setup):
     rd, ri = r2.kneighbors(Q, n_probe=2)
     bd, bi = fitted.kneighbors(Q)
     assert (ri == bi).all()
-    print(f"neighbors-target fallback at n_probe=2: {r2.report['n_fallback']}")
+    print(f"neighbors-target rescans at n_probe=2: {r2.report['n_rescan']}")
 
 
 def test_router_validation(setup):
