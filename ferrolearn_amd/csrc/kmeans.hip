@@ -7,6 +7,7 @@
 // assignment, the centres are the post-update ones (:468-495).
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <unordered_map>
 #include <string>
 #include <vector>
@@ -88,16 +89,39 @@ class Lloyd {
     bool bf16() const { return passes_ > 0; }
 
     // labels/best for every row against centres C (k*d f64, device).
+    // Phase timers are read after the caller's next synchronisation
+    // (flush), so that timing adds no host round trip to an iteration.
+    int tbegin() {
+        if (!tm_) tm_.reset(new PhaseTimer(st_));
+        return tm_->begin();
+    }
+    void tend(int id, double *slot) {
+        tm_->end(id);
+        pend_.push_back({id, slot});
+    }
+    // Add the pending phase times to their slots and, when given, the
+    // fallback count of the last assignment (device) to *n_fb.
+    void flush(int64_t *n_fb = nullptr) {
+        if (n_fb) {
+            int32_t c = 0;
+            FL_HIP(hipMemcpyAsync(&c, ws_.get<int32_t>("fcnt", 1), 4, hipMemcpyDeviceToHost, st_));
+            FL_HIP(hipStreamSynchronize(st_));
+            *n_fb += c;
+        }
+        for (const auto &p : pend_) *p.second += tm_->ms(p.first);
+        pend_.clear();
+        tm_.reset();
+    }
+
     void assign(const double *C, int32_t *labels, double *best, KmAcc &acc) {
-        PhaseTimer tm(st_);
         acc.n_rows += n_;
         if (direct_) {
-            const int t = tm.begin();
+            const int t = tbegin();
             hipLaunchKernelGGL(km_assign_exact_kernel<TX>, dim3(grid1(n_, 256)), dim3(256), 0, st_,
                                X_, C, n_, d_, k_, labels, best);
             check_launch();
-            tm.end(t);
-            acc.ms_dist += tm.ms(t);
+            tend(t, &acc.ms_dist);
+            FL_HIP(hipMemsetAsync(ws_.get<int32_t>("fcnt", 1), 0, sizeof(int32_t), st_));
             acc.n_launch += 1;
             return;
         }
@@ -153,9 +177,9 @@ class Lloyd {
             a.flags = ws_.get<int32_t>("flags", n_);
             a.fcount = ws_.get<int32_t>("fcnt", 1);
             FL_HIP(hipMemsetAsync(a.fcount, 0, sizeof(int32_t), st_));
-            t0 = tm.begin();
+            t0 = tbegin();
             launch_bf16(passes_, 0, nkc_, a, 1, st_);   // argmin form, decision fused
-            tm.end(t0);
+            tend(t0, &acc.ms_dist);
             fused = true;
         } else if (d4_ > 0) {
             // f64 filter: centre terms, padding and the largest centre norm
@@ -189,9 +213,9 @@ class Lloyd {
             a.qoff = qoff;
             a.xnp = cxnp;
             a.out = cand;
-            t0 = tm.begin();
+            t0 = tbegin();
             launch_f64_km<TX>(d4_, a, st_);   // argmin form: 4 candidates per row
-            tm.end(t0);
+            tend(t0, &acc.ms_dist);
         } else {
             TopmArgs a;
             a.nA = n_;
@@ -207,37 +231,41 @@ class Lloyd {
             a.beta = bd.beta;
             a.sqrt_key = 0;
             a.out = cand;
-            t0 = tm.begin();
+            t0 = tbegin();
             launch_topm<TX, double>(kKmM, MODE_MFMA64, X_, C, a, 1, st_);
-            tm.end(t0);
+            tend(t0, &acc.ms_dist);
         }
         acc.n_launch += 1;
         int32_t *flags = ws_.get<int32_t>("flags", n_);
         int32_t *fcnt = ws_.get<int32_t>("fcnt", 1);
-        int t1 = -1;
         if (!fused) {
             FL_HIP(hipMemsetAsync(fcnt, 0, sizeof(int32_t), st_));
-            t1 = tm.begin();
+            const int t1 = tbegin();
             hipLaunchKernelGGL(km_finalize_kernel<TX>, dim3(grid1(n_, 256)), dim3(256), 0, st_, X_,
                                C, n_, d_, k_, kKmM, cand, labels, best, flags, fcnt);
             check_launch();
-            tm.end(t1);
+            tend(t1, &acc.ms_fin);
         }
-        int32_t nflag = 0;
-        FL_HIP(hipMemcpyAsync(&nflag, fcnt, sizeof(int32_t), hipMemcpyDeviceToHost, st_));
-        FL_HIP(hipStreamSynchronize(st_));
-        acc.ms_dist += tm.ms(t0);
-        if (t1 >= 0) acc.ms_fin += tm.ms(t1);
         acc.n_cand += n_ * (int64_t)(fused ? 1 : (k_ < kKmM ? k_ : kKmM));
-        if (nflag > 0) {
-            acc.n_fb += nflag;
-            const int t2 = tm.begin();
-            hipLaunchKernelGGL(km_fallback_kernel<TX>, dim3(grid1(nflag, 4)), dim3(256), 0, st_,
-                               X_, C, d_, k_, flags, fcnt, labels, best);
-            check_launch();
-            tm.end(t2);
-            acc.ms_fb += tm.ms(t2);
+        // exact scan of the uncertified rows; the grid strides over the
+        // device-side count (zero rows: every wave exits at once)
+        const int t2 = tbegin();
+        const dim3 fgrid((unsigned)std::min<int64_t>(grid1(n_, 4), 2048));
+        if (d_ <= 2048) {
+            double *Ct = ws_.get<double>("ctrans", k_ * d_);
+            hipLaunchKernelGGL(km_transpose_kernel, dim3(grid1(k_ * d_, 256)), dim3(256), 0, st_,
+                               C, k_, d_, Ct);
+            const size_t sm = (size_t)4 * d_ * 8;
+            FL_HIP(hipFuncSetAttribute((const void *)km_fallback_lds_kernel<TX>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+            hipLaunchKernelGGL(km_fallback_lds_kernel<TX>, fgrid, dim3(256), sm, st_, X_, Ct, d_,
+                               k_, flags, fcnt, labels, best);
+        } else {
+            hipLaunchKernelGGL(km_fallback_kernel<TX>, fgrid, dim3(256), 0, st_, X_, C, d_, k_,
+                               flags, fcnt, labels, best);
         }
+        check_launch();
+        tend(t2, &acc.ms_fb);
     }
 
     // Deterministic sum of best[0..n) into *out (device).
@@ -252,8 +280,7 @@ class Lloyd {
     // Reference-order centroid update; shift (device) = max centre movement.
     void update(const int32_t *labels, const double *oldc, double *newc, double *shift,
                 KmAcc &acc) {
-        PhaseTimer tm(st_);
-        const int t = tm.begin();
+        const int t = tbegin();
         const int64_t nblk = ceil_div(n_, UPD_RB);
         int32_t *blockcnt = ws_.get<int32_t>("blockcnt", nblk * k_);
         int32_t *blockbase = ws_.get<int32_t>("blockbase", nblk * k_);
@@ -271,14 +298,13 @@ class Lloyd {
                            nblk, k_, blockbase, count);
         hipLaunchKernelGGL(km_offsets_kernel, dim3(1), dim3(256), 0, st_, count, k_, offset);
         hipLaunchKernelGGL(km_scatter_kernel, dim3((unsigned)nblk), dim3(64), ctr_b, st_, labels,
-                           n_, k_, blockbase, offset, order);
+                           n_, k_, label_bits(k_), blockbase, offset, order);
         hipLaunchKernelGGL(km_sum_kernel<TX>, dim3((unsigned)k_, (unsigned)ceil_div(d_, 64)),
                            dim3(64), 0, st_, X_, d_, order, offset, count, oldc, newc);
         hipLaunchKernelGGL(km_shift_kernel, dim3(1), dim3(256), 0, st_, newc, oldc, k_, d_,
                            shift);
         check_launch();
-        tm.end(t);
-        acc.ms_upd += tm.ms(t);
+        tend(t, &acc.ms_upd);
     }
 
     // The library's k-means++ into out (k*d f64, device).
@@ -377,6 +403,8 @@ class Lloyd {
     int d4_ = 0;                 // f64 MFMA filter (d4_ > 0)
     double *xn_ = nullptr, *xs_ = nullptr;
     unsigned short *xh_ = nullptr, *xl_ = nullptr;
+    std::unique_ptr<PhaseTimer> tm_;
+    std::vector<std::pair<int, double *>> pend_;
     Workspace ws_;
 };
 
@@ -456,7 +484,7 @@ fl_status fit_device(const TX *X, int64_t n, int d, const fl_kmeans_params *p,
             L.update(labels, cur, nxt, dsc + 1, acc);
             double h[2];
             FL_HIP(hipMemcpyAsync(h, dsc, 16, hipMemcpyDeviceToHost, st));
-            FL_HIP(hipStreamSynchronize(st));
+            L.flush(&acc.n_fb);   // the iteration's one synchronisation
             inertia = h[0];
             std::swap(cur, nxt);
             n_iter = it + 1;
@@ -618,7 +646,7 @@ fl_status mb_fit_device(const TX *X, int64_t n, int d, const fl_minibatch_params
         check_launch();
         double inertia = 0.0;
         FL_HIP(hipMemcpyAsync(&inertia, dsc + 1, 8, hipMemcpyDeviceToHost, st));
-        FL_HIP(hipStreamSynchronize(st));
+        L.flush(&acc.n_fb);
         if (run == 0 || inertia < best_inertia) {
             best_inertia = inertia;
             best_iter = n_iter;
@@ -820,9 +848,11 @@ fl_status fl_kmeans_predict(const double *centers, int64_t k, int64_t d_centers,
         if (dtype == FL_F32) {
             Lloyd<float> L((const float *)dX, n, (int)d, k, FL_PREC_FP64, sg.s);
             L.assign(dC, dl, db, acc);
+            L.flush();
         } else {
             Lloyd<double> L((const double *)dX, n, (int)d, k, FL_PREC_FP64, sg.s);
             L.assign(dC, dl, db, acc);
+            L.flush();
         }
         hipLaunchKernelGGL(labels_to_i64_kernel, dim3(grid1(n, 256)), dim3(256), 0, sg.s, dl, n,
                            d64);

@@ -119,23 +119,105 @@ __device__ __forceinline__ void wave_exact_argmin(const TX *__restrict__ x,
     dist = bd;
 }
 
-// Exact scan for the rows the certificate could not settle (four rows per
-// 256-thread block, one wave each).
+// Exact scan for the rows the certificate could not settle: one wave per
+// listed row, four per 256-thread block, the grid striding over the count
+// the filter left on the device (so the host need not read it first).
 template <typename TX>
 __global__ __launch_bounds__(256) void km_fallback_kernel(
     const TX *__restrict__ X, const double *__restrict__ C, int d, int64_t k,
     const int32_t *__restrict__ list, const int32_t *__restrict__ count,
     int32_t *__restrict__ labels, double *__restrict__ best) {
-    const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    if (t >= *count) return;
-    const int64_t i = list[t];
-    int32_t l;
-    double v;
-    wave_exact_argmin(X + i * d, C, d, k, lane, l, v);
-    if (lane == 0) {
-        labels[i] = l;
-        best[i] = v;
+    const int64_t m = *count;
+    for (int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); t < m;
+         t += (int64_t)gridDim.x * 4) {
+        const int64_t i = list[t];
+        int32_t l;
+        double v;
+        wave_exact_argmin(X + i * d, C, d, k, lane, l, v);
+        if (lane == 0) {
+            labels[i] = l;
+            best[i] = v;
+        }
+    }
+}
+
+// Feature-major copy of the centres: Ct[j][c] = C[c][j].
+static __global__ __attribute__((unused)) void km_transpose_kernel(const double *__restrict__ C, int64_t k, int d,
+                                    double *__restrict__ Ct) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= k * d) return;
+    const int64_t j = t / k, c = t % k;
+    Ct[t] = C[c * d + j];
+}
+
+// Same scan with the row staged in LDS as f64 (dynamic LDS: 4 * d
+// doubles), the centres read from a feature-major copy Ct (d x k, so the
+// 64 lanes' centres are adjacent and every load is coalesced), and two
+// centres folded per lane step (c and c + 64, two independent left folds).
+// Within a lane the centres are visited in increasing order and replace
+// the best only on a strictly smaller fold, as in wave_exact_argmin.
+template <typename TX>
+__global__ __launch_bounds__(256) void km_fallback_lds_kernel(
+    const TX *__restrict__ X, const double *__restrict__ Ct, int d, int64_t k,
+    const int32_t *__restrict__ list, const int32_t *__restrict__ count,
+    int32_t *__restrict__ labels, double *__restrict__ best) {
+    extern __shared__ double rowbuf[];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double *xr = rowbuf + (int64_t)w * d;
+    const int64_t m = *count;
+    for (int64_t t = (int64_t)blockIdx.x * 4 + w; t < m; t += (int64_t)gridDim.x * 4) {
+        const int64_t i = list[t];
+        for (int j = lane; j < d; j += 64) xr[j] = (double)X[i * d + j];
+        wave_sync_lds();
+        double bd = 1.7976931348623157e308;
+        int64_t bl = -1;
+        int64_t c = lane;
+        for (; c + 64 < k; c += 128) {
+            const double *c0 = Ct + c, *c1 = Ct + c + 64;
+            double a0 = 0.0, a1 = 0.0;
+#pragma unroll 8
+            for (int j = 0; j < d; ++j) {
+                const double x = xr[j];
+                const double t0 = x - c0[(int64_t)j * k];
+                const double t1 = x - c1[(int64_t)j * k];
+                a0 = a0 + t0 * t0;
+                a1 = a1 + t1 * t1;
+            }
+            if (a0 < bd) {
+                bd = a0;
+                bl = c;
+            }
+            if (a1 < bd) {
+                bd = a1;
+                bl = c + 64;
+            }
+        }
+        for (; c < k; c += 64) {
+            const double *c0 = Ct + c;
+            double a0 = 0.0;
+            for (int j = 0; j < d; ++j) {
+                const double t0 = xr[j] - c0[(int64_t)j * k];
+                a0 = a0 + t0 * t0;
+            }
+            if (a0 < bd) {
+                bd = a0;
+                bl = c;
+            }
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            const double od = __shfl_xor(bd, o);
+            const int64_t ol = __shfl_xor(bl, o);
+            if (ol >= 0 && (bl < 0 || od < bd || (od == bd && ol < bl))) {
+                bd = od;
+                bl = ol;
+            }
+        }
+        if (lane == 0) {
+            labels[i] = bl < 0 ? 0 : (int32_t)bl;
+            best[i] = bd;
+        }
+        wave_sync_lds();
     }
 }
 
@@ -269,6 +351,13 @@ static __global__ __attribute__((unused)) void runs_sum_kernel(const double *__r
 // feature) folds its cluster's rows left to right from +0.
 constexpr int UPD_RB = 4096;   // rows per counting block
 
+// Bits that distinguish labels 0..k-1.
+inline int label_bits(int64_t k) {
+    int b = 0;
+    while (b < 62 && ((int64_t)1 << b) < k) ++b;
+    return b;
+}
+
 static __global__ __attribute__((unused)) __launch_bounds__(256) void km_block_hist_kernel(const int32_t *__restrict__ labels,
                                                             int64_t n, int64_t k,
                                                             int32_t *__restrict__ blockcnt) {
@@ -339,40 +428,38 @@ static __global__ __attribute__((unused)) __launch_bounds__(256) void km_offsets
     }
 }
 
-// One wave per counting block: lanes that share a label in a 64-row chunk
-// are grouped with ballots (rank = lanes before with that label, so the
-// scatter is stable); the first lane of each group reserves the group's
+// One wave per counting block: the lanes of a 64-row chunk that share a
+// label are found with one ballot per label bit (their mask is the AND of
+// the matching bit masks), rank = lanes before with that label, so the
+// scatter is stable; the first lane of each group reserves the group's
 // slots with one LDS atomic, and the chunks are taken in row order.
 static __global__ __attribute__((unused)) __launch_bounds__(64) void km_scatter_kernel(
-    const int32_t *__restrict__ labels, int64_t n, int64_t k, const int32_t *__restrict__ blockbase,
-    const int64_t *__restrict__ offset, int32_t *__restrict__ order) {
+    const int32_t *__restrict__ labels, int64_t n, int64_t k, int bits,
+    const int32_t *__restrict__ blockbase, const int64_t *__restrict__ offset,
+    int32_t *__restrict__ order) {
     extern __shared__ int64_t ctr[];
     const int lane = threadIdx.x;
     const int64_t b = blockIdx.x;
     for (int64_t c = lane; c < k; c += 64) ctr[c] = offset[c] + blockbase[b * k + c];
     wave_sync_lds();
     const int64_t b0 = b * UPD_RB;
+    const uint64_t below = (1ull << lane) - 1ull;
     for (int r0 = 0; r0 < UPD_RB; r0 += 64) {
         const int64_t i = b0 + r0 + lane;
         const bool valid = i < n;
-        const int32_t lab = valid ? labels[i] : -1;
-        uint64_t todo = __ballot(valid);
-        int rank = 0, cnt = 0, myleader = 0;
-        while (todo) {
-            const int leader = __ffsll((unsigned long long)todo) - 1;
-            const int32_t L = __shfl(lab, leader);
-            const uint64_t m = __ballot(valid && lab == L);
-            if (valid && lab == L) {
-                rank = __popcll(m & ((1ull << lane) - 1ull));
-                cnt = __popcll(m);
-                myleader = leader;
-            }
-            todo &= ~m;
+        const int32_t lab = valid ? labels[i] : 0;
+        uint64_t m = __ballot(valid);
+        for (int bit = 0; bit < bits; ++bit) {
+            const bool on = (lab >> bit) & 1;
+            const uint64_t bb = __ballot(valid && on);
+            m &= on ? bb : ~bb;
         }
+        const int rank = __popcll(m & below);
+        const int leader = __ffsll((unsigned long long)m) - 1;
         unsigned long long base = 0;
-        if (valid && myleader == lane)
-            base = atomicAdd((unsigned long long *)&ctr[lab], (unsigned long long)cnt);
-        const unsigned long long gb = __shfl(base, myleader);
+        if (valid && leader == lane)
+            base = atomicAdd((unsigned long long *)&ctr[lab], (unsigned long long)__popcll(m));
+        const unsigned long long gb = __shfl(base, valid ? leader : lane);
         if (valid) order[(int64_t)gb + rank] = (int32_t)i;
     }
 }

@@ -294,7 +294,7 @@ void counting_sort(const int32_t *labels, int64_t n, int64_t k, int32_t *order, 
                        blockbase, count);
     hipLaunchKernelGGL(km_offsets_kernel, dim3(1), dim3(256), 0, st, count, k, first);
     hipLaunchKernelGGL(km_scatter_kernel, dim3((unsigned)nblk), dim3(64), ctr_b, st, labels, n, k,
-                       blockbase, first, order);
+                       label_bits(k), blockbase, first, order);
     check_launch();
 }
 
