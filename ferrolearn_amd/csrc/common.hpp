@@ -89,6 +89,15 @@ __device__ __forceinline__ double fold_exact(const TA *a, const TB *b, int d) {
     return acc;
 }
 
+// Lane u's double, to every lane of the wave (u wave-uniform): two
+// v_readlane_b32, which cost far less than a shuffle through LDS.
+__device__ __forceinline__ double wave_bcast(double v, int u) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)b, u);
+    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(b >> 32), u);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
 // (key, idx) lexicographic order; NaN keys compare false like Rust's `<`.
 __device__ __forceinline__ bool pair_less(double ka, int64_t ia, double kb,
                                           int64_t ib) {

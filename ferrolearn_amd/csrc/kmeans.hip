@@ -10,6 +10,7 @@
 #include <memory>
 #include <unordered_map>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "host.hpp"
@@ -312,6 +313,99 @@ class Lloyd {
         Rng rng(seed);
         kmeanspp(rng, out);
     }
+    // k-means++ with every trial of a centre in one pass over X (the same
+    // draws, sums, searches and choices as the per-trial passes below):
+    // block-interleaved distance arrays, one array per trial plus md.
+    void kmeanspp_batched(Rng &rng, double *out, int64_t nt) {
+        const int64_t nblk = ceil_div(n_, KPP_BLK);
+        const int64_t stride = nblk * KPP_BLK;
+        double *pool = ws_.get<double>("kpp_pool", (nt + 1) * stride);
+        double *bs = ws_.get<double>("kpp_bs", (nt + 1) * nblk);
+        double *tot = ws_.get<double>("kpp_tot", nt + 1);
+        double *thr = ws_.get<double>("kpp_thr", nt);
+        int64_t *cand = ws_.get<int64_t>("kpp_cand", nt);
+        double *ndarr = ws_.get<double>("kpp_nd", nt * stride);
+        double *ndbs = ws_.get<double>("kpp_ndbs", nt * nblk);
+        double *md = pool;        // current min distances, interleaved
+        double *mdbs = bs;        // their block sums
+        hipLaunchKernelGGL(kpp_first_il_kernel<TX>, dim3(grid1(stride, 256)), dim3(256), 0, st_,
+                           X_, n_, d_, out, nblk, md);
+        hipLaunchKernelGGL(kpp_bsums_il_kernel, dim3(grid1(nblk, 256)), dim3(256), 0, st_, md,
+                           nblk, 1, mdbs);
+        hipLaunchKernelGGL(kpp_totals_kernel, dim3(1), dim3(64), 0, st_, mdbs, nblk, tot);
+        check_launch();
+        double total = 0.0;
+        FL_HIP(hipMemcpyAsync(&total, tot, 8, hipMemcpyDeviceToHost, st_));
+        FL_HIP(hipStreamSynchronize(st_));
+        std::vector<double> hthr((size_t)nt), hpot((size_t)nt);
+        std::vector<int64_t> hcand((size_t)nt);
+        // rows per distance block in one tile: LDS for the tile and the
+        // candidates within 80 KB
+        int I = 4;
+        while (I > 1 && (size_t)64 * I * (d_ + 1) * sizeof(TX) + (size_t)nt * d_ * 8 > 81920)
+            I /= 2;
+        const size_t sm = (size_t)nt * d_ * 8 + (size_t)64 * I * (d_ + 1) * sizeof(TX);
+        const dim3 tgrid((unsigned)ceil_div(nblk, 64), (unsigned)(KPP_BLK / I));
+        auto trials = [&](auto ntc) {
+            constexpr int NT = decltype(ntc)::value;
+            FL_HIP(hipFuncSetAttribute((const void *)kpp_trials_il_kernel<TX, NT>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+            hipLaunchKernelGGL((kpp_trials_il_kernel<TX, NT>), tgrid, dim3(64 * I), sm, st_, X_, n_,
+                               d_, cand, nblk, I, md, ndarr);
+        };
+        auto launch_trials = [&]() {
+            switch (nt) {
+                case 1: return trials(std::integral_constant<int, 1>());
+                case 2: return trials(std::integral_constant<int, 2>());
+                case 3: return trials(std::integral_constant<int, 3>());
+                case 4: return trials(std::integral_constant<int, 4>());
+                case 5: return trials(std::integral_constant<int, 5>());
+                case 6: return trials(std::integral_constant<int, 6>());
+                case 7: return trials(std::integral_constant<int, 7>());
+                case 8: return trials(std::integral_constant<int, 8>());
+                case 9: return trials(std::integral_constant<int, 9>());
+                case 10: return trials(std::integral_constant<int, 10>());
+                case 11: return trials(std::integral_constant<int, 11>());
+            }
+            throw DeviceError("internal: unsupported k-means++ trial count");
+        };
+        for (int64_t c = 1; c < k_; ++c) {
+            if (total <= 0.0) {
+                const int64_t idx = rng.range(n_);
+                hipLaunchKernelGGL(copy_row_f64_kernel<TX>, dim3(grid1(d_, 64)), dim3(64), 0, st_,
+                                   X_, idx, d_, out + c * d_);
+                check_launch();
+                continue;
+            }
+            for (int64_t t = 0; t < nt; ++t) hthr[(size_t)t] = rng.f64() * total;
+            FL_HIP(hipMemcpyAsync(thr, hthr.data(), (size_t)nt * 8, hipMemcpyHostToDevice, st_));
+            hipLaunchKernelGGL(kpp_search_il_kernel, dim3((unsigned)nt), dim3(64), 0, st_, md, n_,
+                               mdbs, nblk, thr, cand);
+            launch_trials();
+            hipLaunchKernelGGL(kpp_bsums_il_kernel, dim3(grid1(nt * nblk, 256)), dim3(256), 0,
+                               st_, ndarr, nblk, (int)nt, ndbs);
+            hipLaunchKernelGGL(kpp_totals_kernel, dim3((unsigned)nt), dim3(64), 0, st_, ndbs,
+                               nblk, tot);
+            check_launch();
+            FL_HIP(hipMemcpyAsync(hcand.data(), cand, (size_t)nt * 8, hipMemcpyDeviceToHost, st_));
+            FL_HIP(hipMemcpyAsync(hpot.data(), tot, (size_t)nt * 8, hipMemcpyDeviceToHost, st_));
+            FL_HIP(hipStreamSynchronize(st_));
+            // kmeans.rs:169-253: the first trial, then any strictly smaller potential
+            int64_t bt = 0;
+            for (int64_t t = 1; t < nt; ++t)
+                if (hpot[(size_t)t] < hpot[(size_t)bt]) bt = t;
+            FL_HIP(hipMemcpyAsync(md, ndarr + bt * stride, (size_t)stride * 8,
+                                  hipMemcpyDeviceToDevice, st_));
+            FL_HIP(hipMemcpyAsync(mdbs, ndbs + bt * nblk, (size_t)nblk * 8,
+                                  hipMemcpyDeviceToDevice, st_));
+            hipLaunchKernelGGL(copy_row_f64_kernel<TX>, dim3(grid1(d_, 64)), dim3(64), 0, st_, X_,
+                               hcand[(size_t)bt], d_, out + c * d_);
+            check_launch();
+            total = hpot[(size_t)bt];
+        }
+        FL_HIP(hipStreamSynchronize(st_));
+    }
+
     // Same, drawing from a generator the caller continues to use afterwards
     // (MiniBatchKMeans samples its batches from the same stream).
     void kmeanspp(Rng &rng, double *out) {
@@ -337,6 +431,11 @@ class Lloyd {
         int64_t n_trials = 2 + (int64_t)std::floor(std::log((double)k_));
         if (n_trials < 1) n_trials = 1;
         if (n_trials > 64) n_trials = 64;
+        if (n_trials <= KPP_MAXT &&
+            (size_t)n_trials * d_ * 8 + (size_t)64 * (d_ + 1) * sizeof(TX) <= 81920) {
+            kmeanspp_batched(rng, out, n_trials);
+            return;
+        }
         std::vector<double> hthr((size_t)n_trials);
         std::vector<int64_t> hcand((size_t)n_trials);
         auto total_of = [&](const double *v) {

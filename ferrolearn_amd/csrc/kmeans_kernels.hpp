@@ -340,7 +340,7 @@ static __global__ __attribute__((unused)) void runs_sum_kernel(const double *__r
     double acc = 0.0;
     for (int64_t i = lo; i < hi; ++i) acc = acc + part[i];
     double tot = 0.0;
-    for (int l = 0; l < 64; ++l) tot = tot + __shfl(acc, l);
+    for (int l = 0; l < 64; ++l) tot = tot + wave_bcast(acc, l);
     if (lane == 0) *out = tot;
 }
 
@@ -602,6 +602,158 @@ static __global__ __attribute__((unused)) void kpp_search_kernel(const double *_
         run = run + s;
     }
     cand[t] = res;
+}
+
+// Batched form of the passes above, for d small enough to stage every
+// trial's candidate row in LDS.  Distances are stored block-interleaved:
+// the value of row b * KPP_BLK + i lives at [i * nblk + b] (rows past n
+// hold +0, which changes no sum and never reaches a threshold first), so
+// that one lane per block folds its block in row order with coalesced
+// loads.  All sums keep the two-level order of the passes above.
+constexpr int KPP_MAXT = 11;   // trials per batched pass (n_trials <= 11 for k <= 8192)
+
+template <typename TX>
+__global__ void kpp_first_il_kernel(const TX *__restrict__ X, int64_t n, int d,
+                                    const double *__restrict__ c0, int64_t nblk,
+                                    double *__restrict__ md) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nblk * KPP_BLK) return;
+    const int64_t r = (t % nblk) * KPP_BLK + t / nblk;
+    md[t] = r < n ? fold_exact(X + r * d, c0, d) : 0.0;
+}
+
+// nd_t = min(md, fold(x, cand_t)) for every trial t in one pass over X
+// (the reference's `if d < nd`).  A block (64 * I threads) takes I rows from
+// each of 64 consecutive distance blocks: the rows are staged in LDS with
+// whole-row coalesced loads (row stride d + 1 against bank conflicts),
+// thread (ii, bb) folds row (b0 + bb) * KPP_BLK + i0 + ii against the
+// candidate rows (LDS, f64), and its results land at consecutive
+// interleaved addresses.
+template <typename TX, int NT>
+__global__ __launch_bounds__(256) void kpp_trials_il_kernel(
+    const TX *__restrict__ X, int64_t n, int d, const int64_t *__restrict__ cand,
+    int64_t nblk, int I, const double *__restrict__ md, double *__restrict__ nd) {
+    constexpr int nt = NT;
+    extern __shared__ double cr[];                            // nt * d, then the row tile
+    TX *tile = reinterpret_cast<TX *>(cr + (int64_t)nt * d);  // (64 * I) x (d + 1)
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    for (int u = t; u < nt * d; u += blockDim.x) cr[u] = (double)X[cand[u / d] * d + u % d];
+    const int64_t b0 = (int64_t)blockIdx.x * 64;
+    const int64_t i0 = (int64_t)blockIdx.y * I;
+    // wave w stages rows slot = ii * 64 + bb for ii == w, 32 rows' loads in
+    // flight before their LDS stores
+    for (int j = lane; j < d; j += 64)
+        for (int bb0 = 0; bb0 < 64; bb0 += 32) {
+            TX v[32];
+#pragma unroll
+            for (int u = 0; u < 32; ++u) {
+                // unconditional load (row 0 stands in for rows past n), then select
+                const int64_t r = (b0 + bb0 + u) * KPP_BLK + i0 + w;
+                const bool ok = b0 + bb0 + u < nblk && r < n;
+                const TX x = X[(ok ? r : 0) * d + j];
+                v[u] = ok ? x : (TX)0;
+            }
+#pragma unroll
+            for (int u = 0; u < 32; ++u) tile[(int64_t)(w * 64 + bb0 + u) * (d + 1) + j] = v[u];
+        }
+    __syncthreads();
+    const int ii = w, bb = lane;
+    const int64_t b = b0 + bb;
+    if (b >= nblk) return;
+    const int64_t stride = nblk * KPP_BLK;
+    const int64_t slot = (i0 + ii) * nblk + b;
+    const int64_t r = b * KPP_BLK + i0 + ii;
+    if (r >= n) {
+        for (int q = 0; q < nt; ++q) nd[q * stride + slot] = 0.0;
+        return;
+    }
+    const TX *x = tile + (int64_t)(ii * 64 + bb) * (d + 1);
+    double acc[NT];
+#pragma unroll
+    for (int q = 0; q < NT; ++q) acc[q] = 0.0;
+    for (int j = 0; j < d; ++j) {
+        const double xj = (double)x[j];
+#pragma unroll
+        for (int q = 0; q < NT; ++q) {
+            const double df = xj - cr[q * d + j];
+            acc[q] = acc[q] + df * df;
+        }
+    }
+    const double m = md[slot];
+#pragma unroll
+    for (int q = 0; q < NT; ++q) nd[q * stride + slot] = acc[q] < m ? acc[q] : m;
+}
+
+// Block sums of nt interleaved arrays: lane (q, b) folds block b of array q
+// in row order.
+static __global__ __attribute__((unused)) void kpp_bsums_il_kernel(const double *__restrict__ v,
+                                                                   int64_t nblk, int nt,
+                                                                   double *__restrict__ bsum) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nblk * nt) return;
+    const int64_t q = t / nblk, b = t % nblk;
+    const double *p = v + q * nblk * KPP_BLK + b;
+    double s = 0.0;
+    for (int i0 = 0; i0 < KPP_BLK; i0 += 32) {
+        double w[32];
+#pragma unroll
+        for (int u = 0; u < 32; ++u) w[u] = p[(int64_t)(i0 + u) * nblk];
+#pragma unroll
+        for (int u = 0; u < 32; ++u) s = s + w[u];
+    }
+    bsum[t] = s;
+}
+
+// tot[q] = the block sums of array q folded in order (one wave per array;
+// 64 values are loaded at once and added in order through shuffles).
+static __global__ __attribute__((unused)) __launch_bounds__(64) void kpp_totals_kernel(
+    const double *__restrict__ bsum, int64_t nblk, double *__restrict__ tot) {
+    const int lane = threadIdx.x;
+    const double *p = bsum + (int64_t)blockIdx.x * nblk;
+    double acc = 0.0;
+    for (int64_t b0 = 0; b0 < nblk; b0 += 64) {
+        const double v = b0 + lane < nblk ? p[b0 + lane] : 0.0;
+        const int m = nblk - b0 < 64 ? (int)(nblk - b0) : 64;
+        for (int u = 0; u < m; ++u) acc = acc + wave_bcast(v, u);
+    }
+    if (lane == 0) tot[blockIdx.x] = acc;
+}
+
+// The search of kpp_search_kernel on an interleaved array: one wave per
+// threshold, values broadcast through shuffles so that every lane follows
+// the same sequential running sums.
+static __global__ __attribute__((unused)) __launch_bounds__(64) void kpp_search_il_kernel(
+    const double *__restrict__ v, int64_t n, const double *__restrict__ bsum, int64_t nblk,
+    const double *__restrict__ thr, int64_t *__restrict__ cand) {
+    const int lane = threadIdx.x;
+    const double th = thr[blockIdx.x];
+    double run = 0.0;
+    int64_t res = n - 1;
+    bool done = false;
+    for (int64_t b0 = 0; b0 < nblk && !done; b0 += 64) {
+        const double sv = b0 + lane < nblk ? bsum[b0 + lane] : 0.0;
+        const int m = nblk - b0 < 64 ? (int)(nblk - b0) : 64;
+        for (int u = 0; u < m && !done; ++u) {
+            const double s = wave_bcast(sv, u);
+            const int64_t b = b0 + u;
+            if (run + s >= th) {
+                double c = run;
+                for (int i0 = 0; i0 < KPP_BLK && !done; i0 += 64) {
+                    const double w = v[(int64_t)(i0 + lane) * nblk + b];
+                    for (int uu = 0; uu < 64; ++uu) {
+                        c = c + wave_bcast(w, uu);
+                        if (c >= th) {
+                            res = b * KPP_BLK + i0 + uu;
+                            done = true;
+                            break;
+                        }
+                    }
+                }
+            }
+            run = run + s;
+        }
+    }
+    if (lane == 0) cand[blockIdx.x] = res;
 }
 
 }  // namespace fl

@@ -155,11 +155,22 @@ def test_empty_cluster_keeps_centre_and_zero_iter():
 
 
 def test_kmeanspp_matches_oracle():
-    for (n, d, k, seed) in ((3000, 5, 9, 42), (10_000, 16, 33, 7), (9000, 3, 1, 5)):
+    # many 4096-row blocks with a partial last one, and a large d that
+    # takes the per-trial passes instead of the batched one
+    for (n, d, k, seed) in ((3000, 5, 9, 42), (10_000, 16, 33, 7), (9000, 3, 1, 5),
+                            (50_001, 8, 20, 3), (6000, 900, 4, 2)):
         X = blobs(n, d, max(k, 2), seed=seed)
         got = F.kmeans_plusplus(X, k, seed)
         want = O.kmeanspp(X, k, seed)
-        assert (bits(got) == bits(want)).all()
+        assert (bits(got) == bits(want)).all(), (n, d, k)
+
+
+def test_kmeanspp_zero_potential():
+    # identical rows: the potential is 0 and the reference draws a random row
+    X = np.ones((5000, 4))
+    got = F.kmeans_plusplus(X, 5, 9)
+    want = O.kmeanspp(X, 5, 9)
+    assert (bits(got) == bits(want)).all()
 
 
 def test_fit_without_init_uses_library_kmeanspp():
