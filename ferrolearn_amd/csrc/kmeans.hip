@@ -251,19 +251,20 @@ class Lloyd {
         // exact scan of the uncertified rows; the grid strides over the
         // device-side count (zero rows: every wave exits at once)
         const int t2 = tbegin();
-        const dim3 fgrid((unsigned)std::min<int64_t>(grid1(n_, 4), 2048));
-        if (d_ <= 2048) {
+        if (d_ <= 8192) {
             double *Ct = ws_.get<double>("ctrans", k_ * d_);
             hipLaunchKernelGGL(km_transpose_kernel, dim3(grid1(k_ * d_, 256)), dim3(256), 0, st_,
                                C, k_, d_, Ct);
-            const size_t sm = (size_t)4 * d_ * 8;
+            const size_t sm = (size_t)d_ * 8;
             FL_HIP(hipFuncSetAttribute((const void *)km_fallback_lds_kernel<TX>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
-            hipLaunchKernelGGL(km_fallback_lds_kernel<TX>, fgrid, dim3(256), sm, st_, X_, Ct, d_,
-                               k_, flags, fcnt, labels, best);
+            hipLaunchKernelGGL(km_fallback_lds_kernel<TX>,
+                               dim3((unsigned)std::min<int64_t>(n_, 8192)), dim3(256), sm, st_, X_,
+                               Ct, d_, k_, flags, fcnt, labels, best);
         } else {
-            hipLaunchKernelGGL(km_fallback_kernel<TX>, fgrid, dim3(256), 0, st_, X_, C, d_, k_,
-                               flags, fcnt, labels, best);
+            hipLaunchKernelGGL(km_fallback_kernel<TX>,
+                               dim3((unsigned)std::min<int64_t>(grid1(n_, 4), 2048)), dim3(256), 0,
+                               st_, X_, C, d_, k_, flags, fcnt, labels, best);
         }
         check_launch();
         tend(t2, &acc.ms_fb);
@@ -302,8 +303,11 @@ class Lloyd {
                            n_, k_, label_bits(k_), blockbase, offset, order);
         hipLaunchKernelGGL(km_sum_kernel<TX>, dim3((unsigned)k_, (unsigned)ceil_div(d_, 64)),
                            dim3(64), 0, st_, X_, d_, order, offset, count, oldc, newc);
-        hipLaunchKernelGGL(km_shift_kernel, dim3(1), dim3(256), 0, st_, newc, oldc, k_, d_,
-                           shift);
+        unsigned long long *mx = ws_.get<unsigned long long>("shift_max", 1);
+        FL_HIP(hipMemsetAsync(mx, 0, 8, st_));
+        hipLaunchKernelGGL(km_shift_kernel, dim3(grid1(k_, 64)), dim3(64), 0, st_, newc, oldc, k_,
+                           d_, mx);
+        hipLaunchKernelGGL(km_shift_finish, dim3(1), dim3(64), 0, st_, mx, shift);
         check_launch();
         tend(t, &acc.ms_upd);
     }
@@ -730,7 +734,11 @@ fl_status mb_fit_device(const TX *X, int64_t n, int d, const fl_minibatch_params
             FL_HIP(hipMemcpyAsync(old, cur, (size_t)(k * d) * 8, hipMemcpyDeviceToDevice, st));
             hipLaunchKernelGGL(mb_update_kernel<TX>, dim3((unsigned)k), dim3(64), 0, st, X, d,
                                bidx, blab, b, cur, counts);
-            hipLaunchKernelGGL(km_shift_kernel, dim3(1), dim3(256), 0, st, cur, old, k, d, dsc);
+            unsigned long long *mx = ws.get<unsigned long long>("shift_max", 1);
+            FL_HIP(hipMemsetAsync(mx, 0, 8, st));
+            hipLaunchKernelGGL(km_shift_kernel, dim3(grid1(k, 64)), dim3(64), 0, st, cur, old, k,
+                               d, mx);
+            hipLaunchKernelGGL(km_shift_finish, dim3(1), dim3(64), 0, st, mx, dsc);
             check_launch();
             tm.end(t0);
             double shift = 0.0;

@@ -151,30 +151,33 @@ static __global__ __attribute__((unused)) void km_transpose_kernel(const double 
     Ct[t] = C[c * d + j];
 }
 
-// Same scan with the row staged in LDS as f64 (dynamic LDS: 4 * d
-// doubles), the centres read from a feature-major copy Ct (d x k, so the
-// 64 lanes' centres are adjacent and every load is coalesced), and two
-// centres folded per lane step (c and c + 64, two independent left folds).
-// Within a lane the centres are visited in increasing order and replace
-// the best only on a strictly smaller fold, as in wave_exact_argmin.
+// Same scan, one 256-thread block per listed row: the row is staged in LDS
+// as f64 (dynamic LDS: d doubles), the centres are read from a
+// feature-major copy Ct (d x k, so adjacent lanes read adjacent centres),
+// and thread t folds centres t, t + 256, ... two at a time (two independent
+// left folds).  Each thread visits its centres in increasing order and
+// replaces its best only on a strictly smaller fold; the waves' and then
+// the block's (fold, index) minima are combined with the lowest index
+// winning ties, as in wave_exact_argmin.  The grid strides over the count.
 template <typename TX>
 __global__ __launch_bounds__(256) void km_fallback_lds_kernel(
     const TX *__restrict__ X, const double *__restrict__ Ct, int d, int64_t k,
     const int32_t *__restrict__ list, const int32_t *__restrict__ count,
     int32_t *__restrict__ labels, double *__restrict__ best) {
-    extern __shared__ double rowbuf[];
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    double *xr = rowbuf + (int64_t)w * d;
+    extern __shared__ double xr[];
+    __shared__ double wd[4];
+    __shared__ int64_t wl[4];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int64_t m = *count;
-    for (int64_t t = (int64_t)blockIdx.x * 4 + w; t < m; t += (int64_t)gridDim.x * 4) {
-        const int64_t i = list[t];
-        for (int j = lane; j < d; j += 64) xr[j] = (double)X[i * d + j];
-        wave_sync_lds();
+    for (int64_t e = blockIdx.x; e < m; e += gridDim.x) {
+        const int64_t i = list[e];
+        for (int j = t; j < d; j += 256) xr[j] = (double)X[i * d + j];
+        __syncthreads();
         double bd = 1.7976931348623157e308;
         int64_t bl = -1;
-        int64_t c = lane;
-        for (; c + 64 < k; c += 128) {
-            const double *c0 = Ct + c, *c1 = Ct + c + 64;
+        int64_t c = t;
+        for (; c + 256 < k; c += 512) {
+            const double *c0 = Ct + c, *c1 = Ct + c + 256;
             double a0 = 0.0, a1 = 0.0;
 #pragma unroll 8
             for (int j = 0; j < d; ++j) {
@@ -190,10 +193,10 @@ __global__ __launch_bounds__(256) void km_fallback_lds_kernel(
             }
             if (a1 < bd) {
                 bd = a1;
-                bl = c + 64;
+                bl = c + 256;
             }
         }
-        for (; c < k; c += 64) {
+        for (; c < k; c += 256) {
             const double *c0 = Ct + c;
             double a0 = 0.0;
             for (int j = 0; j < d; ++j) {
@@ -214,10 +217,23 @@ __global__ __launch_bounds__(256) void km_fallback_lds_kernel(
             }
         }
         if (lane == 0) {
+            wd[w] = bd;
+            wl[w] = bl;
+        }
+        __syncthreads();
+        if (t == 0) {
+            for (int v = 1; v < 4; ++v) {
+                const double od = wd[v];
+                const int64_t ol = wl[v];
+                if (ol >= 0 && (bl < 0 || od < bd || (od == bd && ol < bl))) {
+                    bd = od;
+                    bl = ol;
+                }
+            }
             labels[i] = bl < 0 ? 0 : (int32_t)bl;
             best[i] = bd;
         }
-        wave_sync_lds();
+        __syncthreads();
     }
 }
 
@@ -333,12 +349,24 @@ static __global__ __attribute__((unused)) void seq_sum_kernel(const double *__re
 // launched with one wave).
 static __global__ __attribute__((unused)) void runs_sum_kernel(const double *__restrict__ part,
                                                                 int64_t m, double *__restrict__ out) {
+    __shared__ double stage[8192];
     const int lane = threadIdx.x;
     if (blockIdx.x != 0 || lane >= 64) return;
     const int64_t per = (m + 63) / 64;
     const int64_t lo = lane * per, hi = lo + per < m ? lo + per : m;
     double acc = 0.0;
-    for (int64_t i = lo; i < hi; ++i) acc = acc + part[i];
+    // the partials pass through LDS 8192 at a time (coalesced loads); each
+    // lane folds the part of its run inside the staged window, in order
+    for (int64_t s0 = 0; s0 < m; s0 += 8192) {
+        const int64_t s1 = s0 + 8192 < m ? s0 + 8192 : m;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        for (int64_t i = s0 + lane; i < s1; i += 64) stage[i - s0] = part[i];
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        const int64_t a = lo > s0 ? lo : s0, b = hi < s1 ? hi : s1;
+        for (int64_t i = a; i < b; ++i) acc = acc + stage[i - s0];
+    }
     double tot = 0.0;
     for (int l = 0; l < 64; ++l) tot = tot + wave_bcast(acc, l);
     if (lane == 0) *out = tot;
@@ -503,25 +531,22 @@ __global__ __launch_bounds__(64) void km_sum_kernel(const TX *__restrict__ X, in
 }
 
 // max_shift = sqrt(max_c fold(new_c, old_c)) (kmeans.rs:389-400); the max
-// ignores NaN folds the way `if shift > max_shift` does.
-static __global__ __attribute__((unused)) __launch_bounds__(256) void km_shift_kernel(const double *__restrict__ newc,
-                                                       const double *__restrict__ oldc,
-                                                       int64_t k, int d,
-                                                       double *__restrict__ shift) {
-    __shared__ double s[256];
-    double mx = 0.0;
-    for (int64_t c = threadIdx.x; c < k; c += 256) {
-        const double f = fold_exact(newc + c * d, oldc + c * d, d);
-        if (f > mx) mx = f;
-    }
-    s[threadIdx.x] = mx;
-    __syncthreads();
-    for (int h = 128; h > 0; h >>= 1) {
-        if ((int)threadIdx.x < h && s[threadIdx.x + h] > s[threadIdx.x])
-            s[threadIdx.x] = s[threadIdx.x + h];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) *shift = sqrt(s[0]);
+// ignores NaN folds the way `if shift > max_shift` does.  One thread per
+// centre; the maximum of the non-negative folds is taken on their bit
+// patterns (zeroed before the launch), and km_shift_finish takes the sqrt.
+static __global__ __attribute__((unused)) void km_shift_kernel(const double *__restrict__ newc,
+                                                               const double *__restrict__ oldc,
+                                                               int64_t k, int d,
+                                                               unsigned long long *__restrict__ mx) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= k) return;
+    const double f = fold_exact(newc + c * d, oldc + c * d, d);
+    if (f > 0.0) atomicMax(mx, (unsigned long long)__double_as_longlong(f));
+}
+
+static __global__ __attribute__((unused)) void km_shift_finish(
+    const unsigned long long *__restrict__ mx, double *__restrict__ shift) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *shift = sqrt(__longlong_as_double((long long)*mx));
 }
 
 // --------------------------------------------------------------- transform --

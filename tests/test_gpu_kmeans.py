@@ -103,6 +103,8 @@ def test_sklearn_fixture_assign():
     (30_000, 64, 200, np.float32, "bf16x3"),
     (30_000, 64, 200, np.float32, "bf16"),      # single product: more fallbacks
     (9_000, 128, 300, np.float32, "bf16x3"),
+    (20_000, 20, 30, np.float32, "fp64"),       # f32 centroid sums, 8 lanes per row
+    (20_000, 4, 10, np.float32, "bf16x3"),      # one lane per row
 ])
 def test_lloyd_matches_oracle(n, d, k, dtype, precision):
     X = blobs(n, d, k, seed=n + k, spread=1.0, dtype=dtype)
