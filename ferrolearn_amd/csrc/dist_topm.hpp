@@ -465,10 +465,49 @@ __global__ __launch_bounds__(64) void knn_finalize_kernel(
     }
     __syncthreads();
 
+    // The exact form's keys are the reference distances sqrt(fold) and its
+    // candidate indices are distinct: the answer is the k smallest
+    // (distance, index) pairs, taken by k wave-wide minimum passes, each
+    // above the previous pair.  A NaN key flags the query (the host raises).
+    if (!certify) {
+        bool nan = false;
+        for (int e = lane; e < nc; e += 64) nan |= ck[e] != ck[e];
+        if (__ballot(nan) != 0 || !complete || nc < k) {
+            if (lane == 0) flag_list[atomicAdd(flag_count, 1)] = (int32_t)q;
+            return;
+        }
+        double lk = -__builtin_huge_val();
+        int64_t li = -1;
+        for (int j = 0; j < k; ++j) {
+            double bk = __builtin_huge_val();
+            int64_t bi = INT64_MAX;
+            for (int e = lane; e < nc; e += 64)
+                if (pair_less(lk, li, ck[e], ci[e]) && pair_less(ck[e], ci[e], bk, bi)) {
+                    bk = ck[e];
+                    bi = ci[e];
+                }
+            for (int o = 32; o > 0; o >>= 1) {
+                const double ok2 = __shfl_xor(bk, o);
+                const int64_t oi2 = __shfl_xor(bi, o);
+                if (pair_less(ok2, oi2, bk, bi)) {
+                    bk = ok2;
+                    bi = oi2;
+                }
+            }
+            if (lane == 0) {
+                dist_out[q * k + j] = bk;
+                idx_out[q * k + j] = bi;
+            }
+            lk = bk;
+            li = bi;
+        }
+        if (lane == 0) atomicAdd(n_cand, (unsigned long long)nc);
+        return;
+    }
+
     // Filter keys are lower bounds of the exact folds, so the candidates are
     // re-scored in increasing key order: first the k+6 smallest; the rest
-    // only when the smallest key left could still reach the k-th fold.  (The
-    // exact form's keys are distances: everything is re-scored.)
+    // only when the smallest key left could still reach the k-th fold.
     for (int e = lane; e < nc; e += 64) {
         int r = 0;
         if (certify)
