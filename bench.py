@@ -432,7 +432,7 @@ def main():
                     help="fraction of the router workload's rows, cells and queries")
     ap.add_argument("--router-epochs", type=int, default=3)
     ap.add_argument("--router-brute-sample", type=int, default=100_000)
-    ap.add_argument("--n-probe", type=int, default=4)
+    ap.add_argument("--n-probe", type=int, default=2)
     ap.add_argument("--km-forms", default="bf16x3,fp64")
     ap.add_argument("--km-rows", type=int, default=10_000_000)
     ap.add_argument("--km-iters", type=int, default=50)

@@ -468,9 +468,9 @@ void launch_top_cells(const float *Z, int64_t ldz, int64_t rows, int C, int n_pr
 }
 
 // Routed search of nq device queries (one pass; nq * n_probe < 2^31).
-// Queries whose certificate fails are searched again over the
-// min(31, n_cells) cells with the smallest bounds (depth 1), and what still
-// fails goes to the brute force.
+// Queries whose certificate fails are searched again over more cells with
+// the smallest bounds (8 or twice n_probe, doubling up to min(31,
+// n_cells)), and what still fails goes to the brute force.
 template <typename TQ, typename TX>
 void route_search(fl_router *r, const TQ *Q, int64_t nq, int k, int n_probe, int route,
                   double *dist, int64_t *idx, RtAcc &acc, int depth = 0) {
@@ -592,11 +592,12 @@ void route_search(fl_router *r, const TQ *Q, int64_t nq, int k, int n_probe, int
     hipLaunchKernelGGL(rt_gather_queries_kernel<TQ>, dim3(grid1(m * d, 256)), dim3(256), 0, st, Q,
                        d, dw, m, fq);
     check_launch();
-    const int wide = std::min(kMaxProbe, C);
-    if (depth == 0 && (wide > n_probe || route != FL_ROUTE_BOUNDS)) {
-        acc.n_rescan += m;
+    // widening probes: max(8, 2 n_probe), doubling up to min(31, n_cells)
+    const int wide = std::min(std::min(std::max(8, 2 * n_probe), kMaxProbe), C);
+    if (depth < 3 && (wide > n_probe || route != FL_ROUTE_BOUNDS)) {
+        if (depth == 0) acc.n_rescan += m;
         RtAcc inner;
-        route_search<TQ, TX>(r, fq, m, k, wide, FL_ROUTE_BOUNDS, fd, fi, inner, 1);
+        route_search<TQ, TX>(r, fq, m, k, wide, FL_ROUTE_BOUNDS, fd, fi, inner, depth + 1);
         acc.rows += inner.rows;
         acc.n_fb += inner.n_fb;
     } else {
