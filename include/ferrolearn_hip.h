@@ -335,272 +335,6 @@ typedef struct fl_router_report {
     int64_t n_queries;
     int64_t n_rescan;     /* queries whose first certificate failed: searched
                              again over more cells with the smallest bounds */
-    int64_t n_fallback;     /* rows whose certificate failed (full scan)    */
-    int64_t n_iter;         /* Lloyd iterations run (KMeans)                */
-    int64_t n_dist_launches;/* launches of the distance kernel              */
-} fl_report;
-
-/* Last error context string of the calling thread ("" after success). */
-const char *fl_last_error(void);
-
-/* Library version: major*10000 + minor*100 + patch. */
-int fl_version(void);
-
-/* Number of visible HIP devices (0 when none; never fails). */
-int fl_device_count(void);
-
-/* Device memory helpers for callers that hold no GPU allocator of their own
- * (C / Rust hosts, tests): plain hipMalloc / hipMemcpy / hipFree on the
- * current device.  No reference counterpart. */
-fl_status fl_device_malloc(void **ptr, uint64_t bytes);
-fl_status fl_device_free(void *ptr);
-fl_status fl_memcpy_to_device(void *dst, const void *src, uint64_t bytes);
-fl_status fl_memcpy_to_host(void *dst, const void *src, uint64_t bytes);
-fl_status fl_memcpy_device(void *dst, const void *src, uint64_t bytes);
-fl_status fl_device_synchronize(void);
-
-/* ------------------------------------------------------------------ */
-/* KMeans -- ferrolearn-cluster/src/kmeans.rs                          */
-/* ------------------------------------------------------------------ */
-
-/* kmeans.rs:55-113 (KMeans builder).  Defaults from KMeans::new
- * (kmeans.rs:76-84): max_iter 300, tol 1e-4, n_init 10, no seed. */
-typedef struct fl_kmeans_params {
-    int64_t n_clusters;
-    int64_t max_iter;
-    double tol;
-    int64_t n_init;
-    uint64_t random_state;  /* used when has_random_state != 0 */
-    int32_t has_random_state;
-    int32_t precision;      /* fl_precision for the assignment filter */
-} fl_kmeans_params;
-
-/* kmeans.rs:76-84 (KMeans::new) */
-void fl_kmeans_params_default(fl_kmeans_params *p, int64_t n_clusters);
-
-/* Fit<Array2<F>, ()> for KMeans (kmeans.rs:403-513).
- * init_centers: NULL runs the library's k-means++ per restart (seed
- * random_state + run, kmeans.rs:451,460, with the library's own generator);
- * otherwise n_init*k*d f64 initial centres, one block per restart, which
- * makes the run comparable with any other implementation started from the
- * same centres.  Outputs: centers_out k*d (post-update centres of the kept
- * run), labels_out n, inertia_out, n_iter_out (kmeans.rs:468-503). */
-fl_status fl_kmeans_fit(const void *X, int32_t dtype, int64_t n, int64_t d,
-                        const fl_kmeans_params *params,
-                        const double *init_centers, double *centers_out,
-                        int64_t *labels_out, double *inertia_out,
-                        int64_t *n_iter_out, fl_report *report);
-
-/* Same as fl_kmeans_fit with X, init_centers and every output in device
- * memory (inertia_out and n_iter_out stay host pointers). */
-fl_status fl_kmeans_fit_device(const void *dX, int32_t dtype, int64_t n,
-                               int64_t d, const fl_kmeans_params *params,
-                               const double *d_init_centers,
-                               double *d_centers_out, int64_t *d_labels_out,
-                               double *inertia_out, int64_t *n_iter_out,
-                               fl_report *report);
-
-/* Predict<Array2<F>> for FittedKMeans (kmeans.rs:528-553): labels of X
- * against centres (k*d_centers f64).  ShapeMismatch if d != d_centers. */
-fl_status fl_kmeans_predict(const double *centers, int64_t k,
-                            int64_t d_centers, const void *X, int32_t dtype,
-                            int64_t n, int64_t d, int64_t *labels_out);
-
-/* Transform<Array2<F>> for FittedKMeans (kmeans.rs:555-618):
- * dist_out[i*k + c] = sqrt(fold(x_i, c_c)). */
-fl_status fl_kmeans_transform(const double *centers, int64_t k,
-                              int64_t d_centers, const void *X, int32_t dtype,
-                              int64_t n, int64_t d, double *dist_out);
-
-/* The library's k-means++ (kmeans.rs:169-253 restated with the library's
- * generator; see fl_kmeans_fit).  centers_out: k*d f64. */
-fl_status fl_kmeans_plusplus(const void *X, int32_t dtype, int64_t n,
-                             int64_t d, int64_t k, uint64_t seed,
-                             double *centers_out);
-
-/* ------------------------------------------------------------------ */
-/* MiniBatchKMeans -- ferrolearn-cluster/src/mini_batch_kmeans.rs      */
-/* ------------------------------------------------------------------ */
-
-/* mini_batch_kmeans.rs:77-93 (MiniBatchKMeans builder).  Defaults from
- * MiniBatchKMeans::new (mini_batch_kmeans.rs:108-118): batch_size 1024,
- * max_iter 100, tol 0, n_init 3, no seed. */
-typedef struct fl_minibatch_params {
-    int64_t n_clusters;
-    int64_t batch_size;
-    int64_t max_iter;
-    double tol;
-    int64_t n_init;
-    uint64_t random_state;  /* used when has_random_state != 0 */
-    int32_t has_random_state;
-    int32_t precision;      /* fl_precision for the final full assignment */
-} fl_minibatch_params;
-
-/* mini_batch_kmeans.rs:108-118 (MiniBatchKMeans::new) */
-void fl_minibatch_params_default(fl_minibatch_params *p, int64_t n_clusters);
-
-/* Fit<Array2<F>, ()> for MiniBatchKMeans (mini_batch_kmeans.rs:478-592).
- * Validation order: n_clusters == 0, batch_size == 0, n_init == 0 ->
- * InvalidParameter; n == 0 or n < n_clusters -> InsufficientSamples.  Run r
- * draws from the library's generator seeded with random_state + r*1000003:
- * k-means++ when init_centers is NULL (else n_init*k*d f64 initial centres),
- * then the batches.  Outputs as fl_kmeans_fit; the inertia is the left fold
- * of the final assignment's folds in row order (:366-373).  Predict and
- * transform of the fitted model are fl_kmeans_predict / fl_kmeans_transform
- * (mini_batch_kmeans.rs:607-682 is the same nearest-centre rule). */
-fl_status fl_minibatch_kmeans_fit(const void *X, int32_t dtype, int64_t n, int64_t d,
-                                  const fl_minibatch_params *params,
-                                  const double *init_centers, double *centers_out,
-                                  int64_t *labels_out, double *inertia_out,
-                                  int64_t *n_iter_out, fl_report *report);
-
-/* Same with X, init_centers and the centre / label outputs in device memory
- * (inertia_out and n_iter_out stay host pointers). */
-fl_status fl_minibatch_kmeans_fit_device(const void *dX, int32_t dtype, int64_t n,
-                                         int64_t d, const fl_minibatch_params *params,
-                                         const double *d_init_centers,
-                                         double *d_centers_out, int64_t *d_labels_out,
-                                         double *inertia_out, int64_t *n_iter_out,
-                                         fl_report *report);
-
-/* ------------------------------------------------------------------ */
-/* k-nearest neighbours -- ferrolearn-neighbors/src/knn.rs             */
-/* ------------------------------------------------------------------ */
-
-typedef struct fl_knn *fl_knn_handle;
-
-/* Fit<Array2<F>, Array1<usize>> for KNeighborsClassifier (knn.rs:256-320)
- * with Algorithm::BruteForce (knn.rs:148).  Validation order: y_len != n
- * -> ShapeMismatch; n_neighbors == 0 -> InvalidParameter; n < n_neighbors
- * -> InsufficientSamples.  The handle owns a device copy of X and y, as the
- * fitted Rust model owns clones (knn.rs:311-318).  Supported n_neighbors:
- * 1..64. */
-fl_status fl_knn_fit_classifier(const void *X, int32_t dtype, int64_t n,
-                                int64_t d, const int64_t *y, int64_t y_len,
-                                int64_t n_neighbors, int32_t weights,
-                                int32_t precision, fl_knn_handle *out);
-
-/* Fit<Array2<F>, Array1<F>> for KNeighborsRegressor (knn.rs:660-718). */
-fl_status fl_knn_fit_regressor(const void *X, int32_t dtype, int64_t n,
-                               int64_t d, const double *y, int64_t y_len,
-                               int64_t n_neighbors, int32_t weights,
-                               int32_t precision, fl_knn_handle *out);
-
-/* Predict for FittedKNeighborsClassifier (knn.rs:322-383): class labels. */
-fl_status fl_knn_predict_classify(fl_knn_handle h, const void *Q,
-                                  int32_t dtype, int64_t nq, int64_t d,
-                                  int64_t *labels_out, fl_report *report);
-
-/* Predict for FittedKNeighborsRegressor (knn.rs:720-781). */
-fl_status fl_knn_predict_regress(fl_knn_handle h, const void *Q,
-                                 int32_t dtype, int64_t nq, int64_t d,
-                                 double *y_out, fl_report *report);
-
-/* FittedKNeighborsClassifier::predict_proba (knn.rs:452-486):
- * proba_out is nq*n_classes, classes in fl_knn_classes order. */
-fl_status fl_knn_predict_proba(fl_knn_handle h, const void *Q, int32_t dtype,
-                               int64_t nq, int64_t d, double *proba_out,
-                               fl_report *report);
-
-/* kneighbors (knn.rs:823-834, 875-917): k == 0 means the fitted
- * n_neighbors.  Errors: ShapeMismatch on feature count, InvalidParameter
- * for k == 0 after defaulting, InsufficientSamples for k > n_fit.
- * dist_out / idx_out are nq*k, sorted by (distance, index). */
-fl_status fl_knn_kneighbors(fl_knn_handle h, const void *Q, int32_t dtype,
-                            int64_t nq, int64_t d, int64_t k,
-                            double *dist_out, int64_t *idx_out,
-                            fl_report *report);
-
-/* Device-memory forms of the three predict calls: Q and the outputs are
- * device pointers. */
-fl_status fl_knn_kneighbors_device(fl_knn_handle h, const void *dQ,
-                                   int32_t dtype, int64_t nq, int64_t d,
-                                   int64_t k, double *d_dist_out,
-                                   int64_t *d_idx_out, fl_report *report);
-fl_status fl_knn_predict_classify_device(fl_knn_handle h, const void *dQ,
-                                         int32_t dtype, int64_t nq, int64_t d,
-                                         int64_t *d_labels_out,
-                                         fl_report *report);
-fl_status fl_knn_predict_regress_device(fl_knn_handle h, const void *dQ,
-                                        int32_t dtype, int64_t nq, int64_t d,
-                                        double *d_y_out, fl_report *report);
-
-/* Switch the candidate filter of a fitted handle (fl_precision). */
-fl_status fl_knn_set_precision(fl_knn_handle h, int32_t precision);
-
-/* HasClasses (ferrolearn-core/src/introspection.rs:44-50) and
- * n_samples_fit (knn.rs:838-842): sorted unique classes. */
-int64_t fl_knn_n_classes(fl_knn_handle h);
-fl_status fl_knn_classes(fl_knn_handle h, int64_t *classes_out);
-int64_t fl_knn_n_samples_fit(fl_knn_handle h);
-
-void fl_knn_free(fl_knn_handle h);
-
-/* ------------------------------------------------------------------ */
-/* Cell router for exact kNN (no reference counterpart)                */
-/* ------------------------------------------------------------------ */
-
-/* A learned index over a fitted brute-force handle.  The database rows are
- * partitioned into cells by a coarse KMeans (kmeans.rs:403-513 with
- * initial centres drawn from the rows); each cell keeps its centre and a
- * radius bounding its rows' distance to the centre.  A perceptron
- * d -> hidden -> hidden -> n_cells (the layer, activation, loss and Adam
- * rules of ferrolearn-neural/src/mlp.rs:217-409, 1326-1340) is trained on
- * the cells that hold each training query's exact k nearest neighbours
- * (brute_force_knn, kdtree.rs:364-391).  A routed search scans the rows of
- * n_probe cells with the exact reference distance and certifies each query
- * with the triangle inequality: every unscanned cell's lower bound must
- * exceed the k-th distance found.  A query that fails is searched again over
- * more cells with the smallest bounds (8 or twice n_probe, doubling up to
- * min(31, n_cells)), and one that still fails runs the brute force, so
- * results equal fl_knn_kneighbors on the same handle, bit for bit. */
-typedef struct fl_router *fl_router_handle;
-
-typedef enum fl_route {
-    FL_ROUTE_NETWORK = 0, /* cells with the largest router outputs          */
-    FL_ROUTE_BOUNDS = 1   /* cells with the smallest lower bounds (no network) */
-} fl_route;
-
-typedef struct fl_router_params {
-    int64_t n_cells;    /* coarse cells, 1..min(n, 8192)                     */
-    int64_t cell_iter;  /* Lloyd iterations of the coarse KMeans (>= 1)      */
-    int64_t hidden;     /* width of both hidden layers, multiple of 64, <= 1024 */
-    uint64_t seed;      /* initial centres and network weights               */
-} fl_router_params;
-
-typedef struct fl_router_train_params {
-    int64_t n_neighbors;  /* k of the exact targets, 1..64                  */
-    int64_t epochs;
-    int64_t batch_size;   /* 1..65536                                        */
-    double learning_rate; /* Adam step (beta1 0.9, beta2 0.999, eps 1e-8)    */
-    double alpha;         /* L2 penalty, grad_w += alpha * W                 */
-    uint64_t seed;        /* per-epoch shuffles                              */
-    int32_t target;       /* fl_router_target                                */
-} fl_router_train_params;
-
-typedef enum fl_router_target {
-    /* all cells whose lower bound does not exceed the exact k-th distance:
-     * the cells a search must scan to certify the exact answer (at most 32,
-     * smallest bounds first) */
-    FL_TARGET_CERTIFY = 0,
-    /* the cells holding the exact k nearest rows */
-    FL_TARGET_NEIGHBORS = 1
-} fl_router_target;
-
-typedef struct fl_router_report {
-    double ms_total;
-    double ms_cells;      /* build: coarse KMeans, sort, radii               */
-    double ms_targets;    /* train: exact kNN of the training queries        */
-    double ms_train;      /* train: epochs                                   */
-    double ms_route;      /* search: network inference and cell choice       */
-    double ms_bounds;     /* search: centre distances and lower bounds       */
-    double ms_scan;       /* search: exact scan of the chosen cells          */
-    double ms_merge;      /* search: merge and certificate                   */
-    double ms_fallback;   /* search: brute force for uncertified queries     */
-    int64_t n_queries;
-    int64_t n_rescan;     /* queries whose first certificate failed; they are
-                             searched again over the min(31, n_cells) cells
-                             with the smallest bounds                       */
     int64_t n_fallback;   /* queries that still failed: brute force          */
     int64_t rows_scanned; /* database rows scanned by the routed search      */
     double loss_first;    /* train: mean loss of the first and last epochs   */
