@@ -508,11 +508,39 @@ __global__ __launch_bounds__(64) void knn_finalize_kernel(
     // Filter keys are lower bounds of the exact folds, so the candidates are
     // re-scored in increasing key order: first the k+6 smallest; the rest
     // only when the smallest key left could still reach the k-th fold.
-    for (int e = lane; e < nc; e += 64) {
-        int r = 0;
-        if (certify)
-            for (int e2 = 0; e2 < nc; ++e2) r += pair_less(ck[e2], ci[e2], ck[e], ci[e]) ? 1 : 0;
-        rk[e] = r;
+    // Ranks by (key, index) of the k+7 smallest candidates, by wave-wide
+    // minimum passes (each above the previous pair); the others get
+    // INT32_MAX and are ranked in full only if the lazy step below needs them.
+    for (int e = lane; e < nc; e += 64) rk[e] = INT32_MAX;
+    {
+        const int want = nc < k + 7 ? nc : k + 7;
+        double lk = -__builtin_huge_val();
+        int64_t li = -1;
+        for (int j = 0; j < want; ++j) {
+            double bk = __builtin_huge_val();
+            int64_t bi = INT64_MAX;
+            int be = -1;
+            for (int e = lane; e < nc; e += 64)
+                if (pair_less(lk, li, ck[e], ci[e]) && (be < 0 || pair_less(ck[e], ci[e], bk, bi))) {
+                    bk = ck[e];
+                    bi = ci[e];
+                    be = e;
+                }
+            for (int o = 32; o > 0; o >>= 1) {
+                const double ok2 = __shfl_xor(bk, o);
+                const int64_t oi2 = __shfl_xor(bi, o);
+                const int oe2 = __shfl_xor(be, o);
+                if (oe2 >= 0 && (be < 0 || pair_less(ok2, oi2, bk, bi))) {
+                    bk = ok2;
+                    bi = oi2;
+                    be = oe2;
+                }
+            }
+            if (be < 0) break;
+            if (lane == 0) rk[be] = j;
+            lk = bk;
+            li = bi;
+        }
     }
     __syncthreads();
     const double *qr = Q + q * d;
@@ -546,6 +574,13 @@ __global__ __launch_bounds__(64) void knn_finalize_kernel(
     rescore(0, R);
     kth(R);
     if (R < nc && !(kth_found && !has_nan && next_key > kth_fold + kth_fold * 0x1.0p-48)) {
+        // full ranks (rare): every candidate is re-scored
+        for (int e = lane; e < nc; e += 64) {
+            int r = 0;
+            for (int e2 = 0; e2 < nc; ++e2) r += pair_less(ck[e2], ci[e2], ck[e], ci[e]) ? 1 : 0;
+            rk[e] = r;
+        }
+        __syncthreads();
         rescore(R, nc);
         if (lane == 0) kth_found = 0;
         __syncthreads();
