@@ -833,6 +833,25 @@ fl_status mb_dispatch(const void *dX, int32_t dtype, int64_t n, int64_t d,
 
 }  // namespace
 
+namespace fl {
+// Exact nearest centre (lowest index on ties) and its fold for n device rows
+// of dtype: the Lloyd assignment step on its own (BF16x3 filter with the
+// exact decision, fallbacks as in a fit).  Used by the cell router.
+void nearest_centers(const void *X, int32_t dtype, int64_t n, int d, const double *C, int64_t k,
+                     int32_t *labels, double *best, hipStream_t st) {
+    KmAcc acc;
+    if (dtype == FL_F32) {
+        Lloyd<float> L((const float *)X, n, d, k, FL_PREC_BF16X3, st);
+        L.assign(C, labels, best, acc);
+        L.flush();
+    } else {
+        Lloyd<double> L((const double *)X, n, d, k, FL_PREC_BF16X3, st);
+        L.assign(C, labels, best, acc);
+        L.flush();
+    }
+}
+}  // namespace fl
+
 extern "C" {
 
 void fl_minibatch_params_default(fl_minibatch_params *p, int64_t n_clusters) {

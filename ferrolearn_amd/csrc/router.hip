@@ -25,6 +25,8 @@ using namespace fl;
 namespace fl {
 void knn_describe(const fl_knn *h, const void **X, int32_t *dtype, int64_t *n, int64_t *d);
 int64_t knn_n_neighbors(const fl_knn *h);
+void nearest_centers(const void *X, int32_t dtype, int64_t n, int d, const double *C, int64_t k,
+                     int32_t *labels, double *best, hipStream_t st);
 }
 
 namespace {
@@ -439,9 +441,9 @@ void smallest_bounds(fl_router *r, const TQ *Q, int64_t nq, int kb, double *bkey
     a.Q = Q, a.nq = nq, a.d = (int)r->d, a.centers = (const double *)r->centers.ptr();
     a.radius = (const double *)r->radius.ptr(), a.cdist = (const double *)r->cdist.ptr();
     a.C = r->C, a.ep = bound_slack(r->d);
-    a.near_fold = nullptr, a.near_cell = nullptr, a.kb = 1;
-    a.out_key = nfold, a.out_id = ncell;
-    launch_bounds_k<1, true>(a, qdt, st);
+    // the nearest centre and its exact fold: the KMeans assignment step
+    nearest_centers(Q, qdt, nq, (int)r->d, (const double *)r->centers.ptr(), r->C, ncell, nfold,
+                    st);
     a.near_fold = nfold, a.near_cell = ncell, a.kb = kb;
     a.out_key = bkey, a.out_id = bid;
     if (kb <= 16)

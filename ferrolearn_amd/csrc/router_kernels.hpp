@@ -121,7 +121,7 @@ __device__ __forceinline__ void rt_fold_lists(const TQ *const *qrow, int d, cons
             }
             __syncthreads();
             const int fc = d - f0 < RT_F ? d - f0 : RT_F;
-            for (int f = 0; f < fc; ++f) {
+            auto step = [&](int f) {
                 double qv[4], xv[4];
 #pragma unroll
                 for (int i = 0; i < 4; ++i) qv[i] = Qs[f * RT_Q + qg + 16 * i];
@@ -134,6 +134,12 @@ __device__ __forceinline__ void rt_fold_lists(const TQ *const *qrow, int d, cons
                         const double df = qv[i] - xv[j];
                         acc[i][j] = acc[i][j] + df * df;
                     }
+            };
+            if (fc == RT_F) {
+#pragma unroll 8
+                for (int f = 0; f < RT_F; ++f) step(f);
+            } else {
+                for (int f = 0; f < fc; ++f) step(f);
             }
             __syncthreads();
         }
