@@ -37,6 +37,8 @@ constexpr int kMaxProbe = 31;
 constexpr int kMaxRoutedK = 32;
 constexpr int64_t kInferRows = 32768;   // network inference chunk
 constexpr int64_t kMaxTargets = 32;     // certification targets per training query
+constexpr int kRouteAll = 2;            // internal route: every cell (exact by construction)
+constexpr int64_t kFullScanMax = 2048;  // queries a last all-cells scan takes
 
 // Router network: d -> H -> H -> C with padded shapes (dp: 32, cp: 64).
 struct Mlp {
@@ -472,7 +474,8 @@ void launch_top_cells(const float *Z, int64_t ldz, int64_t rows, int C, int n_pr
 // Routed search of nq device queries (one pass; nq * n_probe < 2^31).
 // Queries whose certificate fails are searched again over more cells with
 // the smallest bounds (8 or twice n_probe, doubling up to min(31,
-// n_cells)), and what still fails goes to the brute force.
+// n_cells)); up to kFullScanMax queries left then scan every cell, and what
+// still fails (non-finite queries) goes to the brute force.
 template <typename TQ, typename TX>
 void route_search(fl_router *r, const TQ *Q, int64_t nq, int k, int n_probe, int route,
                   double *dist, int64_t *idx, RtAcc &acc, int depth = 0) {
@@ -503,12 +506,17 @@ void route_search(fl_router *r, const TQ *Q, int64_t nq, int k, int n_probe, int
         }
     }
     tm.end(t);
-    // 2. lower bounds: the n_probe + 1 smallest per query
-    const int kb = n_probe + 1;
+    // 2. lower bounds: the n_probe + 1 smallest per query (none when every
+    // cell is scanned: nothing is left to bound)
+    const int kb = route == kRouteAll ? 0 : n_probe + 1;
     double *bkey = r->ws.get<double>("bkey", nq * kb);
     int32_t *bid = r->ws.get<int32_t>("bid", nq * kb);
     const int tb = tm.begin();
-    {
+    if (route == kRouteAll) {
+        hipLaunchKernelGGL(rt_routes_all_kernel, dim3(grid1(nq * n_probe, 256)), dim3(256), 0, st,
+                           nq, n_probe, routes);
+        check_launch();
+    } else {
         smallest_bounds(r, Q, nq, kb, bkey, bid);
         if (route == FL_ROUTE_BOUNDS) {
             hipLaunchKernelGGL(rt_routes_from_bounds_kernel, dim3(grid1(nq * n_probe, 256)),
@@ -600,6 +608,14 @@ void route_search(fl_router *r, const TQ *Q, int64_t nq, int k, int n_probe, int
         if (depth == 0) acc.n_rescan += m;
         RtAcc inner;
         route_search<TQ, TX>(r, fq, m, k, wide, FL_ROUTE_BOUNDS, fd, fi, inner, depth + 1);
+        acc.rows += inner.rows;
+        acc.n_fb += inner.n_fb;
+    } else if (route != kRouteAll && n_probe < C && m <= kFullScanMax) {
+        // a few queries left: scan every cell with the same exact kernels
+        // (a brute force organised by cells; no bound is needed)
+        if (depth == 0) acc.n_rescan += m;
+        RtAcc inner;
+        route_search<TQ, TX>(r, fq, m, k, C, kRouteAll, fd, fi, inner, depth + 1);
         acc.rows += inner.rows;
         acc.n_fb += inner.n_fb;
     } else {

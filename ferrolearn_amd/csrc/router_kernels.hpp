@@ -637,4 +637,10 @@ __global__ void rt_kth_kernel(const double *__restrict__ dist, int64_t nq, int k
     if (q < nq) kth[q] = dist[q * k + k - 1];
 }
 
+// Every cell, in order, for each query.
+__global__ void rt_routes_all_kernel(int64_t nq, int C, int32_t *__restrict__ routes) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nq * C) routes[t] = (int32_t)(t % C);
+}
+
 }  // namespace fl

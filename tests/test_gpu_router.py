@@ -173,3 +173,21 @@ def test_router_validation(setup):
         router.kneighbors(X[:3, :5])
     with pytest.raises(F.InvalidParameter):
         F.CellRouter(fitted, n_cells=16, hidden=100)
+
+
+def test_unstructured_data_full_scan():
+    # Uniform data: the bounds rarely certify, so queries go through the
+    # widening rescans and then the all-cells scan; every answer is exact
+    # and none needs the brute force.
+    rng = np.random.default_rng(11)
+    X = rng.uniform(-1, 1, size=(20000, 16)).astype(np.float32)
+    fitted = F.KNeighborsRegressor().with_n_neighbors(10).fit(X, np.zeros(len(X)))
+    r = F.CellRouter(fitted, n_cells=64, cell_iter=5, hidden=64, seed=2)
+    Q = rng.uniform(-1, 1, size=(700, 16)).astype(np.float32)
+    rd, ri = r.kneighbors(Q, n_probe=2, route="bounds")
+    bd, bi = fitted.kneighbors(Q)
+    assert (ri == bi).all()
+    assert (rd.view(np.int64) == bd.view(np.int64)).all()
+    rep = r.report
+    print(f"unstructured: rescans {rep['n_rescan']}, brute force {rep['n_fallback']}")
+    assert rep["n_rescan"] > 0 and rep["n_fallback"] == 0
