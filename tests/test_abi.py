@@ -124,3 +124,56 @@ def test_product_never_imports_the_oracle():
             if f.endswith((".py", ".hip", ".hpp", ".cpp", ".h")) or f == "Makefile":
                 txt = open(os.path.join(dirpath, f), errors="ignore").read()
                 assert not re.search(r"(from|import)\s+oracle|liboracle|oracle/|ref_cpu", txt), f
+
+
+STRUCTS = {
+    "fl_report": L.fl_report,
+    "fl_kmeans_params": L.fl_kmeans_params,
+    "fl_minibatch_params": L.fl_minibatch_params,
+    "fl_router_params": L.fl_router_params,
+    "fl_router_train_params": L.fl_router_train_params,
+    "fl_router_report": L.fl_router_report,
+}
+
+
+def test_ctypes_structs_match_the_header(tmp_path):
+    # sizeof and offsetof of every field, as a C compiler lays them out
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "ferrolearn_hip.h"',
+             "int main(void) {"]
+    for name, cls in STRUCTS.items():
+        lines.append(f'printf("{name} %zu\\n", sizeof({name}));')
+        for field, _ in cls._fields_:
+            lines.append(f'printf("{name}.{field} %zu\\n", offsetof({name}, {field}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    import subprocess
+    subprocess.check_call(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src),
+                           "-o", str(exe)])
+    got = dict(l.split() for l in subprocess.check_output([str(exe)]).decode().splitlines())
+    for name, cls in STRUCTS.items():
+        assert int(got[name]) == C.sizeof(cls), name
+        for field, _ in cls._fields_:
+            assert int(got[f"{name}.{field}"]) == getattr(cls, field).offset, (name, field)
+
+
+def test_router_defaults_and_null_arguments():
+    p = L.fl_router_params()
+    L.lib().fl_router_params_default(C.byref(p), 4096)
+    assert (p.n_cells, p.cell_iter, p.hidden, p.seed) == (4096, 10, 512, 0)
+    t = L.fl_router_train_params()
+    L.lib().fl_router_train_params_default(C.byref(t))
+    assert (t.n_neighbors, t.epochs, t.batch_size, t.target) == (10, 5, 4096, L.FL_TARGET_CERTIFY)
+    h = C.c_void_p()
+    rep = L.fl_router_report()
+    assert L.lib().fl_router_build(None, C.byref(p), C.byref(h), C.byref(rep)) == \
+        L.FL_INVALID_PARAMETER
+    assert "knn" in L.last_error()
+    assert L.lib().fl_router_kneighbors_device(None, None, 0, 0, 4, 0, 2, 0, None, None,
+                                               None) == L.FL_INVALID_PARAMETER
+    assert L.lib().fl_router_n_params(None) == 0
+    L.lib().fl_router_free(None)
+    loss = C.c_double()
+    assert L.lib().fl_router_mlp_gradients(4, 64, 8, None, None, 1, None, 1, 0.0,
+                                           C.byref(loss), None) == L.FL_INVALID_PARAMETER
