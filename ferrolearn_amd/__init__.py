@@ -6,7 +6,7 @@ kernels behind the C ABI in include/ferrolearn_hip.h.  Labels, iteration
 counts and neighbour indices match the reference's f64 CPU path exactly.
 """
 from . import device
-from .cluster import FittedKMeans, KMeans, MiniBatchKMeans, kmeans_plusplus
+from .cluster import FittedKMeans, KMeans, MiniBatchKMeans, kmeans_fit_batched, kmeans_plusplus
 from .device import DeviceArray, to_device
 from .errors import (DeviceError, FerroError, InsufficientSamples, InvalidParameter,
                      NumericalInstability, ShapeMismatch)
@@ -19,5 +19,5 @@ __all__ = [
     "KNeighborsRegressor", "FittedKNeighborsClassifier", "FittedKNeighborsRegressor",
     "Algorithm", "Weights", "FerroError", "ShapeMismatch", "InsufficientSamples",
     "InvalidParameter", "NumericalInstability", "DeviceError", "DeviceArray", "to_device",
-    "device", "CellRouter",
+    "device", "CellRouter", "kmeans_fit_batched",
 ]

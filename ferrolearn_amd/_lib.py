@@ -76,6 +76,21 @@ class fl_minibatch_params(C.Structure):
     ]
 
 
+class fl_kmeans_problem(C.Structure):
+    _fields_ = [
+        ("X", C.c_void_p),
+        ("dtype", C.c_int32),
+        ("n", C.c_int64),
+        ("d", C.c_int64),
+        ("params", fl_kmeans_params),
+        ("init_centers", C.c_void_p),
+        ("centers_out", C.c_void_p),
+        ("labels_out", C.c_void_p),
+        ("inertia", C.c_double),
+        ("n_iter", C.c_int64),
+        ("status", C.c_int32),
+    ]
+
 FL_ROUTE_NETWORK = 0
 FL_ROUTE_BOUNDS = 1
 ROUTES = {"network": FL_ROUTE_NETWORK, "bounds": FL_ROUTE_BOUNDS}
@@ -148,6 +163,7 @@ SIGNATURES = {
     "fl_kmeans_fit_device": (C.c_int, [_vp, _i32, _i64, _i64, C.POINTER(fl_kmeans_params), _vp,
                                        _vp, _vp, _dp, _ip, _rp]),
     "fl_kmeans_predict": (C.c_int, [_vp, _i64, _i64, _vp, _i32, _i64, _i64, _vp]),
+    "fl_kmeans_fit_batched": (C.c_int, [C.POINTER(fl_kmeans_problem), _i64, _i32, _rp]),
     "fl_kmeans_transform": (C.c_int, [_vp, _i64, _i64, _vp, _i32, _i64, _i64, _vp]),
     "fl_kmeans_plusplus": (C.c_int, [_vp, _i32, _i64, _i64, _i64, C.c_uint64, _vp]),
     "fl_minibatch_params_default": (None, [C.POINTER(fl_minibatch_params), _i64]),

@@ -264,3 +264,44 @@ def kmeans_plusplus(x, n_clusters: int, seed: int) -> np.ndarray:
                                      n_clusters, C.c_uint64(seed),
                                      out.ctypes.data_as(C.c_void_p)))
     return out
+
+
+def kmeans_fit_batched(problems, n_workers: int = 8):
+    """Fit many independent KMeans problems in one call (fl_kmeans_fit_batched).
+
+    `problems` is a sequence of (x, model, init_centers) with x a device array
+    (DeviceArray or CUDA tensor), model a configured `KMeans`, and
+    init_centers None (the library's k-means++) or n_init*k*d values.  The
+    problems run concurrently, each on its own stream; each result equals
+    `model.fit(x, init_centers)` bit for bit.  Returns one `FittedKMeans` per
+    problem and the call's report.
+    """
+    from .device import DeviceArray
+    arr = (L.fl_kmeans_problem * max(1, len(problems)))()
+    keep = []
+    outs = []
+    for i, (x, model, init) in enumerate(problems):
+        if not _is_device(x):
+            raise TypeError("kmeans_fit_batched takes device arrays")
+        n, d = x.shape
+        k = model.n_clusters
+        centers = _dev_empty((max(k, 0), d), np.float64, x)
+        labels = _dev_empty((n,), np.int64, x)
+        it = None
+        if init is not None:
+            it = _dev_from_host(np.ascontiguousarray(init, dtype=np.float64), x)
+        keep.append((centers, labels, it))
+        pr = arr[i]
+        pr.X = x.data_ptr()
+        pr.dtype = _dtype_code(x.dtype)
+        pr.n, pr.d = n, d
+        pr.params = model._params()
+        pr.init_centers = it.data_ptr() if it is not None else None
+        pr.centers_out = centers.data_ptr()
+        pr.labels_out = labels.data_ptr()
+        outs.append(model)
+    rep = L.fl_report()
+    check(L.lib().fl_kmeans_fit_batched(arr, len(problems), int(n_workers), C.byref(rep)))
+    fitted = [FittedKMeans(keep[i][0], keep[i][1], arr[i].inertia, arr[i].n_iter, {},
+                           outs[i].precision) for i in range(len(problems))]
+    return fitted, rep.as_dict()

@@ -6,6 +6,8 @@
 // the strictly smallest inertia.  Labels and inertia come from the last
 // assignment, the centres are the post-update ones (:468-495).
 #include <cmath>
+#include <thread>
+#include <atomic>
 #include <cstring>
 #include <memory>
 #include <unordered_map>
@@ -925,6 +927,53 @@ fl_status fl_kmeans_fit_device(const void *dX, int32_t dtype, int64_t n, int64_t
         return fit_dispatch(dX, dtype, n, d, params, d_init, d_centers_out, d_labels_out,
                             inertia_out, n_iter_out, report);
     });
+}
+
+fl_status fl_kmeans_fit_batched(fl_kmeans_problem *problems, int64_t n_problems,
+                                int32_t n_workers, fl_report *report) {
+    if (!problems && n_problems > 0)
+        return fail(FL_INVALID_PARAMETER, "InvalidParameter: problems: must not be NULL");
+    if (n_problems < 0)
+        return fail(FL_INVALID_PARAMETER, "InvalidParameter: n_problems: negative");
+    WallClock wc;
+    const int workers = (int)std::max<int64_t>(
+        1, std::min<int64_t>(n_workers > 0 ? n_workers : 8, std::max<int64_t>(n_problems, 1)));
+    std::atomic<int64_t> next(0);
+    std::vector<std::string> errs((size_t)n_problems);
+    auto work = [&]() {
+        for (;;) {
+            const int64_t i = next.fetch_add(1);
+            if (i >= n_problems) break;
+            fl_kmeans_problem &p = problems[i];
+            p.status = fl_kmeans_fit_device(p.X, p.dtype, p.n, p.d, &p.params, p.init_centers,
+                                            p.centers_out, p.labels_out, &p.inertia, &p.n_iter,
+                                            nullptr);
+            if (p.status != FL_OK) errs[(size_t)i] = fl_last_error();
+        }
+    };
+    std::vector<std::thread> pool;
+    try {
+        for (int t = 1; t < workers; ++t) pool.emplace_back(work);
+    } catch (...) {
+        // fewer threads than asked: the ones started and this one finish
+    }
+    work();
+    for (auto &th : pool) th.join();
+    if (report) {
+        std::memset(report, 0, sizeof(*report));
+        report->ms_total = wc.ms();
+        for (int64_t i = 0; i < n_problems; ++i)
+            if (problems[i].status == FL_OK) {
+                report->n_rows += problems[i].n * problems[i].n_iter;
+                report->n_iter += problems[i].n_iter;
+            }
+    }
+    for (int64_t i = 0; i < n_problems; ++i)
+        if (problems[i].status != FL_OK)
+            return fail((fl_status)problems[i].status,
+                        "problem " + std::to_string(i) + ": " + errs[(size_t)i]);
+    clear_error();
+    return FL_OK;
 }
 
 fl_status fl_kmeans_fit(const void *X, int32_t dtype, int64_t n, int64_t d,

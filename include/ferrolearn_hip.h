@@ -153,6 +153,32 @@ fl_status fl_kmeans_plusplus(const void *X, int32_t dtype, int64_t n,
                              int64_t d, int64_t k, uint64_t seed,
                              double *centers_out);
 
+/* Many independent KMeans fits in one call (no reference counterpart;
+ * each problem is KMeans::fit, kmeans.rs:403-513).  Problems run
+ * concurrently on n_workers host threads, each fit on its own stream, and
+ * every problem's outputs are those of fl_kmeans_fit_device on the same
+ * arguments, bit for bit.  X, init_centers (or NULL), centers_out and
+ * labels_out are device pointers. */
+typedef struct fl_kmeans_problem {
+    const void *X;
+    int32_t dtype;
+    int64_t n;
+    int64_t d;
+    fl_kmeans_params params;
+    const double *init_centers;
+    double *centers_out;
+    int64_t *labels_out;
+    double inertia;   /* out */
+    int64_t n_iter;   /* out */
+    int32_t status;   /* out: this problem's fl_status */
+} fl_kmeans_problem;
+
+/* Returns FL_OK when every problem succeeded, else the first failing
+ * problem's status (message names the problem).  n_workers <= 0 picks 8.
+ * report: ms_total, n_rows = sum of n * n_iter, n_iter = sum of n_iter. */
+fl_status fl_kmeans_fit_batched(fl_kmeans_problem *problems, int64_t n_problems,
+                                int32_t n_workers, fl_report *report);
+
 /* ------------------------------------------------------------------ */
 /* MiniBatchKMeans -- ferrolearn-cluster/src/mini_batch_kmeans.rs      */
 /* ------------------------------------------------------------------ */

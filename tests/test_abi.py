@@ -133,6 +133,7 @@ STRUCTS = {
     "fl_router_params": L.fl_router_params,
     "fl_router_train_params": L.fl_router_train_params,
     "fl_router_report": L.fl_router_report,
+    "fl_kmeans_problem": L.fl_kmeans_problem,
 }
 
 
