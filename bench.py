@@ -16,7 +16,13 @@ std U[0.5,3]), k = 1,024, k-means++ initial centres (the library's, seed 0),
 50 iterations (tol 0), samples resident in HBM; samples*iterations/s of the
 Lloyd phase for the BF16x3 and plain f64 forms, which must agree bit for bit.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload all|knn|kmeans|copy]
+Third and fourth workloads, under "router" and "batched_kmeans": the
+learned-router exact kNN (20M x 96 rows, 4,096 cells, 1M queries) against
+the brute force on the same handle, and 256 uneven KMeans problems with 8
+restarts each in one batched call, BF16x3 and FP64 forms.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W]
+                  [--workload all|knn|kmeans|router|batched|copy]
 """
 from __future__ import annotations
 
@@ -406,6 +412,62 @@ def run_router(args):
     return out
 
 
+def batched_data(n_problems=256, seed=0):
+    """Uneven KMeans problems: n log-uniform in [1e3, 2e5], d from {8, 16,
+    32, 64}, k log-uniform in [4, 256], f32 Gaussian blobs with k true
+    centres (uniform in [-10, 10], unit std)."""
+    g = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_problems):
+        n = int(np.exp(g.uniform(np.log(1e3), np.log(2e5))))
+        d = int(g.choice([8, 16, 32, 64]))
+        k = int(np.exp(g.uniform(np.log(4), np.log(256))))
+        k = min(k, n)
+        cen = g.uniform(-10, 10, size=(k, d)).astype(np.float32)
+        X = cen[g.integers(0, k, n)] + g.standard_normal((n, d), dtype=np.float32)
+        out.append((X, k))
+    return out
+
+
+def run_batched(args):
+    """Many KMeans problems (n_init = 8 each, max_iter 100, tol 1e-4) in one
+    batched call, BF16x3 and FP64 forms; both must agree problem by problem."""
+    import ferrolearn_amd as F
+    probs = batched_data(args.batched_problems)
+    xs = [F.to_device(X) for X, _ in probs]
+    F.device.synchronize()
+    forms = {}
+    for form in ("bf16x3", "fp64"):
+        items = [(x, F.KMeans(k).with_n_init(8).with_max_iter(100).with_tol(1e-4)
+                  .with_random_state(i).with_precision(form), None)
+                 for i, (x, (_, k)) in enumerate(zip(xs, probs))]
+        t0 = time.perf_counter()
+        fitted, rep = F.kmeans_fit_batched(items, n_workers=args.batched_workers)
+        F.device.synchronize()
+        dt = time.perf_counter() - t0
+        forms[form] = {"s": dt, "fitted": fitted, "samples_iter": rep["n_rows"]}
+        log(f"batched {form}: {dt:.2f} s, {rep['n_rows'] / dt:.3g} samples*iter/s")
+    b, f = forms["bf16x3"], forms["fp64"]
+    exact = all(x.n_iter() == y.n_iter() and (x.labels().numpy() == y.labels().numpy()).all()
+                and (x.cluster_centers().numpy().view(np.int64)
+                     == y.cluster_centers().numpy().view(np.int64)).all()
+                for x, y in zip(b["fitted"], f["fitted"]))
+    return {
+        "workload": f"batched KMeans, {len(probs)} problems (n log-U[1e3,2e5], d in "
+                    "{8,16,32,64}, k log-U[4,256]), n_init=8, max_iter=100, tol=1e-4, "
+                    f"{args.batched_workers} workers",
+        "metric": "samples*iter/s",
+        "note": "samples*iter counts every restart's assignments; k-means++ for all "
+                "restarts is inside the timed call",
+        "value": b["samples_iter"] / b["s"],
+        "form": "bf16x3",
+        "s": {"bf16x3": b["s"], "fp64": f["s"]},
+        "speedup_vs_fp64_form": f["s"] / b["s"],
+        "exact_vs_fp64_form": bool(exact),
+        "total_rows": int(sum(X.shape[0] for X, _ in probs)),
+    }
+
+
 def copy_calibration(mib=512):
     """A plain device-to-device copy of known size: its FETCH_SIZE / WRITE_SIZE
     counters give the scale that turns those counters into HBM bytes."""
@@ -427,7 +489,9 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="all",
-                    choices=["all", "knn", "kmeans", "router", "copy"])
+                    choices=["all", "knn", "kmeans", "router", "batched", "copy"])
+    ap.add_argument("--batched-problems", type=int, default=256)
+    ap.add_argument("--batched-workers", type=int, default=8)
     ap.add_argument("--router-scale", type=float, default=1.0,
                     help="fraction of the router workload's rows, cells and queries")
     ap.add_argument("--router-epochs", type=int, default=3)
@@ -457,10 +521,14 @@ def main():
     if args.workload == "router":
         print(json.dumps(run_router(args)), flush=True)
         return
+    if args.workload == "batched":
+        print(json.dumps(run_batched(args)), flush=True)
+        return
     line = run_knn(args)
     if args.workload == "all":
         line["kmeans"] = run_kmeans(args)
         line["router"] = run_router(args)
+        line["batched_kmeans"] = run_batched(args)
     print(json.dumps(line), flush=True)
 
 

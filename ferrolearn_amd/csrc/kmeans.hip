@@ -940,6 +940,7 @@ fl_status fl_kmeans_fit_batched(fl_kmeans_problem *problems, int64_t n_problems,
         1, std::min<int64_t>(n_workers > 0 ? n_workers : 8, std::max<int64_t>(n_problems, 1)));
     std::atomic<int64_t> next(0);
     std::vector<std::string> errs((size_t)n_problems);
+    std::vector<fl_report> reps((size_t)n_problems);
     auto work = [&]() {
         for (;;) {
             const int64_t i = next.fetch_add(1);
@@ -947,7 +948,7 @@ fl_status fl_kmeans_fit_batched(fl_kmeans_problem *problems, int64_t n_problems,
             fl_kmeans_problem &p = problems[i];
             p.status = fl_kmeans_fit_device(p.X, p.dtype, p.n, p.d, &p.params, p.init_centers,
                                             p.centers_out, p.labels_out, &p.inertia, &p.n_iter,
-                                            nullptr);
+                                            &reps[(size_t)i]);
             if (p.status != FL_OK) errs[(size_t)i] = fl_last_error();
         }
     };
@@ -964,8 +965,12 @@ fl_status fl_kmeans_fit_batched(fl_kmeans_problem *problems, int64_t n_problems,
         report->ms_total = wc.ms();
         for (int64_t i = 0; i < n_problems; ++i)
             if (problems[i].status == FL_OK) {
-                report->n_rows += problems[i].n * problems[i].n_iter;
+                // rows assigned over every restart and iteration
+                report->n_rows += reps[(size_t)i].n_rows;
                 report->n_iter += problems[i].n_iter;
+                report->n_fallback += reps[(size_t)i].n_fallback;
+                report->ms_distance += reps[(size_t)i].ms_distance;
+                report->ms_update += reps[(size_t)i].ms_update;
             }
     }
     for (int64_t i = 0; i < n_problems; ++i)

@@ -175,7 +175,9 @@ typedef struct fl_kmeans_problem {
 
 /* Returns FL_OK when every problem succeeded, else the first failing
  * problem's status (message names the problem).  n_workers <= 0 picks 8.
- * report: ms_total, n_rows = sum of n * n_iter, n_iter = sum of n_iter. */
+ * report: ms_total (wall), n_rows = rows assigned over every restart and
+ * iteration of every problem, n_iter = sum of the kept runs' n_iter, and
+ * the summed kernel times of the fits. */
 fl_status fl_kmeans_fit_batched(fl_kmeans_problem *problems, int64_t n_problems,
                                 int32_t n_workers, fl_report *report);
 
