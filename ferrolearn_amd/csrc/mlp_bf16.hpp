@@ -66,11 +66,20 @@ __device__ __forceinline__ void mg_stage(uint16_t *__restrict__ S, const uint16_
     } else {
         const int kk = t >> 3, rc = (t & 7) * 8;
         const uint16_t *src = P + (int64_t)(k0 + kk) * ld + r0 + rc;
+        uint16_t v[8];
+        if ((ld & 7) == 0 && r0 + rc + 8 <= rows) {
+            // eight contiguous, 16-byte aligned elements: one vector load
+            const u16x8_t w = *reinterpret_cast<const u16x8_t *>(src);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) v[j] = w[j];
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) v[j] = r0 + rc + j < rows ? src[j] : (uint16_t)0;
+        }
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             const int r = rc + j;
-            const uint16_t v = r0 + r < rows ? src[j] : (uint16_t)0;
-            S[r * MG_BK + mg_swz(r, kk >> 3) * 8 + (kk & 7)] = v;
+            S[r * MG_BK + mg_swz(r, kk >> 3) * 8 + (kk & 7)] = v[j];
         }
     }
 }
