@@ -413,19 +413,19 @@ void build_cells(fl_router *r, uint64_t seed, int64_t iters) {
 // (d + 4) u on a distance), plus the roundings of rt_lower_bound itself.
 double bound_slack(int64_t d) { return (double)(d + 8) * std::ldexp(1.0, -52); }
 
-template <int K, bool NEAREST>
+template <int K>
 void launch_bounds_k(const RtBoundsArgs &a, int32_t dtype, hipStream_t st) {
     const size_t sm = rt_lds_bytes(K);
     const unsigned grid = (unsigned)ceil_div(a.nq, RT_Q);
     if (dtype == FL_F32) {
-        FL_HIP(hipFuncSetAttribute((const void *)rt_bounds_kernel<float, K, NEAREST>,
+        FL_HIP(hipFuncSetAttribute((const void *)rt_bounds_kernel<float, K>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
-        hipLaunchKernelGGL((rt_bounds_kernel<float, K, NEAREST>), dim3(grid), dim3(256), sm, st,
+        hipLaunchKernelGGL((rt_bounds_kernel<float, K>), dim3(grid), dim3(256), sm, st,
                            a);
     } else {
-        FL_HIP(hipFuncSetAttribute((const void *)rt_bounds_kernel<double, K, NEAREST>,
+        FL_HIP(hipFuncSetAttribute((const void *)rt_bounds_kernel<double, K>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
-        hipLaunchKernelGGL((rt_bounds_kernel<double, K, NEAREST>), dim3(grid), dim3(256), sm, st,
+        hipLaunchKernelGGL((rt_bounds_kernel<double, K>), dim3(grid), dim3(256), sm, st,
                            a);
     }
     check_launch();
@@ -449,9 +449,9 @@ void smallest_bounds(fl_router *r, const TQ *Q, int64_t nq, int kb, double *bkey
     a.near_fold = nfold, a.near_cell = ncell, a.kb = kb;
     a.out_key = bkey, a.out_id = bid;
     if (kb <= 16)
-        launch_bounds_k<16, false>(a, qdt, st);
+        launch_bounds_k<16>(a, qdt, st);
     else
-        launch_bounds_k<32, false>(a, qdt, st);
+        launch_bounds_k<32>(a, qdt, st);
 }
 
 template <typename TQ, typename TX, int K>

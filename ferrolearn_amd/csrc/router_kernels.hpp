@@ -301,10 +301,9 @@ struct RtBoundsArgs {
     int32_t *out_id;
 };
 
-// NEAREST: the list is the exact nearest centre (fold, cell) -- kb = 1, the
-// reference's argmin order.  Otherwise the key of a cell is the larger of
-// its ball bound and its bisector bound against the query's nearest centre.
-template <typename TQ, int K, bool NEAREST>
+// The key of a cell is the larger of its ball bound and its bisector bound
+// against the query's nearest centre; an empty cell's key is +inf.
+template <typename TQ, int K>
 __global__ __launch_bounds__(256) void rt_bounds_kernel(RtBoundsArgs a) {
     extern __shared__ double lds[];
     __shared__ const TQ *qrow[RT_Q];
@@ -322,17 +321,13 @@ __global__ __launch_bounds__(256) void rt_bounds_kernel(RtBoundsArgs a) {
     const double *rad = a.radius;
     const double ep = a.ep;
     const int64_t qo = q0 + (t & 63);
-    const int32_t near = (!NEAREST && qo < a.nq) ? a.near_cell[qo] : 0;
-    const double fnear = (!NEAREST && qo < a.nq) ? a.near_fold[qo] : 0.0;
-    const double *drow = NEAREST ? nullptr : a.cdist + (int64_t)near * a.C;
+    const int32_t near = qo < a.nq ? a.near_cell[qo] : 0;
+    const double fnear = qo < a.nq ? a.near_fold[qo] : 0.0;
+    const double *drow = a.cdist + (int64_t)near * a.C;
     rt_fold_lists<TQ, double, K>(
         qrow, a.d, a.centers, 0, a.C, lds, lk, li,
         [&](double fold, int64_t c, double &key, int32_t &id) {
             id = (int32_t)c;
-            if (NEAREST) {
-                key = fold;
-                return;
-            }
             const double r = rad[c];
             if (r < 0.0) {
                 key = __builtin_huge_val();
