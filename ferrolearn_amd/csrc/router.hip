@@ -475,10 +475,14 @@ void launch_top_cells(const float *Z, int64_t ldz, int64_t rows, int C, int n_pr
 // Queries whose certificate fails are searched again over more cells with
 // the smallest bounds (8 or twice n_probe, doubling up to min(31,
 // n_cells)); up to kFullScanMax queries left then scan every cell, and what
-// still fails (non-finite queries) goes to the brute force.
+// still fails (non-finite queries) goes to the brute force.  A rescan starts
+// from the k best its parent found (prev_d, prev_i) and skips the cells the
+// parent scanned (prev_routes, prev_np per query).
 template <typename TQ, typename TX>
 void route_search(fl_router *r, const TQ *Q, int64_t nq, int k, int n_probe, int route,
-                  double *dist, int64_t *idx, RtAcc &acc, int depth = 0) {
+                  double *dist, int64_t *idx, RtAcc &acc, int depth = 0,
+                  const double *prev_d = nullptr, const int64_t *prev_i = nullptr,
+                  const int32_t *prev_routes = nullptr, int prev_np = 0) {
     hipStream_t st = r->stream;
     const int d = (int)r->d, C = r->C;
     PhaseTimer tm(st);
@@ -525,57 +529,103 @@ void route_search(fl_router *r, const TQ *Q, int64_t nq, int k, int n_probe, int
         }
     }
     tm.end(tb);
-    // 3. pairs (query, slot) grouped by cell, and the scan's work items
-    const int ts = tm.begin();
+    // 3-5. exact scan of the chosen cells and merge with the certificate.
+    // A pass scans the pairs whose scan_routes entry is a cell (bucket C of
+    // the counting sort gets no work items) and merges them with an earlier
+    // k best (pd, pi) when given.
     const int64_t P = nq * n_probe;
     int32_t *pord = r->ws.get<int32_t>("pord", P);
-    int64_t *pfirst = r->ws.get<int64_t>("pfirst", C);
-    int32_t *pcount = r->ws.get<int32_t>("pcount", C);
-    counting_sort(routes, P, C, pord, pfirst, pcount, r->ws, st);
+    int64_t *pfirst = r->ws.get<int64_t>("pfirst", C + 1);
+    int32_t *pcount = r->ws.get<int32_t>("pcount", C + 1);
     int32_t *wfirst = r->ws.get<int32_t>("wfirst", C);
     int32_t *wtotal = r->ws.get<int32_t>("wtotal", 1);
-    hipLaunchKernelGGL(rt_work_scan_kernel, dim3(1), dim3(256), 0, st, pcount, C, wfirst, wtotal);
-    check_launch();
-    int32_t items = 0;
-    FL_HIP(hipMemcpyAsync(&items, wtotal, 4, hipMemcpyDeviceToHost, st));
-    FL_HIP(hipStreamSynchronize(st));
-    int32_t *wcell = r->ws.get<int32_t>("wcell", items);
-    int32_t *wstart = r->ws.get<int32_t>("wstart", items);
-    hipLaunchKernelGGL(rt_work_fill_kernel, dim3(grid1(C, 256)), dim3(256), 0, st, pcount, wfirst,
-                       C, wcell, wstart);
-    check_launch();
-    // 4. exact scan of the chosen cells
     double *pkey = r->ws.get<double>("pkey", P * k);
     int32_t *pid = r->ws.get<int32_t>("pid", P * k);
-    if (items > 0) {
-        RtScanArgs a;
-        a.Q = Q, a.Xs = r->xs.ptr(), a.row_id = (const int32_t *)r->row_id.ptr();
-        a.cell_first = (const int64_t *)r->cell_first.ptr();
-        a.cell_count = (const int32_t *)r->cell_count.ptr();
-        a.pair_order = pord, a.pair_first = pfirst, a.pair_count = pcount;
-        a.work_cell = wcell, a.work_start = wstart;
-        a.d = d, a.n_probe = n_probe, a.k = k, a.out_key = pkey, a.out_id = pid;
-        if (k <= 16)
-            launch_scan_k<TQ, TX, 16>(a, items, st);
-        else
-            launch_scan_k<TQ, TX, 32>(a, items, st);
-    }
-    tm.end(ts);
-    // 5. merge and certificate
-    const int tmg = tm.begin();
     uint8_t *flag = r->ws.get<uint8_t>("flag", nq);
     unsigned long long *rows = r->ws.get<unsigned long long>("rows", 1);
     FL_HIP(hipMemsetAsync(rows, 0, 8, st));
-    if (k <= 16)
-        hipLaunchKernelGGL((rt_merge_certify_kernel<TQ, 16>), dim3(grid1(nq, 128)), dim3(128), 0,
-                           st, pkey, pid, nq, n_probe, k, routes, bkey, bid, kb, Q, d,
-                           (const int32_t *)r->cell_count.ptr(), dist, idx, flag, rows);
-    else
-        hipLaunchKernelGGL((rt_merge_certify_kernel<TQ, 32>), dim3(grid1(nq, 128)), dim3(128), 0,
-                           st, pkey, pid, nq, n_probe, k, routes, bkey, bid, kb, Q, d,
-                           (const int32_t *)r->cell_count.ptr(), dist, idx, flag, rows);
-    check_launch();
-    tm.end(tmg);
+    std::vector<int> scan_ids, merge_ids;
+    auto pass = [&](const int32_t *sroutes, const double *pd, const int64_t *pi) {
+        const int ts = tm.begin();
+        counting_sort(sroutes, P, C + 1, pord, pfirst, pcount, r->ws, st);
+        hipLaunchKernelGGL(rt_work_scan_kernel, dim3(1), dim3(256), 0, st, pcount, C, wfirst,
+                           wtotal);
+        check_launch();
+        int32_t items = 0;
+        FL_HIP(hipMemcpyAsync(&items, wtotal, 4, hipMemcpyDeviceToHost, st));
+        FL_HIP(hipStreamSynchronize(st));
+        int32_t *wcell = r->ws.get<int32_t>("wcell", items);
+        int32_t *wstart = r->ws.get<int32_t>("wstart", items);
+        hipLaunchKernelGGL(rt_work_fill_kernel, dim3(grid1(C, 256)), dim3(256), 0, st, pcount,
+                           wfirst, C, wcell, wstart);
+        check_launch();
+        if (items > 0) {
+            RtScanArgs a;
+            a.Q = Q, a.Xs = r->xs.ptr(), a.row_id = (const int32_t *)r->row_id.ptr();
+            a.cell_first = (const int64_t *)r->cell_first.ptr();
+            a.cell_count = (const int32_t *)r->cell_count.ptr();
+            a.pair_order = pord, a.pair_first = pfirst, a.pair_count = pcount;
+            a.work_cell = wcell, a.work_start = wstart;
+            a.d = d, a.n_probe = n_probe, a.k = k, a.out_key = pkey, a.out_id = pid;
+            if (k <= 16)
+                launch_scan_k<TQ, TX, 16>(a, items, st);
+            else
+                launch_scan_k<TQ, TX, 32>(a, items, st);
+        }
+        tm.end(ts);
+        const int tmg = tm.begin();
+        if (k <= 16)
+            hipLaunchKernelGGL((rt_merge_certify_kernel<TQ, 16>), dim3(grid1(nq, 128)), dim3(128),
+                               0, st, pkey, pid, nq, n_probe, k, routes, sroutes, C, pd, pi, bkey,
+                               bid, kb, Q, d, (const int32_t *)r->cell_count.ptr(), dist, idx,
+                               flag, rows);
+        else
+            hipLaunchKernelGGL((rt_merge_certify_kernel<TQ, 32>), dim3(grid1(nq, 128)), dim3(128),
+                               0, st, pkey, pid, nq, n_probe, k, routes, sroutes, C, pd, pi, bkey,
+                               bid, kb, Q, d, (const int32_t *)r->cell_count.ptr(), dist, idx,
+                               flag, rows);
+        check_launch();
+        tm.end(tmg);
+        scan_ids.push_back(ts);
+        merge_ids.push_back(tmg);
+    };
+    if (prev_routes) {
+        // a rescan: skip the cells the search before it scanned, and cells
+        // whose bound exceeds that search's k-th distance
+        int32_t *sr = r->ws.get<int32_t>("sroutes", P);
+        if (route == kRouteAll)
+            hipLaunchKernelGGL(rt_skip_all_kernel<TQ>, dim3(grid1(P, 256)), dim3(256), 0, st, Q,
+                               d, nq, C, (const double *)r->centers.ptr(),
+                               (const double *)r->radius.ptr(), bound_slack(d), prev_routes,
+                               prev_np, prev_d, k, sr);
+        else
+            hipLaunchKernelGGL(rt_skip_routes_kernel, dim3(grid1(P, 256)), dim3(256), 0, st,
+                               routes, nq, n_probe, prev_routes, prev_np, C,
+                               route == FL_ROUTE_BOUNDS ? bkey : nullptr, kb, prev_d, k, sr);
+        check_launch();
+        pass(sr, prev_d, prev_i);
+    } else if (n_probe >= 2 && route != kRouteAll) {
+        // each query's first cell, then its other cells whose bound does not
+        // exceed the k-th distance the first found
+        int32_t *sr = r->ws.get<int32_t>("sroutes", P);
+        hipLaunchKernelGGL(rt_first_slot_kernel, dim3(grid1(P, 256)), dim3(256), 0, st, routes,
+                           P, n_probe, C, sr);
+        check_launch();
+        pass(sr, nullptr, nullptr);
+        double *p1d = r->ws.get<double>("p1d", nq * k);
+        int64_t *p1i = r->ws.get<int64_t>("p1i", nq * k);
+        FL_HIP(hipMemcpyAsync(p1d, dist, (size_t)(nq * k) * 8, hipMemcpyDeviceToDevice, st));
+        FL_HIP(hipMemcpyAsync(p1i, idx, (size_t)(nq * k) * 8, hipMemcpyDeviceToDevice, st));
+        hipLaunchKernelGGL(rt_skip_by_bound_kernel<TQ>, dim3(grid1(P, 256)), dim3(256), 0, st, Q,
+                           d, nq, n_probe, routes, C, (const double *)r->centers.ptr(),
+                           (const double *)r->radius.ptr(), (const double *)r->cdist.ptr(),
+                           r->ws.get<double>("near_fold", nq), r->ws.get<int32_t>("near_cell", nq),
+                           bound_slack(d), p1d, k, sr);
+        check_launch();
+        pass(sr, p1d, p1i);
+    } else {
+        pass(routes, nullptr, nullptr);
+    }
     std::vector<uint8_t> hflag((size_t)nq);
     unsigned long long hrows = 0;
     FL_HIP(hipMemcpyAsync(hflag.data(), flag, (size_t)nq, hipMemcpyDeviceToHost, st));
@@ -583,8 +633,8 @@ void route_search(fl_router *r, const TQ *Q, int64_t nq, int k, int n_probe, int
     FL_HIP(hipStreamSynchronize(st));
     acc.ms_route += tm.ms(t);
     acc.ms_bounds += tm.ms(tb);
-    acc.ms_scan += tm.ms(ts);
-    acc.ms_merge += tm.ms(tmg);
+    for (int id : scan_ids) acc.ms_scan += tm.ms(id);
+    for (int id : merge_ids) acc.ms_merge += tm.ms(id);
     acc.rows += (int64_t)hrows;
     // 6. uncertified queries: a wider routed scan, then the brute force
     std::vector<int32_t> which;
@@ -602,12 +652,20 @@ void route_search(fl_router *r, const TQ *Q, int64_t nq, int k, int n_probe, int
     hipLaunchKernelGGL(rt_gather_queries_kernel<TQ>, dim3(grid1(m * d, 256)), dim3(256), 0, st, Q,
                        d, dw, m, fq);
     check_launch();
+    // what this search found and scanned: a rescan starts from it
+    double *pd = r->ws.get<double>(fp + "pd", m * k);
+    int64_t *pi = r->ws.get<int64_t>(fp + "pi", m * k);
+    int32_t *pr = r->ws.get<int32_t>(fp + "pr", m * n_probe);
+    hipLaunchKernelGGL(rt_gather_prev_kernel, dim3(grid1(m * std::max(k, n_probe), 256)),
+                       dim3(256), 0, st, dist, idx, routes, n_probe, dw, m, k, pd, pi, pr);
+    check_launch();
     // widening probes: max(8, 2 n_probe), doubling up to min(31, n_cells)
     const int wide = std::min(std::min(std::max(8, 2 * n_probe), kMaxProbe), C);
     if (depth < 3 && (wide > n_probe || route != FL_ROUTE_BOUNDS)) {
         if (depth == 0) acc.n_rescan += m;
         RtAcc inner;
-        route_search<TQ, TX>(r, fq, m, k, wide, FL_ROUTE_BOUNDS, fd, fi, inner, depth + 1);
+        route_search<TQ, TX>(r, fq, m, k, wide, FL_ROUTE_BOUNDS, fd, fi, inner, depth + 1, pd, pi,
+                             pr, n_probe);
         acc.rows += inner.rows;
         acc.n_fb += inner.n_fb;
     } else if (route != kRouteAll && n_probe < C && m <= kFullScanMax) {
@@ -615,7 +673,8 @@ void route_search(fl_router *r, const TQ *Q, int64_t nq, int k, int n_probe, int
         // (a brute force organised by cells; no bound is needed)
         if (depth == 0) acc.n_rescan += m;
         RtAcc inner;
-        route_search<TQ, TX>(r, fq, m, k, C, kRouteAll, fd, fi, inner, depth + 1);
+        route_search<TQ, TX>(r, fq, m, k, C, kRouteAll, fd, fi, inner, depth + 1, pd, pi, pr,
+                             n_probe);
         acc.rows += inner.rows;
         acc.n_fb += inner.n_fb;
     } else {

@@ -402,13 +402,32 @@ __global__ __launch_bounds__(256) void rt_top_cells_kernel(const float *__restri
     }
 }
 
+// Whether (key, id) is already in a list.
+template <int K>
+__device__ __forceinline__ bool rt_contains(const double (&lk)[K], const int32_t (&li)[K],
+                                            double key, int32_t id) {
+    bool f = false;
+#pragma unroll
+    for (int s = 0; s < K; ++s) f |= (lk[s] == key) & (li[s] == id);
+    return f;
+}
+
 // Merge a query's n_probe pair lists, then certify: the query is finite and
 // the smallest lower bound over cells it did not scan exceeds its k-th
 // distance.  Thread per query.  Uncertified queries get flag 1.
+//
+// A rescan passes the earlier search's k best (prev_d, prev_i; index -1 for
+// an empty entry) and the routes of the cells the earlier search scanned
+// are set to C in scan_routes: those slots were not scanned again and are
+// skipped here.  A (distance, index) pair already in the list is not
+// inserted again, so a cell scanned twice cannot duplicate a row.
 template <typename TQ, int K>
 __global__ void rt_merge_certify_kernel(const double *__restrict__ pkey,
                                         const int32_t *__restrict__ pid, int64_t nq, int n_probe,
                                         int k, const int32_t *__restrict__ routes,
+                                        const int32_t *__restrict__ scan_routes, int C,
+                                        const double *__restrict__ prev_d,
+                                        const int64_t *__restrict__ prev_i,
                                         const double *__restrict__ bkey,
                                         const int32_t *__restrict__ bid, int kb,
                                         const TQ *__restrict__ Q, int d,
@@ -425,14 +444,25 @@ __global__ void rt_merge_certify_kernel(const double *__restrict__ pkey,
         lk[s] = __builtin_huge_val();
         li[s] = RT_NOID;
     }
+    if (prev_d)
+        for (int j = 0; j < k; ++j) {
+            const int64_t pi = prev_i[q * k + j];
+            if (pi < 0) break;
+            const double key = prev_d[q * k + j];
+            if (!pair_less(key, (int32_t)pi, lk[K - 1], li[K - 1])) break;
+            rt_insert<K>(lk, li, key, (int32_t)pi);
+        }
     unsigned long long scanned = 0;
     for (int s = 0; s < n_probe; ++s) {
         const int64_t p = q * n_probe + s;
-        scanned += (unsigned long long)cell_count[routes[p]];
+        const int32_t sc = scan_routes[p];
+        if (sc == C) continue;
+        scanned += (unsigned long long)cell_count[sc];
         for (int j = 0; j < k; ++j) {
             const double key = pkey[p * k + j];
             const int32_t id = pid[p * k + j];
             if (!pair_less(key, id, lk[K - 1], li[K - 1])) break;
+            if (prev_d && rt_contains<K>(lk, li, key, id)) continue;
             rt_insert<K>(lk, li, key, id);
         }
     }
@@ -578,6 +608,110 @@ __global__ void rt_scatter_results_kernel(const double *__restrict__ fd,
     const int64_t q = which[t / k];
     dist[q * k + t % k] = fd[t];
     idx[q * k + t % k] = fi[t];
+}
+
+// A rescan's inputs from the search before it: the k best of each
+// uncertified query and the cells that search scanned.
+__global__ void rt_gather_prev_kernel(const double *__restrict__ dist,
+                                      const int64_t *__restrict__ idx,
+                                      const int32_t *__restrict__ routes, int n_probe,
+                                      const int32_t *__restrict__ which, int64_t m, int k,
+                                      double *__restrict__ pd, int64_t *__restrict__ pi,
+                                      int32_t *__restrict__ pr) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < m * k) {
+        const int64_t q = which[t / k];
+        pd[t] = dist[q * k + t % k];
+        pi[t] = idx[q * k + t % k];
+    }
+    if (t < m * n_probe) pr[t] = routes[(int64_t)which[t / n_probe] * n_probe + t % n_probe];
+}
+
+// scan_routes = routes, with C in place of every cell that need not be
+// scanned (the pair goes to bucket C, which gets no work items): a cell the
+// earlier search already scanned for that query, and, when the routes are
+// the smallest bounds in order (bkey non-null), a cell whose bound exceeds
+// the earlier search's k-th distance.  That distance only decreases as rows
+// are added, so such a cell's rows cannot enter the answer and the cell
+// passes the certificate unscanned.
+__global__ void rt_skip_routes_kernel(const int32_t *__restrict__ routes, int64_t nq,
+                                      int n_probe, const int32_t *__restrict__ prev_routes,
+                                      int prev_np, int C, const double *__restrict__ bkey,
+                                      int kb, const double *__restrict__ prev_d, int k,
+                                      int32_t *__restrict__ scan_routes) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nq * n_probe) return;
+    const int64_t q = t / n_probe;
+    const int32_t c = routes[t];
+    bool skip = false;
+    for (int s = 0; s < prev_np; ++s) skip |= prev_routes[q * prev_np + s] == c;
+    if (bkey) skip |= bkey[q * kb + t % n_probe] > prev_d[q * k + k - 1];
+    scan_routes[t] = skip ? C : c;
+}
+
+// The all-cells rescan's scan_routes (routes[q][c] = c): C in place of a
+// cell the earlier search scanned, an empty cell, or a cell whose ball bound
+// exceeds the earlier search's k-th distance (as in rt_skip_routes_kernel).
+template <typename TQ>
+__global__ void rt_skip_all_kernel(const TQ *__restrict__ Q, int d, int64_t nq, int C,
+                                   const double *__restrict__ centers,
+                                   const double *__restrict__ radius, double ep,
+                                   const int32_t *__restrict__ prev_routes, int prev_np,
+                                   const double *__restrict__ prev_d, int k,
+                                   int32_t *__restrict__ scan_routes) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nq * C) return;
+    const int64_t q = t / C;
+    const int32_t c = (int32_t)(t % C);
+    bool skip = radius[c] < 0.0;
+    for (int s = 0; s < prev_np; ++s) skip |= prev_routes[q * prev_np + s] == c;
+    if (!skip) {
+        const double f = fold_exact(Q + q * d, centers + (int64_t)c * d, d);
+        skip = rt_lower_bound(sqrt(f), radius[c], ep) > prev_d[q * k + k - 1];
+    }
+    scan_routes[t] = skip ? C : c;
+}
+
+// First pass of a search over n_probe >= 2 cells: slot 0 of each query.
+__global__ void rt_first_slot_kernel(const int32_t *__restrict__ routes, int64_t P, int n_probe,
+                                     int C, int32_t *__restrict__ scan_routes) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= P) return;
+    scan_routes[t] = t % n_probe == 0 ? routes[t] : C;
+}
+
+// Second pass: slots 1.. whose cell's key (the larger of the ball and
+// bisector bounds, as in rt_bounds_kernel) does not exceed the k-th distance
+// of the first pass; C elsewhere.  The skipped cells' rows cannot enter the
+// answer (that distance only decreases) and the cells pass the certificate
+// unscanned.
+template <typename TQ>
+__global__ void rt_skip_by_bound_kernel(const TQ *__restrict__ Q, int d, int64_t nq, int n_probe,
+                                        const int32_t *__restrict__ routes, int C,
+                                        const double *__restrict__ centers,
+                                        const double *__restrict__ radius,
+                                        const double *__restrict__ cdist,
+                                        const double *__restrict__ near_fold,
+                                        const int32_t *__restrict__ near_cell, double ep,
+                                        const double *__restrict__ prev_d, int k,
+                                        int32_t *__restrict__ scan_routes) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nq * n_probe) return;
+    const int64_t q = t / n_probe;
+    const int32_t c = routes[t];
+    bool skip = t % n_probe == 0 || radius[c] < 0.0;
+    if (!skip) {
+        const double fold = fold_exact(Q + q * d, centers + (int64_t)c * d, d);
+        double key = rt_lower_bound(sqrt(fold), radius[c], ep);
+        const int32_t a = near_cell[q];
+        const double D = cdist[(int64_t)a * C + c];
+        if (c != a && D > 0.0) {
+            const double v = rt_bisector_bound(fold, near_fold[q], D, radius[c], ep);
+            key = v > key ? v : key;
+        }
+        skip = key > prev_d[q * k + k - 1];
+    }
+    scan_routes[t] = skip ? C : c;
 }
 
 // Selected rows widened to f64 (initial cell centres).
