@@ -472,10 +472,11 @@ void launch_top_cells(const float *Z, int64_t ldz, int64_t rows, int C, int n_pr
 }
 
 // Routed search of nq device queries (one pass; nq * n_probe < 2^31).
-// Queries whose certificate fails are searched again over more cells with
-// the smallest bounds (8 or twice n_probe, doubling up to min(31,
-// n_cells)); up to kFullScanMax queries left then scan every cell, and what
-// still fails (non-finite queries) goes to the brute force.  A rescan starts
+// Queries whose certificate fails are searched again: up to kFullScanMax of
+// them over every cell whose ball bound does not exceed the k-th distance
+// found so far, more over the cells with the smallest bounds (8 or twice
+// n_probe, doubling up to min(31, n_cells)); what still fails (non-finite
+// queries) goes to the brute force.  A rescan starts
 // from the k best its parent found (prev_d, prev_i) and skips the cells the
 // parent scanned (prev_routes, prev_np per query).
 template <typename TQ, typename TX>
@@ -661,20 +662,21 @@ void route_search(fl_router *r, const TQ *Q, int64_t nq, int k, int n_probe, int
     check_launch();
     // widening probes: max(8, 2 n_probe), doubling up to min(31, n_cells)
     const int wide = std::min(std::min(std::max(8, 2 * n_probe), kMaxProbe), C);
-    if (depth < 3 && (wide > n_probe || route != FL_ROUTE_BOUNDS)) {
-        if (depth == 0) acc.n_rescan += m;
-        RtAcc inner;
-        route_search<TQ, TX>(r, fq, m, k, wide, FL_ROUTE_BOUNDS, fd, fi, inner, depth + 1, pd, pi,
-                             pr, n_probe);
-        acc.rows += inner.rows;
-        acc.n_fb += inner.n_fb;
-    } else if (route != kRouteAll && n_probe < C && m <= kFullScanMax) {
-        // a few queries left: scan every cell with the same exact kernels
-        // (a brute force organised by cells; no bound is needed)
+    if (route != kRouteAll && n_probe < C && m <= kFullScanMax) {
+        // a few queries left: every cell whose ball bound does not exceed the
+        // k-th distance found so far, with the same exact kernels; that
+        // certifies every finite query
         if (depth == 0) acc.n_rescan += m;
         RtAcc inner;
         route_search<TQ, TX>(r, fq, m, k, C, kRouteAll, fd, fi, inner, depth + 1, pd, pi, pr,
                              n_probe);
+        acc.rows += inner.rows;
+        acc.n_fb += inner.n_fb;
+    } else if (depth < 3 && (wide > n_probe || route != FL_ROUTE_BOUNDS)) {
+        if (depth == 0) acc.n_rescan += m;
+        RtAcc inner;
+        route_search<TQ, TX>(r, fq, m, k, wide, FL_ROUTE_BOUNDS, fd, fi, inner, depth + 1, pd, pi,
+                             pr, n_probe);
         acc.rows += inner.rows;
         acc.n_fb += inner.n_fb;
     } else {
