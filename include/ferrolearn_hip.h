@@ -311,12 +311,15 @@ void fl_knn_free(fl_knn_handle h);
  * rules of ferrolearn-neural/src/mlp.rs:217-409, 1326-1340) is trained on
  * the cells that hold each training query's exact k nearest neighbours
  * (brute_force_knn, kdtree.rs:364-391).  A routed search scans the rows of
- * n_probe cells with the exact reference distance and certifies each query
- * with the triangle inequality: every unscanned cell's lower bound must
- * exceed the k-th distance found.  A query that fails is searched again over
- * more cells with the smallest bounds (8 or twice n_probe, doubling up to
- * min(31, n_cells)), and one that still fails runs the brute force, so
- * results equal fl_knn_kneighbors on the same handle, bit for bit. */
+ * n_probe cells with the exact reference distance (each query's first cell,
+ * then its other cells whose lower bound does not exceed the k-th distance
+ * found) and certifies each query with the triangle inequality: every
+ * unscanned cell's lower bound must exceed the k-th distance found.  Up to
+ * 2048 queries that fail are searched again over every cell whose ball bound
+ * does not exceed their k-th distance; more go first over the cells with the
+ * smallest bounds (8 or twice n_probe, doubling up to min(31, n_cells)).  A
+ * query that still fails (non-finite) runs the brute force, so results equal
+ * fl_knn_kneighbors on the same handle, bit for bit. */
 typedef struct fl_router *fl_router_handle;
 
 typedef enum fl_route {
@@ -359,10 +362,11 @@ typedef struct fl_router_report {
     double ms_bounds;     /* search: centre distances and lower bounds       */
     double ms_scan;       /* search: exact scan of the chosen cells          */
     double ms_merge;      /* search: merge and certificate                   */
-    double ms_fallback;   /* search: brute force for uncertified queries     */
+    double ms_fallback;   /* search: rescans and brute force for queries whose
+                             first certificate failed                      */
     int64_t n_queries;
     int64_t n_rescan;     /* queries whose first certificate failed: searched
-                             again over more cells with the smallest bounds */
+                             again over more cells                          */
     int64_t n_fallback;   /* queries that still failed: brute force          */
     int64_t rows_scanned; /* database rows scanned by the routed search      */
     double loss_first;    /* train: mean loss of the first and last epochs   */
