@@ -317,6 +317,7 @@ struct fl_router {
     int64_t n = 0, d = 0;
     int C = 0;
     DevBuf centers, radius, cdist, cell_first, cell_count, row_id, xs, label32, row_cell;
+    DevBuf cell_by_size;   // cells by row count, largest first (scan work order)
     Mlp net;
     hipStream_t stream = nullptr;
     Workspace ws;
@@ -406,6 +407,16 @@ void build_cells(fl_router *r, uint64_t seed, int64_t iters) {
     hipLaunchKernelGGL(rt_radius_finish_kernel, dim3(grid1(C, 256)), dim3(256), 0, st, mf, count,
                        (int)C, rad);
     check_launch();
+    // the scan's work items go out largest cell first, so that a launch does
+    // not end waiting on a large cell whose items started last
+    std::vector<int32_t> hc((size_t)C), perm((size_t)C);
+    FL_HIP(hipMemcpyAsync(hc.data(), count, (size_t)C * 4, hipMemcpyDeviceToHost, st));
+    FL_HIP(hipStreamSynchronize(st));
+    for (int64_t c = 0; c < C; ++c) perm[(size_t)c] = (int32_t)c;
+    std::stable_sort(perm.begin(), perm.end(),
+                     [&](int32_t a, int32_t b) { return hc[(size_t)a] > hc[(size_t)b]; });
+    int32_t *pd = (int32_t *)r->cell_by_size.get((size_t)C * 4);
+    FL_HIP(hipMemcpyAsync(pd, perm.data(), (size_t)C * 4, hipMemcpyHostToDevice, st));
     FL_HIP(hipStreamSynchronize(st));
 }
 
@@ -549,8 +560,8 @@ void route_search(fl_router *r, const TQ *Q, int64_t nq, int k, int n_probe, int
     auto pass = [&](const int32_t *sroutes, const double *pd, const int64_t *pi) {
         const int ts = tm.begin();
         counting_sort(sroutes, P, C + 1, pord, pfirst, pcount, r->ws, st);
-        hipLaunchKernelGGL(rt_work_scan_kernel, dim3(1), dim3(256), 0, st, pcount, C, wfirst,
-                           wtotal);
+        hipLaunchKernelGGL(rt_work_scan_kernel, dim3(1), dim3(256), 0, st, pcount,
+                           (const int32_t *)r->cell_by_size.ptr(), C, wfirst, wtotal);
         check_launch();
         int32_t items = 0;
         FL_HIP(hipMemcpyAsync(&items, wtotal, 4, hipMemcpyDeviceToHost, st));

@@ -549,9 +549,12 @@ __global__ void rt_target_cells_kernel(const int64_t *__restrict__ nbr, int64_t 
     if (i < count) cells[i] = nbr[i] >= 0 ? label[nbr[i]] : -1;
 }
 
-// Work items of the scan: per cell, ceil(pairs / 64) blocks.  One 256-thread
-// block scans the per-cell block counts; total[0] = number of items.
+// Work items of the scan: per cell, ceil(pairs / 64) blocks, the cells taken
+// in `order` (largest first, so that their items start first).  One
+// 256-thread block scans the per-cell block counts; total[0] = number of
+// items.
 __global__ __launch_bounds__(256) void rt_work_scan_kernel(const int32_t *__restrict__ pair_count,
+                                                           const int32_t *__restrict__ order,
                                                            int C, int32_t *__restrict__ first,
                                                            int32_t *__restrict__ total) {
     __shared__ int32_t part[256];
@@ -559,7 +562,7 @@ __global__ __launch_bounds__(256) void rt_work_scan_kernel(const int32_t *__rest
     const int per = (C + 255) / 256;
     const int lo = t * per, hi = lo + per < C ? lo + per : C;
     int32_t s = 0;
-    for (int c = lo; c < hi; ++c) s += (pair_count[c] + RT_Q - 1) / RT_Q;
+    for (int i = lo; i < hi; ++i) s += (pair_count[order[i]] + RT_Q - 1) / RT_Q;
     part[t] = s;
     __syncthreads();
     for (int o = 1; o < 256; o <<= 1) {
@@ -569,7 +572,8 @@ __global__ __launch_bounds__(256) void rt_work_scan_kernel(const int32_t *__rest
         __syncthreads();
     }
     int32_t run = t > 0 ? part[t - 1] : 0;
-    for (int c = lo; c < hi; ++c) {
+    for (int i = lo; i < hi; ++i) {
+        const int c = order[i];
         first[c] = run;
         run += (pair_count[c] + RT_Q - 1) / RT_Q;
     }
