@@ -491,7 +491,7 @@ def main():
     ap.add_argument("--workload", default="all",
                     choices=["all", "knn", "kmeans", "router", "batched", "copy"])
     ap.add_argument("--batched-problems", type=int, default=256)
-    ap.add_argument("--batched-workers", type=int, default=8)
+    ap.add_argument("--batched-workers", type=int, default=16)
     ap.add_argument("--router-scale", type=float, default=1.0,
                     help="fraction of the router workload's rows, cells and queries")
     ap.add_argument("--router-epochs", type=int, default=3)
