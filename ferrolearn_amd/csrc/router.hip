@@ -683,7 +683,7 @@ void route_search(fl_router *r, const TQ *Q, int64_t nq, int k, int n_probe, int
                              n_probe);
         acc.rows += inner.rows;
         acc.n_fb += inner.n_fb;
-    } else if (depth < 3 && (wide > n_probe || route != FL_ROUTE_BOUNDS)) {
+    } else if (route != kRouteAll && depth < 3 && (wide > n_probe || route != FL_ROUTE_BOUNDS)) {
         if (depth == 0) acc.n_rescan += m;
         RtAcc inner;
         route_search<TQ, TX>(r, fq, m, k, wide, FL_ROUTE_BOUNDS, fd, fi, inner, depth + 1, pd, pi,

@@ -191,3 +191,25 @@ def test_unstructured_data_full_scan():
     rep = r.report
     print(f"unstructured: rescans {rep['n_rescan']}, brute force {rep['n_fallback']}")
     assert rep["n_rescan"] > 0 and rep["n_fallback"] == 0
+
+
+def test_non_finite_queries(setup):
+    # A query with an infinite feature never certifies: after the all-cells
+    # pass it goes to the handle's brute force, so its answer (every
+    # distance inf, ties by index) is the brute force's.  A NaN feature gives
+    # NaN distances, which the reference rejects (kdtree.rs:386); both
+    # searches raise the same error.
+    X, fitted, router, _, _, Q = setup
+    Qi = Q[:64].copy()
+    Qi[10, 0] = np.inf
+    rd, ri = router.kneighbors(Qi, n_probe=2)
+    bd, bi = fitted.kneighbors(Qi)
+    assert (ri == bi).all()
+    assert (rd.view(np.int64) == bd.view(np.int64)).all()
+    assert router.report["n_fallback"] == 1
+    Qn = Q[:64].copy()
+    Qn[3, 5] = np.nan
+    with pytest.raises(F.NumericalInstability):
+        fitted.kneighbors(Qn)
+    with pytest.raises(F.NumericalInstability):
+        router.kneighbors(Qn, n_probe=2)
