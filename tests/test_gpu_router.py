@@ -213,3 +213,21 @@ def test_non_finite_queries(setup):
         fitted.kneighbors(Qn)
     with pytest.raises(F.NumericalInstability):
         router.kneighbors(Qn, n_probe=2)
+
+
+def test_large_cells():
+    # Three cells of about 13,000 rows, one probe, k up to 32: the first
+    # pass, the bound-skipping rescans and the all-cells pass on large
+    # cells.  Answers stay the brute force's.
+    rng = np.random.default_rng(5)
+    X = rng.normal(size=(40000, 12)).astype(np.float32)
+    fitted = F.KNeighborsRegressor().with_n_neighbors(10).fit(X, np.zeros(len(X)))
+    r = F.CellRouter(fitted, n_cells=3, cell_iter=4, hidden=64, seed=1)
+    assert r.cells()[2].max() > 8192
+    Q = rng.normal(size=(500, 12)).astype(np.float32)
+    for route in ("bounds", "network"):
+        for k in (10, 32):
+            rd, ri = r.kneighbors(Q, n_neighbors=k, n_probe=1, route=route)
+            bd, bi = fitted.kneighbors(Q, n_neighbors=k)
+            assert (ri == bi).all(), (route, k)
+            assert (rd.view(np.int64) == bd.view(np.int64)).all(), (route, k)
