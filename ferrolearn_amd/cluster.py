@@ -42,6 +42,25 @@ def _is_device(x) -> bool:
     return hasattr(x, "is_cuda") and bool(x.is_cuda)
 
 
+def _dev2d(x):
+    """A device input as the kernels read it: 2-D, row-major and packed.
+
+    The C entry points take a bare pointer and read n*d consecutive values,
+    so a strided or broadcast CUDA tensor view (a column slice, a transpose,
+    x[::2], an expand) is first copied into a packed tensor; reading the view
+    in place would give wrong rows, or read past its storage.  DeviceArray
+    is always packed."""
+    from .device import DeviceArray
+    if len(x.shape) != 2:
+        from .errors import ShapeMismatch
+        raise ShapeMismatch(f"expected a 2-D array, got shape {tuple(x.shape)}")
+    if isinstance(x, DeviceArray):
+        return x
+    if hasattr(x, "is_contiguous") and not x.is_contiguous():
+        x = x.contiguous()
+    return x
+
+
 def _to_host(x) -> np.ndarray:
     """Copy a DeviceArray or CUDA tensor back to a numpy array."""
     from .device import DeviceArray
@@ -127,6 +146,7 @@ class KMeans:
         inertia = C.c_double()
         n_iter = C.c_int64()
         if _is_device(x):
+            x = _dev2d(x)
             n, d = x.shape
             dt = _dtype_code(x.dtype)
             k = self.n_clusters
@@ -283,6 +303,7 @@ def kmeans_fit_batched(problems, n_workers: int = 8):
     for i, (x, model, init) in enumerate(problems):
         if not _is_device(x):
             raise TypeError("kmeans_fit_batched takes device arrays")
+        x = _dev2d(x)
         n, d = x.shape
         k = model.n_clusters
         centers = _dev_empty((max(k, 0), d), np.float64, x)
@@ -290,7 +311,7 @@ def kmeans_fit_batched(problems, n_workers: int = 8):
         it = None
         if init is not None:
             it = _dev_from_host(np.ascontiguousarray(init, dtype=np.float64), x)
-        keep.append((centers, labels, it))
+        keep.append((centers, labels, it, x))   # x: a packed copy must outlive the call
         pr = arr[i]
         pr.X = x.data_ptr()
         pr.dtype = _dtype_code(x.dtype)

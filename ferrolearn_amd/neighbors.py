@@ -17,7 +17,7 @@ import enum
 import numpy as np
 
 from . import _lib as L
-from .cluster import _dev_empty, _dtype_code, _host2d, _is_device, _to_host
+from .cluster import _dev2d, _dev_empty, _dtype_code, _host2d, _is_device, _to_host
 from .errors import check
 
 
@@ -86,7 +86,7 @@ class _Fitted:
 
     def _query(self, x):
         if _is_device(x):
-            return x, True
+            return _dev2d(x), True
         return _host2d(x), False
 
     def kneighbors(self, x, n_neighbors=None):

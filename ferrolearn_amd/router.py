@@ -18,7 +18,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib as L
-from .cluster import _dev_empty, _dtype_code, _host2d, _is_device
+from .cluster import _dev2d, _dev_empty, _dtype_code, _host2d, _is_device
 from .errors import check
 
 
@@ -53,7 +53,7 @@ class CellRouter:
         target "certify" -- every cell whose lower bound does not exceed the
         exact k-th distance; "neighbors" -- the cells holding the neighbours."""
         from .device import to_device
-        q = queries if _is_device(queries) else to_device(_host2d(queries))
+        q = _dev2d(queries) if _is_device(queries) else to_device(_host2d(queries))
         nq, d = q.shape
         p = L.fl_router_train_params()
         L.lib().fl_router_train_params_default(C.byref(p))
@@ -71,7 +71,7 @@ class CellRouter:
         """(distances, indices), each n_queries x k, equal to the brute force's."""
         from .device import to_device
         dev = _is_device(x)
-        q = x if dev else to_device(_host2d(x))
+        q = _dev2d(x) if dev else to_device(_host2d(x))
         nq, d = q.shape
         k = 0 if n_neighbors is None else int(n_neighbors)
         kk = self._fitted.n_neighbors if k == 0 else k

@@ -204,3 +204,41 @@ def test_device_array_fit():
     assert f.n_iter() == it_o
     assert (f.labels().numpy() == l_o).all()
     assert (bits(f.cluster_centers().numpy()) == bits(c_o)).all()
+
+
+def config3_blobs(n, d=64, comps=1024, seed=0):
+    """The large-KMeans workload's data at reduced n: f32 rows from 1,024
+    Gaussian blobs, means uniform in [-20, 20]^64, per-blob std uniform in
+    [0.5, 3]."""
+    rng = np.random.default_rng(seed)
+    means = rng.uniform(-20, 20, size=(comps, d))
+    sd = rng.uniform(0.5, 3.0, size=comps)
+    c = rng.integers(0, comps, size=n)
+    return (means[c] + rng.normal(size=(n, d)) * sd[c, None]).astype(np.float32)
+
+
+@pytest.fixture(scope="module")
+def config3_case():
+    X = config3_blobs(120_000)
+    k = 1024
+    init = F.kmeans_plusplus(X, k, 0)     # the library's k-means++, seed 0
+    X64 = X.astype(np.float64)
+    # the oracle's k-means++ picks the same rows (checked at this k on a
+    # subsample, where the oracle finishes quickly)
+    sub = X64[:6000]
+    assert (bits(F.kmeans_plusplus(X[:6000], k, 0)) == bits(O.kmeanspp(sub, k, 0))).all()
+    c_o, l_o, i_o, it_o = O.kmeans_lloyd(X64, init, 6, 0.0)
+    return X, init, (c_o, l_o, i_o, it_o)
+
+
+@pytest.mark.parametrize("precision", ["bf16x3", "fp64"])
+def test_config3_shape_k1024(config3_case, precision):
+    # k = 1,024 on the large-KMeans workload's data: six Lloyd iterations with
+    # tol 0 from the same initial centres; labels, n_iter and centres bit for
+    # bit, inertia to 1e-12.
+    X, init, (c_o, l_o, i_o, it_o) = config3_case
+    f = fit_lib(F.to_device(X), init, 6, 0.0, precision)
+    assert f.n_iter() == it_o == 6
+    assert (f.labels().numpy() == l_o).all()
+    assert (bits(f.cluster_centers().numpy()) == bits(c_o)).all()
+    assert abs(f.inertia() - i_o) <= 1e-12 * i_o

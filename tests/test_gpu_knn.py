@@ -153,6 +153,30 @@ f = F.KMeans(5).with_n_init(1).with_max_iter(20).fit(torch.from_numpy(X).cuda(),
     init_centers=X[:5].astype(np.float64).reshape(1, 5, 24))
 c_o, l_o, _, it_o = O.kmeans_lloyd(X.astype(np.float64), X[:5].astype(np.float64), 20, 1e-4)
 assert (f.labels().cpu().numpy() == l_o).all() and f.n_iter() == it_o
+# Strided views: columns 8..31 of wider tensors (the reads stay inside the
+# storage), a step-2 row view and a transposed view must be packed first.
+wideQ = torch.from_numpy(rng.normal(size=(300, 40)).astype(np.float32)).cuda()
+wideQ[:, 8:32] = torch.from_numpy(Q).cuda()
+dist, idx = clf.kneighbors(wideQ[:, 8:32])
+assert (idx.cpu().numpy() == i_o).all() and (dist.cpu().numpy() == d_o).all()
+dist, idx = clf.kneighbors(torch.from_numpy(np.ascontiguousarray(Q.T)).cuda().t())
+assert (idx.cpu().numpy() == i_o).all()
+Q2 = torch.from_numpy(np.repeat(Q, 2, axis=0)).cuda()[::2]
+dist, idx = clf.kneighbors(Q2)
+assert (idx.cpu().numpy() == i_o).all()
+wideX = torch.zeros((5000, 33), dtype=torch.float32, device="cuda")
+wideX[:, 1:25] = torch.from_numpy(X).cuda()
+f = F.KMeans(5).with_n_init(1).with_max_iter(20).fit(wideX[:, 1:25],
+    init_centers=X[:5].astype(np.float64).reshape(1, 5, 24))
+assert (f.labels().cpu().numpy() == l_o).all() and f.n_iter() == it_o
+router = F.CellRouter(clf, n_cells=12, cell_iter=5, hidden=64, seed=0)
+rd, ri = router.kneighbors(wideQ[:, 8:32], n_probe=2, route="bounds")
+assert (ri.cpu().numpy() == i_o).all() and (rd.cpu().numpy() == d_o).all()
+try:
+    clf.kneighbors(torch.zeros((2, 3, 24), device="cuda"))
+    raise AssertionError("3-D input accepted")
+except F.ShapeMismatch:
+    pass
 print("torch interop ok")
 """
 

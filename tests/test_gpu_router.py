@@ -231,3 +231,77 @@ def test_large_cells():
             bd, bi = fitted.kneighbors(Q, n_neighbors=k)
             assert (ri == bi).all(), (route, k)
             assert (rd.view(np.int64) == bd.view(np.int64)).all(), (route, k)
+
+
+def aniso_mixture(N, comps, seed, d=96):
+    """The learned-router workload's data family at reduced size: a mixture of
+    `comps` anisotropic Gaussians in d = 96 (random rotations, axis std
+    log-uniform in [0.2, 3], means uniform in [-10, 10]); the mixture is fixed
+    by seed 0 and `seed` draws the points."""
+    g = np.random.default_rng(0)
+    means = g.uniform(-10, 10, size=(comps, d))
+    sd = np.exp(g.uniform(np.log(0.2), np.log(3.0), size=(comps, d)))
+    rot = np.linalg.qr(g.standard_normal((comps, d, d)))[0]
+    rng = np.random.default_rng(seed)
+    c = rng.integers(0, comps, size=N)
+    z = rng.standard_normal((N, d)) * sd[c]
+    X = np.empty((N, d))
+    for j in range(comps):
+        m = c == j
+        X[m] = z[m] @ rot[j].T + means[j]
+    return X.astype(np.float32)
+
+
+@pytest.fixture(scope="module")
+def config5_setup():
+    X = aniso_mixture(200_000, 256, seed=1)
+    fitted = (F.KNeighborsClassifier().with_n_neighbors(10).with_precision("bf16x3")
+              .fit(X, np.arange(X.shape[0]) % 10))
+    router = F.CellRouter(fitted, n_cells=256, cell_iter=8, hidden=512, seed=0)
+    Qt = aniso_mixture(40_000, 256, seed=2)
+    rep = router.train(Qt, n_neighbors=10, epochs=2, batch_size=2048, learning_rate=1e-3, seed=0)
+    Q = aniso_mixture(5000, 256, seed=3)
+    return X, fitted, router, rep, Q
+
+
+@pytest.mark.parametrize("route", ["network", "bounds"])
+@pytest.mark.parametrize("n_probe", [2, 4])
+def test_config5_shape_routed_equals_brute_force(config5_setup, route, n_probe):
+    # d = 96, 256 cells over 200,000 rows, a 96-512-512-256 network: every
+    # routed answer is the brute force's, bit for bit.
+    X, fitted, router, rep, Q = config5_setup
+    assert rep["loss_last"] < rep["loss_first"]
+    bd, bi = fitted.kneighbors(Q)
+    rd, ri = router.kneighbors(Q, n_probe=n_probe, route=route)
+    assert (ri == bi).all()
+    assert (rd.view(np.int64) == bd.view(np.int64)).all()
+    print(f"d=96 {route} n_probe={n_probe}: rescans {router.report['n_rescan']}, "
+          f"brute force {router.report['n_fallback']} of {Q.shape[0]}")
+
+
+def test_config5_shape_against_oracle(config5_setup):
+    X, fitted, router, rep, Q = config5_setup
+    rd, ri = router.kneighbors(Q[:16], n_probe=2)
+    od, oi = O.kneighbors(X, Q[:16], 10)
+    assert (ri == oi).all()
+    assert (rd.view(np.int64) == od.view(np.int64)).all()
+
+
+def test_gradients_match_oracle_hidden512():
+    # the router network's widths at the workload's hidden size
+    rng = np.random.default_rng(8)
+    d, h, c, B, k = 96, 512, 256, 64, 10
+    n = d * h + h + h * h + h + h * c + c
+    p = (rng.normal(size=n) * 0.05).astype(np.float32)
+    X = aniso_mixture(B, 256, seed=9)
+    cells = rng.integers(0, c, size=(B, k)).astype(np.int32)
+    cells[::5, 6:] = -1
+    loss, g = mlp_gradients(p, X, cells, h, c)
+    ol, og = O.mlp_loss_grad(p, X, cells, d, h, c, emulate_bf16=True)
+    assert abs(loss - ol) <= 1e-5 * abs(ol)
+    o = 0
+    for s in [(d, h), (h,), (h, h), (h,), (h, c), (c,)]:
+        m = int(np.prod(s))
+        gg, oo = g[o:o + m].astype(np.float64), og[o:o + m]
+        assert np.linalg.norm(gg - oo) <= 1e-2 * np.linalg.norm(oo) + 1e-12, s
+        o += m
