@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstring>
 #include <map>
 #include <string>
 #include <vector>
@@ -46,16 +47,54 @@ class DevBuf {
 };
 
 // Named scratch buffers that only grow; reused across calls on one object.
+// Arena form (constructed over caller memory): get() hands out 256-byte
+// aligned slices of that memory by name and never allocates; a slot keeps
+// its first size, and running past the end throws instead of allocating.
 class Workspace {
   public:
+    Workspace() = default;
+    Workspace(void *base, size_t bytes) : arena_((unsigned char *)base), arena_cap_(bytes) {}
+    Workspace(const Workspace &) = delete;
+    Workspace &operator=(const Workspace &) = delete;
+
     template <typename T>
     T *get(const std::string &name, int64_t count) {
-        return static_cast<T *>(bufs_[name].get((size_t)(count > 0 ? count : 1) * sizeof(T)));
+        const size_t bytes = (size_t)(count > 0 ? count : 1) * sizeof(T);
+        if (arena_) return static_cast<T *>(arena_get(name.c_str(), bytes));
+        return static_cast<T *>(bufs_[name].get(bytes));
     }
     void clear() { bufs_.clear(); }
+    size_t arena_used() const { return arena_used_; }
 
   private:
+    static constexpr int kSlots = 48;
+    struct Slot {
+        char name[24];
+        size_t off, bytes;
+    };
+    void *arena_get(const char *name, size_t bytes) {
+        for (int i = 0; i < nslots_; ++i)
+            if (std::strncmp(slots_[i].name, name, sizeof(slots_[i].name)) == 0) {
+                if (bytes > slots_[i].bytes)
+                    throw DeviceError(std::string("internal: scratch slot '") + name + "' grew");
+                return arena_ + slots_[i].off;
+            }
+        const size_t off = (arena_used_ + 255) & ~(size_t)255;
+        if (nslots_ == kSlots || std::strlen(name) >= sizeof(slots_[0].name) ||
+            off + bytes > arena_cap_)
+            throw DeviceError("internal: the fixed scratch buffer is too small for this call");
+        Slot &s = slots_[nslots_++];
+        std::strncpy(s.name, name, sizeof(s.name));
+        s.off = off;
+        s.bytes = bytes;
+        arena_used_ = off + bytes;
+        return arena_ + off;
+    }
     std::map<std::string, DevBuf> bufs_;
+    unsigned char *arena_ = nullptr;
+    size_t arena_cap_ = 0, arena_used_ = 0;
+    Slot slots_[kSlots];
+    int nslots_ = 0;
 };
 
 // Accumulates kernel time between two events recorded on one stream.

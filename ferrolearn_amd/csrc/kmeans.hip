@@ -10,6 +10,7 @@
 #include <atomic>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <unordered_map>
 #include <string>
 #include <type_traits>
@@ -19,6 +20,14 @@
 #include "kmeans_kernels.hpp"
 
 using namespace fl;
+
+// Fixed device scratch of fl_kmeans_predict_device_noalloc.  Each GPU the
+// library runs on holds its own copy until the process ends.  In host code
+// `scratch` names a host-side stand-in, never the device memory: kernels and
+// copies only ever get the address hipGetSymbolAddress returns for the
+// current device (scratch_address below).
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
 namespace {
 
@@ -47,8 +56,14 @@ __global__ void labels_to_i64_kernel(const int32_t *__restrict__ a, int64_t n,
 template <typename TX>
 class Lloyd {
   public:
-    Lloyd(const TX *X, int64_t n, int d, int64_t k, int32_t precision, hipStream_t st)
-        : X_(X), n_(n), d_(d), k_(k), st_(st) {
+    // scratch / scratch_bytes: a fixed buffer that every device buffer of
+    // this object is carved from (nothing is allocated; phase timing is off).
+    // C_range: centres to include in the range check (predict, where the
+    // centres are not means of the samples).
+    Lloyd(const TX *X, int64_t n, int d, int64_t k, int32_t precision, hipStream_t st,
+          void *scratch = nullptr, size_t scratch_bytes = 0, const double *C_range = nullptr)
+        : X_(X), n_(n), d_(d), k_(k), st_(st), ws_(scratch, scratch_bytes) {
+        timing_ = scratch == nullptr;
         // The direct exact scan is cheaper than filter + re-score when there
         // are few centres.
         direct_ = precision == FL_PREC_EXACT || k <= 32;
@@ -67,6 +82,9 @@ class Lloyd {
             FL_HIP(hipMemsetAsync(flag, 0, 4, st));
             hipLaunchKernelGGL(range_flags_kernel<TX>, dim3(1024), dim3(256), 0, st, X, n * d,
                                flag);
+            if (C_range)
+                hipLaunchKernelGGL(range_flags_kernel<double>, dim3(64), dim3(256), 0, st, C_range,
+                                   k * d, flag);
             check_launch();
             unsigned int hflag = 0;
             FL_HIP(hipMemcpyAsync(&hflag, flag, 4, hipMemcpyDeviceToHost, st));
@@ -95,10 +113,12 @@ class Lloyd {
     // Phase timers are read after the caller's next synchronisation
     // (flush), so that timing adds no host round trip to an iteration.
     int tbegin() {
+        if (!timing_) return -1;
         if (!tm_) tm_.reset(new PhaseTimer(st_));
         return tm_->begin();
     }
     void tend(int id, double *slot) {
+        if (id < 0) return;
         tm_->end(id);
         pend_.push_back({id, slot});
     }
@@ -508,6 +528,7 @@ class Lloyd {
     int d4_ = 0;                 // f64 MFMA filter (d4_ > 0)
     double *xn_ = nullptr, *xs_ = nullptr;
     unsigned short *xh_ = nullptr, *xl_ = nullptr;
+    bool timing_ = true;
     std::unique_ptr<PhaseTimer> tm_;
     std::vector<std::pair<int, double *>> pend_;
     Workspace ws_;
@@ -632,6 +653,40 @@ fl_status fit_dispatch(const void *dX, int32_t dtype, int64_t n, int64_t d,
                                  inertia, n_iter, rep);
     return fit_device<double>((const double *)dX, n, (int)d, p, d_init, d_centers, d_labels,
                               inertia, n_iter, rep);
+}
+
+// ------------------------------------------------------ allocation-free --
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxScratchDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device (on its first call there) and kept.
+void *scratch_address() {
+    int dev = 0;
+    FL_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxScratchDevices)
+        throw DeviceError("device index beyond the scratch address table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw DeviceError(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
+}
+
+// Upper bound of the scratch bytes Lloyd::assign takes for a chunk of m rows
+// against k centres of d features (every slot any filter form can take, each
+// rounded up to 256 bytes), plus the chunk's int32 labels and best folds.
+size_t noalloc_bytes(int64_t m, int64_t k, int d) {
+    const int64_t dpad = (int64_t)bf16_nkc(d) * 32;   // BF16 forms (0: not used)
+    const int64_t dp = (int64_t)f64_d4(d) * 4;        // f64 filter padding (0: not used)
+    const int64_t rows = m * (8 + 8 + 4 * dpad + 32 + 16 + 4 + 8 + 4 + 8 + 4 + 4 + 8);
+    const int64_t cent = k * (8 + 8 + 4 * dpad + 8 + 8 * dp + 8 * (int64_t)d);
+    return (size_t)(rows + cent + 32 * 256);
 }
 
 fl_status check_predict(int64_t k, int64_t d_centers, int64_t n, int64_t d, int32_t dtype) {
@@ -1039,6 +1094,60 @@ fl_status fl_kmeans_predict(const double *centers, int64_t k, int64_t d_centers,
         check_launch();
         FL_HIP(hipStreamSynchronize(sg.s));
         FL_HIP(hipMemcpy(labels_out, d64, (size_t)n * 8, hipMemcpyDeviceToHost));
+        return FL_OK;
+    });
+}
+
+fl_status fl_kmeans_predict_device_noalloc(const double *dC, int64_t k, int64_t d_centers,
+                                          const void *dX, int32_t dtype, int64_t n, int64_t d,
+                                          int32_t precision, int64_t *d_labels_out,
+                                          void *stream) {
+    return guarded([&]() -> fl_status {
+        if (fl_status s = check_predict(k, d_centers, n, d, dtype)) return s;
+        if (precision != FL_PREC_FP64 && precision != FL_PREC_EXACT &&
+            precision != FL_PREC_BF16 && precision != FL_PREC_BF16X3)
+            return fail(FL_INVALID_PARAMETER, "InvalidParameter: precision: unknown mode");
+        if (n == 0) return FL_OK;
+        if (!dC || !dX || !d_labels_out)
+            return fail(FL_INVALID_PARAMETER,
+                        "InvalidParameter: centers, X and labels_out must not be NULL");
+        // Rows go through in chunks that fit the fixed scratch; the centres'
+        // share of it does not shrink with the chunk.
+        const int64_t m_min = std::min<int64_t>(n, 256);
+        if (noalloc_bytes(m_min, k, (int)d) > kNoAllocScratchBytes)
+            return fail(FL_INVALID_PARAMETER,
+                        "InvalidParameter: n_clusters x n_features: k=" + std::to_string(k) +
+                            ", d=" + std::to_string(d) + " needs " +
+                            std::to_string(noalloc_bytes(m_min, k, (int)d) >> 20) +
+                            " MiB of scratch; the allocation-free predict has " +
+                            std::to_string(kNoAllocScratchBytes >> 20) +
+                            " MiB (about 16*k*d bytes plus ~400 bytes per 256-row chunk row)");
+        int64_t m = m_min;
+        while (m < n && noalloc_bytes(std::min<int64_t>(n, 2 * m), k, (int)d) <= kNoAllocScratchBytes)
+            m = std::min<int64_t>(n, 2 * m);
+        void *base = scratch_address();
+        const hipStream_t st = (hipStream_t)stream;
+        auto run = [&](auto tag) {
+            using TX = decltype(tag);
+            const TX *X = (const TX *)dX;
+            for (int64_t r0 = 0; r0 < n; r0 += m) {
+                const int64_t mm = std::min<int64_t>(m, n - r0);
+                Lloyd<TX> L(X + r0 * d, mm, (int)d, k, precision, st, base, kNoAllocScratchBytes,
+                            dC);
+                int32_t *lab = L.ws().template get<int32_t>("labels", mm);
+                double *best = L.ws().template get<double>("best", mm);
+                KmAcc acc;
+                L.assign(dC, lab, best, acc);
+                hipLaunchKernelGGL(labels_to_i64_kernel, dim3(grid1(mm, 256)), dim3(256), 0, st,
+                                   lab, mm, d_labels_out + r0);
+                check_launch();
+            }
+        };
+        if (dtype == FL_F32)
+            run(float());
+        else
+            run(double());
+        FL_HIP(hipStreamSynchronize(st));
         return FL_OK;
     });
 }

@@ -141,6 +141,25 @@ fl_status fl_kmeans_predict(const double *centers, int64_t k,
                             int64_t d_centers, const void *X, int32_t dtype,
                             int64_t n, int64_t d, int64_t *labels_out);
 
+/* Allocation-free form of fl_kmeans_predict.  It has no counterpart in the
+ * reference: it is for callers that cannot allocate device memory (serving
+ * loops, harnesses that forbid hipMalloc).  centers (k*d_centers f64), X (n*d
+ * of dtype) and labels_out (n int64) are device pointers; `precision` picks
+ * the candidate filter (fl_precision; labels are exact in every mode);
+ * `stream` is a hipStream_t (NULL: the null stream).  The call allocates
+ * nothing: every scratch buffer lives in one fixed 64 MiB device array of
+ * the library (one per GPU, held until the process ends), and rows are
+ * processed in chunks that fit it.  k*d beyond what that array holds (about
+ * 16*k*d bytes plus the centre splits) gives InvalidParameter.  The call
+ * synchronises its stream before it returns, so it cannot be captured in a
+ * graph.  Calls that overlap on one GPU share the array and race: serialise
+ * them. */
+fl_status fl_kmeans_predict_device_noalloc(const double *centers, int64_t k,
+                                           int64_t d_centers, const void *X,
+                                           int32_t dtype, int64_t n, int64_t d,
+                                           int32_t precision, int64_t *labels_out,
+                                           void *stream);
+
 /* Transform<Array2<F>> for FittedKMeans (kmeans.rs:555-618):
  * dist_out[i*k + c] = sqrt(fold(x_i, c_c)). */
 fl_status fl_kmeans_transform(const double *centers, int64_t k,
