@@ -164,6 +164,11 @@ SIGNATURES = {
                                        _vp, _vp, _dp, _ip, _rp]),
     "fl_kmeans_predict": (C.c_int, [_vp, _i64, _i64, _vp, _i32, _i64, _i64, _vp]),
     "fl_kmeans_fit_batched": (C.c_int, [C.POINTER(fl_kmeans_problem), _i64, _i32, _rp]),
+    "fl_kmeans_batched_n_variants": (_i32, []),
+    "fl_kmeans_batched_variant": (C.c_int, [_i32, C.POINTER(_i32), C.POINTER(_i32),
+                                            C.POINTER(_i32)]),
+    "fl_kmeans_batched_set_tuning": (C.c_int, [_vp]),
+    "fl_kmeans_batched_get_tuning": (C.c_int, [_vp]),
     "fl_kmeans_transform": (C.c_int, [_vp, _i64, _i64, _vp, _i32, _i64, _i64, _vp]),
     "fl_kmeans_predict_device_noalloc": (C.c_int, [_vp, _i64, _i64, _vp, _i32, _i64, _i64, _i32,
                                                    _vp, _vp]),
@@ -232,7 +237,28 @@ def lib():
             fn.restype = res
             fn.argtypes = args
         _lib = L
+        _load_tuning(L)
     return _lib
+
+
+def _load_tuning(L):
+    """Apply the committed tile-shape table of the batched KMeans kernel
+    (tuning_gfx950.json, written by ferrolearn_amd.tuning.autotune)."""
+    import json
+    path = os.path.join(HERE, "tuning_gfx950.json")
+    if not os.path.exists(path):
+        return
+    with open(path) as f:
+        doc = json.load(f)
+    raw = (C.c_int32 * 16)()
+    if L.fl_kmeans_batched_get_tuning(raw) != FL_OK:
+        return
+    dims, ks = (8, 16, 32, 64), (16, 64, 256, 1024)
+    for key, v in doc.get("table", {}).items():
+        D, K = (int(x) for x in key.split("x"))
+        raw[4 * dims.index(D) + ks.index(K)] = int(v)
+    if L.fl_kmeans_batched_set_tuning(raw) != FL_OK:
+        raise ImportError(f"{path}: invalid tuning table: {L.fl_last_error().decode()}")
 
 
 def last_error() -> str:

@@ -6,8 +6,6 @@
 // the strictly smallest inertia.  Labels and inertia come from the last
 // assignment, the centres are the post-update ones (:468-495).
 #include <cmath>
-#include <thread>
-#include <atomic>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -891,6 +889,12 @@ fl_status mb_dispatch(const void *dX, int32_t dtype, int64_t n, int64_t d,
 }  // namespace
 
 namespace fl {
+// kmeans.rs:421-449 validation, then this build's limits (shared with the
+// batched entry point).
+fl_status validate_kmeans_fit(int64_t n, int64_t d, const fl_kmeans_params *p, int32_t dtype) {
+    return validate_fit(n, d, p, dtype);
+}
+
 // Exact nearest centre (lowest index on ties) and its fold for n device rows
 // of dtype: the Lloyd assignment step on its own (BF16x3 filter with the
 // exact decision, fallbacks as in a fit).  Used by the cell router.
@@ -982,58 +986,6 @@ fl_status fl_kmeans_fit_device(const void *dX, int32_t dtype, int64_t n, int64_t
         return fit_dispatch(dX, dtype, n, d, params, d_init, d_centers_out, d_labels_out,
                             inertia_out, n_iter_out, report);
     });
-}
-
-fl_status fl_kmeans_fit_batched(fl_kmeans_problem *problems, int64_t n_problems,
-                                int32_t n_workers, fl_report *report) {
-    if (!problems && n_problems > 0)
-        return fail(FL_INVALID_PARAMETER, "InvalidParameter: problems: must not be NULL");
-    if (n_problems < 0)
-        return fail(FL_INVALID_PARAMETER, "InvalidParameter: n_problems: negative");
-    WallClock wc;
-    const int workers = (int)std::max<int64_t>(
-        1, std::min<int64_t>(n_workers > 0 ? n_workers : 8, std::max<int64_t>(n_problems, 1)));
-    std::atomic<int64_t> next(0);
-    std::vector<std::string> errs((size_t)n_problems);
-    std::vector<fl_report> reps((size_t)n_problems);
-    auto work = [&]() {
-        for (;;) {
-            const int64_t i = next.fetch_add(1);
-            if (i >= n_problems) break;
-            fl_kmeans_problem &p = problems[i];
-            p.status = fl_kmeans_fit_device(p.X, p.dtype, p.n, p.d, &p.params, p.init_centers,
-                                            p.centers_out, p.labels_out, &p.inertia, &p.n_iter,
-                                            &reps[(size_t)i]);
-            if (p.status != FL_OK) errs[(size_t)i] = fl_last_error();
-        }
-    };
-    std::vector<std::thread> pool;
-    try {
-        for (int t = 1; t < workers; ++t) pool.emplace_back(work);
-    } catch (...) {
-        // fewer threads than asked: the ones started and this one finish
-    }
-    work();
-    for (auto &th : pool) th.join();
-    if (report) {
-        std::memset(report, 0, sizeof(*report));
-        report->ms_total = wc.ms();
-        for (int64_t i = 0; i < n_problems; ++i)
-            if (problems[i].status == FL_OK) {
-                // rows assigned over every restart and iteration
-                report->n_rows += reps[(size_t)i].n_rows;
-                report->n_iter += problems[i].n_iter;
-                report->n_fallback += reps[(size_t)i].n_fallback;
-                report->ms_distance += reps[(size_t)i].ms_distance;
-                report->ms_update += reps[(size_t)i].ms_update;
-            }
-    }
-    for (int64_t i = 0; i < n_problems; ++i)
-        if (problems[i].status != FL_OK)
-            return fail((fl_status)problems[i].status,
-                        "problem " + std::to_string(i) + ": " + errs[(size_t)i]);
-    clear_error();
-    return FL_OK;
 }
 
 fl_status fl_kmeans_fit(const void *X, int32_t dtype, int64_t n, int64_t d,

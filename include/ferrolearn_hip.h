@@ -173,10 +173,14 @@ fl_status fl_kmeans_plusplus(const void *X, int32_t dtype, int64_t n,
                              double *centers_out);
 
 /* Many independent KMeans fits in one call (no reference counterpart;
- * each problem is KMeans::fit, kmeans.rs:403-513).  Problems run
- * concurrently on n_workers host threads, each fit on its own stream, and
- * every problem's outputs are those of fl_kmeans_fit_device on the same
- * arguments, bit for bit.  X, init_centers (or NULL), centers_out and
+ * each problem is KMeans::fit, kmeans.rs:403-513).  Every problem's outputs
+ * are those of fl_kmeans_fit_device on the same arguments, bit for bit,
+ * whatever its precision field.  Problems with d <= 64 and k <= 1024 run
+ * together on the device: k-means++ of all their restarts in lockstep, then
+ * Lloyd iterations of every (problem, restart) run as one HIP graph of a
+ * persistent exact-assignment kernel and a per-run update kernel, replayed
+ * until every run has converged.  Other problems run as whole fits on
+ * n_workers host threads.  X, init_centers (or NULL), centers_out and
  * labels_out are device pointers. */
 typedef struct fl_kmeans_problem {
     const void *X;
@@ -195,10 +199,23 @@ typedef struct fl_kmeans_problem {
 /* Returns FL_OK when every problem succeeded, else the first failing
  * problem's status (message names the problem).  n_workers <= 0 picks 8.
  * report: ms_total (wall), n_rows = rows assigned over every restart and
- * iteration of every problem, n_iter = sum of the kept runs' n_iter, and
- * the summed kernel times of the fits. */
+ * iteration of every problem, n_iter = sum of the kept runs' n_iter,
+ * ms_distance = Lloyd iterations (device), ms_update = k-means++ (device),
+ * n_dist_launches = graph replays. */
 fl_status fl_kmeans_fit_batched(fl_kmeans_problem *problems, int64_t n_problems,
                                 int32_t n_workers, fl_report *report);
+
+/* Tile shapes of the batched assignment kernel (no reference counterpart).
+ * Variant v folds U centres side by side for R rows per lane held in D
+ * feature registers; table[4*b_d + b_k] is the variant used for problems
+ * with d in bucket b_d (d <= 8, 16, 32, 64) and k in bucket b_k (k <= 16,
+ * 64, 256, 1024), and must have that bucket's D.  The project's autotuner
+ * (ferrolearn_amd.tuning) times the variants per bucket and loads its table
+ * with fl_kmeans_batched_set_tuning.  Results never depend on the table. */
+int32_t fl_kmeans_batched_n_variants(void);
+fl_status fl_kmeans_batched_variant(int32_t v, int32_t *D, int32_t *U, int32_t *R);
+fl_status fl_kmeans_batched_set_tuning(const int32_t *table);
+fl_status fl_kmeans_batched_get_tuning(int32_t *table);
 
 /* ------------------------------------------------------------------ */
 /* MiniBatchKMeans -- ferrolearn-cluster/src/mini_batch_kmeans.rs      */
