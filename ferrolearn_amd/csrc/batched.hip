@@ -56,6 +56,8 @@ struct BkRun {
     double *best;       // n: fold to the assigned centre
     int32_t *order;     // n: rows grouped by label, ascending index within a label
     double *part;       // ceil(n / SUM_BLK) inertia block sums
+    int32_t *cnt;       // k: rows per cluster
+    int64_t *off;       // k: first position of each cluster in order
     double *pool;       // k-means++ distances: nt + 1 slots of n
     double *bsum;       // k-means++ block sums: nt + 1 slots of nblk
     int64_t n, nblk;    // rows; k-means++ blocks of KPP_BLK rows
@@ -247,6 +249,7 @@ __global__ __launch_bounds__(64) void bk_kpp_search_kernel(const BkRun *__restri
                                                            BkState *__restrict__ st,
                                                            const int32_t *__restrict__ rlist,
                                                            int nruns) {
+    __shared__ double blk[KPP_BLK];
     const int i = blockIdx.x / BK_MAXT, t = blockIdx.x % BK_MAXT;
     const int lane = threadIdx.x;
     if (i >= nruns) return;
@@ -267,20 +270,36 @@ __global__ __launch_bounds__(64) void bk_kpp_search_kernel(const BkRun *__restri
             const double s = wave_bcast(sv, u);
             const int64_t b = b0 + u;
             if (run + s >= th) {
+                // the block's values in LDS (every load in flight at once),
+                // then lane 0 runs the sequential sum
                 const int64_t lo = b * KPP_BLK;
-                const int64_t hi = lo + KPP_BLK < R.n ? lo + KPP_BLK : R.n;
-                double c = run;
-                for (int64_t i0 = lo; i0 < hi && !found; i0 += 64) {
-                    const double w = i0 + lane < hi ? md[i0 + lane] : 0.0;
-                    const int mm = hi - i0 < 64 ? (int)(hi - i0) : 64;
-                    for (int uu = 0; uu < mm; ++uu) {
-                        c = c + wave_bcast(w, uu);
-                        if (c >= th) {
-                            res = i0 + uu;
-                            found = true;
-                            break;
+                const int len = (int)((lo + KPP_BLK < R.n ? lo + KPP_BLK : R.n) - lo);
+                for (int e = lane; e < len; e += 64) blk[e] = md[lo + e];
+                wave_sync_lds();
+                int hit = -1;
+                if (lane == 0) {
+                    double c = run;
+                    int e = 0;
+                    for (; e + 8 <= len && hit < 0; e += 8) {
+                        double v[8];
+#pragma unroll
+                        for (int q = 0; q < 8; ++q) v[q] = blk[e + q];
+#pragma unroll
+                        for (int q = 0; q < 8; ++q) {
+                            c = c + v[q];
+                            if (hit < 0 && c >= th) hit = e + q;
                         }
                     }
+                    for (; e < len && hit < 0; ++e) {
+                        c = c + blk[e];
+                        if (c >= th) hit = e;
+                    }
+                }
+                hit = __shfl(hit, 0);
+                wave_sync_lds();
+                if (hit >= 0) {
+                    res = lo + hit;
+                    found = true;
                 }
             }
             run = run + s;
@@ -292,11 +311,15 @@ __global__ __launch_bounds__(64) void bk_kpp_search_kernel(const BkRun *__restri
 // One wave's share of a k-means++ distance pass over rows [lo, hi) of one
 // KPP_BLK block.  FIRST: md = fold(x, c0) (c0 = the first centre) into slot
 // 0.  Otherwise, for every trial t: nd = min(md, fold(x, cand_t)) (`if d <
-// nd`, kmeans.rs:226-228) into the trial's slot.  Each lane holds its row
-// in registers; the candidate values are wave-uniform loads.  The block
-// sums are the sequential folds of the values in row order: each chunk's 64
-// values pass through LDS and lane t adds trial t's in order.
-template <typename TX, int D, bool FIRST>
+// nd`, kmeans.rs:226-228) into the trial's slot.  Each lane folds its row
+// against NT candidate rows at once (features outermost, one accumulator
+// per candidate, so each feature value is loaded once); the candidates sit
+// in this wave's LDS slice, padded to D features and to NT rows with +0
+// (padded features add +0 to every partial sum; padded rows are computed
+// and dropped).  The block sums are the sequential folds of the values in
+// row order: each chunk's 64 values pass through LDS and lane t adds trial
+// t's in order.
+template <typename TX, int D, int NT, bool FIRST>
 __device__ __forceinline__ void bk_kpp_pass(const BkRun &R, const BkState &S, int b,
                                             double (*lds)[64], double *__restrict__ cs) {
     const int lane = threadIdx.x & 63;
@@ -307,12 +330,10 @@ __device__ __forceinline__ void bk_kpp_pass(const BkRun &R, const BkState &S, in
     const int64_t hi = lo + KPP_BLK < R.n ? lo + KPP_BLK : R.n;
     const int mdslot = S.mdslot;
     const double *md = R.pool + (int64_t)mdslot * R.n;
-    // candidate rows (the first centre for FIRST), padded to D with +0, in
-    // this wave's LDS slice
-    for (int e = lane; e < nt * D; e += 64) {
+    for (int e = lane; e < NT * D; e += 64) {
         const int t = e / D, j = e % D;
         double v = 0.0;
-        if (j < d) v = FIRST ? R.cA[j] : (double)X[S.cand[t] * d + j];
+        if (t < nt && j < d) v = FIRST ? R.cA[j] : (double)X[S.cand[t] * d + j];
         cs[e] = v;
     }
     wave_sync_lds();
@@ -320,29 +341,27 @@ __device__ __forceinline__ void bk_kpp_pass(const BkRun &R, const BkState &S, in
     for (int64_t i0 = lo; i0 < hi; i0 += 64) {
         const int64_t i = i0 + lane;
         const bool ok = i < hi;
-        // Features d..D-1 are +0 on both sides: their terms add +0, which
-        // leaves every partial sum of the fold unchanged.  Loads use a
-        // clamped index so that none leaves the row.
-        const TX *xr = X + (ok ? i : lo) * d;
-        double x[D];
+        const TX *xr = X + (ok ? i : lo) * d;   // clamped: loads stay in the row
+        double acc[NT];
 #pragma unroll
+        for (int t = 0; t < NT; ++t) acc[t] = 0.0;
+#pragma unroll 8
         for (int j = 0; j < D; ++j) {
-            const double v = (double)xr[j < d ? j : 0];
-            x[j] = j < d ? v : 0.0;
+            const double xv = (double)xr[j < d ? j : 0];
+            const double xj = j < d ? xv : 0.0;
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const double tt = xj - cs[t * D + j];
+                acc[t] = acc[t] + tt * tt;
+            }
         }
         const double m = (!FIRST && ok) ? md[i] : 0.0;
 #pragma unroll
-        for (int t = 0; t < (FIRST ? 1 : BK_MAXT); ++t) {
+        for (int t = 0; t < NT; ++t) {
             if (t < nt) {
-            double acc = 0.0;
-#pragma unroll
-            for (int j = 0; j < D; ++j) {
-                const double tt = x[j] - cs[t * D + j];
-                acc = acc + tt * tt;
-            }
-            const double v = FIRST ? acc : (acc < m ? acc : m);
-            if (ok) R.pool[(int64_t)(FIRST ? 0 : trial_slot(t, mdslot)) * R.n + i] = v;
-            lds[t][lane] = ok ? v : 0.0;
+                const double v = FIRST ? acc[t] : (acc[t] < m ? acc[t] : m);
+                if (ok) R.pool[(int64_t)(FIRST ? 0 : trial_slot(t, mdslot)) * R.n + i] = v;
+                lds[t][lane] = ok ? v : 0.0;
             }
         }
         wave_sync_lds();
@@ -363,6 +382,17 @@ __device__ __forceinline__ void bk_kpp_pass(const BkRun &R, const BkState &S, in
     if (lane < nt) R.bsum[(int64_t)(FIRST ? 0 : trial_slot(lane, mdslot)) * R.nblk + b] = s;
 }
 
+template <typename TX, int NT, bool FIRST>
+__device__ __forceinline__ void bk_kpp_pass_d(int db, const BkRun &R, const BkState &S, int b,
+                                              double (*lds)[64], double *cs) {
+    switch (db) {
+        case 0: return bk_kpp_pass<TX, 8, NT, FIRST>(R, S, b, lds, cs);
+        case 1: return bk_kpp_pass<TX, 16, NT, FIRST>(R, S, b, lds, cs);
+        case 2: return bk_kpp_pass<TX, 32, NT, FIRST>(R, S, b, lds, cs);
+        default: return bk_kpp_pass<TX, 64, NT, FIRST>(R, S, b, lds, cs);
+    }
+}
+
 // One wave per (run, block) pair; four waves per block.
 template <bool FIRST>
 __global__ __launch_bounds__(256) void bk_kpp_pass_kernel(const BkRun *__restrict__ runs,
@@ -380,20 +410,16 @@ __global__ __launch_bounds__(256) void bk_kpp_pass_kernel(const BkRun *__restric
     const BkState &S = st[r];
     if (FIRST ? (!R.kpp || R.k == 1) : S.skip != 0) return;
     const int db = d_bucket(R.d);
-    if (R.f32) {
-        switch (db) {
-            case 0: return bk_kpp_pass<float, 8, FIRST>(R, S, b, lds[w], cs[w]);
-            case 1: return bk_kpp_pass<float, 16, FIRST>(R, S, b, lds[w], cs[w]);
-            case 2: return bk_kpp_pass<float, 32, FIRST>(R, S, b, lds[w], cs[w]);
-            default: return bk_kpp_pass<float, 64, FIRST>(R, S, b, lds[w], cs[w]);
-        }
+    if (FIRST) {
+        if (R.f32) return bk_kpp_pass_d<float, 1, true>(db, R, S, b, lds[w], cs[w]);
+        return bk_kpp_pass_d<double, 1, true>(db, R, S, b, lds[w], cs[w]);
     }
-    switch (db) {
-        case 0: return bk_kpp_pass<double, 8, FIRST>(R, S, b, lds[w], cs[w]);
-        case 1: return bk_kpp_pass<double, 16, FIRST>(R, S, b, lds[w], cs[w]);
-        case 2: return bk_kpp_pass<double, 32, FIRST>(R, S, b, lds[w], cs[w]);
-        default: return bk_kpp_pass<double, 64, FIRST>(R, S, b, lds[w], cs[w]);
+    if (R.nt <= 4) {
+        if (R.f32) return bk_kpp_pass_d<float, 4, false>(db, R, S, b, lds[w], cs[w]);
+        return bk_kpp_pass_d<double, 4, false>(db, R, S, b, lds[w], cs[w]);
     }
+    if (R.f32) return bk_kpp_pass_d<float, BK_MAXT, false>(db, R, S, b, lds[w], cs[w]);
+    return bk_kpp_pass_d<double, BK_MAXT, false>(db, R, S, b, lds[w], cs[w]);
 }
 
 // One wave per run: each trial's potential, the first trial with the
@@ -593,38 +619,34 @@ __global__ __launch_bounds__(256) void bk_assign_kernel(const BkProb *__restrict
 }
 
 // -------------------------------------------------------------- update ----
-// One block of BK_UPD_WAVES waves per active run:
-//  1. inertia of the assignment, in the order of block_sum_kernel +
-//     runs_sum_kernel (1024-row blocks reduced by a fixed tree, the block
-//     sums by 64 contiguous runs, then the runs in order);
-//  2. a stable counting sort of the rows by label (each wave counts and
-//     then places a contiguous range of rows, in row order);
-//  3. the reference-order sums (kmeans.rs:370-386): one lane per (cluster,
-//     feature) folds its cluster's rows in ascending index order from +0,
-//     then divides by the count or keeps the old centre;
-//  4. max_shift = sqrt(max_c fold(new_c, old_c)) and the loop control of
-//     kmeans.rs:468-488 (n_iter = iter + 1; stop when shift < tol or after
-//     max_iter iterations).
-// Block 0 also resets the assignment queue for the next iteration.
-__global__ __launch_bounds__(BK_UPD_WAVES * 64) void bk_update_kernel(
-    const BkRun *__restrict__ runs, BkState *__restrict__ st, int32_t *__restrict__ done,
-    unsigned long long *__restrict__ queue) {
+// The update of kmeans.rs:358-401 and the loop control of :468-488 for
+// every active run, in three kernels:
+//  1. bk_sort_kernel, one block per run: inertia of the assignment in the
+//     order of block_sum_kernel + runs_sum_kernel (1024-row blocks reduced by
+//     a fixed tree, the block sums by 64 contiguous runs, then the runs in
+//     order), and a stable counting sort of the rows by label (each wave
+//     counts and then places a contiguous range of rows, in row order);
+//  2. bk_sums_kernel, one lane per (run, cluster, feature): the cluster's
+//     rows folded in ascending index order from +0 (kmeans.rs:370-386),
+//     divided by the count, or the old centre for an empty cluster;
+//  3. bk_control_kernel, one wave per run: max_shift = sqrt(max_c fold(new_c,
+//     old_c)), n_iter = iter + 1, stop when shift < tol or after max_iter
+//     iterations; block 0 also resets the assignment queue.
+__global__ __launch_bounds__(BK_UPD_WAVES * 64) void bk_sort_kernel(
+    const BkRun *__restrict__ runs, BkState *__restrict__ st, const int32_t *__restrict__ done) {
     __shared__ int32_t hist[BK_UPD_WAVES * BK_MAXK];
     __shared__ double red[2][256];
     __shared__ int32_t cnt_sh[BK_MAXK];
     __shared__ int64_t off_sh[BK_MAXK];
-    __shared__ unsigned long long mx;
-    __shared__ double inertia_sh;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    if (blockIdx.x == 0 && tid == 0) *queue = 0ull;
     const int r = blockIdx.x;
     if (done[r]) return;
     const BkRun R = runs[r];
     BkState &S = st[r];
     const int64_t n = R.n;
-    const int d = R.d, k = R.k;
+    const int k = R.k;
 
-    // 1. inertia
+    // inertia
     {
         const int64_t m = (n + SUM_BLK - 1) / SUM_BLK;
         const int g = tid >> 8, t = tid & 255;
@@ -652,11 +674,11 @@ __global__ __launch_bounds__(BK_UPD_WAVES * 64) void bk_update_kernel(
             for (int64_t i = lo; i < hi; ++i) acc = acc + R.part[i];
             double tot = 0.0;
             for (int l = 0; l < 64; ++l) tot = tot + wave_bcast(acc, l);
-            if (lane == 0) inertia_sh = tot;
+            if (lane == 0) S.inertia = tot;
         }
     }
 
-    // 2. stable counting sort by label
+    // stable counting sort by label
     for (int i = tid; i < BK_UPD_WAVES * k; i += blockDim.x) hist[i] = 0;
     __syncthreads();
     const int64_t nch = (n + 63) / 64;
@@ -671,22 +693,24 @@ __global__ __launch_bounds__(BK_UPD_WAVES * 64) void bk_update_kernel(
         int32_t c = 0;
         for (int ww = 0; ww < BK_UPD_WAVES; ++ww) c += hist[ww * k + l];
         cnt_sh[l] = c;
+        R.cnt[l] = c;
     }
     __syncthreads();
     if (w == 0) {
         // exclusive scan of the counts: lane l takes a contiguous run of clusters
         const int per = (k + 63) / 64;
         const int lo = lane * per, hi = lo + per < k ? lo + per : k;
-        int64_t s = 0;
-        for (int l = lo; l < hi; ++l) s += cnt_sh[l];
-        int64_t incl = s;
+        int64_t sum = 0;
+        for (int l = lo; l < hi; ++l) sum += cnt_sh[l];
+        int64_t incl = sum;
         for (int o = 1; o < 64; o <<= 1) {
             const int64_t v = __shfl_up(incl, o);
             if (lane >= o) incl += v;
         }
-        int64_t run = incl - s;
+        int64_t run = incl - sum;
         for (int l = lo; l < hi; ++l) {
             off_sh[l] = run;
+            R.off[l] = run;
             run += cnt_sh[l];
         }
     }
@@ -700,80 +724,101 @@ __global__ __launch_bounds__(BK_UPD_WAVES * 64) void bk_update_kernel(
         }
     }
     __syncthreads();
-    {
-        const int bits = [&] {
-            int b = 0;
-            while ((1 << b) < k) ++b;
-            return b;
-        }();
-        const uint64_t below = (1ull << lane) - 1ull;
-        for (int64_t ch = ch0; ch < ch1; ++ch) {
-            const int64_t i = ch * 64 + lane;
-            const bool valid = i < n;
-            const int32_t lab = valid ? R.labels[i] : 0;
-            uint64_t msk = __ballot(valid);
-            for (int bit = 0; bit < bits; ++bit) {
-                const bool on = (lab >> bit) & 1;
-                const uint64_t bb = __ballot(valid && on);
-                msk &= on ? bb : ~bb;
-            }
-            const int rank = __popcll(msk & below);
-            const int leader = __ffsll((unsigned long long)msk) - 1;
-            int32_t base = 0;
-            if (valid && leader == lane) base = atomicAdd(&hist[w * k + lab], __popcll(msk));
-            const int32_t gb = __shfl(base, valid ? leader : lane);
-            if (valid) R.order[gb + rank] = (int32_t)i;
+    int bits = 0;
+    while ((1 << bits) < k) ++bits;
+    const uint64_t below = (1ull << lane) - 1ull;
+    for (int64_t ch = ch0; ch < ch1; ++ch) {
+        const int64_t i = ch * 64 + lane;
+        const bool valid = i < n;
+        const int32_t lab = valid ? R.labels[i] : 0;
+        uint64_t msk = __ballot(valid);
+        for (int bit = 0; bit < bits; ++bit) {
+            const bool on = (lab >> bit) & 1;
+            const uint64_t bb = __ballot(valid && on);
+            msk &= on ? bb : ~bb;
         }
+        const int rank = __popcll(msk & below);
+        const int leader = __ffsll((unsigned long long)msk) - 1;
+        int32_t base = 0;
+        if (valid && leader == lane) base = atomicAdd(&hist[w * k + lab], __popcll(msk));
+        const int32_t gb = __shfl(base, valid ? leader : lane);
+        if (valid) R.order[gb + rank] = (int32_t)i;
     }
-    __syncthreads();
+}
 
-    // 3. reference-order sums
+// One thread per (cluster, feature) pair of item (run, first pair).
+__global__ __launch_bounds__(256) void bk_sums_kernel(const BkRun *__restrict__ runs,
+                                                      const BkState *__restrict__ st,
+                                                      const int32_t *__restrict__ done,
+                                                      const int2 *__restrict__ items,
+                                                      int64_t nitems) {
+    const int64_t it = blockIdx.x;
+    if (it >= nitems) return;
+    const int r = items[it].x;
+    if (done[r]) return;
+    const BkRun &R = runs[r];
+    const int d = R.d;
+    const int64_t p = (int64_t)items[it].y + threadIdx.x;
+    if (p >= (int64_t)R.k * d) return;
+    const int l = (int)(p / d), j = (int)(p % d);
+    const int32_t cnt = R.cnt[l];
+    const int32_t *ord = R.order + R.off[l];
+    const double *oldc = st[r].cur_b ? R.cB : R.cA;
+    double *newc = st[r].cur_b ? R.cA : R.cB;
+    double acc = 0.0;
+    int32_t m = 0;
+    constexpr int B = 16;
+    if (R.f32) {
+        const float *X = (const float *)R.X;
+        for (; m + B <= cnt; m += B) {
+            double v[B];
+#pragma unroll
+            for (int u = 0; u < B; ++u) v[u] = (double)X[(int64_t)ord[m + u] * d + j];
+#pragma unroll
+            for (int u = 0; u < B; ++u) acc = acc + v[u];
+        }
+        for (; m < cnt; ++m) acc = acc + (double)X[(int64_t)ord[m] * d + j];
+    } else {
+        const double *X = (const double *)R.X;
+        for (; m + B <= cnt; m += B) {
+            double v[B];
+#pragma unroll
+            for (int u = 0; u < B; ++u) v[u] = X[(int64_t)ord[m + u] * d + j];
+#pragma unroll
+            for (int u = 0; u < B; ++u) acc = acc + v[u];
+        }
+        for (; m < cnt; ++m) acc = acc + X[(int64_t)ord[m] * d + j];
+    }
+    newc[p] = cnt > 0 ? acc / (double)cnt : oldc[p];
+}
+
+__global__ __launch_bounds__(64) void bk_control_kernel(const BkRun *__restrict__ runs,
+                                                        BkState *__restrict__ st,
+                                                        int32_t *__restrict__ done,
+                                                        unsigned long long *__restrict__ queue) {
+    const int lane = threadIdx.x;
+    if (blockIdx.x == 0 && lane == 0) *queue = 0ull;
+    const int r = blockIdx.x;
+    if (done[r]) return;
+    const BkRun &R = runs[r];
+    BkState &S = st[r];
     const double *oldc = S.cur_b ? R.cB : R.cA;
-    double *newc = S.cur_b ? R.cA : R.cB;
-    const int64_t pairs = (int64_t)k * d;
-    for (int64_t p = tid; p < pairs; p += blockDim.x) {
-        const int l = (int)(p / d), j = (int)(p % d);
-        const int32_t cnt = cnt_sh[l];
-        const int32_t *ord = R.order + off_sh[l];
-        double acc = 0.0;
-        int32_t mm = 0;
-        constexpr int B = 8;
-        if (R.f32) {
-            const float *X = (const float *)R.X;
-            for (; mm + B <= cnt; mm += B) {
-                double v[B];
-#pragma unroll
-                for (int u = 0; u < B; ++u) v[u] = (double)X[(int64_t)ord[mm + u] * d + j];
-#pragma unroll
-                for (int u = 0; u < B; ++u) acc = acc + v[u];
-            }
-            for (; mm < cnt; ++mm) acc = acc + (double)X[(int64_t)ord[mm] * d + j];
-        } else {
-            const double *X = (const double *)R.X;
-            for (; mm + B <= cnt; mm += B) {
-                double v[B];
-#pragma unroll
-                for (int u = 0; u < B; ++u) v[u] = X[(int64_t)ord[mm + u] * d + j];
-#pragma unroll
-                for (int u = 0; u < B; ++u) acc = acc + v[u];
-            }
-            for (; mm < cnt; ++mm) acc = acc + X[(int64_t)ord[mm] * d + j];
-        }
-        newc[p] = cnt > 0 ? acc / (double)cnt : oldc[p];
+    const double *newc = S.cur_b ? R.cA : R.cB;
+    // the largest positive fold (a NaN fold is skipped, like `if shift >
+    // max_shift`); non-negative doubles order like their bit patterns
+    unsigned long long mx = 0ull;
+    for (int l = lane; l < R.k; l += 64) {
+        const double f = fold_exact(newc + (int64_t)l * R.d, oldc + (int64_t)l * R.d, R.d);
+        const unsigned long long b = f > 0.0 ? (unsigned long long)__double_as_longlong(f) : 0ull;
+        mx = b > mx ? b : mx;
     }
-    if (tid == 0) mx = 0ull;
-    __syncthreads();
-
-    // 4. shift and loop control
-    for (int l = tid; l < k; l += blockDim.x) {
-        const double f = fold_exact(newc + (int64_t)l * d, oldc + (int64_t)l * d, d);
-        if (f > 0.0) atomicMax(&mx, (unsigned long long)__double_as_longlong(f));
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long v = __shfl_xor(mx, o);
+        mx = v > mx ? v : mx;
     }
-    __syncthreads();
-    if (tid == 0) {
+    if (lane == 0) {
         const double shift = sqrt(__longlong_as_double((long long)mx));
         const int it = S.it + 1;
-        S.inertia = inertia_sh;
         S.shift = shift;
         S.it = it;
         S.cur_b ^= 1;
@@ -824,7 +869,7 @@ size_t problem_bytes(const fl_kmeans_problem &p) {
     const int64_t nt = 2 + (int64_t)std::floor(std::log((double)k));
     const bool kpp = p.init_centers == nullptr;
     size_t b = (size_t)ni * ((size_t)(2 * k * d) * 8 + (size_t)n * (4 + 8 + 4) +
-                             (size_t)ceil_div(n, SUM_BLK) * 8);
+                             (size_t)ceil_div(n, SUM_BLK) * 8 + (size_t)k * 12);
     if (kpp) b += (size_t)ni * ((size_t)(nt + 1) * (size_t)(n + nblk) * 8);
     return b + 1024 * 8;
 }
@@ -842,7 +887,7 @@ void run_batch(fl_kmeans_problem *problems, const std::vector<int64_t> &idx, hip
     size_t bytes = 0;
     for (int q = 0; q < np; ++q) {
         const fl_kmeans_problem &p = problems[idx[(size_t)q]];
-        bytes += problem_bytes(p) + (size_t)p.params.n_init * 8 * 256;
+        bytes += problem_bytes(p) + (size_t)p.params.n_init * 10 * 256;
     }
     DevBuf arena;
     unsigned char *base = (unsigned char *)arena.get(bytes);
@@ -893,6 +938,8 @@ void run_batch(fl_kmeans_problem *problems, const std::vector<int64_t> &idx, hip
             R.best = (double *)take((size_t)n * 8);
             R.order = (int32_t *)take((size_t)n * 4);
             R.part = (double *)take((size_t)ceil_div(n, SUM_BLK) * 8);
+            R.cnt = (int32_t *)take((size_t)k * 4);
+            R.off = (int64_t *)take((size_t)k * 8);
             if (kpp) {
                 R.pool = (double *)take((size_t)(nt + 1) * (size_t)n * 8);
                 R.bsum = (double *)take((size_t)(nt + 1) * (size_t)R.nblk * 8);
@@ -997,7 +1044,19 @@ void run_batch(fl_kmeans_problem *problems, const std::vector<int64_t> &idx, hip
     }
     ptm.end(t_kpp);
 
-    // ---- Lloyd iterations: one captured graph of [assign, update]
+    // ---- Lloyd iterations: one captured graph of [assign, sort, sums, control]
+    // centroid-sum items: 256 (cluster, feature) pairs of one run, the runs
+    // with the most rows per cluster first
+    std::vector<int2> sitems;
+    for (int r = 0; r < nruns; ++r)
+        for (int64_t p0 = 0; p0 < (int64_t)runs[(size_t)r].k * runs[(size_t)r].d; p0 += 256)
+            sitems.push_back(make_int2(r, (int)p0));
+    std::stable_sort(sitems.begin(), sitems.end(), [&](const int2 &a, const int2 &b) {
+        return runs[(size_t)a.x].n / runs[(size_t)a.x].k > runs[(size_t)b.x].n / runs[(size_t)b.x].k;
+    });
+    int2 *d_sitems = ws.get<int2>("sitems", (int64_t)sitems.size());
+    FL_HIP(hipMemcpyAsync(d_sitems, sitems.data(), sizeof(int2) * sitems.size(),
+                          hipMemcpyHostToDevice, st));
     const int t_lloyd = ptm.begin();
     if (max_iter > 0 && !sorted.empty()) {
         const unsigned agrid = (unsigned)std::min<int64_t>((int64_t)sorted.size(),
@@ -1006,8 +1065,12 @@ void run_batch(fl_kmeans_problem *problems, const std::vector<int64_t> &idx, hip
         FL_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         hipLaunchKernelGGL(bk_assign_kernel, dim3(agrid), dim3(256), 0, st, d_probs, d_items,
                            (int64_t)sorted.size(), d_runs, d_st, d_done, d_queue);
-        hipLaunchKernelGGL(bk_update_kernel, dim3((unsigned)nruns), dim3(BK_UPD_WAVES * 64), 0,
-                           st, d_runs, d_st, d_done, d_queue);
+        hipLaunchKernelGGL(bk_sort_kernel, dim3((unsigned)nruns), dim3(BK_UPD_WAVES * 64), 0, st,
+                           d_runs, d_st, d_done);
+        hipLaunchKernelGGL(bk_sums_kernel, dim3((unsigned)sitems.size()), dim3(256), 0, st,
+                           d_runs, d_st, d_done, d_sitems, (int64_t)sitems.size());
+        hipLaunchKernelGGL(bk_control_kernel, dim3((unsigned)nruns), dim3(64), 0, st, d_runs,
+                           d_st, d_done, d_queue);
         FL_HIP(hipStreamEndCapture(st, &go.g));
         FL_HIP(hipGraphInstantiate(&go.e, go.g, nullptr, nullptr, 0));
         std::vector<int32_t> hd((size_t)nruns);
