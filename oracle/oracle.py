@@ -68,6 +68,9 @@ def lib():
         L.or_minibatch_run.restype = None
         L.or_minibatch_run.argtypes = [_dp, _i64, _i64, _i64, _dp, _i64, _i64,
                                        C.c_double, C.c_uint64, C.c_int, _dp, _ip, _dp, _ip]
+        L.or_minibatch_fit.restype = None
+        L.or_minibatch_fit.argtypes = [_dp, _i64, _i64, _i64, _i64, _i64, C.c_double, _i64,
+                                       C.c_uint64, C.c_int, _dp, _ip, _dp, _ip]
         L.or_knn_query.restype = _i64
         L.or_knn_query.argtypes = [_dp, _i64, _i64, _dp, _i64, _ip, _dp]
         L.or_kneighbors.restype = None
@@ -241,6 +244,20 @@ def minibatch_run(X, init, batch, max_iter, tol, seed, nthreads=0):
     lib().or_minibatch_run(_p(X), n, d, k, _p(init), batch, max_iter, tol, C.c_uint64(seed),
                            nthreads, _p(centers), _pi(labels), C.byref(inertia),
                            C.byref(n_iter))
+    return centers, labels, float(inertia.value), int(n_iter.value)
+
+
+def minibatch_fit(X, k, batch, max_iter, tol, n_init, seed, nthreads=0):
+    """A whole MiniBatchKMeans fit with the library's k-means++ per run."""
+    X = _f64(X)
+    n, d = X.shape
+    centers = np.zeros((k, d), np.float64)
+    labels = np.zeros(n, np.int64)
+    inertia = C.c_double()
+    n_iter = C.c_int64()
+    lib().or_minibatch_fit(_p(X), n, d, k, batch, max_iter, C.c_double(tol), n_init,
+                           C.c_uint64(seed), nthreads, _p(centers), _pi(labels),
+                           C.byref(inertia), C.byref(n_iter))
     return centers, labels, float(inertia.value), int(n_iter.value)
 
 

@@ -247,7 +247,11 @@ void or_transform(const double *X, int64_t n, int64_t d, const double *C,
  * vendored, so this restatement uses a documented generator instead:
  * SplitMix64 expands the seed into xoshiro256** state; random_f64 takes the
  * top 53 bits; random_range(n) is Lemire's multiply-shift (no rejection).
- * The library's host code uses the same definition so the two agree. */
+ * The library's host code uses the same definition so the two agree.  The
+ * algorithms around the generator (or_kmeanspp, or_sample_batch,
+ * or_minibatch_fit) are pinned by the pure-Python golden vectors of
+ * tests/golden/make_golden.py; only the stream itself is unpinned against
+ * the reference. */
 typedef struct { uint64_t s[4]; } or_rng;
 
 static uint64_t or_splitmix(uint64_t *x) {
@@ -329,16 +333,31 @@ int64_t or_blocked_search(const double *v, int64_t n, double threshold) {
  * centre, if the total potential is <= 0 pick uniformly, else draw n_trials
  * candidates with probability proportional to D(x)^2 and keep the one with
  * the strictly smallest resulting potential (first wins ties). */
+static void or_kmeanspp_rng(const double *X, int64_t n, int64_t d, int64_t k,
+                            or_rng *rngp, double *centers, int nthreads);
+
 void or_kmeanspp(const double *X, int64_t n, int64_t d, int64_t k,
                  uint64_t seed, double *centers, int nthreads) {
-    nthreads = or_threads(nthreads);
     or_rng rng;
     or_rng_seed(&rng, seed);
+    or_kmeanspp_rng(X, n, d, k, &rng, centers, nthreads);
+}
+
+/* The same, drawing from a generator the caller keeps using afterwards
+ * (MiniBatchKMeans takes its batches from the run's generator,
+ * mini_batch_kmeans.rs:528-555). */
+static void or_kmeanspp_rng(const double *X, int64_t n, int64_t d, int64_t k,
+                            or_rng *rngp, double *centers, int nthreads) {
+    nthreads = or_threads(nthreads);
+    or_rng rng = *rngp;
     memset(centers, 0, (size_t)(k * d) * sizeof(double));
     if (n == 0) return;
     int64_t first = or_rng_range(&rng, n);
     memcpy(centers, X + first * d, (size_t)d * sizeof(double));
-    if (k == 1) return;
+    if (k == 1) {
+        *rngp = rng;
+        return;
+    }
     double *md = (double *)malloc((size_t)n * sizeof(double));
     double *nd = (double *)malloc((size_t)n * sizeof(double));
     double *bd = (double *)malloc((size_t)n * sizeof(double));
@@ -383,6 +402,7 @@ void or_kmeanspp(const double *X, int64_t n, int64_t d, int64_t k,
     free(md);
     free(nd);
     free(bd);
+    *rngp = rng;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -479,6 +499,55 @@ void or_minibatch_run(const double *X, int64_t n, int64_t d, int64_t k,
     free(pool);
     free(bidx);
     free(blab);
+}
+
+/* mini_batch_kmeans.rs:478-592, the whole fit without given centres: run r
+ * seeds the generator with seed + r*1000003, draws k-means++ from it and
+ * then its batches; the run with the strictly smallest inertia is kept. */
+void or_minibatch_fit(const double *X, int64_t n, int64_t d, int64_t k, int64_t batch,
+                      int64_t max_iter, double tol, int64_t n_init, uint64_t seed,
+                      int nthreads, double *centers_out, int64_t *labels_out,
+                      double *inertia_out, int64_t *n_iter_out) {
+    double *C = (double *)malloc((size_t)(k * d) * sizeof(double));
+    int64_t *L = (int64_t *)malloc((size_t)n * sizeof(int64_t));
+    int64_t *counts = (int64_t *)malloc((size_t)k * sizeof(int64_t));
+    int64_t *pool = (int64_t *)malloc((size_t)n * sizeof(int64_t));
+    int64_t bmax = batch < n ? batch : n;
+    int64_t *bidx = (int64_t *)malloc((size_t)bmax * sizeof(int64_t));
+    int64_t *blab = (int64_t *)malloc((size_t)bmax * sizeof(int64_t));
+    double *dist = (double *)malloc((size_t)n * sizeof(double));
+    for (int64_t r = 0; r < n_init; ++r) {
+        or_rng rng;
+        or_rng_seed(&rng, seed + (uint64_t)r * 1000003ULL);
+        or_kmeanspp_rng(X, n, d, k, &rng, C, nthreads);
+        for (int64_t c = 0; c < k; ++c) counts[c] = 1;
+        int64_t it = 0;
+        for (int64_t q = 0; q < max_iter; ++q) {
+            int64_t blen;
+            or_sample_batch(n, batch, &rng, pool, bidx, &blen);
+            for (int64_t s = 0; s < blen; ++s)
+                blab[s] = or_nearest_center(X + bidx[s] * d, C, k, d, NULL);
+            double shift = or_minibatch_update(X, d, bidx, blab, blen, C, k, counts);
+            it += 1;
+            if (shift < tol) break;
+        }
+        or_assign_rows(X, n, d, C, k, L, dist, nthreads);
+        double in = 0.0;
+        for (int64_t i = 0; i < n; ++i) in = in + dist[i];
+        if (r == 0 || in < *inertia_out) {
+            *inertia_out = in;
+            *n_iter_out = it;
+            memcpy(centers_out, C, (size_t)(k * d) * sizeof(double));
+            memcpy(labels_out, L, (size_t)n * sizeof(int64_t));
+        }
+    }
+    free(C);
+    free(L);
+    free(counts);
+    free(pool);
+    free(bidx);
+    free(blab);
+    free(dist);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -177,6 +177,149 @@ def rng_stream(seed, count):
     return out
 
 
+class Gen:
+    """The library's generator: SplitMix64 seeding of xoshiro256**;
+    range(n) = high 64 bits of next() * n; f64() = (next() >> 11) * 2^-53."""
+
+    def __init__(self, seed):
+        x = seed & M64
+        self.s = []
+        for _ in range(4):
+            x = (x + 0x9E3779B97F4A7C15) & M64
+            z = x
+            z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M64
+            z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M64
+            self.s.append(z ^ (z >> 31))
+
+    def next(self):
+        s = self.s
+        rotl = lambda v, k: ((v << k) | (v >> (64 - k))) & M64
+        out = rotl((s[1] * 5) & M64, 7) * 9 & M64
+        t = (s[1] << 17) & M64
+        s[2] ^= s[0]
+        s[3] ^= s[1]
+        s[1] ^= s[2]
+        s[0] ^= s[3]
+        s[2] ^= t
+        s[3] = rotl(s[3], 45)
+        return out
+
+    def range(self, n):
+        return (self.next() * n) >> 64
+
+    def f64(self):
+        return float(self.next() >> 11) * 2.0 ** -53
+
+
+BLK = 4096   # the library's two-level sum: sequential within blocks, then over blocks
+
+
+def blocked_sum(v):
+    tot = 0.0
+    for b in range(0, len(v), BLK):
+        s = 0.0
+        for x in v[b:b + BLK]:
+            s = s + x
+        tot = tot + s
+    return tot
+
+
+def blocked_search(v, thr):
+    run = 0.0
+    for b in range(0, len(v), BLK):
+        s = 0.0
+        for x in v[b:b + BLK]:
+            s = s + x
+        if run + s >= thr:
+            c = run
+            for i in range(b, min(b + BLK, len(v))):
+                c = c + v[i]
+                if c >= thr:
+                    return i
+        run = run + s
+    return len(v) - 1
+
+
+def kmeanspp(X, k, g):
+    """kmeans.rs:169-253 with the library's generator g and sums: the first
+    row uniform; n_trials = 2 + floor(ln k); per centre the total of the
+    minimum distances; a uniform row if it is <= 0, else n_trials rows drawn
+    by D^2 thresholds, keeping the first with the strictly smallest
+    potential.  Returns the picked row indices.  For n <= 4096 every sum is
+    the plain left fold of the reference."""
+    n = len(X)
+    picks = [g.range(n)]
+    if k == 1:
+        return picks
+    md = [fold(x, X[picks[0]]) for x in X]
+    n_trials = max(2 + int(math.floor(math.log(k))), 1)
+    for _ in range(1, k):
+        total = blocked_sum(md)
+        if total <= 0.0:
+            picks.append(g.range(n))
+            continue
+        best, best_pot, best_nd = 0, None, None
+        for _ in range(n_trials):
+            thr = g.f64() * total
+            cand = blocked_search(md, thr)
+            nd = []
+            for i, x in enumerate(X):
+                v = fold(x, X[cand])
+                nd.append(v if v < md[i] else md[i])
+            pot = blocked_sum(nd)
+            if best_pot is None or pot < best_pot:
+                best, best_pot, best_nd = cand, pot, nd
+        picks.append(best)
+        md = best_nd
+    return picks
+
+
+def sample_batch(g, n, batch):
+    """mini_batch_kmeans.rs:451-472: a full Fisher-Yates shuffle when
+    batch >= n, else the first `batch` steps of one over 0..n."""
+    pool = list(range(n))
+    if batch >= n:
+        for i in range(n):
+            j = i + g.range(n - i)
+            pool[i], pool[j] = pool[j], pool[i]
+        return pool
+    out = []
+    for i in range(batch):
+        j = i + g.range(n - i)
+        pool[i], pool[j] = pool[j], pool[i]
+        out.append(pool[i])
+    return out
+
+
+def minibatch_fit(X, k, batch, max_iter, tol, n_init, base):
+    """mini_batch_kmeans.rs:478-592: run r draws from the generator seeded
+    base + r*1000003 (k-means++ first, then the batches); counts start at 1;
+    per iteration sample, assign (:381-404), update online (:410-447),
+    n_iter += 1, stop when the shift < tol; then label every row, inertia the
+    left fold in row order (:343-376); keep the strictly smallest inertia."""
+    best = None
+    for r in range(n_init):
+        g = Gen(base + r * 1000003)
+        C = [list(X[i]) for i in kmeanspp(X, k, g)]
+        counts = [1] * k
+        it = 0
+        for _ in range(max_iter):
+            bidx = sample_batch(g, len(X), batch)
+            blab = [nearest(X[i], C)[0] for i in bidx]
+            C, counts, sh = mb_update(X, bidx, blab, C, counts)
+            it += 1
+            if sh < tol:
+                break
+        labels, inertia = [], 0.0
+        for x in X:
+            l, v = nearest(x, C)
+            labels.append(l)
+            inertia = inertia + v
+        if best is None or inertia < best[2]:
+            best = (C, labels, inertia, it)
+    return best
+
+
 def main():
     rnd = random.Random(20260101)
     out = {}
@@ -283,6 +426,42 @@ def main():
 
     # --- generator ------------------------------------------------------
     out["rng"] = {str(s): [str(v) for v in rng_stream(s, 8)] for s in (0, 1, 42, 2 ** 63 + 5)}
+
+    # --- k-means++ (library generator) ------------------------------------
+    # n <= 4096: one block, so every sum is the reference's plain left fold;
+    # n = 9000: three blocks, which pins the library's two-level order;
+    # identical rows: the zero-potential branch; k = 1; k = n.
+    rk = random.Random(20261204)
+    kpp = []
+    for (n, d, k, seed, spread, dup) in ((300, 3, 5, 7, 1.0, False), (1000, 2, 12, 3, 3.0, False),
+                                          (50, 4, 1, 5, 1.0, False), (40, 2, 6, 9, 0.0, True),
+                                          (9000, 2, 6, 11, 2.0, False), (12, 3, 12, 2, 1.0, False),
+                                          (2000, 5, 17, 2 ** 40 + 3, 0.5, False)):
+        cen = [[rk.uniform(-10, 10) for _ in range(d)] for _ in range(max(k, 2))]
+        if dup:
+            X = [[1.5] * d for _ in range(n - 3)] + [[2.5] * d, [1.5] * d, [-1.0] * d]
+        else:
+            X = [[cen[i % len(cen)][j] + rk.gauss(0, spread) for j in range(d)] for i in range(n)]
+        picks = kmeanspp(X, k, Gen(seed))
+        kpp.append({"X": [[hx(v) for v in r] for r in X], "k": k, "seed": str(seed),
+                    "picks": picks})
+    out["kmeanspp"] = kpp
+
+    # --- whole MiniBatchKMeans fits (k-means++ and batches from each run's
+    # generator) -------------------------------------------------------------
+    mbf = []
+    for (n, d, k, batch, max_iter, tol, n_init, base) in ((200, 3, 4, 50, 10, 0.0, 2, 11),
+                                                          (60, 2, 3, 100, 5, 0.0, 3, 5),
+                                                          (150, 2, 5, 40, 30, 0.05, 2, 8),
+                                                          (90, 4, 3, 30, 6, 0.0, 1, 0)):
+        cen = [[rk.uniform(-8, 8) for _ in range(d)] for _ in range(k)]
+        X = [[cen[i % k][j] + rk.gauss(0, 1.5) for j in range(d)] for i in range(n)]
+        C, L, inertia, it = minibatch_fit(X, k, batch, max_iter, tol, n_init, base)
+        mbf.append({"X": [[hx(v) for v in r] for r in X], "k": k, "batch": batch,
+                    "max_iter": max_iter, "tol": tol, "n_init": n_init, "seed": base,
+                    "centers": [[hx(v) for v in r] for r in C], "labels": L,
+                    "inertia": hx(inertia), "n_iter": it})
+    out["minibatch_fit"] = mbf
 
     with open(os.path.join(HERE, "golden_vectors.json"), "w") as f:
         json.dump(out, f, indent=0, separators=(",", ":"))

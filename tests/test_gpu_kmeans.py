@@ -242,3 +242,28 @@ def test_config3_shape_k1024(config3_case, precision):
     assert (f.labels().numpy() == l_o).all()
     assert (bits(f.cluster_centers().numpy()) == bits(c_o)).all()
     assert abs(f.inertia() - i_o) <= 1e-12 * i_o
+
+
+def test_kmeanspp_golden_vectors():
+    # the library's k-means++ against the pure-Python restatement of
+    # kmeans.rs:169-253 (same generator; n <= 4096 is the reference's own
+    # left-fold order), including the zero-potential branch, k = 1 and k = n
+    for case in G["kmeanspp"]:
+        X = arr(case["X"])
+        got = F.kmeans_plusplus(X, case["k"], int(case["seed"]))
+        assert (bits(got) == bits(X[case["picks"]])).all(), (X.shape, case["k"])
+
+
+def test_batched_kmeanspp_golden_vectors():
+    # the batched path's device-side k-means++ (one restart, max_iter 0 keeps
+    # the initial centres) against the same vectors
+    items = []
+    for case in G["kmeanspp"]:
+        X = arr(case["X"])
+        if X.shape[1] > 64:
+            continue
+        m = F.KMeans(case["k"]).with_n_init(1).with_max_iter(0).with_random_state(int(case["seed"]))
+        items.append((F.to_device(X), m, None, X, case))
+    fitted, _ = F.kmeans_fit_batched([(x, m, i) for x, m, i, _, _ in items])
+    for f, (_, _, _, X, case) in zip(fitted, items):
+        assert (bits(f.cluster_centers().numpy()) == bits(X[case["picks"]])).all()

@@ -71,3 +71,19 @@ def test_minibatch_without_init_is_deterministic_and_consistent():
     assert (a.labels() == lab).all()
     assert abs(a.inertia() - inertia) <= 1e-12 * inertia
     assert (a.predict(X) == lab).all()
+
+
+def test_minibatch_fit_golden_vectors():
+    # whole fits without given centres (k-means++ and batches from each run's
+    # generator) against the pure-Python restatement of mini_batch_kmeans.rs
+    from tests._golden import arr, golden
+    for case in golden()["minibatch_fit"]:
+        X = arr(case["X"])
+        m = (F.MiniBatchKMeans(case["k"]).with_batch_size(case["batch"])
+             .with_max_iter(case["max_iter"]).with_tol(case["tol"]).with_n_init(case["n_init"])
+             .with_random_state(case["seed"]))
+        f = m.fit(X)
+        assert f.n_iter() == case["n_iter"]
+        assert list(f.labels()) == case["labels"]
+        assert (bits(f.cluster_centers()) == bits(arr(case["centers"]))).all()
+        assert f.inertia() == float.fromhex(case["inertia"])

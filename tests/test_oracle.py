@@ -441,3 +441,45 @@ def test_mlp_oracle_zero_network():
 def test_bf16_round_ties_to_even():
     v = np.array([1.0, 1.0 + 2**-8, 1.0 + 3 * 2**-8, -2.0 - 2**-7 - 2**-9, 0.0])
     assert list(O.bf16_round(v)) == [1.0, 1.0, 1.0 + 2**-6, -2.0 - 2**-6, 0.0]
+
+
+# ------------------------------------------- k-means++ / MiniBatch fits ------
+# The golden generator restates kmeans.rs:169-253 and mini_batch_kmeans.rs:
+# 410-592 in pure Python with the library's generator; these pin the C
+# oracle's or_kmeanspp / or_minibatch_fit (first pick, n_trials = 2 +
+# floor(ln k), D^2 thresholds, strict-< potential, the zero-potential branch,
+# k = 1, batch >= n, run seeds base + r*1000003, row-order inertia) bit for
+# bit.  A wrong n_trials or keeping the largest potential changes the picks.
+def test_kmeanspp_golden_bit_exact():
+    for case in G["kmeanspp"]:
+        X = arr(case["X"])
+        got = O.kmeanspp(X, case["k"], int(case["seed"]))
+        want = X[case["picks"]]
+        assert (bits(got) == bits(want)).all(), (X.shape, case["k"])
+
+
+def test_kmeanspp_golden_is_sensitive():
+    # the restatement itself: one trial instead of 2 + floor(ln k), or the
+    # largest potential instead of the smallest, picks other rows
+    from tests.golden import make_golden as MG
+    case = G["kmeanspp"][1]
+    X = [[float.fromhex(v) for v in r] for r in case["X"]]
+    picks = MG.kmeanspp(X, case["k"], MG.Gen(int(case["seed"])))
+    assert picks == case["picks"]
+    orig_log = MG.math.log
+    try:
+        MG.math.log = lambda v: -10.0          # n_trials -> max(2 - 10, 1) = 1
+        assert MG.kmeanspp(X, case["k"], MG.Gen(int(case["seed"]))) != case["picks"]
+    finally:
+        MG.math.log = orig_log
+
+
+def test_minibatch_fit_golden_bit_exact():
+    for case in G["minibatch_fit"]:
+        X = arr(case["X"])
+        c, l, inertia, it = O.minibatch_fit(X, case["k"], case["batch"], case["max_iter"],
+                                            case["tol"], case["n_init"], case["seed"])
+        assert it == case["n_iter"]
+        assert list(l) == case["labels"]
+        assert (bits(c) == bits(arr(case["centers"]))).all()
+        assert inertia == float.fromhex(case["inertia"])
