@@ -37,7 +37,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK = {"f64": 78.6e12, "bf16": 2.5e15, "hbm": 8.0e12}   # MI355X spec: FLOP/s, B/s
+PEAK = {"f64": 78.6e12, "bf16": 2.5e15, "fp8": 5.0e15, "hbm": 8.0e12}   # MI355X spec: FLOP/s, B/s
 
 # HBM bytes per launch of the filter kernel at these workloads, from two
 # rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch),
@@ -243,15 +243,18 @@ def run_kmeans(args):
         forms[form] = {
             "samples_iter_per_s": n * f.n_iter() / dt, "s": dt, "n_iter": f.n_iter(),
             "ms_per_iter": dt * 1e3 / max(1, f.n_iter()),
-            "ms_distance_kernel": r["ms_distance"] / max(1, r["n_dist_launches"]),
-            "ms_finalize_per_iter": r["ms_finalize"] / max(1, f.n_iter()),
+            "ms_distance_kernel": r["ms_distance"] / max(1, f.n_iter()),
+            "ms_second_filter_per_iter": r["ms_finalize"] / max(1, f.n_iter()),
             "ms_update_per_iter": r["ms_update"] / max(1, f.n_iter()),
+            "rows_to_second_filter": int(r["n_second"]),
             "fallback_rows": int(r["n_fallback"]),
             "labels": f.labels().numpy(), "centers": f.cluster_centers().numpy(),
             "inertia": f.inertia(),
         }
         log(f"kmeans {form}: {forms[form]['samples_iter_per_s']:.3g} samples*iter/s, "
-            f"{forms[form]['ms_per_iter']:.2f} ms/iter")
+            f"{forms[form]['ms_per_iter']:.2f} ms/iter, distance kernel "
+            f"{forms[form]['ms_distance_kernel']:.2f} ms, second filter rows "
+            f"{forms[form]['rows_to_second_filter']}, exact-scan rows {forms[form]['fallback_rows']}")
     head = args.km_forms[0]
     h = forms[head]
     out = {
@@ -261,20 +264,31 @@ def run_kmeans(args):
         "forms": {f: {kk: vv for kk, vv in v.items() if kk not in ("labels", "centers")}
                   for f, v in forms.items()},
     }
-    if "fp64" in forms and head != "fp64":
+    if "fp64" in forms:
         ref = forms["fp64"]
-        out["speedup_vs_fp64_form"] = h["samples_iter_per_s"] / ref["samples_iter_per_s"]
-        out["exact_vs_fp64_form"] = bool(
-            (h["labels"] == ref["labels"]).all() and h["n_iter"] == ref["n_iter"]
-            and (h["centers"].view(np.int64) == ref["centers"].view(np.int64)).all()
-            and h["inertia"] == ref["inertia"])
+        for f, v in forms.items():
+            if f == "fp64":
+                continue
+            o = out["forms"][f]
+            o["speedup_vs_fp64_form"] = v["samples_iter_per_s"] / ref["samples_iter_per_s"]
+            o["label_agreement_vs_fp64"] = float((v["labels"] == ref["labels"]).mean())
+            o["inertia_rel_error_vs_fp64"] = abs(v["inertia"] - ref["inertia"]) / ref["inertia"]
+            o["exact_vs_fp64_form"] = bool(
+                (v["labels"] == ref["labels"]).all() and v["n_iter"] == ref["n_iter"]
+                and (v["centers"].view(np.int64) == ref["centers"].view(np.int64)).all()
+                and v["inertia"] == ref["inertia"])
+        if head != "fp64":
+            out["speedup_vs_fp64_form"] = out["forms"][head]["speedup_vs_fp64_form"]
+            out["exact_vs_fp64_form"] = out["forms"][head]["exact_vs_fp64_form"]
     flops = 2.0 * n * k * d
-    peak_key = "f64" if head in ("fp64", "exact") else "bf16"
+    peak_key = {"fp64": "f64", "exact": "f64", "fp8": "fp8"}.get(head, "bf16")
+    kname = {"f64": "dist_f64_kernel<0,...> (argmin form)",
+             "fp8": "dist_bf16_kernel<0,2,0> (FP8 E4M3 argmin form, decision fused)",
+             "bf16": "dist_bf16_kernel<0,2,3> (BF16x3 argmin form, decision fused)"}[peak_key]
     out["roofline"] = {"bound": "mfma", "achieved": flops / (h["ms_distance_kernel"] * 1e-3) / 1e12,
                        "peak": PEAK[peak_key] / 1e12, "unit": "TFLOP/s",
                        "frac": flops / (h["ms_distance_kernel"] * 1e-3) / PEAK[peak_key],
-                       "kernel": "dist_bf16_kernel<0,...> (argmin form, decision fused)"
-                       if peak_key == "bf16" else "dist_f64_kernel<0,...> (argmin form)",
+                       "kernel": kname,
                        "kernel_ms": h["ms_distance_kernel"],
                        "algorithmic_flops_per_launch": flops,
                        "traffic": TRAFFIC.get(("kmeans", head)) if n == 10_000_000 else None,
@@ -521,7 +535,7 @@ def main():
     ap.add_argument("--router-epochs", type=int, default=3)
     ap.add_argument("--router-brute-sample", type=int, default=100_000)
     ap.add_argument("--n-probe", type=int, default=2)
-    ap.add_argument("--km-forms", default="bf16x3,fp64")
+    ap.add_argument("--km-forms", default="fp8,bf16x3,fp64")
     ap.add_argument("--km-rows", type=int, default=10_000_000)
     ap.add_argument("--km-iters", type=int, default=50)
     ap.add_argument("--forms", default="bf16x3,fp64",

@@ -25,12 +25,13 @@ FL_PREC_FP64 = 0
 FL_PREC_BF16 = 1
 FL_PREC_BF16X3 = 2
 FL_PREC_EXACT = 3
+FL_PREC_FP8 = 4
 
 FL_UNIFORM = 0
 FL_DISTANCE = 1
 
 PRECISIONS = {"fp64": FL_PREC_FP64, "bf16": FL_PREC_BF16, "bf16x3": FL_PREC_BF16X3,
-              "exact": FL_PREC_EXACT}
+              "exact": FL_PREC_EXACT, "fp8": FL_PREC_FP8}
 
 
 class fl_report(C.Structure):
@@ -45,6 +46,7 @@ class fl_report(C.Structure):
         ("n_fallback", C.c_int64),
         ("n_iter", C.c_int64),
         ("n_dist_launches", C.c_int64),
+        ("n_second", C.c_int64),
     ]
 
     def as_dict(self):

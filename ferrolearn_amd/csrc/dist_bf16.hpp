@@ -40,6 +40,8 @@ namespace fl {
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef unsigned short u16x8_t __attribute__((ext_vector_type(8)));
+typedef int i32x8_t __attribute__((ext_vector_type(8)));
+typedef int i32x4_t __attribute__((ext_vector_type(4)));
 
 constexpr int BF_WAVES = 8;
 constexpr int BF_QW = 32;                 // resident rows per wave (two 16-col tiles)
@@ -148,7 +150,114 @@ struct Bf16Args {
     double *best = nullptr;
     int32_t *flags = nullptr;
     int32_t *fcount = nullptr;
+    // FP8 form (PASSES == 0): per resident row key factor -2 s_C s_x (f32,
+    // indexed by source row); ah / bh hold E4M3 bytes, 128 per row.
+    const float *afac = nullptr;
+    // Resident rows gathered from a list (argmin form): launch row q is
+    // source row rows[q], for q < *rows_count (device); qoff, afac, the
+    // split rows, labels / best and flags use source rows.
+    const int32_t *rows = nullptr;
+    const int32_t *rows_count = nullptr;
 };
+
+// FP8 E4M3 (OCP: bias 7, largest finite 448, subnormal spacing 2^-9) of a
+// value with |v| <= 448, rounded to nearest even.
+__host__ __device__ __forceinline__ uint8_t fp8_e4m3_rne(double v) {
+    const uint8_t sgn = v < 0.0 ? 0x80 : 0x00;
+    const double a = fabs(v);
+    if (a == 0.0) return sgn;
+    int e = 0;
+    (void)frexp(a, &e);   // a in [2^(e-1), 2^e)
+    int E = e - 1;
+    if (E < -6) {
+        const int m = (int)rint(a * 512.0);   // multiples of 2^-9, exact scaling
+        return m >= 8 ? (uint8_t)(sgn | 0x08) : (uint8_t)(sgn | m);
+    }
+    int m = (int)rint(ldexp(a, 3 - E));       // in [8, 16]
+    if (m == 16) {
+        m = 8;
+        ++E;
+    }
+    return (uint8_t)(sgn | ((E + 7) << 3) | (m - 8));
+}
+
+// Power-of-two scale exponent of a row whose largest magnitude is mx: the
+// smallest E with mx / 2^E < 448, kept in [-60, 60] so that products of two
+// scales stay normal in f32.
+__host__ __device__ __forceinline__ int fp8_scale_exp(double mx) {
+    if (!(mx > 0.0)) return 0;
+    int e = 0;
+    (void)frexp(mx / 448.0, &e);   // mx/448 < 2^e
+    return e < -60 ? -60 : (e > 60 ? 60 : e);
+}
+
+// Rows to E4M3 bytes (128 per row, zero padded) with per-row scale
+// exponents: one thread per row.
+template <typename T>
+__global__ void fp8_split_rows_kernel(const T *__restrict__ X, int64_t n, int d,
+                                      uint8_t *__restrict__ out, int32_t *__restrict__ sexp) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const T *x = X + r * d;
+    double mx = 0.0;
+    for (int j = 0; j < d; ++j) mx = fmax(mx, fabs((double)x[j]));
+    const int E = fp8_scale_exp(mx);
+    uint8_t *o = out + r * 128;
+    for (int j = 0; j < 128; ++j) o[j] = j < d ? fp8_e4m3_rne(ldexp((double)x[j], -E)) : 0;
+    sexp[r] = E;
+}
+
+// Rows to E4M3 bytes with one scale exponent for all of them (*sexp).
+static __global__ __attribute__((unused)) void fp8_split_common_kernel(
+    const double *__restrict__ X, int64_t n, int d, const int32_t *__restrict__ sexp,
+    uint8_t *__restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * 128) return;
+    const int64_t r = t / 128;
+    const int j = (int)(t % 128);
+    out[t] = j < d ? fp8_e4m3_rne(ldexp(X[r * d + j], -*sexp)) : 0;
+}
+
+// *sexp = fp8_scale_exp of the largest magnitude among v[0..count), read
+// from *mxbits (max of the non-negative magnitudes' bit patterns).
+static __global__ __attribute__((unused)) void fp8_common_exp_kernel(
+    const unsigned long long *__restrict__ mxbits, int32_t *__restrict__ sexp) {
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        *sexp = fp8_scale_exp(__longlong_as_double((long long)*mxbits));
+}
+
+// *out = max(*out, bits of max_i |v[i]|) (zeroed before the launch).
+static __global__ __attribute__((unused)) void max_abs_bits_kernel(
+    const double *__restrict__ v, int64_t n, unsigned long long *__restrict__ out) {
+    double m = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        m = fmax(m, fabs(v[i]));
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
+}
+
+// Per resident row of the FP8 form: qoff = (1-b)|q|^2 - c|q| Smax - eta with
+// eta = (d+1) 2^-60 (1 + |q| + Smax): scale exponents are kept >= -60, so
+// elements far below a row's largest one (flushed or subnormal in E4M3 at
+// that scale) are off by at most 2^-70 each, and f32 flushing is covered too.
+static __global__ __attribute__((unused)) void fp8_resident_terms_kernel(
+    const double *__restrict__ n2, const double *__restrict__ n1, int64_t n, int d, double c,
+    double b, const double *__restrict__ smax, double *__restrict__ qoff) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double sm = *smax;
+    const double eta = (double)(d + 1) * 0x1.0p-60 * (1.0 + n1[i] + sm);
+    qoff[i] = (1.0 - b) * n2[i] - c * n1[i] * sm - eta;
+}
+
+// Per resident row: afac = -2 * 2^(eC + ex) (exact in f32).
+static __global__ __attribute__((unused)) void fp8_resident_factor_kernel(
+    const int32_t *__restrict__ sexp_rows, const int32_t *__restrict__ sexp_c, int64_t n,
+    float *__restrict__ afac) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) afac[i] = ldexpf(-2.0f, sexp_rows[i] + *sexp_c);
+}
 
 // Raw buffer resource over [base, base + bytes): loads at offsets past the
 // end return zero (the hardware range check), so a tile that runs past the
@@ -279,6 +388,11 @@ void dist_bf16_kernel(Bf16Args args) {
     // per resident row and the smallest key it dropped; the four lanes of a
     // row output their four bests and theta = the smallest dropped key.
     constexpr bool ARGMIN = MP == 0;
+    // PASSES == 0: FP8 E4M3 operands (128 bytes per row, the layout of NKC = 2
+    // bf16 rows), one v_mfma_scale_f32_16x16x128_f8f6f4 per 16 x 16 tile with
+    // unit scales, key = fma(afac_row, acc, xn'); argmin form only.
+    constexpr bool FP8 = PASSES == 0;
+    static_assert(!FP8 || (ARGMIN && NKC == 2), "FP8 form: argmin, 128-byte rows");
     constexpr int L = ARGMIN ? 1 : MP / 2;
     constexpr int DPAD = NKC * 32;
     float *xs = reinterpret_cast<float *>(smem_raw + TL::NST * TL::BYTES);   // [NST][waves][BF_T]
@@ -287,7 +401,12 @@ void dist_bf16_kernel(Bf16Args args) {
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave-uniform
     const int g = lane >> 4;
     const int c16 = lane & 15;
-    const int64_t nA = args.nA;
+    int64_t nA = args.nA;
+    if (args.rows_count) {
+        const int64_t c = *args.rows_count;
+        nA = c < nA ? c : nA;
+        if ((int64_t)blockIdx.x * BF_Q >= nA) return;   // the whole block is past the list
+    }
     const int64_t q0 = (int64_t)blockIdx.x * BF_Q + w * BF_QW;
     const int split = blockIdx.y;
     const int64_t bbeg = (int64_t)split * args.b_per_split;
@@ -297,16 +416,27 @@ void dist_bf16_kernel(Bf16Args args) {
     // Resident operand fragments: for column tile jt, lane l holds row
     // q0 + 16*jt + l%16, features kc*32 + 8*(l/16) .. +7.
     bf16x8_t qh[2][NKC], ql[2][NKC];
+    i32x8_t q8[2];
     bool qok[2];
+    int64_t qsrc[2];   // source row of each of this lane's resident rows
 #pragma unroll
     for (int jt = 0; jt < 2; ++jt) {
         const int64_t q = q0 + 16 * jt + c16;
         qok[jt] = q < nA;
+        qsrc[jt] = qok[jt] ? (args.rows ? (int64_t)args.rows[q] : q) : 0;
+        if (FP8) {
+            // bytes 32 g .. 32 g + 31 of the row: the k slice of lane group g
+            i32x8_t v = (i32x8_t){0, 0, 0, 0, 0, 0, 0, 0};
+            if (qok[jt])
+                v = *reinterpret_cast<const i32x8_t *>(
+                    reinterpret_cast<const unsigned char *>(args.ah) + qsrc[jt] * 128 + 32 * g);
+            q8[jt] = v;
+        }
 #pragma unroll
         for (int kc = 0; kc < NKC; ++kc) {
             u16x8_t vh = (u16x8_t){0, 0, 0, 0, 0, 0, 0, 0}, vl = vh;
-            if (qok[jt]) {
-                const int64_t off = q * DPAD + kc * 32 + 8 * g;
+            if (qok[jt] && !FP8) {
+                const int64_t off = qsrc[jt] * DPAD + kc * 32 + 8 * g;
                 vh = *reinterpret_cast<const u16x8_t *>(args.ah + off);
                 if (PASSES == 3) vl = *reinterpret_cast<const u16x8_t *>(args.al + off);
             }
@@ -314,6 +444,10 @@ void dist_bf16_kernel(Bf16Args args) {
             ql[jt][kc] = __builtin_bit_cast(bf16x8_t, vl);
         }
     }
+    float fx[2] = {-2.f, -2.f};   // key factor per resident row
+    if (FP8)
+#pragma unroll
+        for (int jt = 0; jt < 2; ++jt) fx[jt] = qok[jt] ? args.afac[qsrc[jt]] : -2.f;
 
     float lk[2][L];
     int32_t li[2][L];
@@ -396,6 +530,23 @@ void dist_bf16_kernel(Bf16Args args) {
         for (int it = 0; it < 4; ++it)
 #pragma unroll
             for (int jt = 0; jt < 2; ++jt) acc[it][jt] = (float4_t){0.f, 0.f, 0.f, 0.f};
+        if (FP8) {
+            // lane group g reads chunks 2g and 2g+1 (bytes 32g..32g+31) of
+            // row 16 it + c16, the same k slice as its resident fragment
+            const int p0 = (2 * g) ^ (c16 & TL::SWZ), p1 = (2 * g + 1) ^ (c16 & TL::SWZ);
+#pragma unroll
+            for (int it = 0; it < 4; ++it) {
+                const unsigned char *rw = tb + (16 * it + c16) * TL::ROWB;
+                const i32x4_t lo = *reinterpret_cast<const i32x4_t *>(rw + p0 * 16);
+                const i32x4_t hi = *reinterpret_cast<const i32x4_t *>(rw + p1 * 16);
+                const i32x8_t x8 = (i32x8_t){lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+#pragma unroll
+                for (int jt = 0; jt < 2; ++jt)
+                    acc[it][jt] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
+                        x8, q8[jt], acc[it][jt], 0, 0, 0, 0, 0, 0);
+            }
+            return;
+        }
 #pragma unroll
         for (int kc = 0; kc < NKC; ++kc) {
             const int pos = (kc * 4 + g) ^ (c16 & TL::SWZ);
@@ -506,7 +657,7 @@ void dist_bf16_kernel(Bf16Args args) {
             for (int it = 0; it < 4; ++it)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
-                    float kv = __builtin_fmaf(-2.f, acc[it][jt][v], xr[it][v]);
+                    float kv = __builtin_fmaf(fx[jt], acc[it][jt][v], xr[it][v]);
                     if (MASK) kv = 16 * it + 4 * g + v < nvalid ? kv : __builtin_huge_valf();
                     const bool lt = kv < bk[jt];
                     sec[jt] = __builtin_amdgcn_fmed3f(bk[jt], sec[jt], kv);
@@ -628,7 +779,8 @@ void dist_bf16_kernel(Bf16Args args) {
             // key fails it and the row takes the exact scan).
 #pragma unroll
             for (int jt = 0; jt < 2; ++jt) {
-                const int64_t q = q0 + 16 * jt + c16;
+                const int64_t qi = q0 + 16 * jt + c16;   // launch row
+                const int64_t q = qsrc[jt];              // source row
                 const float kmin = xred<false, 32>(xred<false, 16>(bk[jt]));
                 // indices < 2^24 (k <= 8192) are exact in f32
                 const float icand = bk[jt] == kmin ? (float)bi[jt] : 0x1.0p30f;
@@ -637,7 +789,7 @@ void dist_bf16_kernel(Bf16Args args) {
                 const bool mine = bk[jt] == kmin && bi[jt] == imin;
                 const float other = mine ? sec[jt] : fminf(sec[jt], bk[jt]);
                 const float thk = xred<false, 32>(xred<false, 16>(other));
-                if (q >= nA || g != jt) continue;
+                if (qi >= nA || g != jt) continue;
                 const int d = args.d;
                 bool ok = imin >= 0;
                 double bf = 0.0;

@@ -168,6 +168,30 @@ inline Bound bound_bf16(int d, int passes) {
     return {2.0 * (2.0 * cdot + 3.0 * u32 + 2.0 * gf), 2.0 * (2.0 * u32 + 2.0 * gf)};
 }
 
+// Same role for the FP8 E4M3 form: rows x~ = s_x q(x / s_x) with per-row
+// power-of-two scales (centres share one scale s_C).  Per element |x - x~| <=
+// u8 |x| + s 2^-10 (u8 = 2^-4; 2^-10 is half the subnormal spacing times s),
+// and s <= 2 max|x_j| / 448, so |x - x~| <= (u8 + rho)|x| with rho =
+// 2 sqrt(d) / (448 * 1024) for a sample row, and the centres' term is
+// bounded with Smax, the largest centre norm.  E4M3 x E4M3 products are
+// exact in f32; the MFMA's accumulation of them is taken as <= eacc times
+// the sum of |products| (<= |x~||c~|).  eacc = 2^-8: the instruction's
+// accumulation error was measured at <= 2.1e-4 of that sum over 400 x 256
+// dot products of mixed-magnitude E4M3 data, and 2^-8 leaves a factor 16.
+// As for BF16, the f32 epilogue roundings and the exact fold's own error
+// are added and everything is doubled.
+inline Bound bound_fp8(int d) {
+    const double u64 = std::ldexp(1.0, -53), u32 = std::ldexp(1.0, -24);
+    const double u8 = std::ldexp(1.0, -4);
+    const double rho = 2.0 * std::sqrt((double)d) / (448.0 * 1024.0);
+    const double eacc = std::ldexp(1.0, -8);
+    const double rx = u8 + rho;
+    const double rep = rx + (1.0 + rx) * (u8 + rho);
+    const double cdot = rep + eacc * (1.0 + rx) * (1.0 + u8 + rho) * 1.01;
+    const double gf = gamma_n(d + 2, u64);
+    return {2.0 * (2.0 * cdot + 3.0 * u32 + 2.0 * gf), 2.0 * (2.0 * u32 + 2.0 * gf)};
+}
+
 // The library's generator (SplitMix64 seeding xoshiro256**), identical to
 // the oracle's restatement; kmeans.rs:178,203,213 draw from rand's StdRng,
 // whose stream is not reproduced.
@@ -251,6 +275,11 @@ void launch_bf16_p(int MP, int nkc, const Bf16Args &a, int S, hipStream_t st) {
 
 inline void launch_bf16(int passes, int MP, int nkc, const Bf16Args &a, int S, hipStream_t st) {
     if (passes == 3) return launch_bf16_p<3>(MP, nkc, a, S, st);
+    if (passes == 0) {
+        // FP8 form: argmin with the decision fused, 128-byte rows
+        if (MP != 0 || nkc != 2) throw DeviceError("internal: FP8 form is argmin over 128-byte rows");
+        return launch_bf16_t<0, 2, 0>(a, S, st);
+    }
     return launch_bf16_p<1>(MP, nkc, a, S, st);
 }
 

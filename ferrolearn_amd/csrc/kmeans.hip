@@ -35,6 +35,7 @@ constexpr int kKmM = 4;   // candidates kept per row by the filter
 struct KmAcc {
     double ms_dist = 0, ms_fin = 0, ms_fb = 0, ms_upd = 0;
     int64_t n_rows = 0, n_cand = 0, n_fb = 0, n_launch = 0;
+    int64_t n_second = 0;   // rows the first filter (FP8) left to the BF16x3 one
 };
 
 template <typename TX>
@@ -89,9 +90,12 @@ class Lloyd {
             FL_HIP(hipStreamSynchronize(st));
             const bool fits = k < ((int64_t)1 << 31) - 64;
             // BF16 filter: samples are split into hi / lo bf16 once per fit.
-            if ((precision == FL_PREC_BF16 || precision == FL_PREC_BF16X3) && bf16_nkc(d) > 0 &&
-                fits && (hflag & 1u) == 0) {
-                passes_ = precision == FL_PREC_BF16X3 ? 3 : 1;
+            // The FP8 form (d <= 128) filters with E4M3 first and sends the
+            // rows it cannot certify through the BF16x3 filter.
+            if ((precision == FL_PREC_BF16 || precision == FL_PREC_BF16X3 ||
+                 precision == FL_PREC_FP8) &&
+                bf16_nkc(d) > 0 && fits && (hflag & 1u) == 0) {
+                passes_ = precision == FL_PREC_BF16 ? 1 : 3;
                 nkc_ = bf16_nkc(d);
                 const int dpad = nkc_ * 32;
                 xh_ = ws_.get<unsigned short>("xh", n * dpad);
@@ -99,6 +103,14 @@ class Lloyd {
                 hipLaunchKernelGGL(bf16_split_kernel<TX>, dim3(grid1(n * dpad, 256)), dim3(256), 0,
                                    st, X, n, d, dpad, xh_, xl_);
                 check_launch();
+                if (precision == FL_PREC_FP8 && d <= 128) {
+                    fp8_ = true;
+                    x8_ = ws_.get<uint8_t>("x8", n * 128);
+                    sx_ = ws_.get<int32_t>("sx", n);
+                    hipLaunchKernelGGL(fp8_split_rows_kernel<TX>, dim3(grid1(n, 256)), dim3(256),
+                                       0, st, X, n, d, x8_, sx_);
+                    check_launch();
+                }
             } else if (f64_d4(d) > 0 && fits && (hflag & 2u) == 0) {
                 d4_ = f64_d4(d);   // f64 MFMA argmin filter
             }
@@ -122,12 +134,17 @@ class Lloyd {
     }
     // Add the pending phase times to their slots and, when given, the
     // fallback count of the last assignment (device) to *n_fb.
-    void flush(int64_t *n_fb = nullptr) {
+    // With n_second, also the rows the FP8 filter sent to the BF16x3 one.
+    void flush(int64_t *n_fb = nullptr, int64_t *n_second = nullptr) {
         if (n_fb) {
-            int32_t c = 0;
-            FL_HIP(hipMemcpyAsync(&c, ws_.get<int32_t>("fcnt", 1), 4, hipMemcpyDeviceToHost, st_));
+            int32_t c[2] = {0, 0};
+            FL_HIP(hipMemcpyAsync(&c[0], ws_.get<int32_t>("fcnt", 1), 4, hipMemcpyDeviceToHost, st_));
+            if (fp8_ && n_second)
+                FL_HIP(hipMemcpyAsync(&c[1], ws_.get<int32_t>("fcnt8", 1), 4,
+                                      hipMemcpyDeviceToHost, st_));
             FL_HIP(hipStreamSynchronize(st_));
-            *n_fb += c;
+            *n_fb += c[0];
+            if (n_second) *n_second += c[1];
         }
         for (const auto &p : pend_) *p.second += tm_->ms(p.first);
         pend_.clear();
@@ -178,8 +195,62 @@ class Lloyd {
             hipLaunchKernelGGL(bf16_resident_terms_kernel, dim3(grid1(n_, 256)), dim3(256), 0, st_,
                                xn_, xs_, n_, d_, bd.alpha, bd.beta, (const double *)smax, qoff);
             check_launch();
+            int32_t *flags8 = nullptr, *fcnt8 = nullptr;
+            if (fp8_) {
+                // FP8 filter with the decision fused over every row; the rows
+                // it cannot certify are listed in flags8 for the BF16x3 pass.
+                const Bound b8 = bound_fp8(d_);
+                uint8_t *c8 = ws_.get<uint8_t>("c8", k_ * 128);
+                int32_t *sc = ws_.get<int32_t>("sc", 1);
+                unsigned long long *cmax = ws_.get<unsigned long long>("cmax", 1);
+                float *cxnp8 = ws_.get<float>("cxnp8", k_);
+                float *afac = ws_.get<float>("afac", n_);
+                double *qoff8 = ws_.get<double>("qoff8", n_);
+                flags8 = ws_.get<int32_t>("flags8", n_);
+                fcnt8 = ws_.get<int32_t>("fcnt8", 1);
+                FL_HIP(hipMemsetAsync(cmax, 0, 8, st_));
+                hipLaunchKernelGGL(max_abs_bits_kernel, dim3(64), dim3(256), 0, st_, C, k_ * d_,
+                                   cmax);
+                hipLaunchKernelGGL(fp8_common_exp_kernel, dim3(1), dim3(64), 0, st_,
+                                   (const unsigned long long *)cmax, sc);
+                hipLaunchKernelGGL(fp8_split_common_kernel, dim3(grid1(k_ * 128, 256)), dim3(256),
+                                   0, st_, C, k_, d_, (const int32_t *)sc, c8);
+                hipLaunchKernelGGL(bf16_stream_terms_kernel, dim3(grid1(k_, 256)), dim3(256), 0,
+                                   st_, cn, k_, b8.beta, cxnp8);
+                hipLaunchKernelGGL(fp8_resident_factor_kernel, dim3(grid1(n_, 256)), dim3(256), 0,
+                                   st_, (const int32_t *)sx_, (const int32_t *)sc, n_, afac);
+                hipLaunchKernelGGL(fp8_resident_terms_kernel, dim3(grid1(n_, 256)), dim3(256), 0,
+                                   st_, xn_, xs_, n_, d_, b8.alpha, b8.beta, (const double *)smax,
+                                   qoff8);
+                check_launch();
+                Bf16Args a8;
+                a8.nA = n_;
+                a8.nB = k_;
+                a8.b_per_split = k_;
+                a8.ah = a8.al = (const unsigned short *)x8_;
+                a8.bh = a8.bl = (const unsigned short *)c8;
+                a8.qoff = qoff8;
+                a8.xnp = cxnp8;
+                a8.afac = afac;
+                a8.out = cand;
+                a8.dec_X = X_;
+                a8.dec_x_f32 = sizeof(TX) == 4;
+                a8.d = d_;
+                a8.dec_C = C;
+                a8.labels = labels;
+                a8.best = best;
+                a8.flags = flags8;
+                a8.fcount = fcnt8;
+                FL_HIP(hipMemsetAsync(fcnt8, 0, sizeof(int32_t), st_));
+                const int t8 = tbegin();
+                launch_bf16(0, 0, 2, a8, 1, st_);
+                tend(t8, &acc.ms_dist);
+                acc.n_launch += 1;
+            }
             Bf16Args a;
             a.nA = n_;
+            a.rows = flags8;          // FP8 form: only the rows FP8 left
+            a.rows_count = fcnt8;
             a.nB = k_;
             a.b_per_split = k_;
             a.ah = xh_;
@@ -200,7 +271,7 @@ class Lloyd {
             FL_HIP(hipMemsetAsync(a.fcount, 0, sizeof(int32_t), st_));
             t0 = tbegin();
             launch_bf16(passes_, 0, nkc_, a, 1, st_);   // argmin form, decision fused
-            tend(t0, &acc.ms_dist);
+            tend(t0, fp8_ ? &acc.ms_fin : &acc.ms_dist);
             fused = true;
         } else if (d4_ > 0) {
             // f64 filter: centre terms, padding and the largest centre norm
@@ -523,6 +594,9 @@ class Lloyd {
     hipStream_t st_;
     bool direct_ = true;
     int passes_ = 0, nkc_ = 0;   // BF16 filter (passes_ > 0)
+    bool fp8_ = false;           // FP8 filter first (BF16x3 for what it leaves)
+    uint8_t *x8_ = nullptr;      // samples as E4M3, 128 bytes per row
+    int32_t *sx_ = nullptr;      // their per-row scale exponents
     int d4_ = 0;                 // f64 MFMA filter (d4_ > 0)
     double *xn_ = nullptr, *xs_ = nullptr;
     unsigned short *xh_ = nullptr, *xl_ = nullptr;
@@ -569,7 +643,8 @@ fl_status validate_fit(int64_t n, int64_t d, const fl_kmeans_params *p, int32_t 
     if (n > 0x7fffffffLL)
         return fail(FL_INVALID_PARAMETER, "InvalidParameter: n_samples: at most 2^31-1 rows");
     if (p->precision != FL_PREC_FP64 && p->precision != FL_PREC_EXACT &&
-        p->precision != FL_PREC_BF16 && p->precision != FL_PREC_BF16X3)
+        p->precision != FL_PREC_BF16 && p->precision != FL_PREC_BF16X3 &&
+        p->precision != FL_PREC_FP8)
         return fail(FL_INVALID_PARAMETER, "InvalidParameter: precision: unknown mode");
     return FL_OK;
 }
@@ -608,7 +683,7 @@ fl_status fit_device(const TX *X, int64_t n, int d, const fl_kmeans_params *p,
             L.update(labels, cur, nxt, dsc + 1, acc);
             double h[2];
             FL_HIP(hipMemcpyAsync(h, dsc, 16, hipMemcpyDeviceToHost, st));
-            L.flush(&acc.n_fb);   // the iteration's one synchronisation
+            L.flush(&acc.n_fb, &acc.n_second);   // the iteration's one synchronisation
             inertia = h[0];
             std::swap(cur, nxt);
             n_iter = it + 1;
@@ -639,6 +714,7 @@ fl_status fit_device(const TX *X, int64_t n, int d, const fl_kmeans_params *p,
         rep->n_fallback = acc.n_fb;
         rep->n_iter = best_iter;
         rep->n_dist_launches = acc.n_launch;
+        rep->n_second = acc.n_second;
     }
     return FL_OK;
 }
@@ -871,7 +947,8 @@ fl_status validate_mb(int64_t n, int64_t d, const fl_minibatch_params *p, int32_
     if (n > 0x7fffffffLL)
         return fail(FL_INVALID_PARAMETER, "InvalidParameter: n_samples: at most 2^31-1 rows");
     if (p->precision != FL_PREC_FP64 && p->precision != FL_PREC_EXACT &&
-        p->precision != FL_PREC_BF16 && p->precision != FL_PREC_BF16X3)
+        p->precision != FL_PREC_BF16 && p->precision != FL_PREC_BF16X3 &&
+        p->precision != FL_PREC_FP8)
         return fail(FL_INVALID_PARAMETER, "InvalidParameter: precision: unknown mode");
     return FL_OK;
 }
@@ -1057,7 +1134,7 @@ fl_status fl_kmeans_predict_device_noalloc(const double *dC, int64_t k, int64_t 
     return guarded([&]() -> fl_status {
         if (fl_status s = check_predict(k, d_centers, n, d, dtype)) return s;
         if (precision != FL_PREC_FP64 && precision != FL_PREC_EXACT &&
-            precision != FL_PREC_BF16 && precision != FL_PREC_BF16X3)
+            precision != FL_PREC_BF16 && precision != FL_PREC_BF16X3 && precision != FL_PREC_FP8)
             return fail(FL_INVALID_PARAMETER, "InvalidParameter: precision: unknown mode");
         if (n == 0) return FL_OK;
         if (!dC || !dX || !d_labels_out)

@@ -310,6 +310,7 @@ void knn_search(fl_knn *h, const double *Q, int64_t nq, int k, int mode, double 
 // shapes use the f64 GEMM-form filter (same results).
 int filter_mode(int32_t precision, int64_t d, int64_t k, int64_t n) {
     if (precision == FL_PREC_EXACT) return MODE_EXACT;
+    if (precision == FL_PREC_FP8) precision = FL_PREC_BF16X3;   // FP8 is a KMeans form
     // BF16 kernels index streamed rows with int32.
     if ((precision == FL_PREC_BF16 || precision == FL_PREC_BF16X3) && bf16_nkc(d) > 0 && k <= 24 &&
         n < ((int64_t)1 << 31) - 64)
@@ -384,7 +385,8 @@ fl_status check_dtype(int32_t dtype) {
 }
 
 fl_status check_precision(int32_t p) {
-    if (p != FL_PREC_FP64 && p != FL_PREC_EXACT && p != FL_PREC_BF16 && p != FL_PREC_BF16X3)
+    if (p != FL_PREC_FP64 && p != FL_PREC_EXACT && p != FL_PREC_BF16 && p != FL_PREC_BF16X3 &&
+        p != FL_PREC_FP8)
         return fail(FL_INVALID_PARAMETER, "InvalidParameter: precision: unknown mode");
     return FL_OK;
 }

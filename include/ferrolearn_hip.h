@@ -48,13 +48,19 @@ typedef enum fl_dtype { FL_F32 = 0, FL_F64 = 1 } fl_dtype;
  *   FL_PREC_BF16  : a.b on BF16 MFMA, FP32 accumulation
  *   FL_PREC_BF16X3: a.b as hi*hi + hi*lo + lo*hi BF16 MFMA products
  *   FL_PREC_EXACT : direct f64 fold for every pair (no filter)
+ *   FL_PREC_FP8   : KMeans assignment: a.b on FP8 E4M3 MFMA (per-row
+ *                   power-of-two scales for the samples, one scale for the
+ *                   centres, FP32 accumulation), then the BF16x3 filter for
+ *                   the rows its certificate leaves (d <= 128; otherwise,
+ *                   and for kNN, it behaves as FL_PREC_BF16X3)
  * Results are identical in every mode; only speed and the share of rows
  * that need an exact re-score differ. */
 typedef enum fl_precision {
     FL_PREC_FP64 = 0,
     FL_PREC_BF16 = 1,
     FL_PREC_BF16X3 = 2,
-    FL_PREC_EXACT = 3
+    FL_PREC_EXACT = 3,
+    FL_PREC_FP8 = 4
 } fl_precision;
 
 /* knn.rs:72-79 (Weights) */
@@ -73,6 +79,8 @@ typedef struct fl_report {
     int64_t n_fallback;     /* rows whose certificate failed (full scan)    */
     int64_t n_iter;         /* Lloyd iterations run (KMeans)                */
     int64_t n_dist_launches;/* launches of the distance kernel              */
+    int64_t n_second;       /* KMeans FP8 form: rows the FP8 certificate
+                               left to the BF16x3 filter                    */
 } fl_report;
 
 /* Last error context string of the calling thread ("" after success). */

@@ -64,7 +64,7 @@ def test_filter_path_ties_lowest_index(k):
     assert (got == want).all()
 
 
-@pytest.mark.parametrize("precision", ["bf16x3", "bf16"])
+@pytest.mark.parametrize("precision", ["bf16x3", "bf16", "fp8"])
 def test_bf16_filter_ties_lowest_index(precision):
     # Integer data and duplicated initial centres: exact ties everywhere; the
     # argmin filter keeps the lowest index of equal keys per lane and the
@@ -105,6 +105,10 @@ def test_sklearn_fixture_assign():
     (9_000, 128, 300, np.float32, "bf16x3"),
     (20_000, 20, 30, np.float32, "fp64"),       # f32 centroid sums, 8 lanes per row
     (20_000, 4, 10, np.float32, "bf16x3"),      # one lane per row
+    (40_000, 32, 64, np.float64, "fp8"),        # FP8 E4M3 filter, BF16x3 for what it leaves
+    (30_000, 64, 200, np.float32, "fp8"),
+    (9_000, 128, 300, np.float32, "fp8"),
+    (12_000, 130, 40, np.float32, "fp8"),       # d > 128: the FP8 form runs as BF16x3
 ])
 def test_lloyd_matches_oracle(n, d, k, dtype, precision):
     X = blobs(n, d, k, seed=n + k, spread=1.0, dtype=dtype)
@@ -231,13 +235,16 @@ def config3_case():
     return X, init, (c_o, l_o, i_o, it_o)
 
 
-@pytest.mark.parametrize("precision", ["bf16x3", "fp64"])
+@pytest.mark.parametrize("precision", ["bf16x3", "fp64", "fp8"])
 def test_config3_shape_k1024(config3_case, precision):
     # k = 1,024 on the large-KMeans workload's data: six Lloyd iterations with
     # tol 0 from the same initial centres; labels, n_iter and centres bit for
     # bit, inertia to 1e-12.
     X, init, (c_o, l_o, i_o, it_o) = config3_case
     f = fit_lib(F.to_device(X), init, 6, 0.0, precision)
+    if precision == "fp8":
+        print(f"fp8 at k=1024: {f.report['n_second']} of {6 * X.shape[0]} row-iterations "
+              f"left to BF16x3, {f.report['n_fallback']} to the exact scan")
     assert f.n_iter() == it_o == 6
     assert (f.labels().numpy() == l_o).all()
     assert (bits(f.cluster_centers().numpy()) == bits(c_o)).all()
