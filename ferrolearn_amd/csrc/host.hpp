@@ -13,6 +13,7 @@
 
 #include "common.hpp"
 #include "dist_bf16.hpp"
+#include "dist_fp8.hpp"
 #include "dist_f64.hpp"
 #include "dist_topm.hpp"
 
@@ -372,6 +373,20 @@ inline int device_cu_count() {
         cache[dev] = n > 0 ? n : 256;
     }
     return cache[dev];
+}
+
+// FP8 argmin with every centre in LDS (k <= FR_MAXK): one block per compute
+// unit (its LDS holds the centres), at most one wave-group per 32 samples.
+inline void launch_fp8_res(const Bf16Args &a, hipStream_t st) {
+    if (a.nB > FR_MAXK) throw DeviceError("internal: too many centres for the LDS-resident filter");
+    const size_t shmem = fp8_res_shmem(a.nB);
+    FL_HIP(hipFuncSetAttribute((const void *)dist_fp8_res_kernel,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
+    const int64_t blocks =
+        std::max<int64_t>(1, std::min<int64_t>(device_cu_count(), ceil_div(ceil_div(a.nA, 32), FR_WAVES)));
+    hipLaunchKernelGGL(dist_fp8_res_kernel, dim3((unsigned)blocks), dim3(FR_WAVES * 64), shmem, st,
+                       a);
+    check_launch();
 }
 
 // Number of splits of the streamed rows for the BF16 filter.  A block holds

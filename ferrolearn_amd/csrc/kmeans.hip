@@ -243,7 +243,10 @@ class Lloyd {
                 a8.fcount = fcnt8;
                 FL_HIP(hipMemsetAsync(fcnt8, 0, sizeof(int32_t), st_));
                 const int t8 = tbegin();
-                launch_bf16(0, 0, 2, a8, 1, st_);
+                if (k_ <= FR_MAXK)
+                    launch_fp8_res(a8, st_);   // every centre resident in LDS
+                else
+                    launch_bf16(0, 0, 2, a8, 1, st_);
                 tend(t8, &acc.ms_dist);
                 acc.n_launch += 1;
             }
