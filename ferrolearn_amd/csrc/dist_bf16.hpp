@@ -192,19 +192,27 @@ __host__ __device__ __forceinline__ int fp8_scale_exp(double mx) {
 }
 
 // Rows to E4M3 bytes (128 per row, zero padded) with per-row scale
-// exponents: one thread per row.
+// exponents (d <= 128): one wave per row, lane j converts elements 2j and
+// 2j+1 after a wave-wide max of the row's magnitudes.  Grid: ceil(n / 4)
+// blocks of 256.
 template <typename T>
-__global__ void fp8_split_rows_kernel(const T *__restrict__ X, int64_t n, int d,
-                                      uint8_t *__restrict__ out, int32_t *__restrict__ sexp) {
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
+__global__ __launch_bounds__(256) void fp8_split_rows_kernel(const T *__restrict__ X, int64_t n,
+                                                             int d, uint8_t *__restrict__ out,
+                                                             int32_t *__restrict__ sexp) {
+    const int lane = threadIdx.x & 63;
+    const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= n) return;   // the whole wave
     const T *x = X + r * d;
-    double mx = 0.0;
-    for (int j = 0; j < d; ++j) mx = fmax(mx, fabs((double)x[j]));
+    const int j0 = 2 * lane;
+    const double a = j0 < d ? (double)x[j0] : 0.0;
+    const double b = j0 + 1 < d ? (double)x[j0 + 1] : 0.0;
+    double mx = fmax(fabs(a), fabs(b));
+    for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
     const int E = fp8_scale_exp(mx);
-    uint8_t *o = out + r * 128;
-    for (int j = 0; j < 128; ++j) o[j] = j < d ? fp8_e4m3_rne(ldexp((double)x[j], -E)) : 0;
-    sexp[r] = E;
+    const uint16_t v = (uint16_t)(fp8_e4m3_rne(ldexp(a, -E)) |
+                                  ((uint16_t)fp8_e4m3_rne(ldexp(b, -E)) << 8));
+    reinterpret_cast<uint16_t *>(out + r * 128)[lane] = v;
+    if (lane == 0) sexp[r] = E;
 }
 
 // Rows to E4M3 bytes with one scale exponent for all of them (*sexp).

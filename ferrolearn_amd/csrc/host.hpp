@@ -98,7 +98,9 @@ class Workspace {
     int nslots_ = 0;
 };
 
-// Accumulates kernel time between two events recorded on one stream.
+// Accumulates kernel time between two events recorded on one stream.  The
+// event pairs are pooled: release(id) returns a pair once its time has been
+// read (or is no longer wanted), so a loop of phases creates its events once.
 class PhaseTimer {
   public:
     explicit PhaseTimer(hipStream_t s) : s_(s) {}
@@ -109,12 +111,19 @@ class PhaseTimer {
         }
     }
     int begin() {
-        hipEvent_t a, b;
-        FL_HIP(hipEventCreate(&a));
-        FL_HIP(hipEventCreate(&b));
-        pairs_.push_back({a, b});
-        FL_HIP(hipEventRecord(a, s_));
-        return (int)pairs_.size() - 1;
+        int id;
+        if (free_.empty()) {
+            hipEvent_t a, b;
+            FL_HIP(hipEventCreate(&a));
+            FL_HIP(hipEventCreate(&b));
+            pairs_.push_back({a, b});
+            id = (int)pairs_.size() - 1;
+        } else {
+            id = free_.back();
+            free_.pop_back();
+        }
+        FL_HIP(hipEventRecord(pairs_[id].first, s_));
+        return id;
     }
     void end(int id) { FL_HIP(hipEventRecord(pairs_[id].second, s_)); }
     double ms(int id) {
@@ -123,10 +132,44 @@ class PhaseTimer {
         FL_HIP(hipEventElapsedTime(&t, pairs_[id].first, pairs_[id].second));
         return (double)t;
     }
+    void release(int id) { free_.push_back(id); }
 
   private:
     hipStream_t s_;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pairs_;
+    std::vector<int> free_;
+};
+
+// Zeroed page-locked host array (asynchronous device-to-host copies).
+template <typename T>
+class PinnedHost {
+  public:
+    explicit PinnedHost(size_t n) {
+        FL_HIP(hipHostMalloc((void **)&p_, n * sizeof(T), hipHostMallocDefault));
+        std::memset((void *)p_, 0, n * sizeof(T));
+    }
+    ~PinnedHost() { (void)hipHostFree(p_); }
+    PinnedHost(const PinnedHost &) = delete;
+    PinnedHost &operator=(const PinnedHost &) = delete;
+    T &operator[](size_t i) { return p_[i]; }
+
+  private:
+    T *p_ = nullptr;
+};
+
+// Two events, destroyed with the object.
+struct EventPair {
+    hipEvent_t e[2] = {nullptr, nullptr};
+    EventPair() {
+        FL_HIP(hipEventCreate(&e[0]));
+        FL_HIP(hipEventCreate(&e[1]));
+    }
+    ~EventPair() {
+        for (auto x : e)
+            if (x) (void)hipEventDestroy(x);
+    }
+    EventPair(const EventPair &) = delete;
+    EventPair &operator=(const EventPair &) = delete;
 };
 
 struct WallClock {

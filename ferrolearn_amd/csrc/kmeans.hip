@@ -107,8 +107,8 @@ class Lloyd {
                     fp8_ = true;
                     x8_ = ws_.get<uint8_t>("x8", n * 128);
                     sx_ = ws_.get<int32_t>("sx", n);
-                    hipLaunchKernelGGL(fp8_split_rows_kernel<TX>, dim3(grid1(n, 256)), dim3(256),
-                                       0, st, X, n, d, x8_, sx_);
+                    hipLaunchKernelGGL(fp8_split_rows_kernel<TX>, dim3((unsigned)ceil_div(n, 4)),
+                                       dim3(256), 0, st, X, n, d, x8_, sx_);
                     check_launch();
                 }
             } else if (f64_d4(d) > 0 && fits && (hflag & 2u) == 0) {
@@ -138,17 +138,36 @@ class Lloyd {
     void flush(int64_t *n_fb = nullptr, int64_t *n_second = nullptr) {
         if (n_fb) {
             int32_t c[2] = {0, 0};
-            FL_HIP(hipMemcpyAsync(&c[0], ws_.get<int32_t>("fcnt", 1), 4, hipMemcpyDeviceToHost, st_));
-            if (fp8_ && n_second)
-                FL_HIP(hipMemcpyAsync(&c[1], ws_.get<int32_t>("fcnt8", 1), 4,
-                                      hipMemcpyDeviceToHost, st_));
+            copy_counts(c);
             FL_HIP(hipStreamSynchronize(st_));
             *n_fb += c[0];
             if (n_second) *n_second += c[1];
         }
-        for (const auto &p : pend_) *p.second += tm_->ms(p.first);
-        pend_.clear();
-        tm_.reset();
+        flush_upto(pend_.size());
+    }
+    // Enqueue copies of the last assignment's counts into h[0] (rows of the
+    // exact scan) and h[1] (rows the FP8 filter left; untouched otherwise).
+    // h must be pinned host memory for the copies to stay asynchronous.
+    void copy_counts(int32_t *h) {
+        FL_HIP(hipMemcpyAsync(&h[0], ws_.get<int32_t>("fcnt", 1), 4, hipMemcpyDeviceToHost, st_));
+        if (fp8_)
+            FL_HIP(hipMemcpyAsync(&h[1], ws_.get<int32_t>("fcnt8", 1), 4, hipMemcpyDeviceToHost,
+                                  st_));
+    }
+    // Pending phase timers: mark() names the position after the phases
+    // enqueued so far; flush_upto(m) adds the first m to their slots (waiting
+    // only for those), drop_from(m) forgets those after m.
+    size_t mark() const { return pend_.size(); }
+    void flush_upto(size_t m) {
+        for (size_t i = 0; i < m; ++i) {
+            *pend_[i].second += tm_->ms(pend_[i].first);
+            tm_->release(pend_[i].first);
+        }
+        pend_.erase(pend_.begin(), pend_.begin() + (std::ptrdiff_t)m);
+    }
+    void drop_from(size_t m) {
+        for (size_t i = m; i < pend_.size(); ++i) tm_->release(pend_[i].first);
+        pend_.resize(m);
     }
 
     void assign(const double *C, int32_t *labels, double *best, KmAcc &acc) {
@@ -662,42 +681,77 @@ fl_status fit_device(const TX *X, int64_t n, int d, const fl_kmeans_params *p,
     const int64_t k = p->n_clusters;
     Lloyd<TX> L(X, n, d, k, p->precision, st);
     Workspace &ws = L.ws();
-    double *cur = ws.get<double>("cur", k * d);
-    double *nxt = ws.get<double>("nxt", k * d);
-    int32_t *labels = ws.get<int32_t>("labels", n);
+    // Iteration i reads centres C[i&1] and writes C[(i+1)&1], labels lab[i&1]
+    // and scalars [inertia, shift] at sc + 2 (i&1).  Iteration i+1 is
+    // enqueued before the host waits for iteration i's shift (through pinned
+    // memory), so the device never idles on the convergence test; an
+    // iteration enqueued after convergence touches none of the buffers the
+    // result is read from, and its counts and phase times are discarded.
+    double *C[2] = {ws.get<double>("cur", k * d), ws.get<double>("nxt", k * d)};
+    int32_t *lab[2] = {ws.get<int32_t>("labels", n), ws.get<int32_t>("labels2", n)};
     double *best = ws.get<double>("best", n);
-    double *dsc = ws.get<double>("scalars", 2);   // [inertia, shift]
-    FL_HIP(hipMemsetAsync(labels, 0, (size_t)n * 4, st));
+    double *sc = ws.get<double>("scalars", 4);
+    FL_HIP(hipMemsetAsync(lab[0], 0, (size_t)n * 4, st));
+    struct IterOut {
+        double inertia, shift;
+        int32_t counts[2];
+    };
+    PinnedHost<IterOut> hout(2);   // zeroed: counts[1] stays 0 without the FP8 filter
+    EventPair ev;
     const uint64_t base = p->has_random_state ? p->random_state : 0;
     KmAcc acc;
     double best_inertia = 0.0;
     int64_t best_iter = 0;
     for (int64_t run = 0; run < p->n_init; ++run) {
         if (d_init)
-            FL_HIP(hipMemcpyAsync(cur, d_init + run * k * d, (size_t)(k * d) * 8,
+            FL_HIP(hipMemcpyAsync(C[0], d_init + run * k * d, (size_t)(k * d) * 8,
                                   hipMemcpyDeviceToDevice, st));
         else
-            L.kmeanspp(base + (uint64_t)run, cur);
+            L.kmeanspp(base + (uint64_t)run, C[0]);
         double inertia = 1.7976931348623157e308;
         int64_t n_iter = 0;
-        for (int64_t it = 0; it < p->max_iter; ++it) {
-            L.assign(cur, labels, best, acc);
-            L.inertia(best, dsc);
-            L.update(labels, cur, nxt, dsc + 1, acc);
-            double h[2];
-            FL_HIP(hipMemcpyAsync(h, dsc, 16, hipMemcpyDeviceToHost, st));
-            L.flush(&acc.n_fb, &acc.n_second);   // the iteration's one synchronisation
-            inertia = h[0];
-            std::swap(cur, nxt);
-            n_iter = it + 1;
-            if (h[1] < p->tol) break;
+        for (int64_t it = 0;; ++it) {
+            const bool enq = it < p->max_iter;
+            const KmAcc before = acc;
+            const size_t mk = L.mark();
+            if (enq) {
+                const int s = (int)(it & 1);
+                L.assign(C[s], lab[s], best, acc);
+                L.inertia(best, sc + 2 * s);
+                L.update(lab[s], C[s], C[s ^ 1], sc + 2 * s + 1, acc);
+                L.copy_counts(hout[s].counts);
+                FL_HIP(hipMemcpyAsync(&hout[s].inertia, sc + 2 * s, 16, hipMemcpyDeviceToHost, st));
+                FL_HIP(hipEventRecord(ev.e[s], st));
+            }
+            if (it == 0) {
+                if (!enq) break;   // max_iter == 0
+                continue;
+            }
+            // iteration it-1 is complete once its event is
+            const int ps = (int)((it - 1) & 1);
+            FL_HIP(hipEventSynchronize(ev.e[ps]));
+            L.flush_upto(mk);
+            acc.n_fb += hout[ps].counts[0];
+            acc.n_second += hout[ps].counts[1];
+            inertia = hout[ps].inertia;
+            n_iter = it;
+            if (hout[ps].shift < p->tol || !enq) {
+                if (enq) {   // discard iteration it
+                    L.drop_from(0);
+                    acc.n_rows = before.n_rows;
+                    acc.n_cand = before.n_cand;
+                    acc.n_launch = before.n_launch;
+                }
+                break;
+            }
         }
         if (run == 0 || inertia < best_inertia) {
             best_inertia = inertia;
             best_iter = n_iter;
-            FL_HIP(hipMemcpyAsync(d_centers, cur, (size_t)(k * d) * 8, hipMemcpyDeviceToDevice,
-                                  st));
-            hipLaunchKernelGGL(labels_to_i64_kernel, dim3(grid1(n, 256)), dim3(256), 0, st, labels,
+            FL_HIP(hipMemcpyAsync(d_centers, C[n_iter & 1], (size_t)(k * d) * 8,
+                                  hipMemcpyDeviceToDevice, st));
+            const int32_t *fl = n_iter > 0 ? lab[(n_iter - 1) & 1] : lab[0];
+            hipLaunchKernelGGL(labels_to_i64_kernel, dim3(grid1(n, 256)), dim3(256), 0, st, fl,
                                n, d_labels);
             check_launch();
         }
