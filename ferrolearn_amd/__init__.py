@@ -1,7 +1,7 @@
 """ferrolearn_amd -- MI355X (gfx950) squared-Euclidean path of ferrolearn.
 
 KMeans Lloyd (assignment, inertia, centroid update) and brute-force
-KNeighborsClassifier / KNeighborsRegressor predict on hand-written HIP
+KNeighborsClassifier / KNeighborsRegressor / NearestNeighbors queries on hand-written HIP
 kernels behind the C ABI in include/ferrolearn_hip.h.  Labels, iteration
 counts and neighbour indices match the reference's f64 CPU path exactly.
 """
@@ -10,8 +10,10 @@ from .cluster import FittedKMeans, KMeans, MiniBatchKMeans, kmeans_fit_batched, 
 from .device import DeviceArray, to_device
 from .errors import (DeviceError, FerroError, InsufficientSamples, InvalidParameter,
                      NumericalInstability, ShapeMismatch)
-from .neighbors import (Algorithm, FittedKNeighborsClassifier, FittedKNeighborsRegressor,
-                        KNeighborsClassifier, KNeighborsRegressor, Weights)
+from .neighbors import (Algorithm, CsrMatrix, FittedKNeighborsClassifier,
+                        FittedKNeighborsRegressor, FittedNearestNeighbors, GraphMode,
+                        KNeighborsClassifier, KNeighborsRegressor, NearestNeighbors, Weights,
+                        kneighbors_graph)
 from .router import CellRouter
 
 __all__ = [
@@ -19,5 +21,6 @@ __all__ = [
     "KNeighborsRegressor", "FittedKNeighborsClassifier", "FittedKNeighborsRegressor",
     "Algorithm", "Weights", "FerroError", "ShapeMismatch", "InsufficientSamples",
     "InvalidParameter", "NumericalInstability", "DeviceError", "DeviceArray", "to_device",
-    "device", "CellRouter", "kmeans_fit_batched",
+    "device", "CellRouter", "kmeans_fit_batched", "NearestNeighbors", "FittedNearestNeighbors",
+    "GraphMode", "CsrMatrix", "kneighbors_graph",
 ]

@@ -29,6 +29,8 @@ FL_PREC_FP8 = 4
 
 FL_UNIFORM = 0
 FL_DISTANCE = 1
+FL_GRAPH_CONNECTIVITY = 0
+FL_GRAPH_DISTANCE = 1
 
 PRECISIONS = {"fp64": FL_PREC_FP64, "bf16": FL_PREC_BF16, "bf16x3": FL_PREC_BF16X3,
               "exact": FL_PREC_EXACT, "fp8": FL_PREC_FP8}
@@ -190,6 +192,11 @@ SIGNATURES = {
     "fl_knn_predict_proba": (C.c_int, [_vp, _vp, _i32, _i64, _i64, _vp, _rp]),
     "fl_knn_kneighbors": (C.c_int, [_vp, _vp, _i32, _i64, _i64, _i64, _vp, _vp, _rp]),
     "fl_knn_kneighbors_device": (C.c_int, [_vp, _vp, _i32, _i64, _i64, _i64, _vp, _vp, _rp]),
+    "fl_nn_fit": (C.c_int, [_vp, _i32, _i64, _i64, _i64, _i32, C.POINTER(_vp)]),
+    "fl_knn_kneighbors_graph": (C.c_int, [_vp, _vp, _i32, _i64, _i64, _i64, _i32, _vp, _vp, _vp,
+                                          _rp]),
+    "fl_knn_kneighbors_graph_device": (C.c_int, [_vp, _vp, _i32, _i64, _i64, _i64, _i32, _vp, _vp,
+                                                 _vp, _rp]),
     "fl_knn_predict_classify_device": (C.c_int, [_vp, _vp, _i32, _i64, _i64, _vp, _rp]),
     "fl_knn_predict_regress_device": (C.c_int, [_vp, _vp, _i32, _i64, _i64, _vp, _rp]),
     "fl_knn_set_precision": (C.c_int, [_vp, _i32]),

@@ -470,6 +470,7 @@ template <>
 inline const double *as_f64<float>(const float *X, int64_t count, Workspace &ws, const char *slot,
                                    hipStream_t st) {
     double *Y = ws.get<double>(slot, count);
+    if (count == 0) return Y;   // no launch with an empty grid
     hipLaunchKernelGGL(to_f64_kernel<float>, dim3(grid1(count, 256)), dim3(256), 0, st, X, count,
                        Y);
     check_launch();

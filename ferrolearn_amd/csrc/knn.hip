@@ -10,12 +10,15 @@
 #include <string>
 #include <vector>
 
+#include <hipcub/hipcub.hpp>
+
 #include "host.hpp"
 
 using namespace fl;
 
 struct fl_knn {
     int is_classifier = 0;
+    int unsupervised = 0;          // NearestNeighbors (no targets)
     int32_t dtype = FL_F64;
     int64_t n = 0, d = 0, n_neighbors = 5;
     int32_t weights = FL_UNIFORM;
@@ -45,7 +48,7 @@ struct fl_knn {
 
 namespace {
 
-constexpr int64_t kMaxNeighbors = 64;
+constexpr int64_t kMaxFilterK = 64;   // largest k of the filter kernels' lists
 
 std::string shape_msg(int64_t expected, int64_t actual, const char *ctx) {
     return "ShapeMismatch: expected [" + std::to_string(expected) + "], actual [" +
@@ -87,6 +90,111 @@ __global__ void scatter_results_kernel(const double *__restrict__ dsrc,
     const int64_t r = t / k, m = t % k;
     dist[(int64_t)list[r] * k + m] = dsrc[t];
     idx[(int64_t)list[r] * k + m] = isrc[t];
+}
+
+// ---------------------------------------------------------------------------
+// Exact top-k by sorting every distance (any k <= n; also the path for the
+// few queries a filter leaves, which it spreads over the whole device).
+//
+// Keys are the bits of sqrt(fold) -- non-negative doubles order as their bit
+// patterns, NaN after +inf -- and values the row positions; a stable
+// segmented radix sort per query leaves equal distances in index order, so
+// the first k of a segment are the reference's k smallest in (distance,
+// index) order (kdtree.rs:364-391, ties as the oracle defines them).
+// ---------------------------------------------------------------------------
+
+template <typename TB>
+__global__ void knn_all_dist_kernel(const double *__restrict__ Q, const TB *__restrict__ X,
+                                    int64_t N, int d, uint64_t *__restrict__ keys,
+                                    int *__restrict__ pos) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t qq = blockIdx.y;
+    if (i >= N) return;
+    const double f = fold_exact(X + i * d, Q + qq * d, d);
+    keys[qq * N + i] = (uint64_t)__double_as_longlong(sqrt(f));
+    pos[qq * N + i] = (int)i;
+}
+
+__global__ void knn_seg_offsets_kernel(int64_t qc, int64_t N, int *__restrict__ offs) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t <= qc) offs[t] = (int)(t * N);
+}
+
+// First k of each sorted segment into rows q0.. of the outputs; a NaN
+// distance (sorted last) sets *nan.
+__global__ void knn_take_sorted_kernel(const uint64_t *__restrict__ keys,
+                                       const int *__restrict__ pos, int64_t qc, int64_t N, int k,
+                                       double *__restrict__ dist, int64_t *__restrict__ idx,
+                                       int *__restrict__ nan) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= qc * k) return;
+    const int64_t qq = t / k, m = t % k;
+    const int64_t p = qq * N + m;
+    dist[t] = __longlong_as_double((long long)keys[p]);
+    idx[t] = pos[p];
+    if (m == 0) {
+        const double last = __longlong_as_double((long long)keys[qq * N + N - 1]);
+        if (last != last) atomicOr(nan, 1);
+    }
+}
+
+// Queries per sort: about 64M keys (about 40 bytes each with the sort's buffers).
+// At most 65535 (the grid's y extent).
+inline int64_t sorted_chunk(int64_t N) {
+    return std::min<int64_t>(65535,
+                             std::max<int64_t>(1, ((int64_t)1 << 26) / std::max<int64_t>(N, 1)));
+}
+
+template <typename TB>
+void knn_sorted(fl_knn *h, const double *Q, int64_t nq, int k, double *dist, int64_t *idx,
+                Acc &acc, bool fallback) {
+    if (nq == 0) return;
+    hipStream_t st = h->stream;
+    Workspace &ws = h->ws;
+    const int64_t N = h->n;
+    const int d = (int)h->d;
+    const int64_t chunk = std::min<int64_t>(nq, sorted_chunk(N));
+    const int64_t cap = chunk * N;
+    uint64_t *kin = ws.get<uint64_t>("srt_kin", cap);
+    uint64_t *kout = ws.get<uint64_t>("srt_kout", cap);
+    int *pin = ws.get<int>("srt_pin", cap);
+    int *pout = ws.get<int>("srt_pout", cap);
+    int *offs = ws.get<int>("srt_offs", chunk + 1);
+    int *nan = ws.get<int>("srt_nan", 1);
+    size_t tmp_bytes = 0;
+    FL_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(
+        nullptr, tmp_bytes, kin, kout, pin, pout, (int)cap,
+        (int)chunk, offs, offs + 1, 0, 64, st));
+    void *tmp = ws.get<unsigned char>("srt_tmp", (int64_t)tmp_bytes + 256);
+    FL_HIP(hipMemsetAsync(nan, 0, 4, st));
+    PhaseTimer tm(st);
+    const int t0 = tm.begin();
+    for (int64_t q0 = 0; q0 < nq; q0 += chunk) {
+        const int64_t qc = std::min<int64_t>(chunk, nq - q0);
+        hipLaunchKernelGGL(knn_all_dist_kernel<TB>, dim3((unsigned)grid1(N, 256), (unsigned)qc),
+                           dim3(256), 0, st, Q + q0 * d, (const TB *)h->X.ptr(), N, d, kin, pin);
+        hipLaunchKernelGGL(knn_seg_offsets_kernel, dim3(grid1(qc + 1, 256)), dim3(256), 0, st, qc,
+                           N, offs);
+        check_launch();
+        size_t tb = tmp_bytes;
+        FL_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(
+            tmp, tb, kin, kout, pin, pout, (int)(qc * N),
+            (int)qc, offs, offs + 1, 0, 64, st));
+        hipLaunchKernelGGL(knn_take_sorted_kernel, dim3(grid1(qc * k, 256)), dim3(256), 0, st,
+                           kout, pout, qc, N, k, dist + q0 * k, idx + q0 * k, nan);
+        check_launch();
+    }
+    tm.end(t0);
+    int hn = 0;
+    FL_HIP(hipMemcpyAsync(&hn, nan, 4, hipMemcpyDeviceToHost, st));
+    FL_HIP(hipStreamSynchronize(st));
+    (fallback ? acc.ms_fb : acc.ms_dist) += tm.ms(t0);
+    acc.n_cand += nq * N;
+    acc.n_launch += 1;
+    if (hn)
+        throw StatusError{FL_NUMERICAL_INSTABILITY,
+                          "NumericalInstability: a distance is NaN (the reference panics "
+                          "here, kdtree.rs:386)"};
 }
 
 // Exact top-k of nq f64 queries against the fitted rows.  mode selects the
@@ -300,7 +408,12 @@ void knn_search(fl_knn *h, const double *Q, int64_t nq, int k, int mode, double 
     check_launch();
     double *df = ws.get<double>(fp + "d", (int64_t)nflag * k);
     int64_t *xf = ws.get<int64_t>(fp + "i", (int64_t)nflag * k);
-    knn_search<TB>(h, Qf, nflag, k, next_mode, df, xf, acc, depth + 1);
+    // A few queries (up to 2^24 distances) are sorted exactly over the whole
+    // device; more go through the next filter, whose grid follows the queries.
+    if ((int64_t)nflag * h->n <= ((int64_t)1 << 24))
+        knn_sorted<TB>(h, Qf, nflag, k, df, xf, acc, true);
+    else
+        knn_search<TB>(h, Qf, nflag, k, next_mode, df, xf, acc, depth + 1);
     hipLaunchKernelGGL(scatter_results_kernel, dim3(grid1((int64_t)nflag * k, 256)), dim3(256), 0,
                        st, df, xf, flags, (int64_t)nflag, k, dist, idx);
     check_launch();
@@ -324,6 +437,13 @@ void search_device(fl_knn *h, const void *dQ, int32_t dtype, int64_t nq, int k, 
     const double *Q = dtype == FL_F32
                           ? as_f64<float>((const float *)dQ, nq * h->d, h->ws, "q64", h->stream)
                           : (const double *)dQ;
+    if (k > kMaxFilterK) {   // beyond the filters' candidate lists
+        if (h->dtype == FL_F32)
+            knn_sorted<float>(h, Q, nq, k, dist, idx, acc, false);
+        else
+            knn_sorted<double>(h, Q, nq, k, dist, idx, acc, false);
+        return;
+    }
     int mode = filter_mode(h->precision, h->d, k, h->n);
     if (mode != MODE_EXACT) {
         // Range of the data: the BF16 bound needs finite values below 2^56 in
@@ -408,9 +528,8 @@ fl_status knn_fit_common(const void *X, int32_t dtype, int64_t n, int64_t d, int
                     "InsufficientSamples: required " + std::to_string(n_neighbors) + ", actual " +
                         std::to_string(n) + ": " + what + " requires at least n_neighbors=" +
                         std::to_string(n_neighbors) + " samples");
-    if (n_neighbors < 0 || n_neighbors > kMaxNeighbors)
-        return fail(FL_INVALID_PARAMETER,
-                    "InvalidParameter: n_neighbors: the GPU brute-force path supports 1..64");
+    if (n_neighbors < 0)
+        return fail(FL_INVALID_PARAMETER, "InvalidParameter: n_neighbors: must be at least 1");
     if (d <= 0 || d > (1 << 20))
         return fail(FL_INVALID_PARAMETER, "InvalidParameter: n_features: must be in 1..2^20");
     if (n > 0x7fffffffLL)
@@ -458,6 +577,28 @@ fl_status check_query(fl_knn *h, const void *Q, int32_t dtype, int64_t nq, int64
                     shape_msg(h->d, d, "number of features must match training data"));
     if (nq < 0) return fail(FL_INVALID_PARAMETER, "InvalidParameter: n_queries: negative");
     if (nq > 0 && !Q) return fail(FL_INVALID_PARAMETER, "InvalidParameter: Q: must not be NULL");
+    return FL_OK;
+}
+
+// k of a kneighbors call after check_query: 0 means the fitted n_neighbors,
+// a negative k (a caller's explicit zero) and k == 0 after defaulting are
+// InvalidParameter; k > n is InsufficientSamples for the supervised models
+// (knn.rs:826-834) and InvalidParameter for NearestNeighbors
+// (nearest_neighbors.rs:274-287).
+fl_status check_k(fl_knn *h, int64_t &k) {
+    if (k == 0) k = h->n_neighbors;
+    if (k <= 0)
+        return fail(FL_INVALID_PARAMETER, "InvalidParameter: n_neighbors: must be at least 1");
+    if (k > h->n) {
+        if (h->unsupervised)
+            return fail(FL_INVALID_PARAMETER, "InvalidParameter: n_neighbors: n_neighbors=" +
+                                                  std::to_string(k) +
+                                                  " exceeds number of training samples=" +
+                                                  std::to_string(h->n));
+        return fail(FL_INSUFFICIENT_SAMPLES,
+                    "InsufficientSamples: required " + std::to_string(k) + ", actual " +
+                        std::to_string(h->n) + ": n_neighbors exceeds number of training samples");
+    }
     return FL_OK;
 }
 
@@ -533,6 +674,87 @@ fl_status regress_dev(fl_knn *h, const void *dQ, int32_t dtype, int64_t nq, doub
     return FL_OK;
 }
 
+// ---------------------------------------------------------------------------
+// kneighbors_graph (graph.rs:91-123): row i of the CSR holds the k neighbours
+// of query i with their columns ascending, valued 1 (connectivity) or the
+// distance.  Columns of one row are distinct, so the order is unique.
+// ---------------------------------------------------------------------------
+
+__global__ void graph_pairs_kernel(const int64_t *__restrict__ idx,
+                                   const double *__restrict__ dist, int64_t total, int mode,
+                                   int *__restrict__ cols, double *__restrict__ vals) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    cols[t] = (int)idx[t];
+    vals[t] = mode == FL_GRAPH_DISTANCE ? dist[t] : 1.0;
+}
+
+__global__ void graph_offsets_kernel(int64_t rows, int k, int *__restrict__ offs) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t <= rows) offs[t] = (int)(t * k);
+}
+
+__global__ void graph_out_kernel(const int *__restrict__ cols, int64_t q0, int64_t rows,
+                                 int k, int64_t *__restrict__ indptr,
+                                 int64_t *__restrict__ indices) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < rows * k) indices[q0 * k + t] = cols[t];
+    if (t <= rows) indptr[q0 + t] = (q0 + t) * k;   // chunk ends written twice, same value
+}
+
+fl_status kneighbors_graph_dev(fl_knn *h, const void *dQ, int32_t dtype, int64_t nq, int64_t k,
+                               int32_t mode, int64_t *d_indptr, int64_t *d_indices,
+                               double *d_data, fl_report *rep) {
+    WallClock wc;
+    Acc acc;
+    hipStream_t st = h->stream;
+    Workspace &ws = h->ws;
+    double *dist = ws.get<double>("gdist", nq * k);
+    int64_t *idx = ws.get<int64_t>("gidx", nq * k);
+    search_device(h, dQ, dtype, nq, (int)k, dist, idx, acc);
+    if (nq == 0) {
+        hipLaunchKernelGGL(graph_out_kernel, dim3(1), dim3(64), 0, st, (const int *)nullptr,
+                           (int64_t)0, (int64_t)0, (int)k, d_indptr, d_indices);
+        check_launch();
+    }
+    int bits = 1;
+    while (bits < 31 && ((int64_t)1 << bits) <= h->n) ++bits;
+    const int64_t rows_per = std::max<int64_t>(1, ((int64_t)1 << 30) / k);
+    for (int64_t q0 = 0; q0 < nq; q0 += rows_per) {
+        const int64_t rows = std::min<int64_t>(rows_per, nq - q0);
+        const int64_t tot = rows * k;
+        int *cols = ws.get<int>("gcols", tot);
+        int *cols_s = ws.get<int>("gcols_s", tot);
+        double *vals = ws.get<double>("gvals", tot);
+        int *offs = ws.get<int>("goffs", rows + 1);
+        hipLaunchKernelGGL(graph_pairs_kernel, dim3(grid1(tot, 256)), dim3(256), 0, st,
+                           idx + q0 * k, dist + q0 * k, tot, mode, cols, vals);
+        hipLaunchKernelGGL(graph_offsets_kernel, dim3(grid1(rows + 1, 256)), dim3(256), 0, st,
+                           rows, (int)k, offs);
+        check_launch();
+        size_t tb = 0;
+        FL_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(nullptr, tb, cols, cols_s, vals,
+                                                           d_data + q0 * k, (int)tot, (int)rows,
+                                                           offs, offs + 1, 0, bits, st));
+        void *tmp = ws.get<unsigned char>("gtmp", (int64_t)tb + 256);
+        FL_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(tmp, tb, cols, cols_s, vals,
+                                                           d_data + q0 * k, (int)tot, (int)rows,
+                                                           offs, offs + 1, 0, bits, st));
+        hipLaunchKernelGGL(graph_out_kernel, dim3(grid1(std::max(tot, rows + 1), 256)), dim3(256),
+                           0, st, cols_s, q0, rows, (int)k, d_indptr, d_indices);
+        check_launch();
+    }
+    FL_HIP(hipStreamSynchronize(st));
+    fill_report(rep, acc, nq, wc.ms());
+    return FL_OK;
+}
+
+fl_status check_graph_mode(int32_t mode) {
+    if (mode != FL_GRAPH_CONNECTIVITY && mode != FL_GRAPH_DISTANCE)
+        return fail(FL_INVALID_PARAMETER, "InvalidParameter: mode: unknown graph mode");
+    return FL_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -596,17 +818,7 @@ fl_status fl_knn_kneighbors_device(fl_knn_handle h, const void *dQ, int32_t dtyp
                                    fl_report *report) {
     return guarded([&]() -> fl_status {
         if (fl_status s = check_query(h, dQ, dtype, nq, d)) return s;
-        if (k == 0) k = h->n_neighbors;
-        if (k <= 0)
-            return fail(FL_INVALID_PARAMETER, "InvalidParameter: n_neighbors: must be at least 1");
-        if (k > h->n)
-            return fail(FL_INSUFFICIENT_SAMPLES,
-                        "InsufficientSamples: required " + std::to_string(k) + ", actual " +
-                            std::to_string(h->n) +
-                            ": n_neighbors exceeds number of training samples");
-        if (k > kMaxNeighbors)
-            return fail(FL_INVALID_PARAMETER,
-                        "InvalidParameter: n_neighbors: the GPU brute-force path supports 1..64");
+        if (fl_status s = check_k(h, k)) return s;
         return kneighbors_dev(h, dQ, dtype, nq, k, d_dist_out, d_idx_out, report);
     });
 }
@@ -615,17 +827,7 @@ fl_status fl_knn_kneighbors(fl_knn_handle h, const void *Q, int32_t dtype, int64
                             int64_t k, double *dist_out, int64_t *idx_out, fl_report *report) {
     return guarded([&]() -> fl_status {
         if (fl_status s = check_query(h, Q, dtype, nq, d)) return s;
-        if (k == 0) k = h->n_neighbors;
-        if (k <= 0)
-            return fail(FL_INVALID_PARAMETER, "InvalidParameter: n_neighbors: must be at least 1");
-        if (k > h->n)
-            return fail(FL_INSUFFICIENT_SAMPLES,
-                        "InsufficientSamples: required " + std::to_string(k) + ", actual " +
-                            std::to_string(h->n) +
-                            ": n_neighbors exceeds number of training samples");
-        if (k > kMaxNeighbors)
-            return fail(FL_INVALID_PARAMETER,
-                        "InvalidParameter: n_neighbors: the GPU brute-force path supports 1..64");
+        if (fl_status s = check_k(h, k)) return s;
         return host_call(h, Q, dtype, nq, [&](const void *dQ) -> fl_status {
             double *dd = h->ws.get<double>("odist", nq * k);
             int64_t *di = h->ws.get<int64_t>("oidx", nq * k);
@@ -728,6 +930,59 @@ fl_status fl_knn_classes(fl_knn_handle h, int64_t *classes_out) {
     if (!h) return fail(FL_INVALID_PARAMETER, "InvalidParameter: handle: must not be NULL");
     std::copy(h->classes.begin(), h->classes.end(), classes_out);
     return FL_OK;
+}
+
+fl_status fl_nn_fit(const void *X, int32_t dtype, int64_t n, int64_t d, int64_t n_neighbors,
+                    int32_t precision, fl_knn_handle *out) {
+    return guarded([&]() -> fl_status {
+        fl_knn *h = nullptr;
+        fl_status s = knn_fit_common(X, dtype, n, d, n, n_neighbors, FL_UNIFORM, precision,
+                                     "NearestNeighbors", &h);
+        if (s) return s;
+        h->unsupervised = 1;
+        *out = h;
+        return FL_OK;
+    });
+}
+
+fl_status fl_knn_kneighbors_graph_device(fl_knn_handle h, const void *dQ, int32_t dtype,
+                                         int64_t nq, int64_t d, int64_t k, int32_t mode,
+                                         int64_t *d_indptr_out, int64_t *d_indices_out,
+                                         double *d_data_out, fl_report *report) {
+    return guarded([&]() -> fl_status {
+        if (fl_status s = check_query(h, dQ, dtype, nq, d)) return s;
+        if (fl_status s = check_k(h, k)) return s;
+        if (fl_status s = check_graph_mode(mode)) return s;
+        if (!d_indptr_out || (nq > 0 && (!d_indices_out || !d_data_out)))
+            return fail(FL_INVALID_PARAMETER, "InvalidParameter: outputs: must not be NULL");
+        return kneighbors_graph_dev(h, dQ, dtype, nq, k, mode, d_indptr_out, d_indices_out,
+                                    d_data_out, report);
+    });
+}
+
+fl_status fl_knn_kneighbors_graph(fl_knn_handle h, const void *Q, int32_t dtype, int64_t nq,
+                                  int64_t d, int64_t k, int32_t mode, int64_t *indptr_out,
+                                  int64_t *indices_out, double *data_out, fl_report *report) {
+    return guarded([&]() -> fl_status {
+        if (fl_status s = check_query(h, Q, dtype, nq, d)) return s;
+        if (fl_status s = check_k(h, k)) return s;
+        if (fl_status s = check_graph_mode(mode)) return s;
+        if (!indptr_out || (nq > 0 && (!indices_out || !data_out)))
+            return fail(FL_INVALID_PARAMETER, "InvalidParameter: outputs: must not be NULL");
+        return host_call(h, Q, dtype, nq, [&](const void *dQ) -> fl_status {
+            int64_t *dp = h->ws.get<int64_t>("g_indptr", nq + 1);
+            int64_t *di = h->ws.get<int64_t>("g_indices", nq * k);
+            double *dd = h->ws.get<double>("g_data", nq * k);
+            fl_status s = kneighbors_graph_dev(h, dQ, dtype, nq, k, mode, dp, di, dd, report);
+            if (s) return s;
+            FL_HIP(hipMemcpy(indptr_out, dp, (size_t)(nq + 1) * 8, hipMemcpyDeviceToHost));
+            if (nq > 0) {
+                FL_HIP(hipMemcpy(indices_out, di, (size_t)(nq * k) * 8, hipMemcpyDeviceToHost));
+                FL_HIP(hipMemcpy(data_out, dd, (size_t)(nq * k) * 8, hipMemcpyDeviceToHost));
+            }
+            return FL_OK;
+        });
+    });
 }
 
 int64_t fl_knn_n_samples_fit(fl_knn_handle h) { return h ? h->n : 0; }

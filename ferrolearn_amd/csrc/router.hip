@@ -957,8 +957,7 @@ fl_status fl_router_kneighbors_device(fl_router_handle r, const void *dQ, int32_
             return fail(FL_INSUFFICIENT_SAMPLES,
                         "InsufficientSamples: required " + std::to_string(k) + ", actual " +
                             std::to_string(r->n) + ": kneighbors requires k <= n_samples");
-        if (k < 1 || k > kMaxRoutedK)
-            return fail(FL_INVALID_PARAMETER, "InvalidParameter: k: the routed search supports 1..32");
+        if (k < 1) return fail(FL_INVALID_PARAMETER, "InvalidParameter: k: must be at least 1");
         if (n_probe < 1 || n_probe > kMaxProbe || n_probe > r->C)
             return fail(FL_INVALID_PARAMETER, "InvalidParameter: n_probe: 1..min(31, n_cells)");
         if (route != FL_ROUTE_NETWORK && route != FL_ROUTE_BOUNDS)
@@ -966,6 +965,21 @@ fl_status fl_router_kneighbors_device(fl_router_handle r, const void *dQ, int32_
         if (nq > 0 && (!dQ || !d_dist || !d_idx))
             return fail(FL_INVALID_PARAMETER, "InvalidParameter: buffers: must not be NULL");
         WallClock wc;
+        if (k > kMaxRoutedK) {
+            // beyond the routed scan's lists: every query takes the exact
+            // brute force of the underlying handle (same results)
+            const fl_status s =
+                fl_knn_kneighbors_device(r->knn, dQ, dtype, nq, d, k, d_dist, d_idx, nullptr);
+            if (s != FL_OK) return s;
+            if (rep) {
+                zero_report(rep);
+                rep->ms_total = wc.ms();
+                rep->ms_fallback = rep->ms_total;
+                rep->n_queries = nq;
+                rep->n_fallback = nq;
+            }
+            return FL_OK;
+        }
         RtAcc acc;
         const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(1 << 20, (1LL << 30) / n_probe));
         for (int64_t q0 = 0; q0 < nq; q0 += chunk) {

@@ -66,6 +66,10 @@ typedef enum fl_precision {
 /* knn.rs:72-79 (Weights) */
 typedef enum fl_weights { FL_UNIFORM = 0, FL_DISTANCE = 1 } fl_weights;
 
+/* GraphMode (ferrolearn-neighbors/src/graph.rs:26-32): the value stored for
+ * every neighbour edge of a kneighbors graph. */
+typedef enum fl_graph_mode { FL_GRAPH_CONNECTIVITY = 0, FL_GRAPH_DISTANCE = 1 } fl_graph_mode;
+
 /* Per-call measurements, filled when a non-NULL pointer is passed.  Kernel
  * times come from HIP events recorded on the stream the kernels run on. */
 typedef struct fl_report {
@@ -280,8 +284,9 @@ typedef struct fl_knn *fl_knn_handle;
  * with Algorithm::BruteForce (knn.rs:148).  Validation order: y_len != n
  * -> ShapeMismatch; n_neighbors == 0 -> InvalidParameter; n < n_neighbors
  * -> InsufficientSamples.  The handle owns a device copy of X and y, as the
- * fitted Rust model owns clones (knn.rs:311-318).  Supported n_neighbors:
- * 1..64. */
+ * fitted Rust model owns clones (knn.rs:311-318).  Any 1 <= n_neighbors
+ * <= n: k <= 64 runs the BF16 / f64 candidate filters, larger k an exact
+ * sort of every distance per query (the same (distance, index) order). */
 fl_status fl_knn_fit_classifier(const void *X, int32_t dtype, int64_t n,
                                 int64_t d, const int64_t *y, int64_t y_len,
                                 int64_t n_neighbors, int32_t weights,
@@ -331,6 +336,35 @@ fl_status fl_knn_predict_classify_device(fl_knn_handle h, const void *dQ,
 fl_status fl_knn_predict_regress_device(fl_knn_handle h, const void *dQ,
                                         int32_t dtype, int64_t nq, int64_t d,
                                         double *d_y_out, fl_report *report);
+
+/* Fit<Array2<F>, ()> for NearestNeighbors (nearest_neighbors.rs:198-228):
+ * n_neighbors == 0 -> InvalidParameter; n < n_neighbors ->
+ * InsufficientSamples.  Every Algorithm runs the GPU brute force (the tree
+ * indexes return the same neighbours, nearest_neighbors.rs:657-700).  The
+ * handle serves fl_knn_kneighbors(_device) and the graph calls below, where
+ * k > n is InvalidParameter as in nearest_neighbors.rs:282-287 (the
+ * supervised handles report InsufficientSamples, knn.rs:826-834).  A
+ * negative k is a caller's explicit zero: InvalidParameter. */
+fl_status fl_nn_fit(const void *X, int32_t dtype, int64_t n, int64_t d,
+                    int64_t n_neighbors, int32_t precision, fl_knn_handle *out);
+
+/* kneighbors_graph (graph.rs:91-123 via nearest_neighbors.rs / graph.rs
+ * :225-304, any handle): the CSR matrix of nq rows by n_samples_fit columns
+ * whose row i holds query i's k neighbours with ascending columns, valued
+ * 1.0 (FL_GRAPH_CONNECTIVITY) or the distance (FL_GRAPH_DISTANCE).
+ * indptr_out has nq+1 entries (i*k), indices_out and data_out nq*k.
+ * Validation as fl_knn_kneighbors, then the mode. */
+fl_status fl_knn_kneighbors_graph(fl_knn_handle h, const void *Q, int32_t dtype,
+                                  int64_t nq, int64_t d, int64_t k, int32_t mode,
+                                  int64_t *indptr_out, int64_t *indices_out,
+                                  double *data_out, fl_report *report);
+/* Same with Q and the three outputs in device memory. */
+fl_status fl_knn_kneighbors_graph_device(fl_knn_handle h, const void *dQ,
+                                         int32_t dtype, int64_t nq, int64_t d,
+                                         int64_t k, int32_t mode,
+                                         int64_t *d_indptr_out,
+                                         int64_t *d_indices_out,
+                                         double *d_data_out, fl_report *report);
 
 /* Switch the candidate filter of a fitted handle (fl_precision). */
 fl_status fl_knn_set_precision(fl_knn_handle h, int32_t precision);
@@ -434,8 +468,10 @@ fl_status fl_router_train_device(fl_router_handle r, const void *dQ,
                                  fl_router_report *report);
 
 /* Routed kneighbors: same contract and results as fl_knn_kneighbors_device
- * (k == 0 means the handle's n_neighbors; 1 <= k <= 32 here), scanning
- * n_probe cells (1..31, <= n_cells) chosen by `route` (fl_route). */
+ * (k == 0 means the handle's n_neighbors), scanning n_probe cells (1..31,
+ * <= n_cells) chosen by `route` (fl_route).  k > 32 is beyond the routed
+ * scan's lists: those calls run the exact brute force of the underlying
+ * handle and report every query as a fallback. */
 fl_status fl_router_kneighbors_device(fl_router_handle r, const void *dQ,
                                       int32_t dtype, int64_t nq, int64_t d,
                                       int64_t k, int64_t n_probe, int32_t route,

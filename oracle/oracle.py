@@ -281,6 +281,24 @@ def kneighbors(data, Q, k, nthreads=0):
     return dist, idx
 
 
+def knn_to_csr(dist, idx, n_cols, distance_mode):
+    """graph.rs:91-123 (knn_to_csr): per row, the (column, value) pairs of
+    the k neighbours -- value 1.0 for connectivity, the distance otherwise --
+    sorted by column (sort_by_key, stable), then concatenated.  Returns
+    (indptr, indices, data, shape)."""
+    nq, k = idx.shape
+    indptr = np.arange(nq + 1, dtype=np.int64) * k
+    indices = np.zeros(nq * k, np.int64)
+    data = np.zeros(nq * k, np.float64)
+    for i in range(nq):
+        pairs = sorted(((int(idx[i, j]), float(dist[i, j]) if distance_mode else 1.0)
+                        for j in range(k)), key=lambda p: p[0])
+        for j, (c, v) in enumerate(pairs):
+            indices[i * k + j] = c
+            data[i * k + j] = v
+    return indptr, indices, data, (nq, int(n_cols))
+
+
 def class_positions(y):
     """Sorted unique classes and the position of each label among them
     (knn.rs:304-306: sort_unstable + dedup)."""

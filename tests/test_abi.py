@@ -94,10 +94,12 @@ def test_knn_validation_order():
         F.KNeighborsRegressor().fit(x, [0.0, 1.0, 2.0])
 
 
-def test_knn_gpu_limit_is_reported_after_reference_checks():
+def test_knn_large_k_is_only_bounded_by_n():
+    # knn.rs:288-301: any 1 <= n_neighbors <= n is accepted; k > n is the
+    # reference's InsufficientSamples, raised before any device work.
     x = np.ones((100, 2))
-    with pytest.raises(F.InvalidParameter, match="1..64"):
-        F.KNeighborsClassifier().with_n_neighbors(65).fit(x, np.zeros(100))
+    with pytest.raises(F.InsufficientSamples, match="required 101, actual 100"):
+        F.KNeighborsClassifier().with_n_neighbors(101).fit(x, np.zeros(100))
 
 
 def test_gemm_shape_mismatch():
@@ -178,3 +180,13 @@ def test_router_defaults_and_null_arguments():
     loss = C.c_double()
     assert L.lib().fl_router_mlp_gradients(4, 64, 8, None, None, 1, None, 1, 0.0,
                                            C.byref(loss), None) == L.FL_INVALID_PARAMETER
+
+
+def test_nearest_neighbors_fit_validation():
+    # nearest_neighbors.rs:203-222: n_neighbors == 0, then n < n_neighbors
+    x = np.ones((2, 2))
+    with pytest.raises(F.InvalidParameter, match="n_neighbors"):
+        F.NearestNeighbors().with_n_neighbors(0).fit(x)
+    with pytest.raises(F.InsufficientSamples,
+                       match="NearestNeighbors requires at least n_neighbors=5 samples"):
+        F.NearestNeighbors().with_n_neighbors(5).fit(x)

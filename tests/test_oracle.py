@@ -483,3 +483,23 @@ def test_minibatch_fit_golden_bit_exact():
         assert list(l) == case["labels"]
         assert (bits(c) == bits(arr(case["centers"]))).all()
         assert inertia == float.fromhex(case["inertia"])
+
+
+def test_knn_to_csr_pinned_by_its_dense_matrix():
+    # graph.rs:91-123: the CSR with ascending columns of the matrix whose
+    # row i holds value v at column idx[i, j] -- a CSR with sorted columns is
+    # unique for a given dense matrix, so these three facts pin it
+    rng = np.random.default_rng(0)
+    nq, n, k = 40, 25, 6
+    idx = np.stack([rng.permutation(n)[:k] for _ in range(nq)]).astype(np.int64)
+    dist = rng.uniform(0.5, 3.0, size=(nq, k))
+    for distance_mode in (False, True):
+        indptr, indices, data, shape = O.knn_to_csr(dist, idx, n, distance_mode)
+        assert shape == (nq, n) and (indptr == np.arange(nq + 1) * k).all()
+        rows = indices.reshape(nq, k)
+        assert (np.diff(rows, axis=1) > 0).all()
+        dense = np.zeros(shape)
+        dense[np.repeat(np.arange(nq), k), indices] = data
+        want = np.zeros(shape)
+        want[np.repeat(np.arange(nq), k), idx.ravel()] = dist.ravel() if distance_mode else 1.0
+        assert (dense == want).all()
