@@ -193,6 +193,10 @@ SIGNATURES = {
     "fl_knn_kneighbors": (C.c_int, [_vp, _vp, _i32, _i64, _i64, _i64, _vp, _vp, _rp]),
     "fl_knn_kneighbors_device": (C.c_int, [_vp, _vp, _i32, _i64, _i64, _i64, _vp, _vp, _rp]),
     "fl_nn_fit": (C.c_int, [_vp, _i32, _i64, _i64, _i64, _i32, C.POINTER(_vp)]),
+    "fl_pairwise_distances_argmin_min": (C.c_int, [_vp, _i32, _i64, _i64, _vp, _i64, _i64, _vp,
+                                                   _vp]),
+    "fl_pairwise_distances_argmin_min_device": (C.c_int, [_vp, _i32, _i64, _i64, _vp, _i64, _i64,
+                                                          _vp, _vp]),
     "fl_knn_kneighbors_graph": (C.c_int, [_vp, _vp, _i32, _i64, _i64, _i64, _i32, _vp, _vp, _vp,
                                           _rp]),
     "fl_knn_kneighbors_graph_device": (C.c_int, [_vp, _vp, _i32, _i64, _i64, _i64, _i32, _vp, _vp,

@@ -140,6 +140,15 @@ class PhaseTimer {
     std::vector<int> free_;
 };
 
+// A stream of its own, destroyed with the object.
+struct StreamGuard {
+    hipStream_t s = nullptr;
+    StreamGuard() { FL_HIP(hipStreamCreate(&s)); }
+    ~StreamGuard() {
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+
 // Zeroed page-locked host array (asynchronous device-to-host copies).
 template <typename T>
 class PinnedHost {

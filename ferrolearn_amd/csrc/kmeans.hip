@@ -628,14 +628,6 @@ class Lloyd {
     Workspace ws_;
 };
 
-struct StreamGuard {
-    hipStream_t s = nullptr;
-    StreamGuard() { FL_HIP(hipStreamCreate(&s)); }
-    ~StreamGuard() {
-        if (s) (void)hipStreamDestroy(s);
-    }
-};
-
 // kmeans.rs:421-449, same order and conditions.
 fl_status validate_fit(int64_t n, int64_t d, const fl_kmeans_params *p, int32_t dtype) {
     if (!p) return fail(FL_INVALID_PARAMETER, "InvalidParameter: params: must not be NULL");

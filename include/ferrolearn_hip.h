@@ -378,6 +378,32 @@ int64_t fl_knn_n_samples_fit(fl_knn_handle h);
 void fl_knn_free(fl_knn_handle h);
 
 /* ------------------------------------------------------------------ */
+/* pairwise_distances_argmin(_min) -- ferrolearn-metrics/src/pairwise.rs */
+/* ------------------------------------------------------------------ */
+
+/* pairwise_distances_argmin_min (pairwise.rs:596-629) with
+ * Metric::Euclidean (also pairwise_distances_argmin, :564-588, with
+ * min_out == NULL): for each of the n rows of X the index of the closest of
+ * the m rows of Y and that distance, where distances are
+ * euclidean_distances' expansion (pairwise.rs:157-185)
+ * sqrt(max((|x|^2 + |y|^2) - 2 x.y, 0)) with left folds, and ties go to the
+ * lowest index.  X and Y share dtype (f32 data widened to f64, exactly).
+ * Errors in the reference's order: dx != dy -> ShapeMismatch; n == 0 or
+ * m == 0, then dx == 0 -> InsufficientSamples.  idx_out and min_out hold
+ * n entries. */
+fl_status fl_pairwise_distances_argmin_min(const void *X, int32_t dtype,
+                                           int64_t n, int64_t dx,
+                                           const void *Y, int64_t m,
+                                           int64_t dy, int64_t *idx_out,
+                                           double *min_out);
+/* Same with X, Y and the outputs in device memory. */
+fl_status fl_pairwise_distances_argmin_min_device(const void *dX, int32_t dtype,
+                                                  int64_t n, int64_t dx,
+                                                  const void *dY, int64_t m,
+                                                  int64_t dy, int64_t *d_idx_out,
+                                                  double *d_min_out);
+
+/* ------------------------------------------------------------------ */
 /* Cell router for exact kNN (no reference counterpart)                */
 /* ------------------------------------------------------------------ */
 

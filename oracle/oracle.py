@@ -75,6 +75,8 @@ def lib():
         L.or_knn_query.argtypes = [_dp, _i64, _i64, _dp, _i64, _ip, _dp]
         L.or_kneighbors.restype = None
         L.or_kneighbors.argtypes = [_dp, _i64, _i64, _dp, _i64, _i64, _ip, _dp, C.c_int]
+        L.or_pairwise_argmin_min.restype = None
+        L.or_pairwise_argmin_min.argtypes = [_dp, _i64, _dp, _i64, _i64, _ip, _dp, C.c_int]
         L.or_class_scores.restype = None
         L.or_class_scores.argtypes = [_ip, _dp, _i64, _ip, _i64, C.c_int, _dp]
         L.or_vote.restype = _i64
@@ -297,6 +299,18 @@ def knn_to_csr(dist, idx, n_cols, distance_mode):
             indices[i * k + j] = c
             data[i * k + j] = v
     return indptr, indices, data, (nq, int(n_cols))
+
+
+def pairwise_argmin_min(X, Y, nthreads=0):
+    """pairwise.rs:564-629 with Metric::Euclidean: (indices, minimum
+    distances) of each row of X over the rows of Y."""
+    X, Y = _f64(X), _f64(Y)
+    n, d = X.shape
+    m = Y.shape[0]
+    idx = np.zeros(n, np.int64)
+    mins = np.zeros(n, np.float64)
+    lib().or_pairwise_argmin_min(_p(X), n, _p(Y), m, d, _pi(idx), _p(mins), nthreads)
+    return idx, mins
 
 
 def class_positions(y):

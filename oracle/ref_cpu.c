@@ -729,3 +729,44 @@ void or_knn_regress(const double *data, int64_t N, int64_t d, const double *y,
 
 /* Thread count the OpenMP runtime would use by default (for reports). */
 int or_max_threads(void) { return or_threads(0); }
+
+/* ------------------------------------------------------------------------ */
+/* f4: pairwise_distances_argmin(_min), Metric::Euclidean                     */
+/* ------------------------------------------------------------------------ */
+
+/* ferrolearn-metrics/src/pairwise.rs:157-185 (euclidean_distances) and
+ * :564-629: x_sq / y_sq are left folds of v*v, dot the left fold of x*y,
+ * the distance sqrt(max((x_sq + y_sq) - 2*dot, 0)) where max returns the
+ * non-NaN operand (f64::max, as C's fmax), and the index the first j with
+ * the smallest distance (best starts at j = 0, then strict <).  min_out may
+ * be NULL. */
+void or_pairwise_argmin_min(const double *X, int64_t n, const double *Y, int64_t m, int64_t d,
+                            int64_t *idx_out, double *min_out, int nthreads) {
+    nthreads = or_threads(nthreads);
+    double *ysq = (double *)malloc((size_t)(m > 0 ? m : 1) * sizeof(double));
+    for (int64_t j = 0; j < m; ++j) {
+        double s = 0.0;
+        for (int64_t k = 0; k < d; ++k) s = s + Y[j * d + k] * Y[j * d + k];
+        ysq[j] = s;
+    }
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads)
+    for (int64_t i = 0; i < n; ++i) {
+        const double *x = X + i * d;
+        double xsq = 0.0;
+        for (int64_t k = 0; k < d; ++k) xsq = xsq + x[k] * x[k];
+        int64_t bi = 0;
+        double bv = 0.0;
+        for (int64_t j = 0; j < m; ++j) {
+            double dot = 0.0;
+            for (int64_t k = 0; k < d; ++k) dot = dot + x[k] * Y[j * d + k];
+            const double v = sqrt(fmax((xsq + ysq[j]) - 2.0 * dot, 0.0));
+            if (j == 0 || v < bv) {
+                bv = v;
+                bi = j;
+            }
+        }
+        idx_out[i] = bi;
+        if (min_out) min_out[i] = bv;
+    }
+    free(ysq);
+}

@@ -190,3 +190,29 @@ def test_nearest_neighbors_fit_validation():
     with pytest.raises(F.InsufficientSamples,
                        match="NearestNeighbors requires at least n_neighbors=5 samples"):
         F.NearestNeighbors().with_n_neighbors(5).fit(x)
+
+
+def test_pairwise_argmin_validation_order():
+    # pairwise.rs:77-117: feature dimensions first, then empty inputs
+    with pytest.raises(F.ShapeMismatch, match="feature dimensions must match"):
+        F.pairwise_distances_argmin(np.ones((0, 2)), np.ones((3, 3)))
+    with pytest.raises(F.InsufficientSamples, match="at least one row"):
+        F.pairwise_distances_argmin(np.ones((0, 2)), np.ones((3, 2)))
+    with pytest.raises(F.InsufficientSamples, match="feature dimension must be at least 1"):
+        F.pairwise_distances_argmin_min(np.ones((2, 0)), np.ones((3, 0)))
+    with pytest.raises(F.InvalidParameter, match="metric"):
+        F.pairwise_distances_argmin(np.ones((2, 2)), np.ones((3, 2)), F.Metric.Manhattan)
+
+
+def test_spectral_validation():
+    # spectral.rs:240-258
+    x = np.ones((10, 2))
+    with pytest.raises(F.InvalidParameter, match="n_clusters"):
+        F.SpectralClustering(0).fit(x)
+    for g in (0.0, -1.0):
+        with pytest.raises(F.InvalidParameter, match="gamma"):
+            F.SpectralClustering(2).with_gamma(g).fit(x)
+    with pytest.raises(F.InsufficientSamples):
+        F.SpectralClustering(2).fit(np.zeros((0, 2)))
+    with pytest.raises(F.InsufficientSamples):
+        F.SpectralClustering(3).fit(np.zeros((1, 2)))

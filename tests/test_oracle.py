@@ -503,3 +503,52 @@ def test_knn_to_csr_pinned_by_its_dense_matrix():
         want = np.zeros(shape)
         want[np.repeat(np.arange(nq), k), idx.ravel()] = dist.ravel() if distance_mode else 1.0
         assert (dense == want).all()
+
+
+def test_pairwise_argmin_known_answers_and_integers():
+    # pairwise.rs:732-738 (euclidean_basic): distances 4 and 5
+    idx, mins = O.pairwise_argmin_min(np.array([[0.0, 0.0], [3.0, 0.0]]), np.array([[0.0, 4.0]]))
+    assert list(idx) == [0, 0] and list(mins) == [4.0, 5.0]
+    # small integers: every operation of the expansion is exact, so the
+    # answer is the integer brute force with the lowest index among ties and
+    # the correctly rounded root of the exact squared distance
+    rng = np.random.default_rng(1)
+    X = rng.integers(-6, 7, size=(60, 5)).astype(np.float64)
+    Y = rng.integers(-6, 7, size=(23, 5)).astype(np.float64)
+    Y[11] = Y[4]
+    idx, mins = O.pairwise_argmin_min(X, Y)
+    for i in range(len(X)):
+        sq = [int(((X[i] - Y[j]) ** 2).sum()) for j in range(len(Y))]
+        j = sq.index(min(sq))
+        assert idx[i] == j and mins[i] == math.sqrt(sq[j])
+
+
+def test_pairwise_argmin_cancellation_and_nan():
+    # far from the origin the expansion cancels: a plain-Python restatement
+    # of pairwise.rs:170-185 / :564-588 (floats are IEEE doubles) gives the
+    # same bits, and differs from the squared_euclidean fold's answer
+    rng = np.random.default_rng(2)
+    X = 1e8 + rng.normal(size=(40, 3))
+    Y = 1e8 + rng.normal(size=(30, 3))
+    idx, mins = O.pairwise_argmin_min(X, Y)
+    differs = False
+    for i in range(len(X)):
+        xs = 0.0
+        for v in X[i]:
+            xs = xs + float(v) * float(v)
+        best, bj = 0.0, 0
+        for j in range(len(Y)):
+            ys = dot = 0.0
+            for a, b in zip(X[i], Y[j]):
+                ys = ys + float(b) * float(b)
+                dot = dot + float(a) * float(b)
+            v = math.sqrt(max((xs + ys) - 2.0 * dot, 0.0))
+            if j == 0 or v < best:
+                best, bj = v, j
+        assert idx[i] == bj and mins[i] == best
+        differs |= bj != int(np.argmin(((X[i] - Y) ** 2).sum(1)))
+    assert differs
+    # f64::max returns the non-NaN operand: a NaN row has distance 0 to all
+    Xn = np.array([[np.nan, 1.0], [0.0, 0.0]])
+    idx, mins = O.pairwise_argmin_min(Xn, np.array([[5.0, 5.0], [0.0, 1.0]]))
+    assert list(idx) == [0, 1] and mins[0] == 0.0 and mins[1] == 1.0
