@@ -216,3 +216,29 @@ def test_spectral_validation():
         F.SpectralClustering(2).fit(np.zeros((0, 2)))
     with pytest.raises(F.InsufficientSamples):
         F.SpectralClustering(3).fit(np.zeros((1, 2)))
+
+
+def test_rs_surface_errors_and_defaults():
+    # ferrolearn-python: constructor defaults (clusterers.rs:25-26,
+    # classifiers.rs:306-307, extras.rs:700-701), RuntimeError before fit,
+    # ValueError with FerroError's Display text (error.rs:34-76)
+    from ferrolearn_amd import _ferrolearn_rs as R
+    km = R._RsKMeans()
+    assert (km.n_clusters, km.max_iter, km.tol, km.n_init, km.random_state) == \
+        (8, 300, 1e-4, 10, None)
+    assert R._RsKNeighborsClassifier().n_neighbors == 5
+    assert R._RsKNeighborsRegressor().n_neighbors == 5
+    mb = R._RsMiniBatchKMeans()
+    assert (mb.n_clusters, mb.max_iter, mb.random_state) == (8, 100, None)
+    for obj, call in ((km, lambda: km.predict(np.ones((2, 2)))),
+                      (km, lambda: km.labels_),
+                      (mb, lambda: mb.labels_)):
+        with pytest.raises(RuntimeError, match="not fitted"):
+            call()
+    with pytest.raises(ValueError, match=r"^Invalid parameter `n_clusters`: must be at least 1$"):
+        R._RsKMeans(n_clusters=0).fit(np.ones((5, 2)))
+    with pytest.raises(ValueError,
+                       match=r"^Insufficient samples: need at least 5, got 3 \(KNeighborsClassifier"):
+        R._RsKNeighborsClassifier().fit(np.ones((3, 2)), np.zeros(3))
+    with pytest.raises(ValueError, match=r"^Shape mismatch in y length must match"):
+        R._RsKNeighborsRegressor().fit(np.ones((6, 2)), np.zeros(4))
