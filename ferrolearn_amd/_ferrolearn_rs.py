@@ -7,7 +7,8 @@ the same constructor keywords and defaults, `fit` / `predict` / `transform`,
 the fitted getters, float64 inputs, int64 label outputs, `ValueError` with
 the reference's error text when a model call fails (the PyO3 mapping of
 FerroError's Display, ferrolearn-core/src/error.rs:34-89) and
-`RuntimeError("not fitted")` before fit.  The sklearn wrappers of
+`RuntimeError("not fitted")` before fit.  The clustering classes draw from
+the reference's StdRng stream (KMeans.with_rng("stdrng")).  The sklearn wrappers of
 ferrolearn-python/python/ferrolearn import these names from
 `ferrolearn._ferrolearn_rs`; binding that module name to this one runs them
 on the GPU.
@@ -75,7 +76,7 @@ class _RsKMeans(_Fitted):
 
     def fit(self, x):
         m = (KMeans(self.n_clusters).with_max_iter(self.max_iter).with_tol(self.tol)
-             .with_n_init(self.n_init))
+             .with_n_init(self.n_init).with_rng("stdrng"))
         if self.random_state is not None:
             m = m.with_random_state(int(self.random_state))
         self._fitted = _call(m.fit, _f64_2d(x))
@@ -112,7 +113,7 @@ class _RsMiniBatchKMeans(_Fitted):
         self._fitted = None
 
     def fit(self, x):
-        m = MiniBatchKMeans(self.n_clusters).with_max_iter(self.max_iter)
+        m = MiniBatchKMeans(self.n_clusters).with_max_iter(self.max_iter).with_rng("stdrng")
         if self.random_state is not None:
             m = m.with_random_state(int(self.random_state))
         self._fitted = _call(m.fit, _f64_2d(x))

@@ -30,6 +30,9 @@ FL_PREC_FP8 = 4
 FL_UNIFORM = 0
 FL_DISTANCE = 1
 FL_GRAPH_CONNECTIVITY = 0
+FL_RNG_LIBRARY = 0
+FL_RNG_STDRNG = 1
+RNGS = {"library": FL_RNG_LIBRARY, "stdrng": FL_RNG_STDRNG}
 FL_GRAPH_DISTANCE = 1
 
 PRECISIONS = {"fp64": FL_PREC_FP64, "bf16": FL_PREC_BF16, "bf16x3": FL_PREC_BF16X3,
@@ -64,6 +67,8 @@ class fl_kmeans_params(C.Structure):
         ("random_state", C.c_uint64),
         ("has_random_state", C.c_int32),
         ("precision", C.c_int32),
+        ("rng", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 
@@ -77,6 +82,8 @@ class fl_minibatch_params(C.Structure):
         ("random_state", C.c_uint64),
         ("has_random_state", C.c_int32),
         ("precision", C.c_int32),
+        ("rng", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 

@@ -101,6 +101,7 @@ class KMeans:
         self.n_init = 10
         self.random_state = None
         self.precision = "fp64"
+        self.rng = "library"
 
     def _with(self, **kw):
         o = copy.copy(self)
@@ -123,6 +124,15 @@ class KMeans:
     def with_precision(self, precision: str):
         return self._with(precision=precision)
 
+    def with_rng(self, rng: str):
+        """'library' (default): the library's generator with parallel
+        two-level sums in k-means++; 'stdrng': the reference's StdRng stream
+        (ChaCha12, rand 0.9) with its plain left folds, so that a seeded fit
+        draws the reference's initial centres."""
+        if rng not in L.RNGS:
+            raise ValueError(f"unknown rng {rng!r}: use 'library' or 'stdrng'")
+        return self._with(rng=rng)
+
     def _params(self):
         p = L.fl_kmeans_params()
         L.lib().fl_kmeans_params_default(C.byref(p), self.n_clusters)
@@ -133,6 +143,7 @@ class KMeans:
             p.random_state = self.random_state
             p.has_random_state = 1
         p.precision = L.PRECISIONS[self.precision]
+        p.rng = L.RNGS[self.rng]
         return p
 
     def fit(self, x, y=None, init_centers=None) -> "FittedKMeans":
@@ -265,6 +276,7 @@ class MiniBatchKMeans(KMeans):
             p.random_state = self.random_state
             p.has_random_state = 1
         p.precision = L.PRECISIONS[self.precision]
+        p.rng = L.RNGS[self.rng]
         return p
 
     def fit(self, x, y=None, init_centers=None) -> "FittedKMeans":

@@ -850,9 +850,10 @@ struct GraphOwner {
     }
 };
 
-// Problems the batched kernels take: validated, d <= 64, k <= 1024.
+// Problems the batched kernels take: validated, d <= 64, k <= 1024, the
+// library's generator (the reference's stream runs on the per-problem path).
 bool batchable(const fl_kmeans_problem &p) {
-    return p.d >= 1 && p.d <= BK_MAXD && p.n <= 0x7fffffffLL && p.params.n_clusters >= 1 &&
+    return p.params.rng == FL_RNG_LIBRARY && p.d >= 1 && p.d <= BK_MAXD && p.n <= 0x7fffffffLL && p.params.n_clusters >= 1 &&
            p.params.n_clusters <= BK_MAXK && p.n >= p.params.n_clusters && p.params.n_init >= 1 &&
            p.params.n_init <= 4096 && p.X && p.centers_out && p.labels_out;
 }

@@ -14,6 +14,7 @@
 #include "common.hpp"
 #include "dist_bf16.hpp"
 #include "dist_fp8.hpp"
+#include "rand09.hpp"
 #include "dist_f64.hpp"
 #include "dist_topm.hpp"
 
@@ -275,6 +276,7 @@ struct Rng {
     int64_t range(int64_t n) {
         return (int64_t)(((unsigned __int128)next() * (uint64_t)n) >> 64);
     }
+    int64_t range_from(int64_t lo, int64_t hi) { return lo + range(hi - lo); }
 };
 
 inline void check_launch() { FL_HIP(hipGetLastError()); }

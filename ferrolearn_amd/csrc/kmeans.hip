@@ -523,6 +523,80 @@ class Lloyd {
         FL_HIP(hipStreamSynchronize(st_));
     }
 
+    // k-means++ exactly as kmeans.rs:169-253 draws and sums it: the
+    // generator's range / f64 draws in the reference's order, the total,
+    // the threshold search and every trial's potential as plain left folds
+    // (one device thread each), the strictly smallest potential kept.  The
+    // caller's generator is left where the reference's would be.
+    template <typename G>
+    void kmeanspp_seq(G &rng, double *out) {
+        const int64_t first = rng.range(n_);
+        hipLaunchKernelGGL(copy_row_f64_kernel<TX>, dim3(grid1(d_, 64)), dim3(64), 0, st_, X_,
+                           first, d_, out);
+        check_launch();
+        if (k_ == 1) {
+            FL_HIP(hipStreamSynchronize(st_));
+            return;
+        }
+        int64_t n_trials = 2 + (int64_t)std::floor(std::log((double)k_));
+        if (n_trials < 1) n_trials = 1;
+        if (n_trials > 64) n_trials = 64;
+        double *md = ws_.get<double>("kps_md", n_);
+        double *nd = ws_.get<double>("kps_nd", n_trials * n_);
+        double *sc = ws_.get<double>("kps_scal", 2 * n_trials + 1);   // [thr | pot | total]
+        int64_t *cand = ws_.get<int64_t>("kps_cand", n_trials);
+        hipLaunchKernelGGL(kpp_first_kernel<TX>, dim3(grid1(n_, 256)), dim3(256), 0, st_, X_, n_,
+                           d_, out, md);
+        check_launch();
+        std::vector<double> hthr((size_t)n_trials), hpot((size_t)n_trials);
+        std::vector<int64_t> hcand((size_t)n_trials);
+        for (int64_t c = 1; c < k_; ++c) {
+            double total = 0.0;
+            hipLaunchKernelGGL(seq_folds_kernel, dim3(1), dim3(64), 0, st_, md, n_, 1,
+                               sc + 2 * n_trials);
+            check_launch();
+            FL_HIP(hipMemcpyAsync(&total, sc + 2 * n_trials, 8, hipMemcpyDeviceToHost, st_));
+            FL_HIP(hipStreamSynchronize(st_));
+            if (total <= 0.0) {
+                const int64_t idx = rng.range(n_);
+                hipLaunchKernelGGL(copy_row_f64_kernel<TX>, dim3(grid1(d_, 64)), dim3(64), 0, st_,
+                                   X_, idx, d_, out + c * d_);
+                check_launch();
+                continue;
+            }
+            for (int64_t t = 0; t < n_trials; ++t) hthr[(size_t)t] = rng.f64() * total;
+            FL_HIP(hipMemcpyAsync(sc, hthr.data(), (size_t)n_trials * 8, hipMemcpyHostToDevice,
+                                  st_));
+            hipLaunchKernelGGL(kpp_seq_search_kernel, dim3(1), dim3(64), 0, st_, md, n_, sc,
+                               (int)n_trials, cand);
+            hipLaunchKernelGGL(kpp_trials_multi_kernel<TX>,
+                               dim3((unsigned)grid1(n_, 256), (unsigned)n_trials), dim3(256), 0,
+                               st_, X_, n_, d_, cand, md, nd);
+            hipLaunchKernelGGL(seq_folds_kernel, dim3(1), dim3(64), 0, st_, nd, n_,
+                               (int)n_trials, sc + n_trials);
+            check_launch();
+            FL_HIP(hipMemcpyAsync(hpot.data(), sc + n_trials, (size_t)n_trials * 8,
+                                  hipMemcpyDeviceToHost, st_));
+            FL_HIP(hipMemcpyAsync(hcand.data(), cand, (size_t)n_trials * 8,
+                                  hipMemcpyDeviceToHost, st_));
+            FL_HIP(hipStreamSynchronize(st_));
+            int64_t bt = 0;
+            for (int64_t t = 1; t < n_trials; ++t)
+                if (hpot[(size_t)t] < hpot[(size_t)bt]) bt = t;
+            hipLaunchKernelGGL(copy_row_f64_kernel<TX>, dim3(grid1(d_, 64)), dim3(64), 0, st_, X_,
+                               hcand[(size_t)bt], d_, out + c * d_);
+            FL_HIP(hipMemcpyAsync(md, nd + bt * n_, (size_t)n_ * 8, hipMemcpyDeviceToDevice,
+                                  st_));
+            check_launch();
+        }
+        FL_HIP(hipStreamSynchronize(st_));
+    }
+
+    // k-means++ from a generator the caller keeps using: the library's
+    // stream with its two-level sums, or the reference's with plain folds.
+    void kmeanspp_from(Rng &rng, double *out) { kmeanspp(rng, out); }
+    void kmeanspp_from(StdRng09 &rng, double *out) { kmeanspp_seq(rng, out); }
+
     // Same, drawing from a generator the caller continues to use afterwards
     // (MiniBatchKMeans samples its batches from the same stream).
     void kmeanspp(Rng &rng, double *out) {
@@ -656,6 +730,8 @@ fl_status validate_fit(int64_t n, int64_t d, const fl_kmeans_params *p, int32_t 
         return fail(FL_INVALID_PARAMETER, "InvalidParameter: n_features: must be in 1..2^20");
     if (n > 0x7fffffffLL)
         return fail(FL_INVALID_PARAMETER, "InvalidParameter: n_samples: at most 2^31-1 rows");
+    if (p->rng != FL_RNG_LIBRARY && p->rng != FL_RNG_STDRNG)
+        return fail(FL_INVALID_PARAMETER, "InvalidParameter: rng: unknown generator");
     if (p->precision != FL_PREC_FP64 && p->precision != FL_PREC_EXACT &&
         p->precision != FL_PREC_BF16 && p->precision != FL_PREC_BF16X3 &&
         p->precision != FL_PREC_FP8)
@@ -698,7 +774,10 @@ fl_status fit_device(const TX *X, int64_t n, int d, const fl_kmeans_params *p,
         if (d_init)
             FL_HIP(hipMemcpyAsync(C[0], d_init + run * k * d, (size_t)(k * d) * 8,
                                   hipMemcpyDeviceToDevice, st));
-        else
+        else if (p->rng == FL_RNG_STDRNG) {
+            StdRng09 g(base + (uint64_t)run);   // kmeans.rs:460
+            L.kmeanspp_seq(g, C[0]);
+        } else
             L.kmeanspp(base + (uint64_t)run, C[0]);
         double inertia = 1.7976931348623157e308;
         int64_t n_iter = 0;
@@ -833,13 +912,14 @@ fl_status check_predict(int64_t k, int64_t d_centers, int64_t n, int64_t d, int3
 // generator: a full Fisher-Yates shuffle when batch >= n, else the first
 // `batch` steps of one over 0..n (the swapped entries kept in a map, so a
 // step costs O(1) instead of rebuilding the pool).
-std::vector<int32_t> sample_batch(Rng &rng, int64_t n, int64_t batch) {
+template <typename G>
+std::vector<int32_t> sample_batch(G &rng, int64_t n, int64_t batch) {
     std::vector<int32_t> out;
     if (batch >= n) {
         out.resize((size_t)n);
         for (int64_t i = 0; i < n; ++i) out[(size_t)i] = (int32_t)i;
         for (int64_t i = 0; i < n; ++i) {
-            const int64_t j = i + rng.range(n - i);
+            const int64_t j = rng.range_from(i, n);
             std::swap(out[(size_t)i], out[(size_t)j]);
         }
         return out;
@@ -852,7 +932,7 @@ std::vector<int32_t> sample_batch(Rng &rng, int64_t n, int64_t batch) {
     };
     out.resize((size_t)batch);
     for (int64_t i = 0; i < batch; ++i) {
-        const int64_t j = i + rng.range(n - i);
+        const int64_t j = rng.range_from(i, n);
         const int64_t vi = at(i), vj = at(j);
         pool[i] = vj;
         pool[j] = vi;
@@ -893,13 +973,13 @@ fl_status mb_fit_device(const TX *X, int64_t n, int d, const fl_minibatch_params
     int64_t best_iter = 0;
     PhaseTimer tm(st);
     double ms_batch = 0.0;
-    for (int64_t run = 0; run < p->n_init; ++run) {
-        Rng rng(base + (uint64_t)run * 1000003ULL);
+    // one run from its generator (the library's or the reference's stream)
+    auto one_run = [&](int64_t run, auto &rng) {
         if (d_init)
             FL_HIP(hipMemcpyAsync(cur, d_init + run * k * d, (size_t)(k * d) * 8,
                                   hipMemcpyDeviceToDevice, st));
         else
-            L.kmeanspp(rng, cur);
+            L.kmeanspp_from(rng, cur);
         hipLaunchKernelGGL(fill_i64_kernel, dim3(grid1(k, 256)), dim3(256), 0, st, counts, k,
                            (int64_t)1);
         check_launch();
@@ -942,6 +1022,16 @@ fl_status mb_fit_device(const TX *X, int64_t n, int d, const fl_minibatch_params
             hipLaunchKernelGGL(labels_to_i64_kernel, dim3(grid1(n, 256)), dim3(256), 0, st, labels,
                                n, d_labels);
             check_launch();
+        }
+    };
+    for (int64_t run = 0; run < p->n_init; ++run) {
+        const uint64_t seed = base + (uint64_t)run * 1000003ULL;   // mini_batch_kmeans.rs:532
+        if (p->rng == FL_RNG_STDRNG) {
+            StdRng09 g(seed);
+            one_run(run, g);
+        } else {
+            Rng g(seed);
+            one_run(run, g);
         }
     }
     FL_HIP(hipStreamSynchronize(st));
@@ -995,6 +1085,8 @@ fl_status validate_mb(int64_t n, int64_t d, const fl_minibatch_params *p, int32_
         return fail(FL_INVALID_PARAMETER, "InvalidParameter: n_features: must be in 1..2^20");
     if (n > 0x7fffffffLL)
         return fail(FL_INVALID_PARAMETER, "InvalidParameter: n_samples: at most 2^31-1 rows");
+    if (p->rng != FL_RNG_LIBRARY && p->rng != FL_RNG_STDRNG)
+        return fail(FL_INVALID_PARAMETER, "InvalidParameter: rng: unknown generator");
     if (p->precision != FL_PREC_FP64 && p->precision != FL_PREC_EXACT &&
         p->precision != FL_PREC_BF16 && p->precision != FL_PREC_BF16X3 &&
         p->precision != FL_PREC_FP8)

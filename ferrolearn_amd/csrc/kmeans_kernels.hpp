@@ -582,6 +582,79 @@ __global__ void kpp_trial_kernel(const TX *__restrict__ X, int64_t n, int d, int
     nd[i] = v < m ? v : m;
 }
 
+// k-means++ with the reference's plain left folds (kmeans.rs:200-228): one
+// thread per trial.  Loads are issued 32 ahead of the running sum, which
+// still adds in index order.
+//
+// out[t] = left fold of v[t*n .. t*n + n) for t < nt.
+static __global__ __attribute__((unused)) void seq_folds_kernel(const double *__restrict__ v,
+                                                                int64_t n, int nt,
+                                                                double *__restrict__ out) {
+    const int t = threadIdx.x;
+    if (blockIdx.x != 0 || t >= nt) return;
+    const double *a = v + (int64_t)t * n;
+    double s = 0.0;
+    int64_t i = 0;
+    for (; i + 32 <= n; i += 32) {
+        double b[32];
+#pragma unroll
+        for (int u = 0; u < 32; ++u) b[u] = a[i + u];
+#pragma unroll
+        for (int u = 0; u < 32; ++u) s = s + b[u];
+    }
+    for (; i < n; ++i) s = s + a[i];
+    out[t] = s;
+}
+
+// cand[t] = first i whose running left fold of md reaches thr[t], else n-1.
+static __global__ __attribute__((unused)) void kpp_seq_search_kernel(
+    const double *__restrict__ md, int64_t n, const double *__restrict__ thr, int nt,
+    int64_t *__restrict__ cand) {
+    const int t = threadIdx.x;
+    if (blockIdx.x != 0 || t >= nt) return;
+    const double th = thr[t];
+    double c = 0.0;
+    int64_t found = n - 1;
+    int64_t i = 0;
+    bool done = false;
+    for (; i + 32 <= n && !done; i += 32) {
+        double b[32];
+#pragma unroll
+        for (int u = 0; u < 32; ++u) b[u] = md[i + u];
+#pragma unroll
+        for (int u = 0; u < 32; ++u) {   // no break: b stays in registers
+            if (!done) {
+                c = c + b[u];
+                if (c >= th) {
+                    found = i + u;
+                    done = true;
+                }
+            }
+        }
+    }
+    for (; i < n && !done; ++i) {
+        c = c + md[i];
+        if (c >= th) {
+            found = i;
+            done = true;
+        }
+    }
+    cand[t] = found;
+}
+
+// nd[t*n + i] = min(md[i], fold(x_i, x_cand[t])) for every trial t (grid.y).
+template <typename TX>
+__global__ void kpp_trials_multi_kernel(const TX *__restrict__ X, int64_t n, int d,
+                                        const int64_t *__restrict__ cand,
+                                        const double *__restrict__ md, double *__restrict__ nd) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t t = blockIdx.y;
+    const double v = fold_exact(X + i * d, X + cand[t] * d, d);
+    const double m = md[i];
+    nd[t * n + i] = v < m ? v : m;
+}
+
 // Sequential fold of each KPP_BLK block (one lane per block).
 static __global__ __attribute__((unused)) void kpp_block_sums_kernel(const double *__restrict__ v, int64_t n,
                                       double *__restrict__ bsum) {

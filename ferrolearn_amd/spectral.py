@@ -8,7 +8,8 @@ from LAPACK rather than faer, so the embedding -- and with it the labels --
 can differ from the reference's wherever an eigenvector's sign or a
 degenerate eigenspace's basis does.  Step 5 is the reference's call
 KMeans::new(k).with_n_init(n_init)[.with_random_state(seed)], fit, predict
-(:283-291), made on this library's KMeans.
+(:283-291), made on this library's KMeans with the reference's StdRng
+stream.
 """
 from __future__ import annotations
 
@@ -104,7 +105,8 @@ class SpectralClustering:
 
     def kmeans(self):
         """The K-Means of step 5 (spectral.rs:283-287)."""
-        km = KMeans(self.n_clusters).with_n_init(self.n_init).with_precision(self.precision)
+        km = (KMeans(self.n_clusters).with_n_init(self.n_init).with_precision(self.precision)
+              .with_rng("stdrng"))
         if self.random_state is not None:
             km = km.with_random_state(self.random_state)
         return km

@@ -112,6 +112,19 @@ fl_status fl_device_synchronize(void);
 
 /* kmeans.rs:55-113 (KMeans builder).  Defaults from KMeans::new
  * (kmeans.rs:76-84): max_iter 300, tol 1e-4, n_init 10, no seed. */
+/* The random stream behind k-means++ and MiniBatchKMeans' batches.
+ * FL_RNG_LIBRARY: the library's own generator, with k-means++'s sums over
+ * the distances taken in a fixed two-level order (blocks of 4096, then the
+ * block sums) so that they run in parallel; seeds follow the reference
+ * (random_state + run, random_state + run*1000003).
+ * FL_RNG_STDRNG: the reference's stream itself -- StdRng::seed_from_u64 of
+ * rand 0.9.4, i.e. ChaCha12 with rand_core's PCG32 key expansion, and its
+ * random_range / random::<f64> draws -- with the reference's plain left
+ * folds for the sums (kmeans.rs:200-228), so that a seeded fit draws what
+ * the reference draws.  The sums are then sequential on the device, which
+ * costs a few milliseconds per centre at n = 10^6. */
+typedef enum fl_rng { FL_RNG_LIBRARY = 0, FL_RNG_STDRNG = 1 } fl_rng;
+
 typedef struct fl_kmeans_params {
     int64_t n_clusters;
     int64_t max_iter;
@@ -120,6 +133,8 @@ typedef struct fl_kmeans_params {
     uint64_t random_state;  /* used when has_random_state != 0 */
     int32_t has_random_state;
     int32_t precision;      /* fl_precision for the assignment filter */
+    int32_t rng;            /* fl_rng: generator of the k-means++ draws */
+    int32_t reserved;       /* zero */
 } fl_kmeans_params;
 
 /* kmeans.rs:76-84 (KMeans::new) */
@@ -245,6 +260,8 @@ typedef struct fl_minibatch_params {
     uint64_t random_state;  /* used when has_random_state != 0 */
     int32_t has_random_state;
     int32_t precision;      /* fl_precision for the final full assignment */
+    int32_t rng;            /* fl_rng: generator of k-means++ and the batches */
+    int32_t reserved;       /* zero */
 } fl_minibatch_params;
 
 /* mini_batch_kmeans.rs:108-118 (MiniBatchKMeans::new) */

@@ -75,6 +75,23 @@ def lib():
         L.or_knn_query.argtypes = [_dp, _i64, _i64, _dp, _i64, _ip, _dp]
         L.or_kneighbors.restype = None
         L.or_kneighbors.argtypes = [_dp, _i64, _i64, _dp, _i64, _i64, _ip, _dp, C.c_int]
+        _u32p = C.POINTER(C.c_uint32)
+        L.or_chacha_block.restype = None
+        L.or_chacha_block.argtypes = [_u32p, C.c_int, _u32p]
+        L.or_stdrng_stream.restype = None
+        L.or_stdrng_stream.argtypes = [C.c_uint64, _i64, _u32p]
+        L.or_stdrng_ranges.restype = None
+        L.or_stdrng_ranges.argtypes = [C.c_uint64, _ip, _ip, _i64, _ip]
+        L.or_stdrng_f64s.restype = None
+        L.or_stdrng_f64s.argtypes = [C.c_uint64, _i64, _dp]
+        L.or_kmeanspp_stdrng.restype = None
+        L.or_kmeanspp_stdrng.argtypes = [_dp, _i64, _i64, _i64, C.c_uint64, _dp, C.c_int]
+        L.or_kmeans_fit_stdrng.restype = None
+        L.or_kmeans_fit_stdrng.argtypes = [_dp, _i64, _i64, _i64, _i64, C.c_double, _i64,
+                                           C.c_uint64, C.c_int, _dp, _ip, _dp, _ip]
+        L.or_minibatch_fit_stdrng.restype = None
+        L.or_minibatch_fit_stdrng.argtypes = [_dp, _i64, _i64, _i64, _i64, _i64, C.c_double,
+                                              _i64, C.c_uint64, C.c_int, _dp, _ip, _dp, _ip]
         L.or_pairwise_argmin_min.restype = None
         L.or_pairwise_argmin_min.argtypes = [_dp, _i64, _dp, _i64, _i64, _ip, _dp, C.c_int]
         L.or_class_scores.restype = None
@@ -311,6 +328,73 @@ def pairwise_argmin_min(X, Y, nthreads=0):
     mins = np.zeros(n, np.float64)
     lib().or_pairwise_argmin_min(_p(X), n, _p(Y), m, d, _pi(idx), _p(mins), nthreads)
     return idx, mins
+
+
+# ------------------------------------------------------- reference stream --
+# StdRng of rand 0.9.4 (ChaCha12) as restated in ref_cpu.c.
+
+def chacha_block(state, rounds):
+    """One ChaCha block (state: 16 u32 words)."""
+    a = np.ascontiguousarray(np.asarray(state, dtype=np.uint32))
+    out = np.zeros(16, np.uint32)
+    lib().or_chacha_block(a.ctypes.data_as(C.POINTER(C.c_uint32)), rounds,
+                          out.ctypes.data_as(C.POINTER(C.c_uint32)))
+    return out
+
+
+def stdrng_stream(seed, count):
+    out = np.zeros(count, np.uint32)
+    lib().or_stdrng_stream(C.c_uint64(seed), count, out.ctypes.data_as(C.POINTER(C.c_uint32)))
+    return out
+
+
+def stdrng_ranges(seed, lo, hi):
+    """random_range(lo[i]..hi[i]) for i in order, from one generator."""
+    lo = _i64a(lo)
+    hi = _i64a(hi)
+    out = np.zeros(lo.shape[0], np.int64)
+    lib().or_stdrng_ranges(C.c_uint64(seed), _pi(lo), _pi(hi), lo.shape[0], _pi(out))
+    return out
+
+
+def stdrng_f64s(seed, count):
+    out = np.zeros(count, np.float64)
+    lib().or_stdrng_f64s(C.c_uint64(seed), count, _p(out))
+    return out
+
+
+def kmeanspp_stdrng(X, k, seed, nthreads=0):
+    X = _f64(X)
+    n, d = X.shape
+    out = np.zeros((k, d), np.float64)
+    lib().or_kmeanspp_stdrng(_p(X), n, d, k, C.c_uint64(seed), _p(out), nthreads)
+    return out
+
+
+def kmeans_fit_stdrng(X, k, max_iter, tol, n_init, seed, nthreads=0):
+    X = _f64(X)
+    n, d = X.shape
+    centers = np.zeros((k, d), np.float64)
+    labels = np.zeros(n, np.int64)
+    inertia = C.c_double()
+    n_iter = C.c_int64()
+    lib().or_kmeans_fit_stdrng(_p(X), n, d, k, max_iter, C.c_double(tol), n_init,
+                               C.c_uint64(seed), nthreads, _p(centers), _pi(labels),
+                               C.byref(inertia), C.byref(n_iter))
+    return centers, labels, float(inertia.value), int(n_iter.value)
+
+
+def minibatch_fit_stdrng(X, k, batch, max_iter, tol, n_init, seed, nthreads=0):
+    X = _f64(X)
+    n, d = X.shape
+    centers = np.zeros((k, d), np.float64)
+    labels = np.zeros(n, np.int64)
+    inertia = C.c_double()
+    n_iter = C.c_int64()
+    lib().or_minibatch_fit_stdrng(_p(X), n, d, k, batch, max_iter, C.c_double(tol), n_init,
+                                  C.c_uint64(seed), nthreads, _p(centers), _pi(labels),
+                                  C.byref(inertia), C.byref(n_iter))
+    return centers, labels, float(inertia.value), int(n_iter.value)
 
 
 def class_positions(y):

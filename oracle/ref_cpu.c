@@ -291,6 +291,139 @@ void or_rng_stream(uint64_t seed, int64_t count, uint64_t *out) {
     for (int64_t i = 0; i < count; ++i) out[i] = or_rng_next(&r);
 }
 
+/* ------------------------------------------------------------------------ */
+/* f1: the reference's own stream -- StdRng of rand 0.9.4 (Cargo.lock:2413),  */
+/* i.e. rand_chacha 0.9.0's ChaCha12Rng seeded by rand_core 0.9.5's default   */
+/* seed_from_u64, restated from the crates' published algorithms (the crates  */
+/* are not vendored): PCG32 (XSH-RR) key expansion, 64-bit block counter and  */
+/* zero stream id, 12 rounds plus feed-forward, u32 words in block order,     */
+/* next_u64 = low word first; random::<f64>() = (next_u64 >> 11) * 2^-53;    */
+/* random_range(lo..hi) over usize = UniformUsize: u32 sampling when hi - 1   */
+/* fits in 32 bits, else u64, both with Canon's method.  The ChaCha block is  */
+/* pinned by RFC 8439's vectors; the rest is parity unpinned against the real */
+/* crates and pinned against an independent pure-Python restatement.          */
+/* ------------------------------------------------------------------------ */
+
+static uint32_t or_rotl32(uint32_t v, int k) { return (v << k) | (v >> (32 - k)); }
+
+void or_chacha_block(const uint32_t *in, int rounds, uint32_t *out) {
+    uint32_t x[16];
+    for (int i = 0; i < 16; ++i) x[i] = in[i];
+#define OR_QR(a, b, c, d)                                   \
+    x[a] += x[b]; x[d] = or_rotl32(x[d] ^ x[a], 16);        \
+    x[c] += x[d]; x[b] = or_rotl32(x[b] ^ x[c], 12);        \
+    x[a] += x[b]; x[d] = or_rotl32(x[d] ^ x[a], 8);         \
+    x[c] += x[d]; x[b] = or_rotl32(x[b] ^ x[c], 7);
+    for (int r = 0; r < rounds; r += 2) {
+        OR_QR(0, 4, 8, 12) OR_QR(1, 5, 9, 13) OR_QR(2, 6, 10, 14) OR_QR(3, 7, 11, 15)
+        OR_QR(0, 5, 10, 15) OR_QR(1, 6, 11, 12) OR_QR(2, 7, 8, 13) OR_QR(3, 4, 9, 14)
+    }
+#undef OR_QR
+    for (int i = 0; i < 16; ++i) out[i] = x[i] + in[i];
+}
+
+typedef struct {
+    uint32_t key[8];
+    uint64_t ctr;
+    uint32_t buf[16];
+    int idx;
+} or_stdrng;
+
+static void or_stdrng_seed(or_stdrng *g, uint64_t seed) {
+    uint64_t s = seed;
+    for (int i = 0; i < 8; ++i) {
+        s = s * 6364136223846793005ULL + 11634580027462260723ULL;
+        uint32_t xs = (uint32_t)(((s >> 18) ^ s) >> 27);
+        int rot = (int)(s >> 59);
+        g->key[i] = rot ? (xs >> rot) | (xs << (32 - rot)) : xs;
+    }
+    g->ctr = 0;
+    g->idx = 16;
+}
+static uint32_t or_stdrng_u32(or_stdrng *g) {
+    if (g->idx == 16) {
+        uint32_t st[16] = {0x61707865u, 0x3320646eu, 0x79622d32u, 0x6b206574u};
+        for (int i = 0; i < 8; ++i) st[4 + i] = g->key[i];
+        st[12] = (uint32_t)g->ctr;
+        st[13] = (uint32_t)(g->ctr >> 32);
+        st[14] = st[15] = 0;
+        or_chacha_block(st, 12, g->buf);
+        g->ctr += 1;
+        g->idx = 0;
+    }
+    return g->buf[g->idx++];
+}
+static uint64_t or_stdrng_u64(or_stdrng *g) {
+    uint64_t lo = or_stdrng_u32(g);
+    uint64_t hi = or_stdrng_u32(g);
+    return (hi << 32) | lo;
+}
+static double or_stdrng_f64(or_stdrng *g) {
+    return (double)(or_stdrng_u64(g) >> 11) * 0x1.0p-53;
+}
+static int64_t or_stdrng_range_from(or_stdrng *g, int64_t lo, int64_t hi) {
+    uint64_t low = (uint64_t)lo, high = (uint64_t)hi - 1;
+    if ((high >> 32) == 0) {
+        uint32_t r = (uint32_t)high - (uint32_t)low + 1u;
+        if (r == 0) return (int64_t)(uint32_t)((uint32_t)low + or_stdrng_u32(g));
+        uint64_t m = (uint64_t)or_stdrng_u32(g) * r;
+        uint32_t result = (uint32_t)(m >> 32), lo_order = (uint32_t)m;
+        if (lo_order > 0u - r) {
+            uint32_t nh = (uint32_t)(((uint64_t)or_stdrng_u32(g) * r) >> 32);
+            if ((uint64_t)lo_order + nh > 0xffffffffULL) result += 1u;
+        }
+        return (int64_t)(uint32_t)((uint32_t)low + result);
+    }
+    uint64_t r = high - low + 1;
+    if (r == 0) return (int64_t)(low + or_stdrng_u64(g));
+    __uint128_t m = (__uint128_t)or_stdrng_u64(g) * r;
+    uint64_t result = (uint64_t)(m >> 64), lo_order = (uint64_t)m;
+    if (lo_order > 0ULL - r) {
+        uint64_t nh = (uint64_t)(((__uint128_t)or_stdrng_u64(g) * r) >> 64);
+        if (lo_order + nh < lo_order) result += 1;
+    }
+    return (int64_t)(low + result);
+}
+
+/* Exported for the tests: the first u32 words, and sequences of draws. */
+void or_stdrng_stream(uint64_t seed, int64_t count, uint32_t *out) {
+    or_stdrng g;
+    or_stdrng_seed(&g, seed);
+    for (int64_t i = 0; i < count; ++i) out[i] = or_stdrng_u32(&g);
+}
+void or_stdrng_ranges(uint64_t seed, const int64_t *lo, const int64_t *hi, int64_t count,
+                      int64_t *out) {
+    or_stdrng g;
+    or_stdrng_seed(&g, seed);
+    for (int64_t i = 0; i < count; ++i) out[i] = or_stdrng_range_from(&g, lo[i], hi[i]);
+}
+void or_stdrng_f64s(uint64_t seed, int64_t count, double *out) {
+    or_stdrng g;
+    or_stdrng_seed(&g, seed);
+    for (int64_t i = 0; i < count; ++i) out[i] = or_stdrng_f64(&g);
+}
+
+/* Either generator behind one interface: mode 0 is the library's stream
+ * (two-level sums below), mode 1 the reference's StdRng stream with the
+ * reference's plain left folds. */
+typedef struct {
+    int mode;
+    or_rng lib;
+    or_stdrng std;
+} or_gen;
+
+static void or_gen_seed(or_gen *g, int mode, uint64_t seed) {
+    g->mode = mode;
+    if (mode) or_stdrng_seed(&g->std, seed);
+    else or_rng_seed(&g->lib, seed);
+}
+static double or_gen_f64(or_gen *g) {
+    return g->mode ? or_stdrng_f64(&g->std) : or_rng_f64(&g->lib);
+}
+static int64_t or_gen_range_from(or_gen *g, int64_t lo, int64_t hi) {
+    return g->mode ? or_stdrng_range_from(&g->std, lo, hi) : lo + or_rng_range(&g->lib, hi - lo);
+}
+
 /* The reference's sums over min_dists are plain left folds (kmeans.rs:200,
  * :207-213, :222-228).  This restatement uses a fixed two-level order
  * instead: blocks of OR_BLK consecutive values are folded left to right,
@@ -328,31 +461,62 @@ int64_t or_blocked_search(const double *v, int64_t n, double threshold) {
     return n - 1;
 }
 
+/* kmeans.rs:200, :207-213, :222-228 exactly: the plain left fold, and the
+ * first index whose running left fold reaches threshold (n-1 if none). */
+double or_seq_sum(const double *v, int64_t n) {
+    double s = 0.0;
+    for (int64_t i = 0; i < n; ++i) s = s + v[i];
+    return s;
+}
+int64_t or_seq_search(const double *v, int64_t n, double threshold) {
+    double c = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        c = c + v[i];
+        if (c >= threshold) return i;
+    }
+    return n - 1;
+}
+static double or_gen_sum(const or_gen *g, const double *v, int64_t n) {
+    return g->mode ? or_seq_sum(v, n) : or_blocked_sum(v, n);
+}
+static int64_t or_gen_search(const or_gen *g, const double *v, int64_t n, double thr) {
+    return g->mode ? or_seq_search(v, n, thr) : or_blocked_search(v, n, thr);
+}
+
 /* kmeans.rs:169-253 (kmeans_plus_plus), greedy multi-trial k-means++:
  * first centre uniform; n_trials = max(2 + floor(ln k), 1); for each new
  * centre, if the total potential is <= 0 pick uniformly, else draw n_trials
  * candidates with probability proportional to D(x)^2 and keep the one with
  * the strictly smallest resulting potential (first wins ties). */
-static void or_kmeanspp_rng(const double *X, int64_t n, int64_t d, int64_t k,
-                            or_rng *rngp, double *centers, int nthreads);
+static void or_kmeanspp_gen(const double *X, int64_t n, int64_t d, int64_t k,
+                            or_gen *rngp, double *centers, int nthreads);
 
 void or_kmeanspp(const double *X, int64_t n, int64_t d, int64_t k,
                  uint64_t seed, double *centers, int nthreads) {
-    or_rng rng;
-    or_rng_seed(&rng, seed);
-    or_kmeanspp_rng(X, n, d, k, &rng, centers, nthreads);
+    or_gen rng;
+    or_gen_seed(&rng, 0, seed);
+    or_kmeanspp_gen(X, n, d, k, &rng, centers, nthreads);
+}
+
+/* The same with the reference's StdRng stream and plain left folds
+ * (kmeans.rs:169-253 with StdRng::seed_from_u64(seed), :460). */
+void or_kmeanspp_stdrng(const double *X, int64_t n, int64_t d, int64_t k,
+                        uint64_t seed, double *centers, int nthreads) {
+    or_gen rng;
+    or_gen_seed(&rng, 1, seed);
+    or_kmeanspp_gen(X, n, d, k, &rng, centers, nthreads);
 }
 
 /* The same, drawing from a generator the caller keeps using afterwards
  * (MiniBatchKMeans takes its batches from the run's generator,
  * mini_batch_kmeans.rs:528-555). */
-static void or_kmeanspp_rng(const double *X, int64_t n, int64_t d, int64_t k,
-                            or_rng *rngp, double *centers, int nthreads) {
+static void or_kmeanspp_gen(const double *X, int64_t n, int64_t d, int64_t k,
+                            or_gen *rngp, double *centers, int nthreads) {
     nthreads = or_threads(nthreads);
-    or_rng rng = *rngp;
+    or_gen rng = *rngp;
     memset(centers, 0, (size_t)(k * d) * sizeof(double));
     if (n == 0) return;
-    int64_t first = or_rng_range(&rng, n);
+    int64_t first = or_gen_range_from(&rng, 0, n);
     memcpy(centers, X + first * d, (size_t)d * sizeof(double));
     if (k == 1) {
         *rngp = rng;
@@ -366,9 +530,9 @@ static void or_kmeanspp_rng(const double *X, int64_t n, int64_t d, int64_t k,
     int64_t n_trials = 2 + (int64_t)floor(log((double)k));
     if (n_trials < 1) n_trials = 1;
     for (int64_t c = 1; c < k; ++c) {
-        double total = or_blocked_sum(md, n);
+        double total = or_gen_sum(&rng, md, n);
         if (total <= 0.0) {
-            int64_t idx = or_rng_range(&rng, n);
+            int64_t idx = or_gen_range_from(&rng, 0, n);
             memcpy(centers + c * d, X + idx * d, (size_t)d * sizeof(double));
             continue;
         }
@@ -376,15 +540,15 @@ static void or_kmeanspp_rng(const double *X, int64_t n, int64_t d, int64_t k,
         double best_pot = 0.0;
         int have = 0;
         for (int64_t t = 0; t < n_trials; ++t) {
-            double thr = or_rng_f64(&rng) * total;
-            int64_t cand = or_blocked_search(md, n, thr);
+            double thr = or_gen_f64(&rng) * total;
+            int64_t cand = or_gen_search(&rng, md, n, thr);
             const double *cr = X + cand * d;
 #pragma omp parallel for schedule(static) num_threads(nthreads)
             for (int64_t i = 0; i < n; ++i) {
                 double v = or_sqeuclid(X + i * d, cr, d);
                 nd[i] = v < md[i] ? v : md[i];
             }
-            double pot = or_blocked_sum(nd, n);
+            double pot = or_gen_sum(&rng, nd, n);
             if (!have || pot < best_pot) {
                 have = 1;
                 best_pot = pot;
@@ -436,12 +600,12 @@ double or_minibatch_update(const double *X, int64_t d, const int64_t *batch_idx,
 
 /* mini_batch_kmeans.rs:451-472 (sample_batch_indices) with the generator
  * above: a full shuffle when batch >= n, else a partial Fisher-Yates. */
-void or_sample_batch(int64_t n, int64_t batch, or_rng *rng, int64_t *pool,
-                     int64_t *out, int64_t *out_len) {
+static void or_sample_batch(int64_t n, int64_t batch, or_gen *rng, int64_t *pool,
+                            int64_t *out, int64_t *out_len) {
     for (int64_t i = 0; i < n; ++i) pool[i] = i;
     if (batch >= n) {
         for (int64_t i = 0; i < n; ++i) {
-            int64_t j = i + or_rng_range(rng, n - i);
+            int64_t j = or_gen_range_from(rng, i, n);
             int64_t t = pool[i];
             pool[i] = pool[j];
             pool[j] = t;
@@ -451,7 +615,7 @@ void or_sample_batch(int64_t n, int64_t batch, or_rng *rng, int64_t *pool,
         return;
     }
     for (int64_t i = 0; i < batch; ++i) {
-        int64_t j = i + or_rng_range(rng, n - i);
+        int64_t j = or_gen_range_from(rng, i, n);
         int64_t t = pool[i];
         pool[i] = pool[j];
         pool[j] = t;
@@ -469,8 +633,8 @@ void or_minibatch_run(const double *X, int64_t n, int64_t d, int64_t k,
                       double tol, uint64_t seed, int nthreads,
                       double *centers_out, int64_t *labels_out,
                       double *inertia_out, int64_t *n_iter_out) {
-    or_rng rng;
-    or_rng_seed(&rng, seed);
+    or_gen rng;
+    or_gen_seed(&rng, 0, seed);
     memcpy(centers_out, init, (size_t)(k * d) * sizeof(double));
     int64_t *counts = (int64_t *)malloc((size_t)k * sizeof(int64_t));
     for (int64_t c = 0; c < k; ++c) counts[c] = 1;
@@ -503,11 +667,13 @@ void or_minibatch_run(const double *X, int64_t n, int64_t d, int64_t k,
 
 /* mini_batch_kmeans.rs:478-592, the whole fit without given centres: run r
  * seeds the generator with seed + r*1000003, draws k-means++ from it and
- * then its batches; the run with the strictly smallest inertia is kept. */
-void or_minibatch_fit(const double *X, int64_t n, int64_t d, int64_t k, int64_t batch,
-                      int64_t max_iter, double tol, int64_t n_init, uint64_t seed,
-                      int nthreads, double *centers_out, int64_t *labels_out,
-                      double *inertia_out, int64_t *n_iter_out) {
+ * then its batches; the run with the strictly smallest inertia is kept.
+ * mode: 0 the library's generator, 1 the reference's StdRng stream. */
+static void or_minibatch_fit_gen(int mode, const double *X, int64_t n, int64_t d, int64_t k,
+                                 int64_t batch, int64_t max_iter, double tol, int64_t n_init,
+                                 uint64_t seed, int nthreads, double *centers_out,
+                                 int64_t *labels_out, double *inertia_out,
+                                 int64_t *n_iter_out) {
     double *C = (double *)malloc((size_t)(k * d) * sizeof(double));
     int64_t *L = (int64_t *)malloc((size_t)n * sizeof(int64_t));
     int64_t *counts = (int64_t *)malloc((size_t)k * sizeof(int64_t));
@@ -517,9 +683,9 @@ void or_minibatch_fit(const double *X, int64_t n, int64_t d, int64_t k, int64_t 
     int64_t *blab = (int64_t *)malloc((size_t)bmax * sizeof(int64_t));
     double *dist = (double *)malloc((size_t)n * sizeof(double));
     for (int64_t r = 0; r < n_init; ++r) {
-        or_rng rng;
-        or_rng_seed(&rng, seed + (uint64_t)r * 1000003ULL);
-        or_kmeanspp_rng(X, n, d, k, &rng, C, nthreads);
+        or_gen rng;
+        or_gen_seed(&rng, mode, seed + (uint64_t)r * 1000003ULL);
+        or_kmeanspp_gen(X, n, d, k, &rng, C, nthreads);
         for (int64_t c = 0; c < k; ++c) counts[c] = 1;
         int64_t it = 0;
         for (int64_t q = 0; q < max_iter; ++q) {
@@ -548,6 +714,49 @@ void or_minibatch_fit(const double *X, int64_t n, int64_t d, int64_t k, int64_t 
     free(bidx);
     free(blab);
     free(dist);
+}
+
+void or_minibatch_fit(const double *X, int64_t n, int64_t d, int64_t k, int64_t batch,
+                      int64_t max_iter, double tol, int64_t n_init, uint64_t seed,
+                      int nthreads, double *centers_out, int64_t *labels_out,
+                      double *inertia_out, int64_t *n_iter_out) {
+    or_minibatch_fit_gen(0, X, n, d, k, batch, max_iter, tol, n_init, seed, nthreads,
+                         centers_out, labels_out, inertia_out, n_iter_out);
+}
+
+void or_minibatch_fit_stdrng(const double *X, int64_t n, int64_t d, int64_t k, int64_t batch,
+                             int64_t max_iter, double tol, int64_t n_init, uint64_t seed,
+                             int nthreads, double *centers_out, int64_t *labels_out,
+                             double *inertia_out, int64_t *n_iter_out) {
+    or_minibatch_fit_gen(1, X, n, d, k, batch, max_iter, tol, n_init, seed, nthreads,
+                         centers_out, labels_out, inertia_out, n_iter_out);
+}
+
+/* kmeans.rs:403-513 with StdRng: run r draws k-means++ from
+ * StdRng::seed_from_u64(seed + r) (:460), then runs Lloyd; the run with the
+ * strictly smallest inertia is kept. */
+void or_kmeans_fit_stdrng(const double *X, int64_t n, int64_t d, int64_t k, int64_t max_iter,
+                          double tol, int64_t n_init, uint64_t seed, int nthreads,
+                          double *centers_out, int64_t *labels_out, double *inertia_out,
+                          int64_t *n_iter_out) {
+    double *init = (double *)malloc((size_t)(k * d) * sizeof(double));
+    double *C = (double *)malloc((size_t)(k * d) * sizeof(double));
+    int64_t *L = (int64_t *)malloc((size_t)(n > 0 ? n : 1) * sizeof(int64_t));
+    for (int64_t r = 0; r < n_init; ++r) {
+        double in;
+        int64_t it;
+        or_kmeanspp_stdrng(X, n, d, k, seed + (uint64_t)r, init, nthreads);
+        or_kmeans_lloyd(X, n, d, k, init, max_iter, tol, nthreads, C, L, &in, &it);
+        if (r == 0 || in < *inertia_out) {
+            *inertia_out = in;
+            *n_iter_out = it;
+            memcpy(centers_out, C, (size_t)(k * d) * sizeof(double));
+            memcpy(labels_out, L, (size_t)n * sizeof(int64_t));
+        }
+    }
+    free(init);
+    free(C);
+    free(L);
 }
 
 /* ------------------------------------------------------------------------ */

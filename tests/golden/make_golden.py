@@ -15,6 +15,12 @@ Sources of each case:
   vote / mean     knn.rs:388-440, :952-988
   minibatch       mini_batch_kmeans.rs:410-447
   rng             SplitMix64 + xoshiro256** (the library's own generator)
+  stdrng_*        the reference's StdRng of rand 0.9.4: ChaCha12 keyed by
+                  rand_core 0.9.5's PCG32 seed_from_u64, random_range over
+                  usize (UniformUsize, Canon's method) and random::<f64>;
+                  k-means++ / MiniBatch / KMeans fits drawing from it with
+                  plain left folds (kmeans.rs:169-253, :451-503;
+                  mini_batch_kmeans.rs:230-320, :451-472, :528-592)
 
 Run:  python tests/golden/make_golden.py
 """
@@ -204,6 +210,9 @@ class Gen:
         s[3] = rotl(s[3], 45)
         return out
 
+    def range_from(self, lo, hi):
+        return lo + self.range(hi - lo)
+
     def range(self, n):
         return (self.next() * n) >> 64
 
@@ -240,13 +249,33 @@ def blocked_search(v, thr):
     return len(v) - 1
 
 
-def kmeanspp(X, k, g):
+def plain_sum(v):
+    """kmeans.rs:200: the left fold from 0."""
+    s = 0.0
+    for x in v:
+        s = s + x
+    return s
+
+
+def plain_search(v, thr):
+    """kmeans.rs:205-214: first index whose running left fold reaches thr."""
+    c = 0.0
+    for i, x in enumerate(v):
+        c = c + x
+        if c >= thr:
+            return i
+    return len(v) - 1
+
+
+def kmeanspp(X, k, g, total_of=None, search=None):
     """kmeans.rs:169-253 with the library's generator g and sums: the first
     row uniform; n_trials = 2 + floor(ln k); per centre the total of the
     minimum distances; a uniform row if it is <= 0, else n_trials rows drawn
     by D^2 thresholds, keeping the first with the strictly smallest
     potential.  Returns the picked row indices.  For n <= 4096 every sum is
     the plain left fold of the reference."""
+    total_of = total_of or blocked_sum
+    search = search or blocked_search
     n = len(X)
     picks = [g.range(n)]
     if k == 1:
@@ -254,19 +283,19 @@ def kmeanspp(X, k, g):
     md = [fold(x, X[picks[0]]) for x in X]
     n_trials = max(2 + int(math.floor(math.log(k))), 1)
     for _ in range(1, k):
-        total = blocked_sum(md)
+        total = total_of(md)
         if total <= 0.0:
             picks.append(g.range(n))
             continue
         best, best_pot, best_nd = 0, None, None
         for _ in range(n_trials):
             thr = g.f64() * total
-            cand = blocked_search(md, thr)
+            cand = search(md, thr)
             nd = []
             for i, x in enumerate(X):
                 v = fold(x, X[cand])
                 nd.append(v if v < md[i] else md[i])
-            pot = blocked_sum(nd)
+            pot = total_of(nd)
             if best_pot is None or pot < best_pot:
                 best, best_pot, best_nd = cand, pot, nd
         picks.append(best)
@@ -280,18 +309,18 @@ def sample_batch(g, n, batch):
     pool = list(range(n))
     if batch >= n:
         for i in range(n):
-            j = i + g.range(n - i)
+            j = g.range_from(i, n)
             pool[i], pool[j] = pool[j], pool[i]
         return pool
     out = []
     for i in range(batch):
-        j = i + g.range(n - i)
+        j = g.range_from(i, n)
         pool[i], pool[j] = pool[j], pool[i]
         out.append(pool[i])
     return out
 
 
-def minibatch_fit(X, k, batch, max_iter, tol, n_init, base):
+def minibatch_fit(X, k, batch, max_iter, tol, n_init, base, make=None, sums=()):
     """mini_batch_kmeans.rs:478-592: run r draws from the generator seeded
     base + r*1000003 (k-means++ first, then the batches); counts start at 1;
     per iteration sample, assign (:381-404), update online (:410-447),
@@ -299,8 +328,8 @@ def minibatch_fit(X, k, batch, max_iter, tol, n_init, base):
     left fold in row order (:343-376); keep the strictly smallest inertia."""
     best = None
     for r in range(n_init):
-        g = Gen(base + r * 1000003)
-        C = [list(X[i]) for i in kmeanspp(X, k, g)]
+        g = (make or Gen)(base + r * 1000003)
+        C = [list(X[i]) for i in kmeanspp(X, k, g, *sums)]
         counts = [1] * k
         it = 0
         for _ in range(max_iter):
@@ -317,6 +346,96 @@ def minibatch_fit(X, k, batch, max_iter, tol, n_init, base):
             inertia = inertia + v
         if best is None or inertia < best[2]:
             best = (C, labels, inertia, it)
+    return best
+
+
+class StdRng:
+    """StdRng::seed_from_u64 of rand 0.9.4, written independently of the C
+    oracle.  Seed: eight PCG32 (XSH-RR) outputs, each taken after one LCG
+    step, little-endian into the 32-byte ChaCha key.  Stream: ChaCha with 12
+    rounds, 64-bit block counter from zero in words 12-13, zero stream id,
+    each block's 16 words in order."""
+    SIGMA = (0x61707865, 0x3320646E, 0x79622D32, 0x6B206574)
+    COLUMNS = ((0, 4, 8, 12), (1, 5, 9, 13), (2, 6, 10, 14), (3, 7, 11, 15))
+    DIAGONALS = ((0, 5, 10, 15), (1, 6, 11, 12), (2, 7, 8, 13), (3, 4, 9, 14))
+
+    def __init__(self, seed):
+        state = seed & M64
+        key = b""
+        for _ in range(8):
+            state = (state * 6364136223846793005 + 11634580027462260723) & M64
+            xorshifted = (((state >> 18) ^ state) >> 27) & 0xFFFFFFFF
+            rot = state >> 59
+            word = ((xorshifted >> rot) | (xorshifted << ((32 - rot) & 31))) & 0xFFFFFFFF
+            key += word.to_bytes(4, "little")
+        self.key = [int.from_bytes(key[4 * i:4 * i + 4], "little") for i in range(8)]
+        self.block = 0
+        self.words = []
+
+    @classmethod
+    def chacha(cls, inp, rounds):
+        w = list(inp)
+        def rotl(v, c):
+            return ((v << c) | (v >> (32 - c))) & 0xFFFFFFFF
+        for _ in range(rounds // 2):
+            for group in (cls.COLUMNS, cls.DIAGONALS):
+                for a, b, c, d in group:
+                    w[a] = (w[a] + w[b]) & 0xFFFFFFFF
+                    w[d] = rotl(w[d] ^ w[a], 16)
+                    w[c] = (w[c] + w[d]) & 0xFFFFFFFF
+                    w[b] = rotl(w[b] ^ w[c], 12)
+                    w[a] = (w[a] + w[b]) & 0xFFFFFFFF
+                    w[d] = rotl(w[d] ^ w[a], 8)
+                    w[c] = (w[c] + w[d]) & 0xFFFFFFFF
+                    w[b] = rotl(w[b] ^ w[c], 7)
+        return [(x + y) & 0xFFFFFFFF for x, y in zip(w, inp)]
+
+    def u32(self):
+        if not self.words:
+            inp = list(self.SIGMA) + self.key + [self.block & 0xFFFFFFFF, self.block >> 32, 0, 0]
+            self.words = self.chacha(inp, 12)[::-1]
+            self.block += 1
+        return self.words.pop()
+
+    def u64(self):
+        lo = self.u32()
+        return (self.u32() << 32) | lo
+
+    def f64(self):
+        """random::<f64>(): 53 high bits over 2^53."""
+        return (self.u64() >> 11) / 9007199254740992.0
+
+    def range_from(self, lo, hi):
+        """random_range(lo..hi) over usize: UniformUsize samples u32 when
+        hi - 1 < 2^32, else u64; Canon's method in that width."""
+        high = hi - 1
+        bits = 32 if high < (1 << 32) else 64
+        mask = (1 << bits) - 1
+        draw = self.u32 if bits == 32 else self.u64
+        span = (high - lo + 1) & mask
+        if span == 0:
+            return (lo + draw()) & mask
+        prod = draw() * span
+        result, low = prod >> bits, prod & mask
+        if low > ((1 << bits) - span):
+            extra = (draw() * span) >> bits
+            if low + extra > mask:
+                result += 1
+        return (lo + result) & mask
+
+    def range(self, n):
+        return self.range_from(0, n)
+
+
+def kmeans_fit_stdrng(X, k, max_iter, tol, n_init, base):
+    """kmeans.rs:451-503: run r seeds StdRng with base + r, k-means++ with
+    plain folds, Lloyd; the strictly smallest inertia wins."""
+    best = None
+    for r in range(n_init):
+        picks = kmeanspp(X, k, StdRng(base + r), plain_sum, plain_search)
+        C, L, inertia, it = lloyd(X, [list(X[i]) for i in picks], max_iter, tol)
+        if best is None or inertia < best[2]:
+            best = (C, L, inertia, it)
     return best
 
 
@@ -462,6 +581,62 @@ def main():
                     "centers": [[hx(v) for v in r] for r in C], "labels": L,
                     "inertia": hx(inertia), "n_iter": it})
     out["minibatch_fit"] = mbf
+
+    # --- the reference's StdRng stream ------------------------------------
+    stream = {}
+    for seed in (0, 1, 42, 12345678901234, M64):
+        g = StdRng(seed)
+        stream[str(seed)] = [g.u32() for _ in range(40)]
+    out["stdrng_stream"] = stream
+    spans = [(0, 10), (0, 3), (5, 9), (0, 2 ** 31 - 5), (0, 2 ** 32), (7, 2 ** 32 + 1),
+             (0, 2 ** 40), (2 ** 33, 2 ** 62 + 3), (0, 1), (100, 1000)] * 3
+    g = StdRng(7)
+    out["stdrng_ranges"] = {"seed": 7, "lo": [str(a) for a, _ in spans],
+                            "hi": [str(b) for _, b in spans],
+                            "out": [str(g.range_from(a, b)) for a, b in spans]}
+    g = StdRng(99)
+    out["stdrng_f64"] = {"seed": 99, "out": [hx(g.f64()) for _ in range(20)]}
+
+    rs = random.Random(20261205)
+    kps = []
+    for (n, d, k, seed, spread, dup) in ((300, 3, 5, 7, 1.0, False), (50, 4, 1, 5, 1.0, False),
+                                          (40, 2, 6, 9, 0.0, True), (9000, 2, 6, 11, 2.0, False),
+                                          (12, 3, 12, 2, 1.0, False),
+                                          (1500, 5, 17, 2 ** 40 + 3, 0.5, False)):
+        cen = [[rs.uniform(-10, 10) for _ in range(d)] for _ in range(max(k, 2))]
+        if dup:
+            X = [[1.5] * d for _ in range(n - 3)] + [[2.5] * d, [1.5] * d, [-1.0] * d]
+        else:
+            X = [[cen[i % len(cen)][j] + rs.gauss(0, spread) for j in range(d)] for i in range(n)]
+        picks = kmeanspp(X, k, StdRng(seed), plain_sum, plain_search)
+        kps.append({"X": [[hx(v) for v in r] for r in X], "k": k, "seed": str(seed),
+                    "picks": picks})
+    out["kmeanspp_stdrng"] = kps
+
+    kfs = []
+    for (n, d, k, max_iter, tol, n_init, base) in ((400, 3, 5, 50, 1e-4, 3, 17),
+                                                   (250, 2, 4, 30, 0.0, 2, 0)):
+        cen = [[rs.uniform(-8, 8) for _ in range(d)] for _ in range(k)]
+        X = [[cen[i % k][j] + rs.gauss(0, 1.2) for j in range(d)] for i in range(n)]
+        C, L, inertia, it = kmeans_fit_stdrng(X, k, max_iter, tol, n_init, base)
+        kfs.append({"X": [[hx(v) for v in r] for r in X], "k": k, "max_iter": max_iter,
+                    "tol": tol, "n_init": n_init, "seed": base,
+                    "centers": [[hx(v) for v in r] for r in C], "labels": L,
+                    "inertia": hx(inertia), "n_iter": it})
+    out["kmeans_fit_stdrng"] = kfs
+
+    mbs2 = []
+    for (n, d, k, batch, max_iter, tol, n_init, base) in ((200, 3, 4, 50, 10, 0.0, 2, 11),
+                                                          (60, 2, 3, 100, 5, 0.0, 2, 5)):
+        cen = [[rs.uniform(-8, 8) for _ in range(d)] for _ in range(k)]
+        X = [[cen[i % k][j] + rs.gauss(0, 1.5) for j in range(d)] for i in range(n)]
+        C, L, inertia, it = minibatch_fit(X, k, batch, max_iter, tol, n_init, base, StdRng,
+                                          (plain_sum, plain_search))
+        mbs2.append({"X": [[hx(v) for v in r] for r in X], "k": k, "batch": batch,
+                     "max_iter": max_iter, "tol": tol, "n_init": n_init, "seed": base,
+                     "centers": [[hx(v) for v in r] for r in C], "labels": L,
+                     "inertia": hx(inertia), "n_iter": it})
+    out["minibatch_fit_stdrng"] = mbs2
 
     with open(os.path.join(HERE, "golden_vectors.json"), "w") as f:
         json.dump(out, f, indent=0, separators=(",", ":"))

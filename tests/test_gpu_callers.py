@@ -95,15 +95,15 @@ def test_spectral_reference_cases():
 
 @pytest.mark.parametrize("seed,n_init", [(None, 3), (11, 4)])
 def test_spectral_kmeans_step_matches_oracle(seed, n_init):
-    # step 5 given the embedding: n_init library k-means++ restarts (seeds
-    # base + r), the strictly smallest inertia, then predict on the embedding
+    # step 5 given the embedding: n_init k-means++ restarts from the
+    # reference's StdRng (seeds base + r), the strictly smallest inertia, then
+    # predict on the embedding
     x = blobs(400, 5, 4, 8, spread=1.5)
     sc = F.SpectralClustering(4).with_gamma(0.05).with_n_init(n_init)
     if seed is not None:
         sc = sc.with_random_state(seed)
     e = sc.embed(x)
     base = 0 if seed is None else seed
-    inits = np.stack([O.kmeanspp(e, 4, base + r) for r in range(n_init)])
-    c_o, _, _, _, _ = O.kmeans_fit_from(e, inits, 300, 1e-4)
+    c_o, _, _, _ = O.kmeans_fit_stdrng(e, 4, 300, 1e-4, n_init, base)
     want = O.assign(e, c_o)[0]
     assert (sc.fit(x).labels() == want).all()

@@ -552,3 +552,77 @@ def test_pairwise_argmin_cancellation_and_nan():
     Xn = np.array([[np.nan, 1.0], [0.0, 0.0]])
     idx, mins = O.pairwise_argmin_min(Xn, np.array([[5.0, 5.0], [0.0, 1.0]]))
     assert list(idx) == [0, 1] and mins[0] == 0.0 and mins[1] == 1.0
+
+
+# ---------------------------------------------------------------- StdRng ---
+
+def _block_hex(fn, rounds, ctr, key=(0,) * 8, tail=(0, 0, 0)):
+    st = [0x61707865, 0x3320646E, 0x79622D32, 0x6B206574, *key, ctr, *tail]
+    return np.asarray(fn(st, rounds), dtype="<u4").tobytes().hex()
+
+
+def test_chacha_block_published_vectors():
+    # RFC 8439 section 2.3.2 (key 00..1f, counter 1, nonce 0:4a:0:0:0:9 words)
+    # and the zero-key zero-nonce keystreams of ChaCha20 (blocks 0 and 1) and
+    # ChaCha12 (block 0), both restatements of the block function
+    from tests.golden import make_golden as MG
+    key = [0x03020100, 0x07060504, 0x0B0A0908, 0x0F0E0D0C,
+           0x13121110, 0x17161514, 0x1B1A1918, 0x1F1E1D1C]
+    rfc = ("10f1e7e4d13b5915500fdd1fa32071c4c7d1f4c733c068030422aa9ac3d46c4e"
+           "d2826446079faa0914c2d705d98b02a2b5129cd1de164eb9cbd083e8a2503c4e")
+    zero20 = ("76b8e0ada0f13d90405d6ae55386bd28bdd219b8a08ded1aa836efcc8b770dc7"
+              "da41597c5157488d7724e03fb8d84a376a43b8f41518a11cc387b669b2ee6586")
+    zero20b = ("9f07e7be5551387a98ba977c732d080dcb0f29a048e3656912c6533e32ee7aed"
+               "29b721769ce64e43d57133b074d839d531ed1f28510afb45ace10a1f4b794d6f")
+    zero12 = ("9bf49a6a0755f953811fce125f2683d50429c3bb49e074147e0089a52eae155f"
+              "0564f879d27ae3c02ce82834acfa8c793a629f2ca0de6919610be82f411326be")
+    for fn in (O.chacha_block, MG.StdRng.chacha):
+        assert _block_hex(fn, 20, 1, key, (0x09000000, 0x4A000000, 0)) == rfc
+        assert _block_hex(fn, 20, 0) == zero20
+        assert _block_hex(fn, 20, 1) == zero20b
+        assert _block_hex(fn, 12, 0) == zero12
+
+
+def test_stdrng_draws_golden():
+    for seed, words in G["stdrng_stream"].items():
+        assert list(O.stdrng_stream(int(seed), len(words))) == words, seed
+    r = G["stdrng_ranges"]
+    got = O.stdrng_ranges(r["seed"], [int(v) for v in r["lo"]], [int(v) for v in r["hi"]])
+    assert [str(v) for v in got] == r["out"]
+    f = G["stdrng_f64"]
+    assert [float.hex(float(v)) for v in O.stdrng_f64s(f["seed"], len(f["out"]))] == f["out"]
+
+
+def test_stdrng_range_is_uniform_and_in_bounds():
+    # Canon's method over a span that is not a power of two: every value in
+    # [lo, hi) and none outside; counts near uniform
+    lo, hi, m = 3, 17, 14000
+    got = O.stdrng_ranges(5, [lo] * m, [hi] * m)
+    assert got.min() == lo and got.max() == hi - 1
+    counts = np.bincount(got - lo, minlength=hi - lo)
+    assert counts.min() > 0.8 * m / (hi - lo) and counts.max() < 1.2 * m / (hi - lo)
+
+
+def test_kmeanspp_stdrng_golden():
+    for case in G["kmeanspp_stdrng"]:
+        X = arr(case["X"])
+        got = O.kmeanspp_stdrng(X, case["k"], int(case["seed"]))
+        assert (bits(got) == bits(X[case["picks"]])).all(), (X.shape, case["k"])
+
+
+def test_kmeans_and_minibatch_fit_stdrng_golden():
+    for case in G["kmeans_fit_stdrng"]:
+        X = arr(case["X"])
+        c, l, inertia, it = O.kmeans_fit_stdrng(X, case["k"], case["max_iter"], case["tol"],
+                                                case["n_init"], case["seed"])
+        assert it == case["n_iter"] and list(l) == case["labels"]
+        assert (bits(c) == bits(arr(case["centers"]))).all()
+        assert inertia == float.fromhex(case["inertia"])
+    for case in G["minibatch_fit_stdrng"]:
+        X = arr(case["X"])
+        c, l, inertia, it = O.minibatch_fit_stdrng(X, case["k"], case["batch"],
+                                                   case["max_iter"], case["tol"],
+                                                   case["n_init"], case["seed"])
+        assert it == case["n_iter"] and list(l) == case["labels"]
+        assert (bits(c) == bits(arr(case["centers"]))).all()
+        assert inertia == float.fromhex(case["inertia"])
