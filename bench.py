@@ -305,6 +305,69 @@ def run_kmeans(args):
     return out
 
 
+def config1_data(n=100_000, d=32, comps=16, seed=0):
+    """make_blobs-style f64 data: `comps` isotropic unit-std Gaussians with
+    centres uniform in [-10, 10]^d."""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(-10.0, 10.0, size=(comps, d))
+    lab = rng.integers(0, comps, size=n)
+    return c[lab] + rng.normal(size=(n, d))
+
+
+def run_config1(args):
+    """The reference's CPU-runnable KMeans case, whole seeded fits: 100,000 x
+    32 f64 blobs, k = 16, k-means++ from the reference's StdRng (seed 0),
+    n_init 1, max_iter 100, tol 1e-4.  The GPU fit (k-means++ included) is
+    timed over `steps` fits; the oracle's whole fit runs once on all host
+    threads; both must agree bit for bit (labels, n_iter, centres)."""
+    import ferrolearn_amd as F
+    from oracle import oracle as O
+    X = config1_data()
+    n, d = X.shape
+    k, seed = 16, 0
+    xd = F.to_device(X)
+    km = (F.KMeans(k).with_n_init(1).with_max_iter(100).with_tol(1e-4)
+          .with_rng("stdrng").with_random_state(seed))
+    for _ in range(max(1, args.warmup)):
+        km.fit(xd)
+    F.device.synchronize()
+    steps = max(5, args.steps)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        f = km.fit(xd)
+    F.device.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    labels, centers = f.labels().numpy(), f.cluster_centers().numpy()
+    out = {"workload": f"KMeans Lloyd f64, {n} x {d} blobs (16 centres), k={k}, "
+                       "k-means++ seeded (StdRng seed 0), n_init 1, max_iter 100, tol 1e-4",
+           "metric": "samples*iter/s", "value": n * f.n_iter() / dt,
+           "fits_per_s": 1.0 / dt, "ms_per_fit": dt * 1e3, "n_iter": f.n_iter(),
+           "inertia": f.inertia(), "route": "direct exact folds (k <= 32)",
+           "ms_lloyd_per_iter": f.report["ms_distance"] / max(1, f.n_iter())}
+    threads = O.max_threads()
+    t0 = time.perf_counter()
+    c_o, l_o, i_o, it_o = O.kmeans_fit_stdrng(X, k, 100, 1e-4, 1, seed, nthreads=threads)
+    cdt = time.perf_counter() - t0
+    reps = int(min(200, max(1, 3.0 / max(cdt, 1e-3))))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        O.kmeans_fit_stdrng(X, k, 100, 1e-4, 1, seed, nthreads=threads)
+    cdt = (time.perf_counter() - t0) / reps
+    out["cpu_baseline"] = {"value": n * it_o / cdt, "unit": "samples*iter/s", "cores": threads,
+                           "kind": "port", "s_per_fit": cdt,
+                           "sample": f"the whole fit (k-means++ and {it_o} Lloyd iterations) "
+                                     f"repeated {reps} times, oracle/ref_cpu.c with {threads} "
+                                     "OpenMP threads"}
+    out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    out["exact_vs_oracle"] = bool(
+        it_o == f.n_iter() and (l_o == labels).all()
+        and (c_o.view(np.int64) == centers.view(np.int64)).all()
+        and abs(i_o - f.inertia()) <= 1e-12 * i_o)
+    log(f"config1: {out['ms_per_fit']:.2f} ms per fit ({f.n_iter()} iterations), "
+        f"oracle {cdt:.2f} s, exact {out['exact_vs_oracle']}")
+    return out
+
+
 def router_data(N, d=96, comps=4096, n_train=2_000_000, n_test=1_000_000, seed=0):
     """Mixture of `comps` anisotropic Gaussians in d dimensions: random
     rotations, axis std log-uniform in [0.2, 3], means uniform in [-10, 10].
@@ -525,7 +588,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="all",
-                    choices=["all", "knn", "kmeans", "router", "batched", "copy"])
+                    choices=["all", "knn", "kmeans", "config1", "router", "batched", "copy"])
     ap.add_argument("--batched-problems", type=int, default=256)
     ap.add_argument("--batched-workers", type=int, default=16)
     ap.add_argument("--batched-no-plain", action="store_true",
@@ -556,6 +619,9 @@ def main():
     if args.workload == "kmeans":
         print(json.dumps(run_kmeans(args)), flush=True)
         return
+    if args.workload == "config1":
+        print(json.dumps(run_config1(args)), flush=True)
+        return
     if args.workload == "router":
         print(json.dumps(run_router(args)), flush=True)
         return
@@ -565,6 +631,7 @@ def main():
     line = run_knn(args)
     if args.workload == "all":
         line["kmeans"] = run_kmeans(args)
+        line["kmeans_config1"] = run_config1(args)
         line["router"] = run_router(args)
         line["batched_kmeans"] = run_batched(args)
     print(json.dumps(line), flush=True)

@@ -104,7 +104,8 @@ class fl_kmeans_problem(C.Structure):
 
 FL_ROUTE_NETWORK = 0
 FL_ROUTE_BOUNDS = 1
-ROUTES = {"network": FL_ROUTE_NETWORK, "bounds": FL_ROUTE_BOUNDS}
+FL_ROUTE_CENTRES = 2
+ROUTES = {"network": FL_ROUTE_NETWORK, "bounds": FL_ROUTE_BOUNDS, "centres": FL_ROUTE_CENTRES}
 
 
 class fl_router_params(C.Structure):

@@ -145,6 +145,10 @@ inline int64_t sorted_chunk(int64_t N) {
                              std::max<int64_t>(1, ((int64_t)1 << 26) / std::max<int64_t>(N, 1)));
 }
 
+// Queries sorted one at a time by the device-wide radix sort (every CU on
+// one query's N keys); more use the segmented sort, one block per query.
+constexpr int64_t kSortEachMax = 16;
+
 template <typename TB>
 void knn_sorted(fl_knn *h, const double *Q, int64_t nq, int k, double *dist, int64_t *idx,
                 Acc &acc, bool fallback) {
@@ -153,6 +157,44 @@ void knn_sorted(fl_knn *h, const double *Q, int64_t nq, int k, double *dist, int
     Workspace &ws = h->ws;
     const int64_t N = h->n;
     const int d = (int)h->d;
+    if (nq <= kSortEachMax && N < ((int64_t)1 << 31)) {
+        uint64_t *kin = ws.get<uint64_t>("srt_kin", nq * N);
+        uint64_t *kout = ws.get<uint64_t>("srt_kout", N);
+        int *pin = ws.get<int>("srt_pin", nq * N);
+        int *pout = ws.get<int>("srt_pout", N);
+        int *nan = ws.get<int>("srt_nan", 1);
+        size_t tmp_bytes = 0;
+        FL_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, kin, kout, pin, pout,
+                                                  (int)N, 0, 64, st));
+        void *tmp = ws.get<unsigned char>("srt_tmp1", (int64_t)tmp_bytes + 256);
+        FL_HIP(hipMemsetAsync(nan, 0, 4, st));
+        PhaseTimer tm(st);
+        const int t0 = tm.begin();
+        hipLaunchKernelGGL(knn_all_dist_kernel<TB>, dim3((unsigned)grid1(N, 256), (unsigned)nq),
+                           dim3(256), 0, st, Q, (const TB *)h->X.ptr(), N, d, kin, pin);
+        check_launch();
+        for (int64_t q = 0; q < nq; ++q) {
+            size_t tb = tmp_bytes;
+            // LSD radix sort: stable, so equal distances stay in row order
+            FL_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, kin + q * N, kout, pin + q * N,
+                                                      pout, (int)N, 0, 64, st));
+            hipLaunchKernelGGL(knn_take_sorted_kernel, dim3(grid1(k, 256)), dim3(256), 0, st,
+                               kout, pout, (int64_t)1, N, k, dist + q * k, idx + q * k, nan);
+            check_launch();
+        }
+        tm.end(t0);
+        int hn = 0;
+        FL_HIP(hipMemcpyAsync(&hn, nan, 4, hipMemcpyDeviceToHost, st));
+        FL_HIP(hipStreamSynchronize(st));
+        (fallback ? acc.ms_fb : acc.ms_dist) += tm.ms(t0);
+        acc.n_cand += nq * N;
+        acc.n_launch += 1;
+        if (hn)
+            throw StatusError{FL_NUMERICAL_INSTABILITY,
+                              "NumericalInstability: a distance is NaN (the reference panics "
+                              "here, kdtree.rs:386)"};
+        return;
+    }
     const int64_t chunk = std::min<int64_t>(nq, sorted_chunk(N));
     const int64_t cap = chunk * N;
     uint64_t *kin = ws.get<uint64_t>("srt_kin", cap);

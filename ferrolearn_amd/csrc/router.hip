@@ -37,7 +37,7 @@ constexpr int kMaxProbe = 31;
 constexpr int kMaxRoutedK = 32;
 constexpr int64_t kInferRows = 32768;   // network inference chunk
 constexpr int64_t kMaxTargets = 32;     // certification targets per training query
-constexpr int kRouteAll = 2;            // internal route: every cell (exact by construction)
+constexpr int kRouteAll = 3;            // internal route: every cell (exact by construction)
 constexpr int64_t kFullScanMax = 2048;  // queries a last all-cells scan takes
 
 // Router network: d -> H -> H -> C with padded shapes (dp: 32, cp: 64).
@@ -443,9 +443,11 @@ void launch_bounds_k(const RtBoundsArgs &a, int32_t dtype, hipStream_t st) {
 }
 
 // Per query, the kb smallest (lower bound, cell) over all cells: first the
-// exact nearest centre, then every cell's ball and bisector bounds.
+// exact nearest centre, then every cell's ball and bisector bounds.  With
+// ckey / cid, also the n_probe cells with the nearest centres.
 template <typename TQ>
-void smallest_bounds(fl_router *r, const TQ *Q, int64_t nq, int kb, double *bkey, int32_t *bid) {
+void smallest_bounds(fl_router *r, const TQ *Q, int64_t nq, int kb, double *bkey, int32_t *bid,
+                     int n_probe = 0, double *ckey = nullptr, int32_t *cid = nullptr) {
     hipStream_t st = r->stream;
     const int32_t qdt = std::is_same<TQ, float>::value ? FL_F32 : FL_F64;
     double *nfold = r->ws.get<double>("near_fold", nq);
@@ -457,12 +459,21 @@ void smallest_bounds(fl_router *r, const TQ *Q, int64_t nq, int kb, double *bkey
     // the nearest centre and its exact fold: the KMeans assignment step
     nearest_centers(Q, qdt, nq, (int)r->d, (const double *)r->centers.ptr(), r->C, ncell, nfold,
                     st);
-    a.near_fold = nfold, a.near_cell = ncell, a.kb = kb;
+    a.near_fold = nfold, a.near_cell = ncell, a.kb = kb, a.centre_key = 0;
     a.out_key = bkey, a.out_id = bid;
     if (kb <= 16)
         launch_bounds_k<16>(a, qdt, st);
     else
         launch_bounds_k<32>(a, qdt, st);
+    if (ckey) {
+        // the same pass keyed by the fold with each centre: the n_probe
+        // nearest non-empty cells (ties: lower cell)
+        a.centre_key = 1, a.kb = n_probe, a.out_key = ckey, a.out_id = cid;
+        if (n_probe <= 16)
+            launch_bounds_k<16>(a, qdt, st);
+        else
+            launch_bounds_k<32>(a, qdt, st);
+    }
 }
 
 template <typename TQ, typename TX, int K>
@@ -533,7 +544,16 @@ void route_search(fl_router *r, const TQ *Q, int64_t nq, int k, int n_probe, int
                            nq, n_probe, routes);
         check_launch();
     } else {
-        smallest_bounds(r, Q, nq, kb, bkey, bid);
+        if (route == FL_ROUTE_CENTRES) {
+            double *ckey = r->ws.get<double>("ckey", nq * n_probe);
+            int32_t *cid = r->ws.get<int32_t>("cid", nq * n_probe);
+            smallest_bounds(r, Q, nq, kb, bkey, bid, n_probe, ckey, cid);
+            hipLaunchKernelGGL(rt_routes_from_bounds_kernel, dim3(grid1(nq * n_probe, 256)),
+                               dim3(256), 0, st, cid, n_probe, nq, n_probe, routes);
+            check_launch();
+        } else {
+            smallest_bounds(r, Q, nq, kb, bkey, bid);
+        }
         if (route == FL_ROUTE_BOUNDS) {
             hipLaunchKernelGGL(rt_routes_from_bounds_kernel, dim3(grid1(nq * n_probe, 256)),
                                dim3(256), 0, st, bid, kb, nq, n_probe, routes);
@@ -960,7 +980,7 @@ fl_status fl_router_kneighbors_device(fl_router_handle r, const void *dQ, int32_
         if (k < 1) return fail(FL_INVALID_PARAMETER, "InvalidParameter: k: must be at least 1");
         if (n_probe < 1 || n_probe > kMaxProbe || n_probe > r->C)
             return fail(FL_INVALID_PARAMETER, "InvalidParameter: n_probe: 1..min(31, n_cells)");
-        if (route != FL_ROUTE_NETWORK && route != FL_ROUTE_BOUNDS)
+        if (route != FL_ROUTE_NETWORK && route != FL_ROUTE_BOUNDS && route != FL_ROUTE_CENTRES)
             return fail(FL_INVALID_PARAMETER, "InvalidParameter: route: unknown");
         if (nq > 0 && (!dQ || !d_dist || !d_idx))
             return fail(FL_INVALID_PARAMETER, "InvalidParameter: buffers: must not be NULL");

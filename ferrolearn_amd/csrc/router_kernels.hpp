@@ -297,12 +297,15 @@ struct RtBoundsArgs {
     int C;
     double ep;
     int kb;
+    int centre_key;                // 1: key = fold with the centre (no bounds)
     double *out_key;
     int32_t *out_id;
 };
 
 // The key of a cell is the larger of its ball bound and its bisector bound
-// against the query's nearest centre; an empty cell's key is +inf.
+// against the query's nearest centre; with centre_key, the query's fold with
+// the cell's centre (the inverted-file probe order).  An empty cell's key is
+// +inf.
 template <typename TQ, int K>
 __global__ __launch_bounds__(256) void rt_bounds_kernel(RtBoundsArgs a) {
     extern __shared__ double lds[];
@@ -331,6 +334,10 @@ __global__ __launch_bounds__(256) void rt_bounds_kernel(RtBoundsArgs a) {
             const double r = rad[c];
             if (r < 0.0) {
                 key = __builtin_huge_val();
+                return;
+            }
+            if (a.centre_key) {
+                key = fold;
                 return;
             }
             key = rt_lower_bound(sqrt(fold), r, ep);

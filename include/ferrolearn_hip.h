@@ -445,7 +445,9 @@ typedef struct fl_router *fl_router_handle;
 
 typedef enum fl_route {
     FL_ROUTE_NETWORK = 0, /* cells with the largest router outputs          */
-    FL_ROUTE_BOUNDS = 1   /* cells with the smallest lower bounds (no network) */
+    FL_ROUTE_BOUNDS = 1,  /* cells with the smallest lower bounds (no network) */
+    FL_ROUTE_CENTRES = 2  /* cells with the nearest centres (no network): the
+                             classic inverted-file probe order               */
 } fl_route;
 
 typedef struct fl_router_params {

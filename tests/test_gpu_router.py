@@ -60,7 +60,7 @@ def test_training_lowers_loss(setup):
     assert rep["loss_last"] < rep["loss_first"]
 
 
-@pytest.mark.parametrize("route", ["network", "bounds"])
+@pytest.mark.parametrize("route", ["network", "bounds", "centres"])
 @pytest.mark.parametrize("n_probe", [1, 4])
 def test_routed_equals_brute_force(setup, route, n_probe):
     X, fitted, router, _, _, Q = setup
@@ -264,7 +264,7 @@ def config5_setup():
     return X, fitted, router, rep, Q
 
 
-@pytest.mark.parametrize("route", ["network", "bounds"])
+@pytest.mark.parametrize("route", ["network", "bounds", "centres"])
 @pytest.mark.parametrize("n_probe", [2, 4])
 def test_config5_shape_routed_equals_brute_force(config5_setup, route, n_probe):
     # d = 96, 256 cells over 200,000 rows, a 96-512-512-256 network: every
@@ -305,3 +305,27 @@ def test_gradients_match_oracle_hidden512():
         gg, oo = g[o:o + m].astype(np.float64), og[o:o + m]
         assert np.linalg.norm(gg - oo) <= 1e-2 * np.linalg.norm(oo) + 1e-12, s
         o += m
+
+
+def test_config5_shape_routes_compared(config5_setup):
+    # The same search with three probe orders at equal n_probe: the trained
+    # network, the smallest lower bounds, and the nearest centres (the
+    # classic inverted-file order, no network).  Every answer is the brute
+    # force's; the counts of queries whose first certificate failed are
+    # printed side by side.
+    X, fitted, router, rep, Q = config5_setup
+    bd, bi = fitted.kneighbors(Q)
+    rates = {}
+    for n_probe in (1, 2, 4):
+        for route in ("network", "bounds", "centres"):
+            rd, ri = router.kneighbors(Q, n_probe=n_probe, route=route)
+            assert (ri == bi).all() and (rd.view(np.int64) == bd.view(np.int64)).all()
+            rates[(route, n_probe)] = router.report["n_rescan"]
+    print("first-certificate failures of", Q.shape[0], "queries:",
+          {f"{r}@{p}": v for (r, p), v in rates.items()})
+    for n_probe in (1, 2, 4):
+        # On these mixtures the cells a certified search needs are, for
+        # nearly every query, the cells with the nearest centres: the trained
+        # network matches the nearest-centre order to within 1 % of the
+        # queries and does not beat it.
+        assert rates[("network", n_probe)] <= rates[("centres", n_probe)] + Q.shape[0] // 100
