@@ -541,55 +541,64 @@ class Lloyd {
         int64_t n_trials = 2 + (int64_t)std::floor(std::log((double)k_));
         if (n_trials < 1) n_trials = 1;
         if (n_trials > 64) n_trials = 64;
+        const int nt = (int)n_trials;
         double *md = ws_.get<double>("kps_md", n_);
+        double *pre = ws_.get<double>("kps_pre", n_);
         double *nd = ws_.get<double>("kps_nd", n_trials * n_);
-        double *sc = ws_.get<double>("kps_scal", 2 * n_trials + 1);   // [thr | pot | total]
-        int64_t *cand = ws_.get<int64_t>("kps_cand", n_trials);
+        double *pre_t = ws_.get<double>("kps_pret", n_trials * n_);
+        double *thr = ws_.get<double>("kps_thr", n_trials);
+        double *pot = ws_.get<double>("kps_pot", n_trials);
+        unsigned long long *cand = ws_.get<unsigned long long>("kps_cand", n_trials);
+        KppSeqCtl *ctl = ws_.get<KppSeqCtl>("kps_ctl", 1);
+        double *r = ws_.get<double>("kps_r", (k_ - 1) * n_trials);
+        // Every centre's draws, made now on the assumption that the
+        // reference's total stays > 0; the generator's state before each
+        // centre is kept to replay the tail from if it does not.
+        std::vector<G> snap;
+        snap.reserve((size_t)(k_ - 1));
+        std::vector<double> hr((size_t)((k_ - 1) * n_trials));
+        for (int64_t c = 1; c < k_; ++c) {
+            snap.push_back(rng);
+            for (int t = 0; t < nt; ++t) hr[(size_t)((c - 1) * nt + t)] = rng.f64();
+        }
+        FL_HIP(hipMemcpyAsync(r, hr.data(), hr.size() * 8, hipMemcpyHostToDevice, st_));
+        FL_HIP(hipMemsetAsync(ctl, 0, sizeof(KppSeqCtl), st_));
         hipLaunchKernelGGL(kpp_first_kernel<TX>, dim3(grid1(n_, 256)), dim3(256), 0, st_, X_, n_,
                            d_, out, md);
+        hipLaunchKernelGGL(exact_fold_kernel, dim3(1), dim3(EF_NT), 0, st_, md, n_, n_,
+                           (const int32_t *)nullptr, &ctl->total, pre);
         check_launch();
-        std::vector<double> hthr((size_t)n_trials), hpot((size_t)n_trials);
-        std::vector<int64_t> hcand((size_t)n_trials);
+        const dim3 gn((unsigned)grid1(n_, 256), (unsigned)nt);
         for (int64_t c = 1; c < k_; ++c) {
-            double total = 0.0;
-            hipLaunchKernelGGL(seq_folds_kernel, dim3(1), dim3(64), 0, st_, md, n_, 1,
-                               sc + 2 * n_trials);
+            hipLaunchKernelGGL(kpp_seq_ctl_kernel, dim3(1), dim3(64), 0, st_, ctl,
+                               r + (c - 1) * nt, nt, (int)c, n_, thr, cand);
+            hipLaunchKernelGGL(kpp_seq_search_pre_kernel, gn, dim3(256), 0, st_, ctl, pre, n_,
+                               thr, cand);
+            hipLaunchKernelGGL(kpp_seq_trials_kernel<TX>, dim3(grid1(n_, 256)), dim3(256), 0, st_,
+                               ctl, X_, n_, d_, nt, cand, md, nd);
+            hipLaunchKernelGGL(exact_fold_kernel, dim3((unsigned)nt), dim3(EF_NT), 0, st_, nd,
+                               n_, n_, &ctl->stop, pot, pre_t);
+            hipLaunchKernelGGL(kpp_seq_select_kernel<TX>, dim3(1), dim3(64), 0, st_, ctl, pot, nt,
+                               cand, X_, d_, out + c * d_);
+            hipLaunchKernelGGL(kpp_seq_adopt_kernel, dim3(grid1(n_, 256)), dim3(256), 0, st_,
+                               ctl, n_, nd, pre_t, md, pre);
             check_launch();
-            FL_HIP(hipMemcpyAsync(&total, sc + 2 * n_trials, 8, hipMemcpyDeviceToHost, st_));
-            FL_HIP(hipStreamSynchronize(st_));
-            if (total <= 0.0) {
+        }
+        int32_t stop = 0;
+        FL_HIP(hipMemcpyAsync(&stop, &ctl->stop, 4, hipMemcpyDeviceToHost, st_));
+        FL_HIP(hipStreamSynchronize(st_));
+        if (stop > 0) {
+            // total <= 0 at centre `stop`: the minimum distances never change
+            // again, so every later centre is a uniformly drawn row (:201-205)
+            rng = snap[(size_t)(stop - 1)];
+            for (int64_t c = stop; c < k_; ++c) {
                 const int64_t idx = rng.range(n_);
                 hipLaunchKernelGGL(copy_row_f64_kernel<TX>, dim3(grid1(d_, 64)), dim3(64), 0, st_,
                                    X_, idx, d_, out + c * d_);
                 check_launch();
-                continue;
             }
-            for (int64_t t = 0; t < n_trials; ++t) hthr[(size_t)t] = rng.f64() * total;
-            FL_HIP(hipMemcpyAsync(sc, hthr.data(), (size_t)n_trials * 8, hipMemcpyHostToDevice,
-                                  st_));
-            hipLaunchKernelGGL(kpp_seq_search_kernel, dim3(1), dim3(64), 0, st_, md, n_, sc,
-                               (int)n_trials, cand);
-            hipLaunchKernelGGL(kpp_trials_multi_kernel<TX>,
-                               dim3((unsigned)grid1(n_, 256), (unsigned)n_trials), dim3(256), 0,
-                               st_, X_, n_, d_, cand, md, nd);
-            hipLaunchKernelGGL(seq_folds_kernel, dim3(1), dim3(64), 0, st_, nd, n_,
-                               (int)n_trials, sc + n_trials);
-            check_launch();
-            FL_HIP(hipMemcpyAsync(hpot.data(), sc + n_trials, (size_t)n_trials * 8,
-                                  hipMemcpyDeviceToHost, st_));
-            FL_HIP(hipMemcpyAsync(hcand.data(), cand, (size_t)n_trials * 8,
-                                  hipMemcpyDeviceToHost, st_));
             FL_HIP(hipStreamSynchronize(st_));
-            int64_t bt = 0;
-            for (int64_t t = 1; t < n_trials; ++t)
-                if (hpot[(size_t)t] < hpot[(size_t)bt]) bt = t;
-            hipLaunchKernelGGL(copy_row_f64_kernel<TX>, dim3(grid1(d_, 64)), dim3(64), 0, st_, X_,
-                               hcand[(size_t)bt], d_, out + c * d_);
-            FL_HIP(hipMemcpyAsync(md, nd + bt * n_, (size_t)n_ * 8, hipMemcpyDeviceToDevice,
-                                  st_));
-            check_launch();
         }
-        FL_HIP(hipStreamSynchronize(st_));
     }
 
     // k-means++ from a generator the caller keeps using: the library's
@@ -1009,7 +1018,8 @@ fl_status mb_fit_device(const TX *X, int64_t n, int d, const fl_minibatch_params
             if (shift < p->tol) break;
         }
         L.assign(cur, labels, best, acc);
-        hipLaunchKernelGGL(seq_fold_kernel, dim3(1), dim3(64), 0, st, best, n, dsc + 1);
+        hipLaunchKernelGGL(exact_fold_kernel, dim3(1), dim3(EF_NT), 0, st, best, n, n,
+                           (const int32_t *)nullptr, dsc + 1, (double *)nullptr);
         check_launch();
         double inertia = 0.0;
         FL_HIP(hipMemcpyAsync(&inertia, dsc + 1, 8, hipMemcpyDeviceToHost, st));
@@ -1366,6 +1376,24 @@ fl_status fl_kmeans_plusplus(const void *X, int32_t dtype, int64_t n, int64_t d,
             L.kmeanspp(seed, dC);
         }
         FL_HIP(hipMemcpy(centers_out, dC, (size_t)(k * d) * 8, hipMemcpyDeviceToHost));
+        return FL_OK;
+    });
+}
+
+fl_status fl_left_fold_device(const double *d_v, int64_t n, int64_t count, int64_t ld,
+                              double *d_sums, double *d_prefix) {
+    return guarded([&]() -> fl_status {
+        if (n < 0 || count < 0 || (count > 1 && ld < n) || count > 65535)
+            return fail(FL_INVALID_PARAMETER,
+                        "InvalidParameter: left fold needs n >= 0, ld >= n, count <= 65535");
+        if (count == 0) return FL_OK;
+        if (!d_v || !d_sums)
+            return fail(FL_INVALID_PARAMETER, "InvalidParameter: left fold: null pointer");
+        StreamGuard sg;
+        hipLaunchKernelGGL(exact_fold_kernel, dim3((unsigned)count), dim3(EF_NT), 0, sg.s, d_v, n,
+                           count > 1 ? ld : n, (const int32_t *)nullptr, d_sums, d_prefix);
+        check_launch();
+        FL_HIP(hipStreamSynchronize(sg.s));
         return FL_OK;
     });
 }

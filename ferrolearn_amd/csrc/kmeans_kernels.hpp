@@ -7,6 +7,7 @@
 #pragma once
 
 #include "common.hpp"
+#include "exact_fold.hpp"
 #include "dist_topm.hpp"
 
 namespace fl {
@@ -580,6 +581,123 @@ __global__ void kpp_trial_kernel(const TX *__restrict__ X, int64_t n, int d, int
     const double v = fold_exact(X + i * d, X + cand * d, d);
     const double m = md[i];
     nd[i] = v < m ? v : m;
+}
+
+// ------------------------------------------- k-means++ in the reference's order --
+// kmeans.rs:169-253 with every sum the reference's plain left fold, each
+// centre's steps enqueued without a host round trip.  The generator's draws
+// for every centre are made on the host up front (they do not depend on the
+// data while the reference's `total > 0` holds); the device keeps the
+// running state:
+//   total  = left fold of the current minimum distances md (the previous
+//            centre's best potential, which is the same fold);
+//   pre[i] = running left fold of md up to and including i, so a trial's
+//            threshold search (the reference's cumsum loop) is the first i
+//            with pre[i] >= threshold;
+//   stop   = the first centre whose total is <= 0 (the reference then draws
+//            a row uniformly instead; the host replays that tail).
+struct KppSeqCtl {
+    double total;
+    int32_t stop;
+    int32_t best;
+};
+
+// Centre c (>= 1): thresholds r_t * total for the nt trials, or stop.
+static __global__ __attribute__((unused)) void kpp_seq_ctl_kernel(KppSeqCtl *__restrict__ ctl,
+                                                                  const double *__restrict__ r,
+                                                                  int nt, int c, int64_t n,
+                                                                  double *__restrict__ thr,
+                                                                  unsigned long long *__restrict__ cand) {
+    if (threadIdx.x != 0 || blockIdx.x != 0 || ctl->stop) return;
+    const double total = ctl->total;
+    if (total <= 0.0) {   // kmeans.rs:201
+        ctl->stop = c;
+        return;
+    }
+    for (int t = 0; t < nt; ++t) {
+        thr[t] = r[t] * total;                     // kmeans.rs:209-210
+        cand[t] = (unsigned long long)(n - 1);     // :212, no index reaches it
+    }
+}
+
+// cand[t] = first i with pre[i] >= thr[t] (grid: ceil(n / 256) x nt).  The
+// running sums of non-negative distances never decrease until they turn NaN,
+// after which every comparison is false, so at most one i has pre[i] >=
+// thr[t] and not pre[i-1] >= thr[t]: that thread writes it (no index does
+// when the threshold is never reached, and cand keeps n - 1).
+static __global__ __attribute__((unused)) __launch_bounds__(256) void kpp_seq_search_pre_kernel(
+    const KppSeqCtl *__restrict__ ctl, const double *__restrict__ pre, int64_t n,
+    const double *__restrict__ thr, unsigned long long *__restrict__ cand) {
+    if (ctl->stop) return;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double th = thr[blockIdx.y];
+    if (pre[i] >= th && !(i > 0 && pre[i - 1] >= th)) cand[blockIdx.y] = (unsigned long long)i;
+}
+
+// nd[t*n + i] = min(md[i], fold(x_i, x_cand[t])) with `if d < nd`
+// (:219-226), every trial from one read of the row.
+template <typename TX>
+__global__ __launch_bounds__(256) void kpp_seq_trials_kernel(
+    const KppSeqCtl *__restrict__ ctl, const TX *__restrict__ X, int64_t n, int d, int nt,
+    const unsigned long long *__restrict__ cand, const double *__restrict__ md,
+    double *__restrict__ nd) {
+    if (ctl->stop) return;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double m = md[i];
+    const TX *xi = X + i * d;
+    for (int t0 = 0; t0 < nt; t0 += 4) {
+        const int tc = nt - t0 < 4 ? nt - t0 : 4;
+        const TX *c[4];
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int u = 0; u < 4; ++u) c[u] = X + (int64_t)cand[t0 + (u < tc ? u : 0)] * d;
+        for (int j = 0; j < d; ++j) {
+            const double xv = (double)xi[j];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const double df = xv - (double)c[u][j];
+                acc[u] = acc[u] + df * df;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (u < tc) nd[(int64_t)(t0 + u) * n + i] = acc[u] < m ? acc[u] : m;
+    }
+}
+
+// The trial with the strictly smallest potential, first trial first
+// (:231-235); its candidate becomes centre c.
+template <typename TX>
+__global__ void kpp_seq_select_kernel(KppSeqCtl *__restrict__ ctl, const double *__restrict__ pot,
+                                      int nt, const unsigned long long *__restrict__ cand,
+                                      const TX *__restrict__ X, int d, double *__restrict__ centre) {
+    if (ctl->stop) return;
+    __shared__ int bt;
+    if (threadIdx.x == 0) {
+        int b = 0;
+        for (int t = 1; t < nt; ++t)
+            if (pot[t] < pot[b]) b = t;
+        bt = b;
+        ctl->best = b;
+        ctl->total = pot[b];
+    }
+    __syncthreads();
+    const int64_t row = (int64_t)cand[bt];
+    for (int j = threadIdx.x; j < d; j += blockDim.x) centre[j] = (double)X[row * d + j];
+}
+
+// md = nd[best], pre = the best trial's running sums (:239-241).
+static __global__ __attribute__((unused)) __launch_bounds__(256) void kpp_seq_adopt_kernel(
+    const KppSeqCtl *__restrict__ ctl, int64_t n, const double *__restrict__ nd,
+    const double *__restrict__ pre_t, double *__restrict__ md, double *__restrict__ pre) {
+    if (ctl->stop) return;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t o = (int64_t)ctl->best * n + i;
+    md[i] = nd[o];
+    pre[i] = pre_t[o];
 }
 
 // k-means++ with the reference's plain left folds (kmeans.rs:200-228): one
