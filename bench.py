@@ -14,15 +14,21 @@ Second workload, reported in the same line under "kmeans": KMeans Lloyd on
 10,000,000 x 64 float32 samples from 1,024 Gaussian blobs (means U[-20,20],
 std U[0.5,3]), k = 1,024, k-means++ initial centres (the library's, seed 0),
 50 iterations (tol 0), samples resident in HBM; samples*iterations/s of the
-Lloyd phase for the BF16x3 and plain f64 forms, which must agree bit for bit.
+Lloyd phase for the FP8, BF16x3 and plain f64 forms, which must agree bit for
+bit.
 
-Third and fourth workloads, under "router" and "batched_kmeans": the
-learned-router exact kNN (20M x 96 rows, 4,096 cells, 1M queries) against
-the brute force on the same handle, and 256 uneven KMeans problems with 8
-restarts each in one batched call, BF16x3 and FP64 forms.
+Third, under "kmeans_config1": the reference's own CPU-runnable KMeans case
+(100,000 x 32 f64 blobs, k = 16, k-means++ from the reference's StdRng seed 0,
+n_init 1, max_iter 100, tol 1e-4), whole fits on the GPU against the oracle's
+whole fit on the host's cores, bit for bit.
+
+Fourth and fifth, under "router" and "batched_kmeans": the learned-router
+exact kNN (20M x 96 rows, 4,096 cells, 1M queries) against the brute force on
+the same handle, and 256 uneven KMeans problems with 8 restarts each in one
+batched call against per-problem FP64 fits.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W]
-                  [--workload all|knn|kmeans|router|batched|copy]
+                  [--workload all|knn|kmeans|config1|router|batched|copy]
 """
 from __future__ import annotations
 
@@ -39,16 +45,25 @@ sys.path.insert(0, ROOT)
 
 PEAK = {"f64": 78.6e12, "bf16": 2.5e15, "fp8": 5.0e15, "hbm": 8.0e12}   # MI355X spec: FLOP/s, B/s
 
-# HBM bytes per launch of the filter kernel at these workloads, from two
+# HBM bytes per launch of the dominant kernel at the benchmark sizes, from two
 # rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch),
 # each scaled by what the counter reads for a plain 512 MiB device copy
-# (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).
+# (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0), and the MFMA busy fraction
+# (SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE x 1024 SIMDs)) of the same
+# kernels.  The passes and their commands are summarised in
+# profiles/knn_bruteforce_1Mx128_q100k_k10.md and
+# profiles/kmeans_lloyd_10Mx64_k1024.md.
 TRAFFIC = {
-    ("knn", "bf16x3"): (5.19e6 * 2.0 + 6.74e4) * 1024,
-    ("knn", "fp64"): (8.51e6 * 2.0 + 6.17e4) * 1024,
-    ("kmeans", "bf16x3"): (2.87e6 * 2.0 + 5.95e5) * 1024,
-    ("kmeans", "fp64"): (1.29e6 * 2.0 + 6.25e5) * 1024,
+    ("knn", "bf16x3"): (5.189e6 * 2.0 + 6.64e4) * 1024,
+    ("knn", "fp64"): (8.508e6 * 2.0 + 6.15e4) * 1024,
+    ("kmeans", "fp8"): (1.938e6 * 2.0 + 1.29e5) * 1024,
+    ("kmeans", "bf16x3"): (3.005e6 * 2.0 + 9.08e5) * 1024,
+    ("kmeans", "fp64"): (1.291e6 * 2.0 + 6.25e5) * 1024,
 }
+MFMA_BUSY = {("knn", "bf16x3"): 0.541, ("knn", "fp64"): 0.705, ("kmeans", "fp8"): 0.240,
+             ("kmeans", "fp64"): 0.573}
+TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes of bench.py at these sizes "
+                  "(profiles/), copy-calibrated x2.0 / x1.0; not measured in this run")
 
 
 def log(*a):
@@ -168,6 +183,8 @@ def run_knn(args):
                      "frac": achieved / PEAK[peak_key],
                      "traffic": TRAFFIC.get(("knn", head)) if Q == 100_000 else None,
                      "traffic_unit": "bytes per launch (HBM, rocprofv3 counters)",
+                     "traffic_source": TRAFFIC_SOURCE,
+                     "mfma_busy_profiled": MFMA_BUSY.get(("knn", head)),
                      "algorithmic_bytes_per_launch": N * d * 4 + Q * (d * 4 + k * 12),
                      "kernel": "dist_bf16_kernel<16,4,3>" if peak_key == "bf16" else "dist_f64_kernel<16,32,double,float>",
                      "algorithmic_flops_per_launch": flops,
@@ -293,6 +310,8 @@ def run_kmeans(args):
                        "algorithmic_flops_per_launch": flops,
                        "traffic": TRAFFIC.get(("kmeans", head)) if n == 10_000_000 else None,
                        "traffic_unit": "bytes per launch (HBM, rocprofv3 counters)",
+                       "traffic_source": TRAFFIC_SOURCE,
+                       "mfma_busy_profiled": MFMA_BUSY.get(("kmeans", head)),
                        "algorithmic_bytes_per_launch": n * (d * 4 + 4) + k * d * 8}
     if not args.no_cpu:
         log("kmeans: cpu baseline ...")
