@@ -186,6 +186,8 @@ SIGNATURES = {
                                                    _vp, _vp]),
     "fl_kmeans_plusplus": (C.c_int, [_vp, _i32, _i64, _i64, _i64, C.c_uint64, _vp]),
     "fl_left_fold_device": (C.c_int, [_vp, _i64, _i64, _i64, _vp, _vp]),
+    "fl_router_cell_bounds_device": (C.c_int, [_vp, _vp, _i32, _i64, _i64, _i64, _i32, _vp,
+                                               _vp]),
     "fl_minibatch_params_default": (None, [C.POINTER(fl_minibatch_params), _i64]),
     "fl_minibatch_kmeans_fit": (C.c_int, [_vp, _i32, _i64, _i64, C.POINTER(fl_minibatch_params),
                                           _vp, _vp, _vp, _dp, _ip, _rp]),

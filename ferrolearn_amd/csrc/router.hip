@@ -318,6 +318,11 @@ struct fl_router {
     int C = 0;
     DevBuf centers, radius, cdist, cell_first, cell_count, row_id, xs, label32, row_cell;
     DevBuf cell_by_size;   // cells by row count, largest first (scan work order)
+    // BF16x3 MFMA bounds: the centres' bf16 hi / lo rows (padded to mfma_dp
+    // features) and norms; mfma: -1 not built yet, 0 not usable (d or the
+    // centres' range), 1 ready
+    DevBuf mh, ml, mn2, mn1;
+    int mfma = -1, mfma_dp = 0;
     Mlp net;
     hipStream_t stream = nullptr;
     Workspace ws;
@@ -442,12 +447,59 @@ void launch_bounds_k(const RtBoundsArgs &a, int32_t dtype, hipStream_t st) {
     check_launch();
 }
 
+template <int K>
+void launch_bounds_mfma_k(const RtBoundsMfmaArgs &m, hipStream_t st) {
+    const size_t sm = rtm_lds_bytes(K, m.dp);
+    FL_HIP(hipFuncSetAttribute((const void *)rt_bounds_mfma_kernel<K>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+    hipLaunchKernelGGL((rt_bounds_mfma_kernel<K>), dim3((unsigned)ceil_div(m.b.nq, RT_Q)),
+                       dim3(256), sm, st, m);
+    check_launch();
+}
+
+// The centres' side of the MFMA bounds, once per router: usable when the
+// padded width fits the kernel's LDS plan and every centre is finite with
+// magnitude <= 2^56 (the BF16 split's range).
+void prepare_mfma_bounds(fl_router *r) {
+    if (r->mfma >= 0) return;
+    hipStream_t st = r->stream;
+    const int d = (int)r->d, C = r->C;
+    const int dp = (int)ceil_div(d, 32) * 32;
+    r->mfma = 0;
+    if (dp > RTM_MAXDP) return;
+    const double *cen = (const double *)r->centers.ptr();
+    unsigned int *flag = r->ws.get<unsigned int>("mfma_flag", 1);
+    FL_HIP(hipMemsetAsync(flag, 0, 4, st));
+    hipLaunchKernelGGL(range_flags_kernel<double>, dim3(64), dim3(256), 0, st, cen,
+                       (int64_t)C * d, flag);
+    check_launch();
+    unsigned int hf = 0;
+    FL_HIP(hipMemcpyAsync(&hf, flag, 4, hipMemcpyDeviceToHost, st));
+    FL_HIP(hipStreamSynchronize(st));
+    if (hf & 1u) return;
+    uint16_t *mh = (uint16_t *)r->mh.get((size_t)C * dp * 2);
+    uint16_t *ml = (uint16_t *)r->ml.get((size_t)C * dp * 2);
+    double *n2 = (double *)r->mn2.get((size_t)C * 8);
+    double *n1 = (double *)r->mn1.get((size_t)C * 8);
+    hipLaunchKernelGGL(bf16_split_kernel<double>, dim3(grid1((int64_t)C * dp, 256)), dim3(256), 0,
+                       st, cen, (int64_t)C, d, dp, (unsigned short *)mh, (unsigned short *)ml);
+    hipLaunchKernelGGL(row_norms_kernel<double>, dim3(grid1(C, 256)), dim3(256), 0, st, cen,
+                       (int64_t)C, d, n2, n1);
+    check_launch();
+    FL_HIP(hipStreamSynchronize(st));
+    r->mfma_dp = dp;
+    r->mfma = 1;
+}
+
 // Per query, the kb smallest (lower bound, cell) over all cells: first the
-// exact nearest centre, then every cell's ball and bisector bounds.  With
-// ckey / cid, also the n_probe cells with the nearest centres.
+// exact nearest centre, then every cell's ball and bisector bounds (from
+// BF16x3 MFMA lower bounds of the folds when the data's range allows, else
+// from the exact folds).  With ckey / cid, also the n_probe cells with the
+// nearest centres.
 template <typename TQ>
 void smallest_bounds(fl_router *r, const TQ *Q, int64_t nq, int kb, double *bkey, int32_t *bid,
-                     int n_probe = 0, double *ckey = nullptr, int32_t *cid = nullptr) {
+                     int n_probe = 0, double *ckey = nullptr, int32_t *cid = nullptr,
+                     bool allow_mfma = true) {
     hipStream_t st = r->stream;
     const int32_t qdt = std::is_same<TQ, float>::value ? FL_F32 : FL_F64;
     double *nfold = r->ws.get<double>("near_fold", nq);
@@ -461,18 +513,63 @@ void smallest_bounds(fl_router *r, const TQ *Q, int64_t nq, int kb, double *bkey
                     st);
     a.near_fold = nfold, a.near_cell = ncell, a.kb = kb, a.centre_key = 0;
     a.out_key = bkey, a.out_id = bid;
-    if (kb <= 16)
-        launch_bounds_k<16>(a, qdt, st);
-    else
-        launch_bounds_k<32>(a, qdt, st);
+    if (allow_mfma) prepare_mfma_bounds(r);
+    bool use_mfma = allow_mfma && r->mfma == 1 && nq > 0;
+    RtBoundsMfmaArgs m;
+    if (use_mfma) {
+        // the queries' range, then their bf16 hi / lo rows and norms
+        const int d = (int)r->d, dp = r->mfma_dp;
+        unsigned int *flag = r->ws.get<unsigned int>("mfma_flag", 1);
+        FL_HIP(hipMemsetAsync(flag, 0, 4, st));
+        hipLaunchKernelGGL(range_flags_kernel<TQ>, dim3(256), dim3(256), 0, st, Q, nq * d, flag);
+        check_launch();
+        unsigned int hf = 0;
+        FL_HIP(hipMemcpyAsync(&hf, flag, 4, hipMemcpyDeviceToHost, st));
+        FL_HIP(hipStreamSynchronize(st));
+        use_mfma = (hf & 1u) == 0;
+        if (use_mfma) {
+            uint16_t *qh = r->ws.get<uint16_t>("mq_h", nq * dp);
+            uint16_t *ql = r->ws.get<uint16_t>("mq_l", nq * dp);
+            double *qn2 = r->ws.get<double>("mq_n2", nq);
+            double *qn1 = r->ws.get<double>("mq_n1", nq);
+            hipLaunchKernelGGL(bf16_split_kernel<TQ>, dim3(grid1(nq * dp, 256)), dim3(256), 0, st,
+                               Q, nq, d, dp, (unsigned short *)qh, (unsigned short *)ql);
+            hipLaunchKernelGGL(row_norms_kernel<TQ>, dim3(grid1(nq, 256)), dim3(256), 0, st, Q,
+                               nq, d, qn2, qn1);
+            check_launch();
+            const Bound bd = bound_bf16(d, 3);
+            m.b = a, m.qh = qh, m.ql = ql, m.qn2 = qn2, m.qn1 = qn1;
+            m.ch = (const uint16_t *)r->mh.ptr(), m.cl = (const uint16_t *)r->ml.ptr();
+            m.cn2 = (const double *)r->mn2.ptr(), m.cn1 = (const double *)r->mn1.ptr();
+            m.dp = dp, m.alpha = bd.alpha, m.beta = bd.beta;
+            m.eta = (double)(3 * d + 1) * std::ldexp(1.0, -100);
+        }
+    }
+    auto run = [&](const RtBoundsArgs &ab, int kk) {
+        if (use_mfma) {
+            // each of the four sub-lists keeps K >= kk entries, which is all
+            // the merge of the kk smallest needs
+            m.b = ab;
+            if (kk <= 4)
+                launch_bounds_mfma_k<4>(m, st);
+            else if (kk <= 8)
+                launch_bounds_mfma_k<8>(m, st);
+            else if (kk <= 16)
+                launch_bounds_mfma_k<16>(m, st);
+            else
+                launch_bounds_mfma_k<32>(m, st);
+        } else if (kk <= 16) {
+            launch_bounds_k<16>(ab, qdt, st);
+        } else {
+            launch_bounds_k<32>(ab, qdt, st);
+        }
+    };
+    run(a, kb);
     if (ckey) {
         // the same pass keyed by the fold with each centre: the n_probe
         // nearest non-empty cells (ties: lower cell)
         a.centre_key = 1, a.kb = n_probe, a.out_key = ckey, a.out_id = cid;
-        if (n_probe <= 16)
-            launch_bounds_k<16>(a, qdt, st);
-        else
-            launch_bounds_k<32>(a, qdt, st);
+        run(a, n_probe);
     }
 }
 
@@ -955,6 +1052,33 @@ fl_status fl_router_train_device(fl_router_handle r, const void *dQ, int32_t dty
             rep->loss_first = lf;
             rep->loss_last = ll;
         }
+        return FL_OK;
+    });
+}
+
+fl_status fl_router_cell_bounds_device(fl_router_handle r, const void *dQ, int32_t dtype,
+                                       int64_t nq, int64_t d, int64_t kb, int32_t exact,
+                                       double *d_keys, int32_t *d_ids) {
+    return guarded([&]() -> fl_status {
+        if (!r) return fail(FL_INVALID_PARAMETER, "InvalidParameter: handle: must not be NULL");
+        if (fl_status s = check_dtype(dtype)) return s;
+        if (d != r->d)
+            return fail(FL_SHAPE_MISMATCH, "ShapeMismatch: expected " + std::to_string(r->d) +
+                                               ", actual " + std::to_string(d) +
+                                               ": number of features must match training data");
+        if (nq < 0) return fail(FL_INVALID_PARAMETER, "InvalidParameter: n_queries: negative");
+        if (kb < 1 || kb > 32 || kb > r->C)
+            return fail(FL_INVALID_PARAMETER, "InvalidParameter: kb: 1..min(32, n_cells)");
+        if (nq == 0) return FL_OK;
+        if (!dQ || !d_keys || !d_ids)
+            return fail(FL_INVALID_PARAMETER, "InvalidParameter: buffers: must not be NULL");
+        if (dtype == FL_F32)
+            smallest_bounds(r, (const float *)dQ, nq, (int)kb, d_keys, d_ids, 0, nullptr, nullptr,
+                            exact == 0);
+        else
+            smallest_bounds(r, (const double *)dQ, nq, (int)kb, d_keys, d_ids, 0, nullptr,
+                            nullptr, exact == 0);
+        FL_HIP(hipStreamSynchronize(r->stream));
         return FL_OK;
     });
 }

@@ -11,6 +11,7 @@
 #pragma once
 
 #include "common.hpp"
+#include "dist_bf16.hpp"
 
 namespace fl {
 
@@ -356,6 +357,242 @@ __global__ __launch_bounds__(256) void rt_bounds_kernel(RtBoundsArgs a) {
         if (q0 + own < nq) {
             ok[(q0 + own) * kb + j] = key;
             oi[(q0 + own) * kb + j] = id;
+        }
+    });
+}
+
+// The same lists and keys with every fold replaced by a lower bound from a
+// BF16x3 MFMA dot product: q.mu as hi*hi + hi*lo + lo*hi (bf16 splits of the
+// f64-widened values) in one FP32 accumulator, then in f64
+//   flo = |q|^2 + |mu|^2 - 2 q.mu - (alpha |q||mu| + beta (|q|^2 + |mu|^2))(1 + 2^-50)
+//         - 2^-50 (|q|^2 + |mu|^2) - eta (1 + |q| + |mu|),
+// which the kNN filters' error bound (alpha, beta of bound_bf16 with three
+// passes; eta for FP32 flushing) puts at or below the exact fold; the 2^-50
+// terms cover this expression's own roundings.  flo is kept as the f32 at or
+// below it.  The ball bound grows with sqrt(fold) and the bisector bound with
+// fold(q, mu_c), so both stay valid lower bounds with flo in place of the
+// fold, and the certificate that uses them is unchanged.  Queries and
+// centres must be finite with magnitudes <= 2^56 (the BF16 filters' range);
+// other searches use rt_bounds_kernel.
+//
+// One block of 256 threads per 64 queries, 64-centre tiles: the queries' and
+// the tile's hi / lo rows sit in LDS with a row stride of dp + 8 halfwords
+// (16-byte MFMA operand loads of 16 consecutive rows hit distinct banks);
+// wave w computes the 32 x 32 (query, centre) quarter (w >> 1, w & 1) as
+// 2 x 2 tiles of v_mfma_f32_16x16x32_bf16, writes its lower bounds to the
+// [query][centre] tile in LDS, and the list owners of rt_fold_lists read
+// them.
+constexpr int RTM_MAXDP = 128;   // padded features the LDS plan allows
+constexpr int RTM_FS = 68;       // lower-bound tile row stride (floats): 16-byte rows
+
+struct RtBoundsMfmaArgs {
+    RtBoundsArgs b;                  // queries, cells, nearest centres, lists
+    const uint16_t *qh, *ql;         // nq x dp bf16 hi / lo of the queries
+    const double *qn2, *qn1;         // |q|^2, |q|
+    const uint16_t *ch, *cl;         // C x dp bf16 hi / lo of the centres
+    const double *cn2, *cn1;         // |mu|^2, |mu|
+    int dp;                          // features padded to a multiple of 32
+    double alpha, beta, eta;
+};
+
+inline size_t rtm_lds_bytes(int K, int dp) {
+    const size_t rows = (size_t)2 * 2 * RT_Q * (dp + 8) * 2;   // q and centre tiles, hi / lo
+    const size_t norms = (size_t)6 * RT_Q * 8;
+    const size_t tile = (size_t)RT_Q * RTM_FS * 4;             // f32 lower bounds
+    const size_t merge = (size_t)4 * RT_Q * K * 12;
+    const size_t work = rows + norms + tile;
+    return work > merge ? work : merge;
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void rt_bounds_mfma_kernel(RtBoundsMfmaArgs m) {
+    extern __shared__ __align__(16) unsigned char rtm_lds[];
+    const RtBoundsArgs &a = m.b;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int dp = m.dp, ds = dp + 8;
+    uint16_t *Qh = reinterpret_cast<uint16_t *>(rtm_lds);
+    uint16_t *Ql = Qh + RT_Q * ds;
+    uint16_t *Ch = Ql + RT_Q * ds;
+    uint16_t *Cl = Ch + RT_Q * ds;
+    double *sq2 = reinterpret_cast<double *>(Cl + RT_Q * ds);
+    double *sq1 = sq2 + RT_Q;
+    double *sc2 = sq1 + RT_Q;
+    double *sc1 = sc2 + RT_Q;
+    double *srr = sc1 + RT_Q;                           // r_c (1 + ep) / (1 - ep)
+    double *srad = srr + RT_Q;                          // r_c
+    float *F = reinterpret_cast<float *>(srad + RT_Q);  // [RT_Q][RTM_FS]
+    const int64_t q0 = (int64_t)blockIdx.x * RT_Q;
+    const int chunks = dp / 8;   // 16-byte chunks per row
+    // the block's queries, once
+    for (int e = t; e < RT_Q * chunks; e += 256) {
+        const int r = e / chunks, c = e % chunks;
+        u16x8_t h = {0, 0, 0, 0, 0, 0, 0, 0}, l = h;
+        if (q0 + r < a.nq) {
+            h = *reinterpret_cast<const u16x8_t *>(m.qh + (q0 + r) * dp + c * 8);
+            l = *reinterpret_cast<const u16x8_t *>(m.ql + (q0 + r) * dp + c * 8);
+        }
+        *reinterpret_cast<u16x8_t *>(Qh + r * ds + c * 8) = h;
+        *reinterpret_cast<u16x8_t *>(Ql + r * ds + c * 8) = l;
+    }
+    if (t < RT_Q) {
+        sq2[t] = q0 + t < a.nq ? m.qn2[q0 + t] : 0.0;
+        sq1[t] = q0 + t < a.nq ? m.qn1[q0 + t] : 0.0;
+    }
+    double lk[K];
+    int32_t li[K];
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+        lk[s] = __builtin_huge_val();
+        li[s] = RT_NOID;
+    }
+    const int own = t & 63, sub = t >> 6;
+    const int64_t qo = q0 + own;
+    const int32_t near = qo < a.nq ? a.near_cell[qo] : 0;
+    const double fnear = qo < a.nq ? a.near_fold[qo] : 0.0;
+    const double *drow = a.cdist + (int64_t)near * a.C;
+    const double ep = a.ep;
+    const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+    for (int c0 = 0; c0 < a.C; c0 += RT_R) {
+        for (int e = t; e < RT_R * chunks; e += 256) {
+            const int r = e / chunks, c = e % chunks;
+            u16x8_t h = {0, 0, 0, 0, 0, 0, 0, 0}, l = h;
+            if (c0 + r < a.C) {
+                h = *reinterpret_cast<const u16x8_t *>(m.ch + (int64_t)(c0 + r) * dp + c * 8);
+                l = *reinterpret_cast<const u16x8_t *>(m.cl + (int64_t)(c0 + r) * dp + c * 8);
+            }
+            *reinterpret_cast<u16x8_t *>(Ch + r * ds + c * 8) = h;
+            *reinterpret_cast<u16x8_t *>(Cl + r * ds + c * 8) = l;
+        }
+        if (t < RT_R) {
+            sc2[t] = c0 + t < a.C ? m.cn2[c0 + t] : 0.0;
+            sc1[t] = c0 + t < a.C ? m.cn1[c0 + t] : 0.0;
+            const double rc = c0 + t < a.C ? a.radius[c0 + t] : -1.0;
+            srr[t] = rc < 0.0 ? __builtin_huge_val() : rc * (1.0 + ep) / (1.0 - ep);
+            srad[t] = rc;
+        }
+        __syncthreads();
+        float4_t acc[2][2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[i][j] = (float4_t){0.f, 0.f, 0.f, 0.f};
+        for (int k0 = 0; k0 < dp; k0 += 32) {
+            const int ko = k0 + (lane >> 4) * 8;
+            bf16x8_t ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const int ra = wm + i * 16 + (lane & 15), rb = wn + i * 16 + (lane & 15);
+                ah[i] = __builtin_bit_cast(bf16x8_t,
+                                           *reinterpret_cast<const u16x8_t *>(Qh + ra * ds + ko));
+                al[i] = __builtin_bit_cast(bf16x8_t,
+                                           *reinterpret_cast<const u16x8_t *>(Ql + ra * ds + ko));
+                bh[i] = __builtin_bit_cast(bf16x8_t,
+                                           *reinterpret_cast<const u16x8_t *>(Ch + rb * ds + ko));
+                bl[i] = __builtin_bit_cast(bf16x8_t,
+                                           *reinterpret_cast<const u16x8_t *>(Cl + rb * ds + ko));
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j],
+                                                                        0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j],
+                                                                        0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j],
+                                                                        0, 0, 0);
+                }
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int mq = wm + i * 16 + bf_acc_row(lane, v);
+                    const int nc = wn + j * 16 + (lane & 15);
+                    const double q2 = sq2[mq], q1 = sq1[mq], c2 = sc2[nc], c1 = sc1[nc];
+                    const double s2 = q2 + c2;
+                    const double val = s2 - 2.0 * (double)acc[i][j][v];
+                    const double err = (m.alpha * q1 * c1 + m.beta * s2) * (1.0 + 0x1p-50) +
+                                       0x1p-50 * s2 + m.eta * (1.0 + q1 + c1);
+                    const double flo = val - err;
+                    float f = (float)flo;
+                    if ((double)f > flo) {   // one f32 step down (f is not NaN or -inf here)
+                        const uint32_t fb = __builtin_bit_cast(uint32_t, f);
+                        f = __builtin_bit_cast(
+                            float, f > 0.f ? fb - 1u : (f < 0.f ? fb + 1u : 0x80000001u));
+                    }
+                    F[mq * RTM_FS + nc] = f;
+                }
+        __syncthreads();
+        // A cell whose ball bound alone exceeds the list's last key T cannot
+        // enter: ball > T once sqrt(flo) > S = T / (1-ep)^2 + r (1+ep)/(1-ep).
+        // S is widened by 2^-40 (|T'| + r') before the test flo > S^2 (1 +
+        // 2^-40), which leaves room for the roundings of S (T' + r' may
+        // cancel) and of the key itself, so a skipped pair's computed key is
+        // above T.  The owner's 16 bounds are tested together; only the
+        // candidates left compute their key.
+        const double tpf = 1.0 / ((1.0 - ep) * (1.0 - ep));
+        const float *Fr = F + own * RTM_FS + sub * 16;
+        float fv[16];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const float4_t v4 = *reinterpret_cast<const float4_t *>(Fr + 4 * u);
+            fv[4 * u] = v4[0], fv[4 * u + 1] = v4[1], fv[4 * u + 2] = v4[2], fv[4 * u + 3] = v4[3];
+        }
+        auto skip = [&](double flo, int r, double tl) {
+            if (a.centre_key) return flo > tl;
+            const double tp = tl * tpf;
+            const double S = tp + srr[r] + 0x1p-40 * (__builtin_fabs(tp) + srr[r]);
+            return S > 0.0 && flo > S * S * (1.0 + 0x1p-40);
+        };
+        unsigned cand = 0;
+        {
+            const double tl = lk[K - 1];
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj) {
+                const int r = sub * 16 + jj;
+                const bool keep = c0 + r < a.C && !skip((double)fv[jj], r, tl);
+                cand |= (unsigned)keep << jj;
+            }
+        }
+#pragma unroll 1
+        while (cand) {
+            const int jj = __builtin_ctz(cand);
+            cand &= cand - 1;
+            const int r = sub * 16 + jj;
+            const int c = c0 + r;
+            const double flo = (double)Fr[jj];
+            if (skip(flo, r, lk[K - 1])) continue;   // the list moved on
+            double key;
+            const double rc = srad[r];
+            if (rc < 0.0) {
+                key = __builtin_huge_val();
+            } else if (a.centre_key) {
+                key = flo;
+            } else {
+                const double fl0 = flo < 0.0 ? 0.0 : flo;   // NaN stays NaN: no bound
+                key = rt_lower_bound(sqrt(fl0), rc, ep);
+                const double D = drow[c];
+                if (c != near && D > 0.0) {
+                    const double vb = rt_bisector_bound(fl0, fnear, D, rc, ep);
+                    key = vb > key ? vb : key;
+                }
+            }
+            if (pair_less(key, (int32_t)c, lk[K - 1], li[K - 1])) rt_insert<K>(lk, li, key, c);
+        }
+        __syncthreads();
+    }
+    double *lds = reinterpret_cast<double *>(rtm_lds);
+    double *ok = a.out_key;
+    int32_t *oi = a.out_id;
+    const int kb = a.kb;
+    const int64_t nq = a.nq;
+    rt_merge4<K>(lds, lk, li, kb, [&](int o, int j, double key, int32_t id) {
+        if (q0 + o < nq) {
+            ok[(q0 + o) * kb + j] = key;
+            oi[(q0 + o) * kb + j] = id;
         }
     });
 }

@@ -99,6 +99,21 @@ class CellRouter:
             raise ValueError("parameter vector has the wrong length")
         check(L.lib().fl_router_set_params(self._h, p.ctypes.data_as(C.c_void_p)))
 
+    def cell_bounds(self, x, kb: int, exact: bool = False):
+        """(keys, cells), each n_queries x kb: the kb smallest per-cell lower
+        bounds on the distance of a cell's rows to each query, ascending,
+        computed as the search does (BF16x3 MFMA lower bounds of the folds
+        unless exact, or when the data lie outside the BF16 range)."""
+        from .device import to_device, empty
+        q = _dev2d(x) if _is_device(x) else to_device(_host2d(x))
+        nq, d = q.shape
+        keys = empty((nq, int(kb)), np.float64)
+        ids = empty((nq, int(kb)), np.int32)
+        check(L.lib().fl_router_cell_bounds_device(
+            self._h, C.c_void_p(q.data_ptr()), _dtype_code(q.dtype), nq, d, int(kb),
+            1 if exact else 0, C.c_void_p(keys.data_ptr()), C.c_void_p(ids.data_ptr())))
+        return keys.numpy(), ids.numpy()
+
     def cells(self):
         """(centres n_cells x d, radii, counts, cell of each database row)."""
         n = self._fitted.n_samples_fit()

@@ -523,6 +523,17 @@ fl_status fl_router_kneighbors_device(fl_router_handle r, const void *dQ,
                                       double *d_dist_out, int64_t *d_idx_out,
                                       fl_router_report *report);
 
+/* The kb (<= min(32, n_cells)) smallest per-cell lower bounds of each query
+ * (d_keys, d_ids: nq x kb, ascending (key, cell)), as the search computes
+ * them: from BF16x3 MFMA lower bounds of the folds (exact == 0; searches use
+ * these whenever the data lie within the BF16 range) or from the exact folds
+ * (exact != 0).  Every MFMA key is at most the exact key of the same cell.
+ * For inspection and tests. */
+fl_status fl_router_cell_bounds_device(fl_router_handle r, const void *dQ,
+                                       int32_t dtype, int64_t nq, int64_t d,
+                                       int64_t kb, int32_t exact, double *d_keys,
+                                       int32_t *d_ids);
+
 /* Network parameters in mlp.rs order: W1 (d x hidden), b1, W2 (hidden x
  * hidden), b2, W3 (hidden x n_cells), b3, as f32. */
 int64_t fl_router_n_params(fl_router_handle r);

@@ -329,3 +329,50 @@ def test_config5_shape_routes_compared(config5_setup):
         # network matches the nearest-centre order to within 1 % of the
         # queries and does not beat it.
         assert rates[("network", n_probe)] <= rates[("centres", n_probe)] + Q.shape[0] // 100
+
+
+def _per_cell(keys, ids):
+    return [dict(zip(i.tolist(), k.tolist())) for k, i in zip(keys, ids)]
+
+
+def test_mfma_bounds_are_below_exact_bounds(setup):
+    # Every cell's bound from the BF16x3 MFMA lower bound of the fold is at
+    # most the bound from the exact fold, and close to it; with kb = n_cells
+    # every cell is compared.
+    X, fitted, _, _, _, Q = setup
+    r = F.CellRouter(fitted, n_cells=24, cell_iter=5, hidden=64, seed=3)
+    for q in (Q[:500], Q[:500].astype(np.float64)):
+        km, im = r.cell_bounds(q, 24)
+        ke, ie = r.cell_bounds(q, 24, exact=True)
+        gaps = []
+        for dm, de in zip(_per_cell(km, im), _per_cell(ke, ie)):
+            assert set(dm) == set(de) == set(range(24))
+            for c, v in de.items():
+                assert dm[c] <= v
+                gaps.append((v - dm[c]) / (1.0 + abs(v)))
+        # the BF16x3 error term is about 2e-4 |q||mu| in fold units
+        assert np.median(gaps) < 1e-3 and max(gaps) < 0.1
+
+
+def test_mfma_bounds_config5_shape(config5_setup):
+    X, fitted, router, rep, Q = config5_setup
+    km, im = router.cell_bounds(Q[:2000], 32)
+    ke, ie = router.cell_bounds(Q[:2000], 32, exact=True)
+    assert (km[:, -1] <= ke[:, -1]).all()
+    same = 0
+    for dm, de in zip(_per_cell(km, im), _per_cell(ke, ie)):
+        for c in set(dm) & set(de):
+            assert dm[c] <= de[c]
+            same += 1
+    assert same > 0.9 * 32 * 2000
+
+
+def test_mfma_bounds_fall_back_outside_bf16_range(setup):
+    # a query beyond 2^56 (or non-finite) sends the whole call to the exact
+    # bounds, which then equal the exact path's keys
+    X, fitted, router, *_ , Q = setup
+    q = Q[:64].astype(np.float64).copy()
+    q[3, 0] = 2.0 ** 60
+    km, im = router.cell_bounds(q, 8)
+    ke, ie = router.cell_bounds(q, 8, exact=True)
+    assert (km.view(np.int64) == ke.view(np.int64)).all() and (im == ie).all()
