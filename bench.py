@@ -300,7 +300,7 @@ def run_kmeans(args):
     flops = 2.0 * n * k * d
     peak_key = {"fp64": "f64", "exact": "f64", "fp8": "fp8"}.get(head, "bf16")
     kname = {"f64": "dist_f64_kernel<0,...> (argmin form)",
-             "fp8": "dist_bf16_kernel<0,2,0> (FP8 E4M3 argmin form, decision fused)",
+             "fp8": "dist_fp8_res_kernel (FP8 E4M3 argmin, centres resident in LDS, decision fused)",
              "bf16": "dist_bf16_kernel<0,2,3> (BF16x3 argmin form, decision fused)"}[peak_key]
     out["roofline"] = {"bound": "mfma", "achieved": flops / (h["ms_distance_kernel"] * 1e-3) / 1e12,
                        "peak": PEAK[peak_key] / 1e12, "unit": "TFLOP/s",

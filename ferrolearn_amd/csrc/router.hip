@@ -39,6 +39,7 @@ constexpr int64_t kInferRows = 32768;   // network inference chunk
 constexpr int64_t kMaxTargets = 32;     // certification targets per training query
 constexpr int kRouteAll = 3;            // internal route: every cell (exact by construction)
 constexpr int64_t kFullScanMax = 2048;  // queries a last all-cells scan takes
+constexpr int64_t kPairListBytes = (int64_t)1 << 31;   // per-pair top-k lists of one pass
 
 // Router network: d -> H -> H -> C with padded shapes (dp: 32, cp: 64).
 struct Mlp {
@@ -596,8 +597,11 @@ void launch_top_cells(const float *Z, int64_t ldz, int64_t rows, int C, int n_pr
 // found so far, more over the cells with the smallest bounds (8 or twice
 // n_probe, doubling up to min(31, n_cells)); what still fails (non-finite
 // queries) goes to the brute force.  A rescan starts
-// from the k best its parent found (prev_d, prev_i) and skips the cells the
-// parent scanned (prev_routes, prev_np per query).
+// from the k best its parent found (prev_d, prev_i) and skips the cells its
+// parent scanned (prev_routes, prev_np per query) -- its parent only: an
+// all-cells pass after a widened rescan can scan a cell of the first search
+// again, and the merge then drops the (distance, index) pairs it already
+// holds, so the result is the same.
 template <typename TQ, typename TX>
 void route_search(fl_router *r, const TQ *Q, int64_t nq, int k, int n_probe, int route,
                   double *dist, int64_t *idx, RtAcc &acc, int depth = 0,
@@ -795,11 +799,18 @@ void route_search(fl_router *r, const TQ *Q, int64_t nq, int k, int n_probe, int
         // k-th distance found so far, with the same exact kernels; that
         // certifies every finite query
         if (depth == 0) acc.n_rescan += m;
-        RtAcc inner;
-        route_search<TQ, TX>(r, fq, m, k, C, kRouteAll, fd, fi, inner, depth + 1, pd, pi, pr,
-                             n_probe);
-        acc.rows += inner.rows;
-        acc.n_fb += inner.n_fb;
+        // in chunks whose per-pair lists (C pairs per query, k entries of 12
+        // bytes) stay within kPairListBytes
+        const int64_t mc = std::max<int64_t>(1, kPairListBytes / ((int64_t)C * k * 12));
+        for (int64_t q0 = 0; q0 < m; q0 += mc) {
+            const int64_t mq = std::min(mc, m - q0);
+            RtAcc inner;
+            route_search<TQ, TX>(r, fq + q0 * d, mq, k, C, kRouteAll, fd + q0 * k, fi + q0 * k,
+                                 inner, depth + 1, pd + q0 * k, pi + q0 * k, pr + q0 * n_probe,
+                                 n_probe);
+            acc.rows += inner.rows;
+            acc.n_fb += inner.n_fb;
+        }
     } else if (route != kRouteAll && depth < 3 && (wide > n_probe || route != FL_ROUTE_BOUNDS)) {
         if (depth == 0) acc.n_rescan += m;
         RtAcc inner;
@@ -1125,7 +1136,10 @@ fl_status fl_router_kneighbors_device(fl_router_handle r, const void *dQ, int32_
             return FL_OK;
         }
         RtAcc acc;
-        const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(1 << 20, (1LL << 30) / n_probe));
+        // queries per pass: the per-pair lists (n_probe pairs per query, k
+        // entries of 12 bytes) stay within kPairListBytes
+        const int64_t chunk = std::max<int64_t>(
+            1, std::min<int64_t>(1 << 20, kPairListBytes / (n_probe * k * 12)));
         for (int64_t q0 = 0; q0 < nq; q0 += chunk) {
             const int64_t m = std::min(chunk, nq - q0);
             double *dd = d_dist + q0 * k;
