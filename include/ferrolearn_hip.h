@@ -574,8 +574,10 @@ fl_status fl_left_fold_device(const double *d_v, int64_t n, int64_t count, int64
 /* ------------------------------------------------------------------ */
 
 /* Backend::gemm (backend.rs:41-44; faer impl backend_faer.rs:61-78):
- * C (m*n) = A (m*k) . B (k2*n), f64, host pointers.  ShapeMismatch when
- * k != k2. */
+ * C (m*n) = A (m*k) . B (k2*n), f64, row-major host pointers, on f64 MFMA
+ * (summation order of k unspecified, as for a.dot(b)).  ShapeMismatch
+ * (expected [m, k], actual [k2, n]) when k != k2; InvalidParameter for a
+ * negative dimension or m above 65535 * 64. */
 fl_status fl_backend_gemm_f64(const double *A, int64_t m, int64_t k,
                               const double *B, int64_t k2, int64_t n,
                               double *C);

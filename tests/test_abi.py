@@ -109,7 +109,9 @@ def test_gemm_shape_mismatch():
                                      b.ctypes.data_as(C.c_void_p), 4, 2,
                                      np.zeros(4).ctypes.data_as(C.c_void_p))
     assert st == L.FL_SHAPE_MISMATCH
-    assert "inner dimensions" in L.last_error()
+    # the reference's fields: expected = A's shape, actual = B's shape
+    assert "expected [2, 3], actual [4, 2]" in L.last_error()
+    assert "gemm: A is 2x3 but B is 4x2 (inner dimensions 3 != 4)" in L.last_error()
 
 
 def test_no_device_gives_device_error_not_a_crash():

@@ -195,18 +195,24 @@ def test_torch_tensors_in_fresh_process():
     assert "torch interop ok" in r.stdout
 
 
-def test_gemm_backend():
+@pytest.mark.parametrize("m,k,n", [(37, 19, 23), (1, 1, 1), (64, 16, 64), (65, 17, 129),
+                                   (300, 0, 5), (200, 513, 170), (1000, 96, 1)])
+def test_gemm_backend(m, k, n):
+    # Backend::gemm = a.dot(b): f64 products, k summed in an unspecified
+    # order, so each entry is within a few ulp of sum |a||b| of the exact one.
     import ctypes as C
     from ferrolearn_amd import _lib as L
-    rng = np.random.default_rng(0)
-    A = rng.normal(size=(37, 19))
-    B = rng.normal(size=(19, 23))
-    out = np.zeros((37, 23))
-    st = L.lib().fl_backend_gemm_f64(A.ctypes.data_as(C.c_void_p), 37, 19,
-                                     B.ctypes.data_as(C.c_void_p), 19, 23,
+    rng = np.random.default_rng(m * 7 + k * 3 + n)
+    A = rng.normal(size=(m, k))
+    B = rng.normal(size=(k, n))
+    out = np.full((m, n), np.nan)
+    st = L.lib().fl_backend_gemm_f64(A.ctypes.data_as(C.c_void_p), m, k,
+                                     B.ctypes.data_as(C.c_void_p), k, n,
                                      out.ctypes.data_as(C.c_void_p))
-    assert st == 0
-    np.testing.assert_allclose(out, A @ B, rtol=1e-12, atol=1e-12)
+    assert st == 0, L.last_error()
+    ref = A @ B
+    tol = (k + 2) * 2.0 ** -52 * (np.abs(A) @ np.abs(B)) + 1e-300
+    assert (np.abs(out - ref) <= tol).all()
 
 
 def test_sqrt_and_division_correctly_rounded():
