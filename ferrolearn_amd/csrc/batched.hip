@@ -618,6 +618,19 @@ __global__ __launch_bounds__(256) void bk_assign_kernel(const BkProb *__restrict
     }
 }
 
+// Runs on the reference's serial path (n * d below KM_SERIAL_WORK) that did
+// at least one iteration: the inertia of the last assignment becomes the
+// left fold of its folds in row order (kmeans.rs:310-322), as in the
+// single fit.  One block of EF_NT threads per run.
+__global__ __launch_bounds__(EF_NT) void bk_serial_inertia_kernel(const BkRun *__restrict__ runs,
+                                                                  BkState *__restrict__ st) {
+    const BkRun &R = runs[blockIdx.x];
+    BkState &S = st[blockIdx.x];
+    if (R.n * (int64_t)R.d >= KM_SERIAL_WORK || S.it <= 0) return;   // the whole block
+    const double s = exact_fold_block(R.best, R.n, nullptr);
+    if (threadIdx.x == 0) S.inertia = s;
+}
+
 // -------------------------------------------------------------- update ----
 // The update of kmeans.rs:358-401 and the loop control of :468-488 for
 // every active run, in three kernels:
@@ -1088,6 +1101,11 @@ void run_batch(fl_kmeans_problem *problems, const std::vector<int64_t> &idx, hip
                 if (all) break;
             }
         }
+    }
+    if (max_iter > 0 && nruns > 0) {
+        hipLaunchKernelGGL(bk_serial_inertia_kernel, dim3((unsigned)nruns), dim3(EF_NT), 0, st,
+                           d_runs, d_st);
+        check_launch();
     }
     ptm.end(t_lloyd);
 

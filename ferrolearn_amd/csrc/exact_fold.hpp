@@ -49,30 +49,31 @@ constexpr int EF_P = 16;                   // window elements per thread
 constexpr int EF_W = EF_NT * EF_P;         // window staged in LDS (double-buffered)
 constexpr int EF_SERIAL = 32;              // serial adds after a binade crossing
 
+// The reference assigns serially, folding the inertia in row order, when
+// n * d < PARALLEL_WORK_THRESHOLD (ferrolearn-cluster/src/kmeans.rs:261,
+// 286-290, 310-322); above it rayon sums per-chunk folds in an unspecified
+// order.
+constexpr int64_t KM_SERIAL_WORK = 100000;
+
 // LDS position of window element i: one pad double after every 16, so that
 // a thread's 16 contiguous elements load without bank conflicts.
 __device__ __forceinline__ int ef_pad(int i) { return i + (i >> 4); }
 constexpr int EF_WPAD = EF_W + EF_W / 16;
 
-// grid.x = number of folds: fold b reads v[b*ld .. b*ld + n), writes its
-// result to sums[b] and, when prefix is given, the running sums
-// prefix[b*ld + i] = s_{i+1}.  A non-zero *stop (when given) makes the
-// launch a no-op.  Thread t owns elements [t*16, t*16 + 16) of each window
-// and keeps them in registers through the window's phases.
-static __global__ __attribute__((unused)) __launch_bounds__(EF_NT) void exact_fold_kernel(
-    const double *__restrict__ v, int64_t n, int64_t ld, const int32_t *__restrict__ stop,
-    double *__restrict__ sums, double *__restrict__ prefix) {
+// The fold of src[0..n) by the calling block of EF_NT threads (every
+// thread gets the result); when pre is given, also the running sums
+// pre[i] = s_{i+1}.  Thread t owns elements [t*16, t*16 + 16) of each
+// window and keeps them in registers through the window's phases.
+__device__ __forceinline__ double exact_fold_block(const double *__restrict__ src, int64_t n,
+                                                   double *__restrict__ pre) {
     static_assert(EF_P == 16, "one pad per 16 elements");
     using ScanD = rocprim::block_scan<double, EF_NT>;
     __shared__ double buf[2][EF_WPAD];
     __shared__ typename ScanD::storage_type scan_st;
     __shared__ int sh_xc[EF_NW], sh_mc[EF_NW];
     __shared__ double m_at[EF_NT];
-    if (stop && *stop) return;
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const int rs = t * EF_P;   // this thread's slice of every window
-    const double *src = v + (int64_t)blockIdx.x * ld;
-    double *pre = prefix ? prefix + (int64_t)blockIdx.x * ld : nullptr;
     // window 0 into buffer 0
 #pragma unroll
     for (int u = 0; u < EF_P; ++u) {
@@ -253,7 +254,20 @@ static __global__ __attribute__((unused)) __launch_bounds__(EF_NT) void exact_fo
         }
         __syncthreads();
     }
-    if (t == 0 && sums) sums[blockIdx.x] = s;
+    return s;
+}
+
+// grid.x = number of folds: fold b reads v[b*ld .. b*ld + n), writes its
+// result to sums[b] and, when prefix is given, the running sums
+// prefix[b*ld + i] = s_{i+1}.  A non-zero *stop (when given) makes the
+// launch a no-op.
+static __global__ __attribute__((unused)) __launch_bounds__(EF_NT) void exact_fold_kernel(
+    const double *__restrict__ v, int64_t n, int64_t ld, const int32_t *__restrict__ stop,
+    double *__restrict__ sums, double *__restrict__ prefix) {
+    if (stop && *stop) return;
+    const double s = exact_fold_block(v + (int64_t)blockIdx.x * ld, n,
+                                      prefix ? prefix + (int64_t)blockIdx.x * ld : nullptr);
+    if (threadIdx.x == 0 && sums) sums[blockIdx.x] = s;
 }
 
 }  // namespace fl

@@ -383,8 +383,16 @@ class Lloyd {
         tend(t2, &acc.ms_fb);
     }
 
-    // Deterministic sum of best[0..n) into *out (device).
+    // Sum of best[0..n) into *out (device): on the reference's serial path
+    // its row-order left fold, bit for bit; otherwise 1024-row blocks, a
+    // fixed order (the reference's rayon order is unspecified there).
     void inertia(const double *best, double *out) {
+        if (n_ * d_ < KM_SERIAL_WORK) {
+            hipLaunchKernelGGL(exact_fold_kernel, dim3(1), dim3(EF_NT), 0, st_, best, n_, n_,
+                               (const int32_t *)nullptr, out, (double *)nullptr);
+            check_launch();
+            return;
+        }
         const int64_t nb = ceil_div(n_, SUM_BLK);
         double *part = ws_.get<double>("part", nb);
         hipLaunchKernelGGL(block_sum_kernel, dim3((unsigned)nb), dim3(256), 0, st_, best, n_, part);
