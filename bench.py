@@ -57,11 +57,11 @@ TRAFFIC = {
     ("knn", "bf16x3"): (5.189e6 * 2.0 + 6.64e4) * 1024,
     ("knn", "fp64"): (8.508e6 * 2.0 + 6.15e4) * 1024,
     ("kmeans", "fp8"): (1.938e6 * 2.0 + 1.29e5) * 1024,
-    ("kmeans", "bf16x3"): (3.005e6 * 2.0 + 9.08e5) * 1024,
+    ("kmeans", "bf16x3"): (3.986e6 * 2.0 + 2.63e5) * 1024,
     ("kmeans", "fp64"): (1.291e6 * 2.0 + 6.25e5) * 1024,
 }
 MFMA_BUSY = {("knn", "bf16x3"): 0.541, ("knn", "fp64"): 0.705, ("kmeans", "fp8"): 0.240,
-             ("kmeans", "fp64"): 0.573}
+             ("kmeans", "bf16x3"): 0.468, ("kmeans", "fp64"): 0.573}
 TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes of bench.py at these sizes "
                   "(profiles/), copy-calibrated x2.0 / x1.0; not measured in this run")
 
@@ -301,7 +301,8 @@ def run_kmeans(args):
     peak_key = {"fp64": "f64", "exact": "f64", "fp8": "fp8"}.get(head, "bf16")
     kname = {"f64": "dist_f64_kernel<0,...> (argmin form)",
              "fp8": "dist_fp8_res_kernel (FP8 E4M3 argmin, centres resident in LDS, decision fused)",
-             "bf16": "dist_bf16_kernel<0,2,3> (BF16x3 argmin form, decision fused)"}[peak_key]
+             "bf16": "dist_bf16_res_kernel<2> (BF16x3 argmin, centres resident in LDS in "
+                     "chunks of 512, decision fused)"}[peak_key]
     out["roofline"] = {"bound": "mfma", "achieved": flops / (h["ms_distance_kernel"] * 1e-3) / 1e12,
                        "peak": PEAK[peak_key] / 1e12, "unit": "TFLOP/s",
                        "frac": flops / (h["ms_distance_kernel"] * 1e-3) / PEAK[peak_key],

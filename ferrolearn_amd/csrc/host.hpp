@@ -14,6 +14,7 @@
 #include "common.hpp"
 #include "dist_bf16.hpp"
 #include "dist_fp8.hpp"
+#include "dist_bf16_res.hpp"
 #include "rand09.hpp"
 #include "dist_f64.hpp"
 #include "dist_topm.hpp"
@@ -441,6 +442,29 @@ inline void launch_fp8_res(const Bf16Args &a, hipStream_t st) {
     hipLaunchKernelGGL(dist_fp8_res_kernel, dim3((unsigned)blocks), dim3(FR_WAVES * 64), shmem, st,
                        a);
     check_launch();
+}
+
+// The BF16x3 argmin filter with the centres resident in LDS (chunks of
+// 1024 / nkc centres), one block per compute unit.
+template <int NKC>
+void launch_bf16_res_t(const BfResArgs &ra, hipStream_t st) {
+    const size_t shmem = BfResShape<NKC>::SHMEM;
+    FL_HIP(hipFuncSetAttribute((const void *)dist_bf16_res_kernel<NKC>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
+    const int64_t blocks = std::max<int64_t>(
+        1, std::min<int64_t>(device_cu_count(), ceil_div(ceil_div(ra.a.nA, 32), BR_WAVES)));
+    hipLaunchKernelGGL(dist_bf16_res_kernel<NKC>, dim3((unsigned)blocks), dim3(BR_WAVES * 64),
+                       shmem, st, ra);
+    check_launch();
+}
+
+inline void launch_bf16_res(int nkc, const BfResArgs &ra, hipStream_t st) {
+    switch (nkc) {
+        case 1: return launch_bf16_res_t<1>(ra, st);
+        case 2: return launch_bf16_res_t<2>(ra, st);
+        case 4: return launch_bf16_res_t<4>(ra, st);
+    }
+    throw DeviceError("internal: unsupported feature chunk count for the resident filter");
 }
 
 // Number of splits of the streamed rows for the BF16 filter.  A block holds

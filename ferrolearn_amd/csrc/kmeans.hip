@@ -6,6 +6,7 @@
 // the strictly smallest inertia.  Labels and inertia come from the last
 // assignment, the centres are the post-update ones (:468-495).
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -292,7 +293,17 @@ class Lloyd {
             a.fcount = ws_.get<int32_t>("fcnt", 1);
             FL_HIP(hipMemsetAsync(a.fcount, 0, sizeof(int32_t), st_));
             t0 = tbegin();
-            launch_bf16(passes_, 0, nkc_, a, 1, st_);   // argmin form, decision fused
+            if (!a.rows && passes_ == 3 && nkc_ <= 4) {
+                // every row: the centres resident in LDS
+                BfResArgs ra;
+                ra.a = a;
+                ra.pk = ws_.get<float>("res_pk", n_);
+                ra.psec = ws_.get<float>("res_ps", n_);
+                ra.pidx = ws_.get<int32_t>("res_pi", n_);
+                launch_bf16_res(nkc_, ra, st_);
+            } else {
+                launch_bf16(passes_, 0, nkc_, a, 1, st_);   // argmin form, decision fused
+            }
             tend(t0, fp8_ ? &acc.ms_fin : &acc.ms_dist);
             fused = true;
         } else if (d4_ > 0) {
