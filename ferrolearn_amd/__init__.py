@@ -14,7 +14,8 @@ from .neighbors import (Algorithm, CsrMatrix, FittedKNeighborsClassifier,
                         FittedKNeighborsRegressor, FittedNearestNeighbors, GraphMode,
                         KNeighborsClassifier, KNeighborsRegressor, NearestNeighbors, Weights,
                         kneighbors_graph)
-from .metrics import Metric, pairwise_distances_argmin, pairwise_distances_argmin_min
+from .metrics import (Metric, euclidean_distances, pairwise_distances_argmin,
+                      pairwise_distances_argmin_min)
 from .router import CellRouter
 from .spectral import FittedSpectralClustering, SpectralClustering
 
@@ -24,6 +25,7 @@ __all__ = [
     "Algorithm", "Weights", "FerroError", "ShapeMismatch", "InsufficientSamples",
     "InvalidParameter", "NumericalInstability", "DeviceError", "DeviceArray", "to_device",
     "device", "CellRouter", "kmeans_fit_batched", "NearestNeighbors", "FittedNearestNeighbors",
-    "GraphMode", "CsrMatrix", "kneighbors_graph", "Metric", "pairwise_distances_argmin",
+    "GraphMode", "CsrMatrix", "kneighbors_graph", "Metric", "euclidean_distances",
+    "pairwise_distances_argmin",
     "pairwise_distances_argmin_min", "SpectralClustering", "FittedSpectralClustering",
 ]

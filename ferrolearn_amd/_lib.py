@@ -204,6 +204,8 @@ SIGNATURES = {
     "fl_knn_kneighbors": (C.c_int, [_vp, _vp, _i32, _i64, _i64, _i64, _vp, _vp, _rp]),
     "fl_knn_kneighbors_device": (C.c_int, [_vp, _vp, _i32, _i64, _i64, _i64, _vp, _vp, _rp]),
     "fl_nn_fit": (C.c_int, [_vp, _i32, _i64, _i64, _i64, _i32, C.POINTER(_vp)]),
+    "fl_euclidean_distances": (C.c_int, [_vp, _i32, _i64, _i64, _vp, _i64, _i64, _vp]),
+    "fl_euclidean_distances_device": (C.c_int, [_vp, _i32, _i64, _i64, _vp, _i64, _i64, _vp]),
     "fl_pairwise_distances_argmin_min": (C.c_int, [_vp, _i32, _i64, _i64, _vp, _i64, _i64, _vp,
                                                    _vp]),
     "fl_pairwise_distances_argmin_min_device": (C.c_int, [_vp, _i32, _i64, _i64, _vp, _i64, _i64,

@@ -1,6 +1,7 @@
-// metrics.hip -- pairwise_distances_argmin(_min) with Metric::Euclidean on gfx950.
+// metrics.hip -- euclidean_distances and pairwise_distances_argmin(_min)
+// with Metric::Euclidean on gfx950.
 //
-// Reference: ferrolearn-metrics/src/pairwise.rs:564-629, whose Euclidean
+// Reference: ferrolearn-metrics/src/pairwise.rs:157-189 and :564-629, whose Euclidean
 // distances are the expansion of euclidean_distances (:157-185):
 //   x_sq = fold(x*x), y_sq = fold(y*y), dot = fold(x*y)   (left folds)
 //   dist = sqrt(max((x_sq + y_sq) - 2*dot, 0))            (max: the non-NaN operand)
@@ -24,12 +25,13 @@ constexpr int PW_K = 32;       // features staged per step
 
 // Each thread folds its X row against PW_J staged Y rows at a time; features
 // arrive PW_K at a time through LDS (X transposed, one column per thread),
-// so every accumulator still adds its products in feature order.
-template <typename T>
+// so every accumulator still adds its products in feature order.  FULL:
+// every distance goes to dist_out (row-major n x m) instead of the argmin.
+template <typename T, bool FULL>
 __global__ __launch_bounds__(PW_ROWS) void pw_argmin_kernel(
     const T *__restrict__ X, int64_t n, const T *__restrict__ Y, int64_t m, int d,
     const double *__restrict__ xsq, const double *__restrict__ ysq, int64_t *__restrict__ idx_out,
-    double *__restrict__ min_out) {
+    double *__restrict__ min_out, double *__restrict__ dist_out) {
     __shared__ double xs[PW_K][PW_ROWS + 1];
     __shared__ double ys[PW_K][PW_J];
     const int t = threadIdx.x;
@@ -68,13 +70,15 @@ __global__ __launch_bounds__(PW_ROWS) void pw_argmin_kernel(
         for (int jj = 0; jj < jw; ++jj) {
             const int64_t j = j0 + jj;
             const double v = sqrt(fmax((xq + ysq[j]) - 2.0 * acc[jj], 0.0));
-            if (j == 0 || v < bv) {
+            if (FULL) {
+                if (i < n) dist_out[i * m + j] = v;
+            } else if (j == 0 || v < bv) {
                 bv = v;
                 bi = j;
             }
         }
     }
-    if (i < n) {
+    if (!FULL && i < n) {
         idx_out[i] = bi;
         if (min_out) min_out[i] = bv;
     }
@@ -82,7 +86,7 @@ __global__ __launch_bounds__(PW_ROWS) void pw_argmin_kernel(
 
 template <typename T>
 void pw_argmin_run(const T *X, int64_t n, const T *Y, int64_t m, int d, int64_t *idx,
-                   double *mins, Workspace &ws, hipStream_t st) {
+                   double *mins, double *dist, Workspace &ws, hipStream_t st) {
     double *xsq = ws.get<double>("pw_xsq", n);
     double *xs1 = ws.get<double>("pw_xs1", n);
     double *ysq = ws.get<double>("pw_ysq", m);
@@ -91,8 +95,12 @@ void pw_argmin_run(const T *X, int64_t n, const T *Y, int64_t m, int d, int64_t 
                        xs1);
     hipLaunchKernelGGL(row_norms_kernel<T>, dim3(grid1(m, 256)), dim3(256), 0, st, Y, m, d, ysq,
                        ys1);
-    hipLaunchKernelGGL(pw_argmin_kernel<T>, dim3(grid1(n, PW_ROWS)), dim3(PW_ROWS), 0, st, X, n, Y,
-                       m, d, xsq, ysq, idx, mins);
+    if (dist)
+        hipLaunchKernelGGL((pw_argmin_kernel<T, true>), dim3(grid1(n, PW_ROWS)), dim3(PW_ROWS), 0,
+                           st, X, n, Y, m, d, xsq, ysq, idx, mins, dist);
+    else
+        hipLaunchKernelGGL((pw_argmin_kernel<T, false>), dim3(grid1(n, PW_ROWS)), dim3(PW_ROWS), 0,
+                           st, X, n, Y, m, d, xsq, ysq, idx, mins, dist);
     check_launch();
 }
 
@@ -123,15 +131,15 @@ fl_status pw_validate(int32_t dtype, int64_t n, int64_t dx, int64_t m, int64_t d
 }
 
 fl_status pw_device(const void *dX, int32_t dtype, int64_t n, const void *dY, int64_t m, int d,
-                    int64_t *d_idx, double *d_min) {
+                    int64_t *d_idx, double *d_min, double *d_dist = nullptr) {
     StreamGuard sg;
     Workspace ws;
     if (dtype == FL_F32)
-        pw_argmin_run<float>((const float *)dX, n, (const float *)dY, m, d, d_idx, d_min, ws,
-                             sg.s);
+        pw_argmin_run<float>((const float *)dX, n, (const float *)dY, m, d, d_idx, d_min, d_dist,
+                             ws, sg.s);
     else
-        pw_argmin_run<double>((const double *)dX, n, (const double *)dY, m, d, d_idx, d_min, ws,
-                              sg.s);
+        pw_argmin_run<double>((const double *)dX, n, (const double *)dY, m, d, d_idx, d_min,
+                              d_dist, ws, sg.s);
     FL_HIP(hipStreamSynchronize(sg.s));
     return FL_OK;
 }
@@ -170,6 +178,37 @@ fl_status fl_pairwise_distances_argmin_min(const void *X, int32_t dtype, int64_t
         if (fl_status s = pw_device(dX, dtype, n, dY, m, (int)dx, di, dm)) return s;
         FL_HIP(hipMemcpy(idx_out, di, (size_t)n * 8, hipMemcpyDeviceToHost));
         if (min_out) FL_HIP(hipMemcpy(min_out, dm, (size_t)n * 8, hipMemcpyDeviceToHost));
+        return FL_OK;
+    });
+}
+
+// euclidean_distances (pairwise.rs:157-189): the same validation, then
+// every distance.
+fl_status fl_euclidean_distances_device(const void *dX, int32_t dtype, int64_t n, int64_t dx,
+                                        const void *dY, int64_t m, int64_t dy, double *d_out) {
+    return guarded([&]() -> fl_status {
+        if (fl_status s = pw_validate(dtype, n, dx, m, dy)) return s;
+        if (!dX || !dY || !d_out)
+            return fail(FL_INVALID_PARAMETER, "InvalidParameter: buffers: must not be NULL");
+        return pw_device(dX, dtype, n, dY, m, (int)dx, nullptr, nullptr, d_out);
+    });
+}
+
+fl_status fl_euclidean_distances(const void *X, int32_t dtype, int64_t n, int64_t dx,
+                                 const void *Y, int64_t m, int64_t dy, double *out) {
+    return guarded([&]() -> fl_status {
+        if (fl_status s = pw_validate(dtype, n, dx, m, dy)) return s;
+        if (!X || !Y || !out)
+            return fail(FL_INVALID_PARAMETER, "InvalidParameter: buffers: must not be NULL");
+        const size_t es = dtype_size(dtype);
+        DevBuf bx, by, bo;
+        void *dX = bx.get((size_t)(n * dx) * es);
+        void *dY = by.get((size_t)(m * dy) * es);
+        double *dO = (double *)bo.get((size_t)(n * m) * 8);
+        FL_HIP(hipMemcpy(dX, X, (size_t)(n * dx) * es, hipMemcpyHostToDevice));
+        FL_HIP(hipMemcpy(dY, Y, (size_t)(m * dy) * es, hipMemcpyHostToDevice));
+        if (fl_status s = pw_device(dX, dtype, n, dY, m, (int)dx, nullptr, nullptr, dO)) return s;
+        FL_HIP(hipMemcpy(out, dO, (size_t)(n * m) * 8, hipMemcpyDeviceToHost));
         return FL_OK;
     });
 }

@@ -1,9 +1,10 @@
-"""pairwise_distances_argmin / pairwise_distances_argmin_min on the GPU.
+"""euclidean_distances, pairwise_distances_argmin and
+pairwise_distances_argmin_min on the GPU.
 
-Mirrors ferrolearn-metrics/src/pairwise.rs:564-629 with Metric::Euclidean:
-the distances are euclidean_distances' expansion (pairwise.rs:157-185),
-sqrt(max((|x|^2 + |y|^2) - 2 x.y, 0)) with left folds, and ties go to the
-lowest index.  Results are bit-exact against the reference's f64 path.
+Mirrors ferrolearn-metrics/src/pairwise.rs:157-189 and :564-629 with
+Metric::Euclidean: the distances are euclidean_distances' expansion,
+sqrt(max((|x|^2 + |y|^2) - 2 x.y, 0)) with left folds, and argmin ties go to
+the lowest index.  Results are bit-exact against the reference's f64 path.
 """
 from __future__ import annotations
 
@@ -59,6 +60,43 @@ def _run(x, y, metric, want_min):
         yh.ctypes.data_as(C.c_void_p), m, dy, idx.ctypes.data_as(C.c_void_p),
         mins.ctypes.data_as(C.c_void_p) if want_min else None))
     return idx, mins
+
+
+def _pair(x, y):
+    """The two inputs on one side (device when either is) with one dtype."""
+    if _is_device(x) or _is_device(y):
+        import torch
+        from .device import DeviceArray
+        xd = _dev2d(x if _is_device(x) else torch.from_numpy(_host2d(x)).cuda())
+        yd = _dev2d(y if _is_device(y) else torch.from_numpy(_host2d(y)).cuda())
+        if str(xd.dtype) != str(yd.dtype):   # f32 widened to f64 is exact
+            if isinstance(xd, DeviceArray) or isinstance(yd, DeviceArray):
+                raise TypeError("X and Y must share a dtype")
+            xd, yd = xd.double(), yd.double()
+        return True, xd, yd
+    xh, yh = _host2d(x), _host2d(y)
+    if xh.dtype != yh.dtype:
+        xh, yh = xh.astype(np.float64), yh.astype(np.float64)
+    return False, xh, yh
+
+
+def euclidean_distances(x, y):
+    """pairwise.rs:157-189: the (n, m) f64 matrix of distances between the
+    rows of x and y (a device tensor when either input is on the device)."""
+    dev, xa, ya = _pair(x, y)
+    n, dx = xa.shape
+    m, dy = ya.shape
+    if dev:
+        out = _dev_empty((n, m), np.float64, xa)
+        check(L.lib().fl_euclidean_distances_device(
+            C.c_void_p(xa.data_ptr()), _dtype_code(xa.dtype), n, dx, C.c_void_p(ya.data_ptr()),
+            m, dy, C.c_void_p(out.data_ptr())))
+        return out
+    out = np.zeros((n, m), np.float64)
+    check(L.lib().fl_euclidean_distances(
+        xa.ctypes.data_as(C.c_void_p), _dtype_code(xa.dtype), n, dx,
+        ya.ctypes.data_as(C.c_void_p), m, dy, out.ctypes.data_as(C.c_void_p)))
+    return out
 
 
 def pairwise_distances_argmin(x, y, metric=Metric.Euclidean):

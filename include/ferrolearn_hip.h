@@ -398,6 +398,18 @@ void fl_knn_free(fl_knn_handle h);
 /* pairwise_distances_argmin(_min) -- ferrolearn-metrics/src/pairwise.rs */
 /* ------------------------------------------------------------------ */
 
+/* euclidean_distances (pairwise.rs:157-189): out (n x m, row-major f64)
+ * holds sqrt(max((|x_i|^2 + |y_j|^2) - 2 x_i.y_j, 0)) for every pair, each
+ * term a left fold, max returning the non-NaN operand; bit-exact.  Same
+ * dtype rule and errors, in the same order, as below. */
+fl_status fl_euclidean_distances(const void *X, int32_t dtype, int64_t n,
+                                 int64_t dx, const void *Y, int64_t m,
+                                 int64_t dy, double *out);
+/* Same with X, Y and out in device memory. */
+fl_status fl_euclidean_distances_device(const void *dX, int32_t dtype,
+                                        int64_t n, int64_t dx, const void *dY,
+                                        int64_t m, int64_t dy, double *d_out);
+
 /* pairwise_distances_argmin_min (pairwise.rs:596-629) with
  * Metric::Euclidean (also pairwise_distances_argmin, :564-588, with
  * min_out == NULL): for each of the n rows of X the index of the closest of

@@ -92,6 +92,8 @@ def lib():
         L.or_minibatch_fit_stdrng.restype = None
         L.or_minibatch_fit_stdrng.argtypes = [_dp, _i64, _i64, _i64, _i64, _i64, C.c_double,
                                               _i64, C.c_uint64, C.c_int, _dp, _ip, _dp, _ip]
+        L.or_euclidean_distances.restype = None
+        L.or_euclidean_distances.argtypes = [_dp, _i64, _dp, _i64, _i64, _dp, C.c_int]
         L.or_pairwise_argmin_min.restype = None
         L.or_pairwise_argmin_min.argtypes = [_dp, _i64, _dp, _i64, _i64, _ip, _dp, C.c_int]
         L.or_class_scores.restype = None
@@ -316,6 +318,17 @@ def knn_to_csr(dist, idx, n_cols, distance_mode):
             indices[i * k + j] = c
             data[i * k + j] = v
     return indptr, indices, data, (nq, int(n_cols))
+
+
+def euclidean_distances(X, Y, nthreads=0):
+    """pairwise.rs:157-189: the (n, m) matrix of distances between the rows
+    of X and Y in the expansion arithmetic."""
+    X, Y = _f64(X), _f64(Y)
+    n, d = X.shape
+    m = Y.shape[0]
+    out = np.zeros((n, m), np.float64)
+    lib().or_euclidean_distances(_p(X), n, _p(Y), m, d, _p(out), nthreads)
+    return out
 
 
 def pairwise_argmin_min(X, Y, nthreads=0):

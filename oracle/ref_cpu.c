@@ -949,6 +949,32 @@ int or_max_threads(void) { return or_threads(0); }
  * non-NaN operand (f64::max, as C's fmax), and the index the first j with
  * the smallest distance (best starts at j = 0, then strict <).  min_out may
  * be NULL. */
+/* ferrolearn-metrics/src/pairwise.rs:157-189 (euclidean_distances): every
+ * entry out[i*m + j] = sqrt(max((x_sq + y_sq) - 2*dot, 0)) with the left
+ * folds above, max returning the non-NaN operand. */
+void or_euclidean_distances(const double *X, int64_t n, const double *Y, int64_t m, int64_t d,
+                            double *out, int nthreads) {
+    nthreads = or_threads(nthreads);
+    double *ysq = (double *)malloc((size_t)(m > 0 ? m : 1) * sizeof(double));
+    for (int64_t j = 0; j < m; ++j) {
+        double s = 0.0;
+        for (int64_t k = 0; k < d; ++k) s = s + Y[j * d + k] * Y[j * d + k];
+        ysq[j] = s;
+    }
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads)
+    for (int64_t i = 0; i < n; ++i) {
+        const double *x = X + i * d;
+        double xsq = 0.0;
+        for (int64_t k = 0; k < d; ++k) xsq = xsq + x[k] * x[k];
+        for (int64_t j = 0; j < m; ++j) {
+            double dot = 0.0;
+            for (int64_t k = 0; k < d; ++k) dot = dot + x[k] * Y[j * d + k];
+            out[i * m + j] = sqrt(fmax((xsq + ysq[j]) - 2.0 * dot, 0.0));
+        }
+    }
+    free(ysq);
+}
+
 void or_pairwise_argmin_min(const double *X, int64_t n, const double *Y, int64_t m, int64_t d,
                             int64_t *idx_out, double *min_out, int nthreads) {
     nthreads = or_threads(nthreads);

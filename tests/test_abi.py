@@ -206,6 +206,16 @@ def test_pairwise_argmin_validation_order():
         F.pairwise_distances_argmin(np.ones((2, 2)), np.ones((3, 2)), F.Metric.Manhattan)
 
 
+def test_euclidean_distances_validation_order():
+    # pairwise.rs:163-164 (check_feature_dim, then check_non_empty)
+    with pytest.raises(F.ShapeMismatch, match="feature dimensions must match"):
+        F.euclidean_distances(np.ones((0, 2)), np.ones((3, 3)))
+    with pytest.raises(F.InsufficientSamples, match="at least one row"):
+        F.euclidean_distances(np.ones((2, 2)), np.ones((0, 2)))
+    with pytest.raises(F.InsufficientSamples, match="feature dimension must be at least 1"):
+        F.euclidean_distances(np.ones((2, 0)), np.ones((3, 0)))
+
+
 def test_spectral_validation():
     # spectral.rs:240-258
     x = np.ones((10, 2))

@@ -58,6 +58,32 @@ def test_pairwise_argmin_ties_cancellation_nan():
     assert (idx == idx_o).all() and (bits(mins) == bits(min_o)).all()
 
 
+@pytest.mark.parametrize("shape", [(700, 300, 37), (257, 31, 5), (1, 1, 1), (600, 65, 70)])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_euclidean_distances_matches_oracle(shape, dtype):
+    n, m, d = shape
+    rng = np.random.default_rng(n * 3 + m + d)
+    X = blobs(n, d, 12, 5).astype(dtype)
+    Y = (blobs(m, d, 12, 6) + rng.normal(scale=0.1, size=(m, d))).astype(dtype)
+    want = O.euclidean_distances(X, Y)
+    got = F.euclidean_distances(X, Y)
+    assert got.shape == (n, m) and (bits(got) == bits(want)).all()
+    gd = F.euclidean_distances(torch.from_numpy(X).cuda(), torch.from_numpy(Y).cuda())
+    assert (bits(gd.cpu().numpy()) == bits(want)).all()
+
+
+def test_euclidean_distances_cancellation_nan_doc_example():
+    d = F.euclidean_distances(np.array([[0.0, 0.0], [1.0, 0.0]]),
+                              np.array([[0.0, 0.0], [0.0, 1.0]]))   # pairwise.rs:146-154
+    assert d[0, 0] == 0.0 and d[0, 1] == 1.0 and d[1, 0] == 1.0 and d[1, 1] == np.sqrt(2.0)
+    rng = np.random.default_rng(9)
+    a = 1e8 + rng.normal(size=(300, 3))
+    b = 1e8 + rng.normal(size=(70, 3))
+    b[:5] = a[:5]
+    a[7, 1] = np.nan
+    assert (bits(F.euclidean_distances(a, b)) == bits(O.euclidean_distances(a, b))).all()
+
+
 def test_pairwise_argmin_mixed_dtypes_widen():
     X = blobs(300, 8, 5, 3).astype(np.float32)
     Y = blobs(40, 8, 5, 4)

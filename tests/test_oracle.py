@@ -626,3 +626,62 @@ def test_kmeans_and_minibatch_fit_stdrng_golden():
         assert it == case["n_iter"] and list(l) == case["labels"]
         assert (bits(c) == bits(arr(case["centers"]))).all()
         assert inertia == float.fromhex(case["inertia"])
+
+
+def _euclid_loops(X, Y):
+    """pairwise.rs:166-183 as plain Python loops of binary64 operations."""
+    xs = [0.0] * len(X)
+    for i, x in enumerate(X):
+        acc = 0.0
+        for v in x:
+            acc = acc + v * v
+        xs[i] = acc
+    ys = [0.0] * len(Y)
+    for j, y in enumerate(Y):
+        acc = 0.0
+        for v in y:
+            acc = acc + v * v
+        ys[j] = acc
+    out = []
+    for i, x in enumerate(X):
+        row = []
+        for j, y in enumerate(Y):
+            dot = 0.0
+            for a, b in zip(x, y):
+                dot = dot + a * b
+            v = (xs[i] + ys[j]) - 2.0 * dot
+            v = 0.0 if v != v else max(v, 0.0)   # f64::max: the non-NaN operand
+            row.append(math.sqrt(v))
+        out.append(row)
+    return np.array(out)
+
+
+def test_euclidean_distances_doc_example():
+    # pairwise.rs:146-154
+    d = O.euclidean_distances(np.array([[0.0, 0.0], [1.0, 0.0]]),
+                              np.array([[0.0, 0.0], [0.0, 1.0]]))
+    assert d[0, 0] == 0.0 and d[0, 1] == 1.0 and d[1, 0] == 1.0 and d[1, 1] == math.sqrt(2.0)
+
+
+def test_euclidean_distances_against_loops():
+    rng = np.random.default_rng(11)
+    for n, m, d, scale in ((7, 5, 3, 1.0), (12, 9, 17, 1e8), (4, 6, 40, 1e-3)):
+        X = rng.normal(size=(n, d)) * scale + scale
+        Y = rng.normal(size=(m, d)) * scale + scale
+        Y[0] = X[0]                     # a zero distance that the expansion can miss
+        got = O.euclidean_distances(X, Y)
+        assert (bits(got) == bits(_euclid_loops(X.tolist(), Y.tolist()))).all()
+        # max(., 0) clamps the cancelled negatives, and the row minima are
+        # pairwise_distances_argmin_min's
+        assert (got >= 0.0).all()
+        idx, mins = O.pairwise_argmin_min(X, Y)
+        assert (bits(mins) == bits(got.min(axis=1))).all()
+        assert (idx == got.argmin(axis=1)).all()
+
+
+def test_euclidean_distances_nan_is_clamped_like_f64_max():
+    # f64::max returns the non-NaN operand: a NaN expansion becomes 0
+    X = np.array([[np.nan, 1.0], [1.0, 2.0]])
+    Y = np.array([[0.0, 0.0]])
+    d = O.euclidean_distances(X, Y)
+    assert d[0, 0] == 0.0 and d[1, 0] == math.sqrt(5.0)
