@@ -6,6 +6,7 @@ kernels behind the C ABI in include/ferrolearn_hip.h.  Labels, iteration
 counts and neighbour indices match the reference's f64 CPU path exactly.
 """
 from . import device
+from .bisecting import BisectingKMeans, BisectingStrategy, FittedBisectingKMeans
 from .cluster import FittedKMeans, KMeans, MiniBatchKMeans, kmeans_fit_batched, kmeans_plusplus
 from .device import DeviceArray, to_device
 from .errors import (DeviceError, FerroError, InsufficientSamples, InvalidParameter,
@@ -28,4 +29,5 @@ __all__ = [
     "GraphMode", "CsrMatrix", "kneighbors_graph", "Metric", "euclidean_distances",
     "pairwise_distances_argmin",
     "pairwise_distances_argmin_min", "SpectralClustering", "FittedSpectralClustering",
+    "BisectingKMeans", "BisectingStrategy", "FittedBisectingKMeans",
 ]

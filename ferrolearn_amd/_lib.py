@@ -72,6 +72,17 @@ class fl_kmeans_params(C.Structure):
     ]
 
 
+class fl_bisecting_params(C.Structure):
+    _fields_ = [
+        ("n_clusters", C.c_int64),
+        ("max_iter", C.c_int64),
+        ("n_init", C.c_int64),
+        ("random_state", C.c_uint64),
+        ("has_random_state", C.c_int32),
+        ("strategy", C.c_int32),
+    ]
+
+
 class fl_minibatch_params(C.Structure):
     _fields_ = [
         ("n_clusters", C.c_int64),
@@ -204,6 +215,13 @@ SIGNATURES = {
     "fl_knn_kneighbors": (C.c_int, [_vp, _vp, _i32, _i64, _i64, _i64, _vp, _vp, _rp]),
     "fl_knn_kneighbors_device": (C.c_int, [_vp, _vp, _i32, _i64, _i64, _i64, _vp, _vp, _rp]),
     "fl_nn_fit": (C.c_int, [_vp, _i32, _i64, _i64, _i64, _i32, C.POINTER(_vp)]),
+    "fl_bisecting_params_default": (None, [C.POINTER(fl_bisecting_params), _i64]),
+    "fl_bisecting_kmeans_fit": (C.c_int, [_vp, _i32, _i64, _i64, C.POINTER(fl_bisecting_params),
+                                          _vp, _vp, C.POINTER(C.c_double),
+                                          C.POINTER(C.c_int64)]),
+    "fl_bisecting_kmeans_fit_device": (C.c_int, [_vp, _i32, _i64, _i64,
+                                                 C.POINTER(fl_bisecting_params), _vp, _vp,
+                                                 C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "fl_euclidean_distances": (C.c_int, [_vp, _i32, _i64, _i64, _vp, _i64, _i64, _vp]),
     "fl_euclidean_distances_device": (C.c_int, [_vp, _i32, _i64, _i64, _vp, _i64, _i64, _vp]),
     "fl_pairwise_distances_argmin_min": (C.c_int, [_vp, _i32, _i64, _i64, _vp, _i64, _i64, _vp,

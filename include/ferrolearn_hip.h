@@ -395,6 +395,54 @@ int64_t fl_knn_n_samples_fit(fl_knn_handle h);
 void fl_knn_free(fl_knn_handle h);
 
 /* ------------------------------------------------------------------ */
+/* BisectingKMeans -- ferrolearn-cluster/src/bisecting_kmeans.rs       */
+/* ------------------------------------------------------------------ */
+
+/* BisectingStrategy (bisecting_kmeans.rs:54-61). */
+typedef enum fl_bisect_strategy {
+    FL_BISECT_LARGEST_CLUSTER = 0,
+    FL_BISECT_LARGEST_SSE = 1
+} fl_bisect_strategy;
+
+typedef struct fl_bisecting_params {
+    int64_t n_clusters;
+    int64_t max_iter;        /* per 2-means bisection */
+    int64_t n_init;          /* 2-means runs per bisection, best SSE kept */
+    uint64_t random_state;   /* used when has_random_state != 0; else 0 */
+    int32_t has_random_state;
+    int32_t strategy;        /* fl_bisect_strategy */
+} fl_bisecting_params;
+
+/* BisectingKMeans::new (bisecting_kmeans.rs:93-102): max_iter 300,
+ * n_init 10, no seed, LargestCluster. */
+void fl_bisecting_params_default(fl_bisecting_params *p, int64_t n_clusters);
+
+/* Fit<Array2<F>, ()> for BisectingKMeans (bisecting_kmeans.rs:326-505):
+ * centers_out (n_clusters * d f64; the first *n_clusters_out rows are
+ * written), labels_out (n), inertia_out, n_clusters_out (fewer than
+ * n_clusters when no cluster of two or more samples is left).  Labels,
+ * centres and inertia equal the reference's bit for bit (f32 data widened
+ * to f64).  Errors in the reference's order: n_clusters or n_init zero ->
+ * InvalidParameter; n < n_clusters -> InsufficientSamples.  Where the
+ * reference panics (no 2-means run below f64::MAX) or would loop without
+ * end (1000 bisections in a row with an empty side, identical samples):
+ * NumericalInstability.  Predict and transform of the fitted model are
+ * fl_kmeans_predict / fl_kmeans_transform (bisecting_kmeans.rs:531-605
+ * compute the same). */
+fl_status fl_bisecting_kmeans_fit(const void *X, int32_t dtype, int64_t n,
+                                  int64_t d, const fl_bisecting_params *params,
+                                  double *centers_out, int64_t *labels_out,
+                                  double *inertia_out, int64_t *n_clusters_out);
+/* Same with X, centers_out and labels_out in device memory. */
+fl_status fl_bisecting_kmeans_fit_device(const void *dX, int32_t dtype,
+                                         int64_t n, int64_t d,
+                                         const fl_bisecting_params *params,
+                                         double *d_centers_out,
+                                         int64_t *d_labels_out,
+                                         double *inertia_out,
+                                         int64_t *n_clusters_out);
+
+/* ------------------------------------------------------------------ */
 /* pairwise_distances_argmin(_min) -- ferrolearn-metrics/src/pairwise.rs */
 /* ------------------------------------------------------------------ */
 

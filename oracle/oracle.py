@@ -92,6 +92,9 @@ def lib():
         L.or_minibatch_fit_stdrng.restype = None
         L.or_minibatch_fit_stdrng.argtypes = [_dp, _i64, _i64, _i64, _i64, _i64, C.c_double,
                                               _i64, C.c_uint64, C.c_int, _dp, _ip, _dp, _ip]
+        L.or_bisecting_kmeans.restype = _i64
+        L.or_bisecting_kmeans.argtypes = [_dp, _i64, _i64, _i64, _i64, _i64, C.c_uint64, C.c_int,
+                                          _i64, _dp, _ip, _dp]
         L.or_euclidean_distances.restype = None
         L.or_euclidean_distances.argtypes = [_dp, _i64, _dp, _i64, _i64, _dp, C.c_int]
         L.or_pairwise_argmin_min.restype = None
@@ -318,6 +321,26 @@ def knn_to_csr(dist, idx, n_cols, distance_mode):
             indices[i * k + j] = c
             data[i * k + j] = v
     return indptr, indices, data, (nq, int(n_cols))
+
+
+def bisecting_kmeans(X, n_clusters, max_iter=300, n_init=10, seed=0, strategy="size",
+                     max_stall=1000):
+    """bisecting_kmeans.rs:326-505: (centres, labels, inertia).  strategy
+    "size" = LargestCluster, "sse" = LargestSSE.  Raises where the reference
+    panics (no 2-means run below f64::MAX) or would keep drawing (max_stall
+    bisections in a row with an empty side)."""
+    X = _f64(X)
+    n, d = X.shape
+    cen = np.zeros((n_clusters, d), np.float64)
+    lab = np.zeros(n, np.int64)
+    inertia = np.zeros(1, np.float64)
+    nc = lib().or_bisecting_kmeans(_p(X), n, d, n_clusters, max_iter, n_init, seed & (2**64 - 1),
+                                   0 if strategy == "size" else 1, max_stall, _p(cen), _pi(lab),
+                                   _p(inertia))
+    if nc < 0:
+        raise RuntimeError("bisecting_kmeans: " + ("no 2-means run below f64::MAX" if nc == -1
+                                                   else "bisections keep leaving a side empty"))
+    return cen[:nc].copy(), lab, float(inertia[0])
 
 
 def euclidean_distances(X, Y, nthreads=0):

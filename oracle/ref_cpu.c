@@ -1005,3 +1005,208 @@ void or_pairwise_argmin_min(const double *X, int64_t n, const double *Y, int64_t
     }
     free(ysq);
 }
+
+/* ------------------------------------------------------------------------ */
+/* BisectingKMeans -- ferrolearn-cluster/src/bisecting_kmeans.rs            */
+/* ------------------------------------------------------------------------ */
+
+/* :174-178: acc + (a - b) * (a - b), left to right. */
+static double or_bk_sqe(const double *a, const double *b, int64_t d) {
+    double s = 0.0;
+    for (int64_t k = 0; k < d; ++k) {
+        const double t = a[k] - b[k];
+        s = s + t * t;
+    }
+    return s;
+}
+
+/* :225-322 (run_2means) on the rows idx[0..n): two distinct random rows as
+ * centres (random_range(0..n), redrawn while equal when n > 1), then until
+ * no label changes or max_iter: d0 <= d1 -> 0 else 1; centres = sums in
+ * member order / counts (a centre with no member keeps its value); the
+ * total SSE is the left fold over the members in order. */
+static double or_bk_2means(const double *X, int64_t d, const int64_t *idx, int64_t n,
+                           int64_t max_iter, or_stdrng *g, uint8_t *lab, double *c0,
+                           double *c1) {
+    const int64_t i0 = or_stdrng_range_from(g, 0, n);
+    int64_t i1 = or_stdrng_range_from(g, 0, n);
+    if (n > 1)
+        while (i1 == i0) i1 = or_stdrng_range_from(g, 0, n);
+    memcpy(c0, X + idx[i0] * d, (size_t)d * sizeof(double));
+    memcpy(c1, X + idx[i1] * d, (size_t)d * sizeof(double));
+    memset(lab, 0, (size_t)n);
+    double *s0 = (double *)malloc((size_t)d * sizeof(double));
+    double *s1 = (double *)malloc((size_t)d * sizeof(double));
+    for (int64_t it = 0; it < max_iter; ++it) {
+        int changed = 0;
+        for (int64_t li = 0; li < n; ++li) {
+            const double *x = X + idx[li] * d;
+            const double d0 = or_bk_sqe(x, c0, d), d1 = or_bk_sqe(x, c1, d);
+            const uint8_t nl = d0 <= d1 ? 0 : 1;
+            if (nl != lab[li]) {
+                lab[li] = nl;
+                changed = 1;
+            }
+        }
+        if (!changed) break;
+        memset(s0, 0, (size_t)d * sizeof(double));
+        memset(s1, 0, (size_t)d * sizeof(double));
+        double n0 = 0.0, n1 = 0.0;
+        for (int64_t li = 0; li < n; ++li) {
+            const double *x = X + idx[li] * d;
+            if (lab[li] == 0) {
+                n0 = n0 + 1.0;
+                for (int64_t j = 0; j < d; ++j) s0[j] = s0[j] + x[j];
+            } else {
+                n1 = n1 + 1.0;
+                for (int64_t j = 0; j < d; ++j) s1[j] = s1[j] + x[j];
+            }
+        }
+        if (n0 > 0.0)
+            for (int64_t j = 0; j < d; ++j) c0[j] = s0[j] / n0;
+        if (n1 > 0.0)
+            for (int64_t j = 0; j < d; ++j) c1[j] = s1[j] / n1;
+    }
+    free(s0);
+    free(s1);
+    double tot = 0.0;
+    for (int64_t li = 0; li < n; ++li)
+        tot = tot + or_bk_sqe(X + idx[li] * d, lab[li] == 0 ? c0 : c1, d);
+    return tot;
+}
+
+typedef struct {
+    int64_t *idx;
+    int64_t n;
+    double *center;
+    double sse;
+} or_bk_cluster;
+
+/* :326-505 (fit).  strategy 0 = LargestCluster (max_by_key: the last of
+ * equal sizes), 1 = LargestSSE (max_by with partial_cmp, unordered as
+ * equal: the last of equal or unordered).  Returns the number of clusters,
+ * or -1 where the reference panics (no run with SSE below f64::MAX), or -2
+ * after max_stall bisections in a row that left one side empty (the
+ * reference would keep drawing). */
+int64_t or_bisecting_kmeans(const double *X, int64_t n, int64_t d, int64_t n_clusters,
+                            int64_t max_iter, int64_t n_init, uint64_t seed, int strategy,
+                            int64_t max_stall, double *centers_out, int64_t *labels_out,
+                            double *inertia_out) {
+    or_bk_cluster *cl = (or_bk_cluster *)calloc((size_t)n_clusters + 1, sizeof(or_bk_cluster));
+    int64_t nc = 1;
+    cl[0].idx = (int64_t *)malloc((size_t)n * sizeof(int64_t));
+    for (int64_t i = 0; i < n; ++i) cl[0].idx[i] = i;
+    cl[0].n = n;
+    cl[0].center = (double *)calloc((size_t)d, sizeof(double));
+    /* :192-223 compute_cluster_stats */
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t j = 0; j < d; ++j) cl[0].center[j] = cl[0].center[j] + X[i * d + j];
+    for (int64_t j = 0; j < d; ++j) cl[0].center[j] = cl[0].center[j] / (double)n;
+    double sse = 0.0;
+    for (int64_t i = 0; i < n; ++i) sse = sse + or_bk_sqe(X + i * d, cl[0].center, d);
+    cl[0].sse = sse;
+
+    uint64_t split_count = 0;
+    int64_t stall = 0, ret = 0;
+    uint8_t *lab = (uint8_t *)malloc((size_t)n + 1);
+    uint8_t *blab = (uint8_t *)malloc((size_t)n + 1);
+    double *c0 = (double *)malloc((size_t)d * sizeof(double));
+    double *c1 = (double *)malloc((size_t)d * sizeof(double));
+    while (nc < n_clusters) {
+        int64_t t = -1;
+        for (int64_t i = 0; i < nc; ++i) {
+            if (cl[i].n < 2) continue;
+            if (t < 0) {
+                t = i;
+            } else if (strategy == 0) {
+                if (!(cl[t].n > cl[i].n)) t = i;
+            } else {
+                if (!(cl[t].sse > cl[i].sse)) t = i;
+            }
+        }
+        if (t < 0) break;
+        const int64_t tn = cl[t].n;
+        double best = 1.7976931348623157e308;
+        int have = 0;
+        double *b0 = (double *)malloc((size_t)d * sizeof(double));
+        double *b1 = (double *)malloc((size_t)d * sizeof(double));
+        for (int64_t run = 0; run < n_init; ++run) {
+            or_stdrng g;
+            or_stdrng_seed(&g, seed + split_count * 1000u + (uint64_t)run);
+            const double s = or_bk_2means(X, d, cl[t].idx, tn, max_iter, &g, lab, c0, c1);
+            if (s < best) {
+                best = s;
+                have = 1;
+                memcpy(blab, lab, (size_t)tn);
+                memcpy(b0, c0, (size_t)d * sizeof(double));
+                memcpy(b1, c1, (size_t)d * sizeof(double));
+            }
+        }
+        ++split_count;
+        if (!have) {
+            free(b0);
+            free(b1);
+            ret = -1;
+            break;
+        }
+        int64_t n0 = 0;
+        for (int64_t li = 0; li < tn; ++li) n0 += blab[li] == 0;
+        const int64_t n1 = tn - n0;
+        or_bk_cluster k0 = {NULL, n0, b0, 0.0}, k1 = {NULL, n1, b1, 0.0};
+        k0.idx = (int64_t *)malloc((size_t)(n0 > 0 ? n0 : 1) * sizeof(int64_t));
+        k1.idx = (int64_t *)malloc((size_t)(n1 > 0 ? n1 : 1) * sizeof(int64_t));
+        int64_t p0 = 0, p1 = 0;
+        for (int64_t li = 0; li < tn; ++li) {
+            if (blab[li] == 0)
+                k0.idx[p0++] = cl[t].idx[li];
+            else
+                k1.idx[p1++] = cl[t].idx[li];
+        }
+        for (int64_t i = 0; i < n0; ++i) k0.sse = k0.sse + or_bk_sqe(X + k0.idx[i] * d, b0, d);
+        for (int64_t i = 0; i < n1; ++i) k1.sse = k1.sse + or_bk_sqe(X + k1.idx[i] * d, b1, d);
+        free(cl[t].idx);
+        free(cl[t].center);
+        for (int64_t i = t; i + 1 < nc; ++i) cl[i] = cl[i + 1];
+        --nc;
+        if (n0 > 0) {
+            cl[nc++] = k0;
+        } else {
+            free(k0.idx);
+            free(k0.center);
+        }
+        if (n1 > 0) {
+            cl[nc++] = k1;
+        } else {
+            free(k1.idx);
+            free(k1.center);
+        }
+        if (n0 == 0 || n1 == 0) {
+            if (++stall >= max_stall) {
+                ret = -2;
+                break;
+            }
+        } else {
+            stall = 0;
+        }
+    }
+    if (ret == 0) {
+        double inertia = 0.0;
+        for (int64_t c = 0; c < nc; ++c) {
+            memcpy(centers_out + c * d, cl[c].center, (size_t)d * sizeof(double));
+            for (int64_t i = 0; i < cl[c].n; ++i) labels_out[cl[c].idx[i]] = c;
+            inertia = inertia + cl[c].sse;
+        }
+        *inertia_out = inertia;
+        ret = nc;
+    }
+    for (int64_t c = 0; c < nc; ++c) {
+        free(cl[c].idx);
+        free(cl[c].center);
+    }
+    free(cl);
+    free(lab);
+    free(blab);
+    free(c0);
+    free(c1);
+    return ret;
+}

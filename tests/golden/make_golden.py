@@ -439,6 +439,111 @@ def kmeans_fit_stdrng(X, k, max_iter, tol, n_init, base):
     return best
 
 
+def bk_sqe(a, b):
+    """bisecting_kmeans.rs:174-178."""
+    s = 0.0
+    for x, y in zip(a, b):
+        s = s + (x - y) * (x - y)
+    return s
+
+
+def bk_2means(X, idx, max_iter, g):
+    """bisecting_kmeans.rs:225-322."""
+    n = len(idx)
+    i0 = g.range(n)
+    i1 = g.range(n)
+    if n > 1:
+        while i1 == i0:
+            i1 = g.range(n)
+    c0, c1 = list(X[idx[i0]]), list(X[idx[i1]])
+    lab = [0] * n
+    for _ in range(max_iter):
+        changed = False
+        for li, r in enumerate(idx):
+            nl = 0 if bk_sqe(X[r], c0) <= bk_sqe(X[r], c1) else 1
+            if nl != lab[li]:
+                lab[li] = nl
+                changed = True
+        if not changed:
+            break
+        d = len(c0)
+        s0, s1, n0, n1 = [0.0] * d, [0.0] * d, 0.0, 0.0
+        for li, r in enumerate(idx):
+            if lab[li] == 0:
+                n0 = n0 + 1.0
+                s0 = [a + b for a, b in zip(s0, X[r])]
+            else:
+                n1 = n1 + 1.0
+                s1 = [a + b for a, b in zip(s1, X[r])]
+        if n0 > 0.0:
+            c0 = [v / n0 for v in s0]
+        if n1 > 0.0:
+            c1 = [v / n1 for v in s1]
+    tot = 0.0
+    for li, r in enumerate(idx):
+        tot = tot + bk_sqe(X[r], c0 if lab[li] == 0 else c1)
+    return lab, c0, c1, tot
+
+
+def bisecting_kmeans(X, n_clusters, max_iter, n_init, seed, strategy):
+    """bisecting_kmeans.rs:326-505; strategy "size" (LargestCluster) or
+    "sse" (LargestSSE), the last of equal candidates winning as Rust's
+    max_by_key / max_by do."""
+    n, d = len(X), len(X[0])
+    cen = [0.0] * d
+    for r in X:
+        cen = [a + b for a, b in zip(cen, r)]
+    cen = [v / float(n) for v in cen]
+    sse = 0.0
+    for r in X:
+        sse = sse + bk_sqe(r, cen)
+    clusters = [(list(range(n)), cen, sse)]
+    split = 0
+    while len(clusters) < n_clusters:
+        t = None
+        for i, c in enumerate(clusters):
+            if len(c[0]) < 2:
+                continue
+            if t is None:
+                t = i
+            elif strategy == "size":
+                if not len(clusters[t][0]) > len(c[0]):
+                    t = i
+            elif not clusters[t][2] > c[2]:
+                t = i
+        if t is None:
+            break
+        idx = clusters[t][0]
+        best = None
+        for run in range(n_init):
+            g = StdRng((seed + split * 1000 + run) & M64)
+            lab, c0, c1, s = bk_2means(X, idx, max_iter, g)
+            if s < (best[3] if best else 1.7976931348623157e308):
+                best = (lab, c0, c1, s)
+        split += 1
+        lab, c0, c1, _ = best
+        i0 = [r for li, r in enumerate(idx) if lab[li] == 0]
+        i1 = [r for li, r in enumerate(idx) if lab[li] == 1]
+        s0 = 0.0
+        for r in i0:
+            s0 = s0 + bk_sqe(X[r], c0)
+        s1 = 0.0
+        for r in i1:
+            s1 = s1 + bk_sqe(X[r], c1)
+        del clusters[t]
+        if i0:
+            clusters.append((i0, c0, s0))
+        if i1:
+            clusters.append((i1, c1, s1))
+    labels = [0] * n
+    inertia = 0.0
+    for ci, c in enumerate(clusters):
+        for r in c[0]:
+            labels[r] = ci
+        inertia = inertia + c[2]
+    return [c[1] for c in clusters], labels, inertia
+
+
 def main():
     rnd = random.Random(20260101)
     out = {}
@@ -637,6 +742,22 @@ def main():
                      "centers": [[hx(v) for v in r] for r in C], "labels": L,
                      "inertia": hx(inertia), "n_iter": it})
     out["minibatch_fit_stdrng"] = mbs2
+
+    rb = random.Random(20261204)
+    bks = []
+    for (n, d, blobs, n_clusters, max_iter, n_init, seed, strategy) in (
+            (120, 2, 4, 4, 300, 3, 42, "size"),
+            (150, 3, 5, 5, 300, 2, 7, "sse"),
+            (90, 2, 3, 6, 4, 1, 0, "sse"),
+            (64, 4, 2, 3, 300, 10, 2 ** 64 - 3, "size")):
+        cen = [[rb.uniform(-10, 10) for _ in range(d)] for _ in range(blobs)]
+        X = [[cen[i % blobs][j] + rb.gauss(0, 1.0) for j in range(d)] for i in range(n)]
+        C, L, inertia = bisecting_kmeans(X, n_clusters, max_iter, n_init, seed, strategy)
+        bks.append({"X": [[hx(v) for v in r] for r in X], "n_clusters": n_clusters,
+                    "max_iter": max_iter, "n_init": n_init, "seed": str(seed),
+                    "strategy": strategy, "centers": [[hx(v) for v in r] for r in C],
+                    "labels": L, "inertia": hx(inertia)})
+    out["bisecting"] = bks
 
     with open(os.path.join(HERE, "golden_vectors.json"), "w") as f:
         json.dump(out, f, indent=0, separators=(",", ":"))

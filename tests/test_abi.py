@@ -138,6 +138,7 @@ STRUCTS = {
     "fl_router_train_params": L.fl_router_train_params,
     "fl_router_report": L.fl_router_report,
     "fl_kmeans_problem": L.fl_kmeans_problem,
+    "fl_bisecting_params": L.fl_bisecting_params,
 }
 
 
@@ -214,6 +215,24 @@ def test_euclidean_distances_validation_order():
         F.euclidean_distances(np.ones((2, 2)), np.ones((0, 2)))
     with pytest.raises(F.InsufficientSamples, match="feature dimension must be at least 1"):
         F.euclidean_distances(np.ones((2, 0)), np.ones((3, 0)))
+
+
+def test_bisecting_defaults_and_validation_order():
+    # bisecting_kmeans.rs:93-102 defaults; :326-366 checks in order
+    p = L.fl_bisecting_params()
+    L.lib().fl_bisecting_params_default(C.byref(p), 5)
+    assert (p.n_clusters, p.max_iter, p.n_init, p.has_random_state, p.strategy) == \
+        (5, 300, 10, 0, 0)
+    x = np.ones((3, 2))
+    with pytest.raises(F.InvalidParameter, match="n_clusters"):
+        F.BisectingKMeans(0).with_n_init(0).fit(x)
+    with pytest.raises(F.InvalidParameter, match="n_init"):
+        F.BisectingKMeans(4).with_n_init(0).fit(x)
+    with pytest.raises(F.InsufficientSamples,
+                       match="required 4, actual 3: BisectingKMeans requires at least"):
+        F.BisectingKMeans(4).fit(x)
+    with pytest.raises(F.InsufficientSamples, match="required 2, actual 0"):
+        F.BisectingKMeans(2).fit(np.ones((0, 2)))
 
 
 def test_spectral_validation():

@@ -685,3 +685,37 @@ def test_euclidean_distances_nan_is_clamped_like_f64_max():
     Y = np.array([[0.0, 0.0]])
     d = O.euclidean_distances(X, Y)
     assert d[0, 0] == 0.0 and d[1, 0] == math.sqrt(5.0)
+
+
+def test_bisecting_golden_bit_exact():
+    # pure-Python restatement of bisecting_kmeans.rs:174-505 with its own
+    # StdRng (tests/golden/make_golden.py)
+    for run in golden()["bisecting"]:
+        X = arr(run["X"])
+        C, L, inertia = O.bisecting_kmeans(X, run["n_clusters"], run["max_iter"], run["n_init"],
+                                           int(run["seed"]), run["strategy"])
+        assert list(L) == run["labels"]
+        assert (bits(C) == bits(arr(run["centers"]))).all()
+        assert inertia.hex() == run["inertia"]
+
+
+def test_bisecting_reference_unit_tests():
+    # bisecting_kmeans.rs:612-700: three tight blobs split into the blobs,
+    # two clusters, one cluster
+    x = np.array([[0.0, 0.0], [0.1, 0.1], [-0.1, 0.1], [10.0, 10.0], [10.1, 10.1], [9.9, 10.1],
+                  [0.0, 10.0], [0.1, 10.1], [-0.1, 9.9]])
+    _, L, _ = O.bisecting_kmeans(x, 3, seed=42)
+    assert L[0] == L[1] == L[2] and L[3] == L[4] == L[5] and L[6] == L[7] == L[8]
+    assert len({L[0], L[3], L[6]}) == 3
+    x2 = np.array([[0.0, 0.0], [0.5, 0.0], [0.0, 0.5], [0.5, 0.5], [10.0, 10.0], [10.5, 10.0],
+                   [10.0, 10.5], [10.5, 10.5]])
+    _, L2, _ = O.bisecting_kmeans(x2, 2, seed=42)
+    assert len(set(L2[:4])) == 1 and len(set(L2[4:])) == 1 and L2[0] != L2[4]
+    C1, L1, i1 = O.bisecting_kmeans(np.array([[1.0, 1.0], [2.0, 2.0], [3.0, 3.0], [4.0, 4.0]]), 1,
+                                    seed=42)
+    assert (L1 == 0).all() and C1.shape == (1, 2) and C1[0, 0] == 2.5 and i1 == 10.0
+
+
+def test_bisecting_identical_rows_do_not_loop():
+    with pytest.raises(RuntimeError, match="empty"):
+        O.bisecting_kmeans(np.ones((10, 2)), 2, max_stall=50)
