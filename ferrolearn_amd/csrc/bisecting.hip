@@ -112,9 +112,7 @@ class Bisect {
     // Left fold of v[0..n) (device) into the host.
     double fold1(const double *v, int64_t n) {
         double *out = ws_.get<double>("fold_out", 1);
-        hipLaunchKernelGGL(exact_fold_kernel, dim3(1), dim3(EF_NT), 0, st_, v, n, n,
-                           (const int32_t *)nullptr, out, (double *)nullptr);
-        check_launch();
+        left_fold(v, n, n, 1, nullptr, out, nullptr, ws_, st_);
         double h = 0.0;
         FL_HIP(hipMemcpyAsync(&h, out, 8, hipMemcpyDeviceToHost, st_));
         FL_HIP(hipStreamSynchronize(st_));
@@ -130,8 +128,7 @@ class Bisect {
         const unsigned gy = (unsigned)std::min(d_, 64);
         hipLaunchKernelGGL(bk2_gather_kernel<TX>, dim3(grid1(m, 256), gy), dim3(256), 0, st_, X_,
                            d_, sel, dcnt, m, buf);
-        hipLaunchKernelGGL(exact_fold_kernel, dim3((unsigned)d_), dim3(EF_NT), 0, st_, buf, m, m,
-                           (const int32_t *)nullptr, sums, (double *)nullptr);
+        left_fold(buf, m, m, d_, nullptr, sums, nullptr, ws_, st_);
         hipLaunchKernelGGL(bk2_centre_kernel, dim3(grid1(d_, 256)), dim3(256), 0, st_, sums,
                            dcnt, d_, dc);
         check_launch();

@@ -46,7 +46,6 @@ namespace fl {
 constexpr int EF_NT = 512;                 // threads per fold
 constexpr int EF_NW = EF_NT / 64;          // waves per fold
 constexpr int EF_P = 16;                   // window elements per thread
-constexpr int EF_W = EF_NT * EF_P;         // window staged in LDS (double-buffered)
 constexpr int EF_SERIAL = 32;              // serial adds after a binade crossing
 
 // The reference assigns serially, folding the inertia in row order, when
@@ -58,47 +57,52 @@ constexpr int64_t KM_SERIAL_WORK = 100000;
 // LDS position of window element i: one pad double after every 16, so that
 // a thread's 16 contiguous elements load without bank conflicts.
 __device__ __forceinline__ int ef_pad(int i) { return i + (i >> 4); }
-constexpr int EF_WPAD = EF_W + EF_W / 16;
 
 // The fold of src[0..n) by the calling block of EF_NT threads (every
-// thread gets the result); when pre is given, also the running sums
-// pre[i] = s_{i+1}.  Thread t owns elements [t*16, t*16 + 16) of each
-// window and keeps them in registers through the window's phases.
-__device__ __forceinline__ double exact_fold_block(const double *__restrict__ src, int64_t n,
-                                                   double *__restrict__ pre) {
-    static_assert(EF_P == 16, "one pad per 16 elements");
+// thread gets the result), starting from s0 (+0 for a whole sequence, the
+// running sum before src[0] for a piece of one); when pre is given, also the
+// running sums pre[i] = s_{i+1}.  Thread t owns elements [t*P, t*P + P) of
+// each window of EF_NT*P and keeps them in registers through the window's
+// phases: P = 16 for long sequences, P = 4 for pieces of at most 2048
+// elements (a phase's walk is then 4 elements long).
+template <int P>
+__device__ __forceinline__ double exact_fold_block_p(const double *__restrict__ src, int64_t n,
+                                                     double *__restrict__ pre, double s0) {
+    static_assert(P == 4 || P == 16, "window layout: P elements per thread, P | 16");
+    constexpr int W = EF_NT * P;
+    constexpr int WPAD = W + W / 16;
     using ScanD = rocprim::block_scan<double, EF_NT>;
-    __shared__ double buf[2][EF_WPAD];
+    __shared__ double buf[2][WPAD];
     __shared__ typename ScanD::storage_type scan_st;
     __shared__ int sh_xc[EF_NW], sh_mc[EF_NW];
     __shared__ double m_at[EF_NT];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    const int rs = t * EF_P;   // this thread's slice of every window
+    const int rs = t * P;   // this thread's slice of every window
     // window 0 into buffer 0
 #pragma unroll
-    for (int u = 0; u < EF_P; ++u) {
+    for (int u = 0; u < P; ++u) {
         const int i = u * EF_NT + t;
         if (i < n) buf[0][ef_pad(i)] = src[i];
     }
     __syncthreads();
-    double s = 0.0;
+    double s = s0;
     bool crossed = false;
     int cur = 0;
-    for (int64_t w0 = 0; w0 < n; w0 += EF_W, cur ^= 1) {
-        const int wn = (int)(n - w0 < EF_W ? n - w0 : EF_W);
+    for (int64_t w0 = 0; w0 < n; w0 += W, cur ^= 1) {
+        const int wn = (int)(n - w0 < W ? n - w0 : W);
         // the next window's loads are in flight while this one is folded
-        double nx[EF_P];
-        const int64_t w1 = w0 + EF_W;
+        double nx[P];
+        const int64_t w1 = w0 + W;
 #pragma unroll
-        for (int u = 0; u < EF_P; ++u) {
+        for (int u = 0; u < P; ++u) {
             const int64_t i = w1 + u * EF_NT + t;
             nx[u] = i < n ? src[i] : 0.0;
         }
         const double *bw = buf[cur];
-        double x[EF_P];
+        double x[P];
 #pragma unroll
-        for (int u = 0; u < EF_P; ++u) x[u] = bw[ef_pad(rs + u)];   // past wn: unused
-        const int re = rs + EF_P < wn ? rs + EF_P : wn;
+        for (int u = 0; u < P; ++u) x[u] = bw[ef_pad(rs + u)];   // past wn: unused
+        const int re = rs + P < wn ? rs + P : wn;
         double *pw = pre ? pre + w0 : nullptr;
         int j = 0;
         while (j < wn) {
@@ -107,7 +111,7 @@ __device__ __forceinline__ double exact_fold_block(const double *__restrict__ sr
                 int pz = s != s ? j : INT_MAX;
                 if (s == s)
 #pragma unroll
-                    for (int u = 0; u < EF_P; ++u) {
+                    for (int u = 0; u < P; ++u) {
                         const int i = rs + u;
                         if (pz == INT_MAX && i >= j && i < re && (x[u] != x[u] || x[u] == -s))
                             pz = i;
@@ -125,7 +129,7 @@ __device__ __forceinline__ double exact_fold_block(const double *__restrict__ sr
                 const double sn = s != s ? s : (pz < wn ? s + bw[ef_pad(pz)] : s);
                 if (pw)
 #pragma unroll
-                    for (int u = 0; u < EF_P; ++u) {
+                    for (int u = 0; u < P; ++u) {
                         const int i = rs + u;
                         if (i >= j && i < re) pw[i] = i < pz ? s : sn;
                     }
@@ -172,12 +176,12 @@ __device__ __forceinline__ double exact_fold_block(const double *__restrict__ sr
             // pass 1: a = floor(x / u) and [f > 1/2]; the thread's first
             // element the phase cannot take (f == 1/2, NaN, infinite or too
             // large); the increments before it
-            double a[EF_P];
+            double a[P];
             int upm = 0, live = 0;
             bool open = true;
             double qs = 0.0;
 #pragma unroll
-            for (int u = 0; u < EF_P; ++u) {
+            for (int u = 0; u < P; ++u) {
                 const int i = rs + u;
                 const double y = ldexp(x[u], -le);
                 const double fa = floor(y);
@@ -198,10 +202,10 @@ __device__ __forceinline__ double exact_fold_block(const double *__restrict__ sr
             // pass 2: running sums (in registers) up to the first element the
             // phase cannot take or whose exact sum would leave the binade
             int mc = INT_MAX;
-            double pv[EF_P];
+            double pv[P];
             bool go = true;
 #pragma unroll
-            for (int u = 0; u < EF_P; ++u) {
+            for (int u = 0; u < P; ++u) {
                 const int i = rs + u;
                 const bool in = i >= j && i < re;
                 const double tt = m + a[u];
@@ -228,12 +232,12 @@ __device__ __forceinline__ double exact_fold_block(const double *__restrict__ sr
                 if (end == INT_MAX) end = sh_mc[i];
             if (pw) {
 #pragma unroll
-                for (int u = 0; u < EF_P; ++u) {
+                for (int u = 0; u < P; ++u) {
                     const int i = rs + u;
                     if (i >= j && i < re && i < end) pw[i] = pv[u];
                 }
             }
-            const int owner = (end < wn ? end : wn - 1) / EF_P;
+            const int owner = (end < wn ? end : wn - 1) / P;
             const double se = ldexp(m_at[owner], le);
             if (end >= wn) {
                 s = se;
@@ -250,11 +254,16 @@ __device__ __forceinline__ double exact_fold_block(const double *__restrict__ sr
         if (w1 < n) {
             double *bn = buf[cur ^ 1];
 #pragma unroll
-            for (int u = 0; u < EF_P; ++u) bn[ef_pad(u * EF_NT + t)] = nx[u];
+            for (int u = 0; u < P; ++u) bn[ef_pad(u * EF_NT + t)] = nx[u];
         }
         __syncthreads();
     }
     return s;
+}
+
+__device__ __forceinline__ double exact_fold_block(const double *__restrict__ src, int64_t n,
+                                                   double *__restrict__ pre, double s0 = 0.0) {
+    return exact_fold_block_p<EF_P>(src, n, pre, s0);
 }
 
 // grid.x = number of folds: fold b reads v[b*ld .. b*ld + n), writes its

@@ -8,6 +8,8 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -16,6 +18,7 @@
 #include "dist_fp8.hpp"
 #include "dist_bf16_res.hpp"
 #include "rand09.hpp"
+#include "split_fold.hpp"
 #include "dist_f64.hpp"
 #include "dist_topm.hpp"
 
@@ -38,6 +41,7 @@ class DevBuf {
         return p_;
     }
     void *ptr() const { return p_; }
+    size_t cap() const { return cap_; }
     void release() {
         if (p_) (void)hipFree(p_);
         p_ = nullptr;
@@ -49,14 +53,106 @@ class DevBuf {
     size_t cap_ = 0;
 };
 
-// Named scratch buffers that only grow; reused across calls on one object.
+// Streams and named-buffer sets are kept per device after a call and handed
+// to the next one: a new stream costs about a millisecond and each hipMalloc
+// tens of microseconds, against a few milliseconds for a whole small fit.
+// Only sets of at most kPoolKeepBytes are kept (larger ones are freed, so a
+// large fit gives its memory back), and at most kPoolMax of each per device.
+// The pool lives until the process ends.
+using BufSet = std::map<std::string, DevBuf>;
+
+class DevicePool {
+  public:
+    static constexpr size_t kPoolKeepBytes = (size_t)64 << 20;
+    static constexpr size_t kPoolMax = 16;
+
+    static DevicePool &get() {
+        static DevicePool *p = new DevicePool;   // never destroyed: no hipFree at exit
+        return *p;
+    }
+    hipStream_t take_stream(int dev) {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            auto &v = streams_[dev];
+            if (!v.empty()) {
+                hipStream_t s = v.back();
+                v.pop_back();
+                return s;
+            }
+        }
+        hipStream_t s = nullptr;
+        FL_HIP(hipStreamCreate(&s));
+        return s;
+    }
+    void give_stream(int dev, hipStream_t s) {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            auto &v = streams_[dev];
+            if (v.size() < kPoolMax) {
+                v.push_back(s);
+                return;
+            }
+        }
+        (void)hipStreamDestroy(s);
+    }
+    BufSet *take_bufs(int dev) {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            auto &v = bufs_[dev];
+            if (!v.empty()) {
+                BufSet *b = v.back();
+                v.pop_back();
+                return b;
+            }
+        }
+        return new BufSet;
+    }
+    // The caller's work on these buffers may still be in flight on a stream
+    // this object does not know: the device is synchronised before the set
+    // goes back (hipFree would have synchronised too).
+    void give_bufs(int dev, BufSet *b) {
+        size_t bytes = 0;
+        for (auto &kv : *b) bytes += kv.second.cap();
+        if (bytes <= kPoolKeepBytes && hipDeviceSynchronize() == hipSuccess) {
+            std::lock_guard<std::mutex> g(mu_);
+            auto &v = bufs_[dev];
+            if (v.size() < kPoolMax) {
+                v.push_back(b);
+                return;
+            }
+        }
+        delete b;
+    }
+
+  private:
+    std::mutex mu_;
+    std::map<int, std::vector<hipStream_t>> streams_;
+    std::map<int, std::vector<BufSet *>> bufs_;
+};
+
+inline int current_device() {
+    int dev = 0;
+    FL_HIP(hipGetDevice(&dev));
+    return dev;
+}
+
+// Named scratch buffers that only grow; reused across calls on one object,
+// and the set itself taken from and returned to the device's pool.
 // Arena form (constructed over caller memory): get() hands out 256-byte
 // aligned slices of that memory by name and never allocates; a slot keeps
 // its first size, and running past the end throws instead of allocating.
 class Workspace {
   public:
-    Workspace() = default;
-    Workspace(void *base, size_t bytes) : arena_((unsigned char *)base), arena_cap_(bytes) {}
+    Workspace() : dev_(current_device()), bufs_(DevicePool::get().take_bufs(dev_)) {}
+    Workspace(void *base, size_t bytes) : arena_((unsigned char *)base), arena_cap_(bytes) {
+        if (!arena_) {
+            dev_ = current_device();
+            bufs_ = DevicePool::get().take_bufs(dev_);
+        }
+    }
+    ~Workspace() {
+        if (bufs_) DevicePool::get().give_bufs(dev_, bufs_);
+    }
     Workspace(const Workspace &) = delete;
     Workspace &operator=(const Workspace &) = delete;
 
@@ -64,9 +160,11 @@ class Workspace {
     T *get(const std::string &name, int64_t count) {
         const size_t bytes = (size_t)(count > 0 ? count : 1) * sizeof(T);
         if (arena_) return static_cast<T *>(arena_get(name.c_str(), bytes));
-        return static_cast<T *>(bufs_[name].get(bytes));
+        return static_cast<T *>((*bufs_)[name].get(bytes));
     }
-    void clear() { bufs_.clear(); }
+    void clear() {
+        if (bufs_) bufs_->clear();
+    }
     size_t arena_used() const { return arena_used_; }
 
   private:
@@ -93,7 +191,8 @@ class Workspace {
         arena_used_ = off + bytes;
         return arena_ + off;
     }
-    std::map<std::string, DevBuf> bufs_;
+    int dev_ = 0;
+    BufSet *bufs_ = nullptr;
     unsigned char *arena_ = nullptr;
     size_t arena_cap_ = 0, arena_used_ = 0;
     Slot slots_[kSlots];
@@ -142,13 +241,17 @@ class PhaseTimer {
     std::vector<int> free_;
 };
 
-// A stream of its own, destroyed with the object.
+// A stream for one call: taken from the device's pool and returned to it
+// (work still queued on it stays ordered before the next user's).
 struct StreamGuard {
     hipStream_t s = nullptr;
-    StreamGuard() { FL_HIP(hipStreamCreate(&s)); }
+    int dev = 0;
+    StreamGuard() : dev(current_device()) { s = DevicePool::get().take_stream(dev); }
     ~StreamGuard() {
-        if (s) (void)hipStreamDestroy(s);
+        if (s) DevicePool::get().give_stream(dev, s);
     }
+    StreamGuard(const StreamGuard &) = delete;
+    StreamGuard &operator=(const StreamGuard &) = delete;
 };
 
 // Zeroed page-locked host array (asynchronous device-to-host copies).
@@ -528,5 +631,49 @@ fl_status guarded(F &&f) {
 }
 
 size_t dtype_size(int32_t dtype);
+
+// Sequences at least this long are folded over many compute units
+// (split_fold.hpp); shorter ones by one workgroup each.
+constexpr int64_t kSplitFoldMin = 4 * SF_L;
+
+// The left fold of count sequences v[q*ld .. q*ld + n) into sums[q] and,
+// when prefix is given, the running sums prefix[q*ld + i] = s_{i+1}: bit for
+// bit the serial loop, in stream order on st.  A non-zero *stop (when given)
+// makes the whole fold a no-op.
+inline void left_fold(const double *v, int64_t n, int64_t ld, int64_t count,
+                      const int32_t *stop, double *sums, double *prefix, Workspace &ws,
+                      hipStream_t st) {
+    if (count <= 0) return;
+    if (n < kSplitFoldMin) {
+        hipLaunchKernelGGL(exact_fold_kernel, dim3((unsigned)count), dim3(EF_NT), 0, st, v, n,
+                           ld, stop, sums, prefix);
+        check_launch();
+        return;
+    }
+    const int64_t nb = ceil_div(n, (int64_t)SF_L);
+    const int64_t cc = count < 65535 ? count : 65535;   // sequences per pass (grid.y)
+    double *part = ws.get<double>("sf_part", nb * cc);
+    int32_t *gle = ws.get<int32_t>("sf_gle", nb * cc);
+    SfBlock *blk = ws.get<SfBlock>("sf_blk", nb * cc);
+    SfStart *starts = ws.get<SfStart>("sf_start", nb * cc);
+    for (int64_t q0 = 0; q0 < count; q0 += cc) {
+        const int64_t c = count - q0 < cc ? count - q0 : cc;
+        const double *vq = v + q0 * ld;
+        double *sq = sums ? sums + q0 : nullptr;
+        double *pq = prefix ? prefix + q0 * ld : nullptr;
+        const dim3 g2((unsigned)nb, (unsigned)c);
+        hipLaunchKernelGGL(sf_part_kernel, g2, dim3(SF_NT), 0, st, vq, n, ld, nb, stop, part);
+        hipLaunchKernelGGL(sf_guess_kernel, dim3((unsigned)c), dim3(SF_NT), 0, st, part, nb,
+                           stop, gle);
+        hipLaunchKernelGGL(sf_spec_kernel, g2, dim3(SF_NT), 0, st, vq, n, ld, nb, stop, gle, blk,
+                           pq);
+        hipLaunchKernelGGL(sf_combine_kernel, dim3((unsigned)c), dim3(EF_NT), 0, st, vq, n, ld,
+                           nb, stop, blk, starts, sq, pq);
+        if (pq)
+            hipLaunchKernelGGL(sf_prefix_kernel, g2, dim3(SF_NT), 0, st, n, ld, nb, stop, starts,
+                               pq);
+        check_launch();
+    }
+}
 
 }  // namespace fl

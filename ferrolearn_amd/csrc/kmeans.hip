@@ -399,9 +399,7 @@ class Lloyd {
     // fixed order (the reference's rayon order is unspecified there).
     void inertia(const double *best, double *out) {
         if (n_ * d_ < KM_SERIAL_WORK) {
-            hipLaunchKernelGGL(exact_fold_kernel, dim3(1), dim3(EF_NT), 0, st_, best, n_, n_,
-                               (const int32_t *)nullptr, out, (double *)nullptr);
-            check_launch();
+            left_fold(best, n_, n_, 1, nullptr, out, nullptr, ws_, st_);
             return;
         }
         const int64_t nb = ceil_div(n_, SUM_BLK);
@@ -584,9 +582,7 @@ class Lloyd {
         FL_HIP(hipMemsetAsync(ctl, 0, sizeof(KppSeqCtl), st_));
         hipLaunchKernelGGL(kpp_first_kernel<TX>, dim3(grid1(n_, 256)), dim3(256), 0, st_, X_, n_,
                            d_, out, md);
-        hipLaunchKernelGGL(exact_fold_kernel, dim3(1), dim3(EF_NT), 0, st_, md, n_, n_,
-                           (const int32_t *)nullptr, &ctl->total, pre);
-        check_launch();
+        left_fold(md, n_, n_, 1, nullptr, &ctl->total, pre, ws_, st_);
         const dim3 gn((unsigned)grid1(n_, 256), (unsigned)nt);
         for (int64_t c = 1; c < k_; ++c) {
             hipLaunchKernelGGL(kpp_seq_ctl_kernel, dim3(1), dim3(64), 0, st_, ctl,
@@ -595,8 +591,7 @@ class Lloyd {
                                thr, cand);
             hipLaunchKernelGGL(kpp_seq_trials_kernel<TX>, dim3(grid1(n_, 256)), dim3(256), 0, st_,
                                ctl, X_, n_, d_, nt, cand, md, nd);
-            hipLaunchKernelGGL(exact_fold_kernel, dim3((unsigned)nt), dim3(EF_NT), 0, st_, nd,
-                               n_, n_, &ctl->stop, pot, pre_t);
+            left_fold(nd, n_, n_, nt, &ctl->stop, pot, pre_t, ws_, st_);
             hipLaunchKernelGGL(kpp_seq_select_kernel<TX>, dim3(1), dim3(64), 0, st_, ctl, pot, nt,
                                cand, X_, d_, out + c * d_);
             hipLaunchKernelGGL(kpp_seq_adopt_kernel, dim3(grid1(n_, 256)), dim3(256), 0, st_,
@@ -1037,9 +1032,7 @@ fl_status mb_fit_device(const TX *X, int64_t n, int d, const fl_minibatch_params
             if (shift < p->tol) break;
         }
         L.assign(cur, labels, best, acc);
-        hipLaunchKernelGGL(exact_fold_kernel, dim3(1), dim3(EF_NT), 0, st, best, n, n,
-                           (const int32_t *)nullptr, dsc + 1, (double *)nullptr);
-        check_launch();
+        left_fold(best, n, n, 1, nullptr, dsc + 1, nullptr, ws, st);
         double inertia = 0.0;
         FL_HIP(hipMemcpyAsync(&inertia, dsc + 1, 8, hipMemcpyDeviceToHost, st));
         L.flush(&acc.n_fb);
@@ -1409,9 +1402,8 @@ fl_status fl_left_fold_device(const double *d_v, int64_t n, int64_t count, int64
         if (!d_v || !d_sums)
             return fail(FL_INVALID_PARAMETER, "InvalidParameter: left fold: null pointer");
         StreamGuard sg;
-        hipLaunchKernelGGL(exact_fold_kernel, dim3((unsigned)count), dim3(EF_NT), 0, sg.s, d_v, n,
-                           count > 1 ? ld : n, (const int32_t *)nullptr, d_sums, d_prefix);
-        check_launch();
+        Workspace ws;
+        left_fold(d_v, n, count > 1 ? ld : n, count, nullptr, d_sums, d_prefix, ws, sg.s);
         FL_HIP(hipStreamSynchronize(sg.s));
         return FL_OK;
     });

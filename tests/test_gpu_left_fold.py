@@ -102,3 +102,82 @@ def test_left_fold_long_sequence_and_count():
         want = serial(seqs[b])
         assert same_bits(pre[b], want) and same_bits(sums[b], want[-1]), b
     assert math.isfinite(float(sums[0]))
+
+
+def device_fold_ld(seqs, ld):
+    """Sequences placed ld elements apart (ld >= n) in one device array."""
+    count, n = seqs.shape
+    buf = np.full((count, ld), 12345.0)
+    buf[:, :n] = seqs
+    dv = F.to_device(buf)
+    sums = F.device.empty((count,), np.float64)
+    pre = F.device.empty((count, ld), np.float64)
+    check(L.lib().fl_left_fold_device(C.c_void_p(dv.data_ptr()), n, count, ld,
+                                      C.c_void_p(sums.data_ptr()), C.c_void_p(pre.data_ptr())))
+    return sums.numpy(), pre.numpy()[:, :n]
+
+
+@pytest.mark.parametrize("n", [8191, 8192, 8193, 2048 * 9 + 1, 200_000])
+def test_left_fold_split_sizes(n):
+    # sequences of 8192 elements and more are cut into 2048-element blocks
+    # done in parallel; every case of the small sizes again at these sizes
+    rng = np.random.default_rng(n + 11)
+    cs = cases(n, rng)
+    names = list(cs)
+    seqs = np.stack([cs[k] for k in names])
+    sums, pre = device_fold(seqs)
+    for b, name in enumerate(names):
+        want = serial(seqs[b])
+        assert same_bits(pre[b], want), (name, n)
+        assert same_bits(sums[b], want[-1]), (name, n)
+
+
+def test_left_fold_split_block_edge_cases():
+    n = 2048 * 40 + 123
+    rng = np.random.default_rng(99)
+    seqs = []
+    # an exact tie deep inside a middle block (the block is folded serially)
+    a = np.ones(n)
+    a[0] = 2.0 ** 53
+    a[2048 * 17 + 900] = 1.0
+    seqs.append(a)
+    # NaN in a middle block, then infinities
+    b = rng.random(n) * 10.0
+    b[2048 * 5 + 3] = np.nan
+    b[2048 * 30] = np.inf
+    seqs.append(b)
+    # negative running sums throughout
+    seqs.append(-(rng.random(n) * 50.0) ** 2)
+    # the running sum lands on a power of two exactly at a block boundary
+    c = np.full(n, 0.25)
+    c[:2048 * 8] = 2.0 ** 11 / (2048 * 8) * 4.0
+    seqs.append(c)
+    # sums that cancel to tiny values, so the plain block sums guess a wrong
+    # binade for the blocks after the cancellation
+    d = rng.uniform(1e6, 2e6, n)
+    d[1::2] = -d[0::2][: n // 2]
+    d[-1] = 1e-300
+    seqs.append(d)
+    # mixed magnitudes spanning the grid limit of a block (|x/u| near 2^51)
+    e = rng.random(n)
+    e[2048 * 3 + 7] = 2.0 ** 40
+    e[2048 * 20 + 1] = 2.0 ** 45
+    seqs.append(e)
+    seqs = np.stack(seqs)
+    sums, pre = device_fold(seqs)
+    for i in range(seqs.shape[0]):
+        want = serial(seqs[i])
+        assert same_bits(pre[i], want), i
+        assert same_bits(sums[i], want[-1]), i
+
+
+def test_left_fold_split_strided():
+    # count > 1 with ld > n: only each sequence's n elements are read and
+    # only its n running sums written
+    n, ld = 30_000, 30_037
+    rng = np.random.default_rng(5)
+    seqs = rng.gamma(2.0, 3.0, (3, n)) ** 2
+    sums, pre = device_fold_ld(seqs, ld)
+    for i in range(3):
+        want = serial(seqs[i])
+        assert same_bits(pre[i], want) and same_bits(sums[i], want[-1]), i
