@@ -652,26 +652,27 @@ inline void left_fold(const double *v, int64_t n, int64_t ld, int64_t count,
     }
     const int64_t nb = ceil_div(n, (int64_t)SF_L);
     const int64_t cc = count < 65535 ? count : 65535;   // sequences per pass (grid.y)
-    double *part = ws.get<double>("sf_part", nb * cc);
-    int32_t *gle = ws.get<int32_t>("sf_gle", nb * cc);
-    SfBlock *blk = ws.get<SfBlock>("sf_blk", nb * cc);
-    SfStart *starts = ws.get<SfStart>("sf_start", nb * cc);
+    double *bsum = ws.get<double>("sf_bsum", nb * cc);
+    double *apx = ws.get<double>("sf_apx", nb * cc);
+    SfPart *parts = ws.get<SfPart>("sf_parts", nb * cc * SF_NP);
+    int32_t *npart = ws.get<int32_t>("sf_npart", nb * cc);
+    SfStart *starts = ws.get<SfStart>("sf_starts", nb * cc * SF_NP);
     for (int64_t q0 = 0; q0 < count; q0 += cc) {
         const int64_t c = count - q0 < cc ? count - q0 : cc;
         const double *vq = v + q0 * ld;
         double *sq = sums ? sums + q0 : nullptr;
         double *pq = prefix ? prefix + q0 * ld : nullptr;
         const dim3 g2((unsigned)nb, (unsigned)c);
-        hipLaunchKernelGGL(sf_part_kernel, g2, dim3(SF_NT), 0, st, vq, n, ld, nb, stop, part);
-        hipLaunchKernelGGL(sf_guess_kernel, dim3((unsigned)c), dim3(SF_NT), 0, st, part, nb,
-                           stop, gle);
-        hipLaunchKernelGGL(sf_spec_kernel, g2, dim3(SF_NT), 0, st, vq, n, ld, nb, stop, gle, blk,
-                           pq);
+        hipLaunchKernelGGL(sf_part_kernel, g2, dim3(SF_NT), 0, st, vq, n, ld, nb, stop, bsum);
+        hipLaunchKernelGGL(sf_guess_kernel, dim3((unsigned)c), dim3(SF_NT), 0, st, bsum, nb,
+                           stop, apx);
+        hipLaunchKernelGGL(sf_spec_kernel, g2, dim3(SF_NT), 0, st, vq, n, ld, nb, stop, apx, parts,
+                           npart, pq);
         hipLaunchKernelGGL(sf_combine_kernel, dim3((unsigned)c), dim3(EF_NT), 0, st, vq, n, ld,
-                           nb, stop, blk, starts, sq, pq);
+                           nb, stop, parts, npart, starts, sq, pq);
         if (pq)
-            hipLaunchKernelGGL(sf_prefix_kernel, g2, dim3(SF_NT), 0, st, n, ld, nb, stop, starts,
-                               pq);
+            hipLaunchKernelGGL(sf_prefix_kernel, g2, dim3(SF_NT), 0, st, n, ld, nb, stop, parts,
+                               npart, starts, pq);
         check_launch();
     }
 }
