@@ -15,6 +15,8 @@
 #include <type_traits>
 #include <vector>
 
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include "host.hpp"
 #include "kmeans_kernels.hpp"
 
@@ -30,7 +32,11 @@ static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
 namespace {
 
-constexpr int64_t kMaxClusters = 8192;
+// Largest cluster count (labels are int32; the filters' tiles index
+// centres below 2^31 - 64).
+constexpr int64_t kMaxClusters = ((int64_t)1 << 31) - 65;
+// Up to this many clusters the centroid update counts members in LDS.
+constexpr int64_t kLdsClusters = 8192;
 constexpr int kKmM = 4;   // candidates kept per row by the filter
 
 struct KmAcc {
@@ -410,27 +416,47 @@ class Lloyd {
     }
 
     // Reference-order centroid update; shift (device) = max centre movement.
+    // The member order (rows of each cluster in ascending index) comes from
+    // per-block counts in LDS up to kLdsClusters clusters, and from a stable
+    // radix sort of the row indices by label above that.
     void update(const int32_t *labels, const double *oldc, double *newc, double *shift,
                 KmAcc &acc) {
         const int t = tbegin();
-        const int64_t nblk = ceil_div(n_, UPD_RB);
-        int32_t *blockcnt = ws_.get<int32_t>("blockcnt", nblk * k_);
-        int32_t *blockbase = ws_.get<int32_t>("blockbase", nblk * k_);
         int32_t *count = ws_.get<int32_t>("count", k_);
         int64_t *offset = ws_.get<int64_t>("offset", k_);
         int32_t *order = ws_.get<int32_t>("order", n_);
-        const size_t hist_b = (size_t)k_ * 4, ctr_b = (size_t)k_ * 8;
-        FL_HIP(hipFuncSetAttribute((const void *)km_block_hist_kernel,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)hist_b));
-        FL_HIP(hipFuncSetAttribute((const void *)km_scatter_kernel,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)ctr_b));
-        hipLaunchKernelGGL(km_block_hist_kernel, dim3((unsigned)nblk), dim3(256), hist_b, st_,
-                           labels, n_, k_, blockcnt);
-        hipLaunchKernelGGL(km_col_scan_kernel, dim3((unsigned)k_), dim3(256), 0, st_, blockcnt,
-                           nblk, k_, blockbase, count);
-        hipLaunchKernelGGL(km_offsets_kernel, dim3(1), dim3(256), 0, st_, count, k_, offset);
-        hipLaunchKernelGGL(km_scatter_kernel, dim3((unsigned)nblk), dim3(64), ctr_b, st_, labels,
-                           n_, k_, label_bits(k_), blockbase, offset, order);
+        if (k_ > kLdsClusters) {
+            int32_t *ids = ws_.get<int32_t>("row_ids", n_);
+            uint32_t *keys = ws_.get<uint32_t>("labels_sorted", n_);
+            hipLaunchKernelGGL(km_fill_i32_kernel, dim3(grid1(k_, 256)), dim3(256), 0, st_, count,
+                               k_, 0);
+            hipLaunchKernelGGL(km_count_ids_kernel, dim3(grid1(n_, 256)), dim3(256), 0, st_,
+                               labels, n_, count, ids);
+            hipLaunchKernelGGL(km_offsets_kernel, dim3(1), dim3(256), 0, st_, count, k_, offset);
+            size_t tb = 0;
+            const uint32_t *lk = reinterpret_cast<const uint32_t *>(labels);
+            FL_HIP(rocprim::radix_sort_pairs(nullptr, tb, lk, keys, ids, order, (size_t)n_, 0,
+                                             (unsigned)label_bits(k_), st_));
+            void *tmp = ws_.get<unsigned char>("sort_tmp", (int64_t)tb);
+            FL_HIP(rocprim::radix_sort_pairs(tmp, tb, lk, keys, ids, order, (size_t)n_, 0,
+                                             (unsigned)label_bits(k_), st_));
+        } else {
+            const int64_t nblk = ceil_div(n_, UPD_RB);
+            int32_t *blockcnt = ws_.get<int32_t>("blockcnt", nblk * k_);
+            int32_t *blockbase = ws_.get<int32_t>("blockbase", nblk * k_);
+            const size_t hist_b = (size_t)k_ * 4, ctr_b = (size_t)k_ * 8;
+            FL_HIP(hipFuncSetAttribute((const void *)km_block_hist_kernel,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)hist_b));
+            FL_HIP(hipFuncSetAttribute((const void *)km_scatter_kernel,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)ctr_b));
+            hipLaunchKernelGGL(km_block_hist_kernel, dim3((unsigned)nblk), dim3(256), hist_b, st_,
+                               labels, n_, k_, blockcnt);
+            hipLaunchKernelGGL(km_col_scan_kernel, dim3((unsigned)k_), dim3(256), 0, st_,
+                               blockcnt, nblk, k_, blockbase, count);
+            hipLaunchKernelGGL(km_offsets_kernel, dim3(1), dim3(256), 0, st_, count, k_, offset);
+            hipLaunchKernelGGL(km_scatter_kernel, dim3((unsigned)nblk), dim3(64), ctr_b, st_,
+                               labels, n_, k_, label_bits(k_), blockbase, offset, order);
+        }
         hipLaunchKernelGGL(km_sum_kernel<TX>, dim3((unsigned)k_, (unsigned)ceil_div(d_, 64)),
                            dim3(64), 0, st_, X_, d_, order, offset, count, oldc, newc);
         unsigned long long *mx = ws_.get<unsigned long long>("shift_max", 1);
@@ -744,7 +770,7 @@ fl_status validate_fit(int64_t n, int64_t d, const fl_kmeans_params *p, int32_t 
     // Limits of this implementation (no reference counterpart).
     if (p->n_clusters < 0 || p->n_clusters > kMaxClusters)
         return fail(FL_INVALID_PARAMETER,
-                    "InvalidParameter: n_clusters: the GPU path supports 1..8192 clusters");
+                    "InvalidParameter: n_clusters: the GPU path supports 1..2^31-65 clusters");
     if (p->n_init < 0 || p->max_iter < 0)
         return fail(FL_INVALID_PARAMETER, "InvalidParameter: n_init/max_iter: negative");
     if (dtype != FL_F32 && dtype != FL_F64)
@@ -924,7 +950,7 @@ fl_status check_predict(int64_t k, int64_t d_centers, int64_t n, int64_t d, int3
         return fail(FL_INVALID_PARAMETER, "InvalidParameter: dtype: must be FL_F32 or FL_F64");
     if (k <= 0 || k > kMaxClusters)
         return fail(FL_INVALID_PARAMETER,
-                    "InvalidParameter: n_clusters: the GPU path supports 1..8192 clusters");
+                    "InvalidParameter: n_clusters: the GPU path supports 1..2^31-65 clusters");
     if (n < 0 || n > 0x7fffffffLL)
         return fail(FL_INVALID_PARAMETER, "InvalidParameter: n_samples: out of range");
     return FL_OK;
@@ -1097,7 +1123,7 @@ fl_status validate_mb(int64_t n, int64_t d, const fl_minibatch_params *p, int32_
     // Limits of this implementation (no reference counterpart).
     if (p->n_clusters < 0 || p->n_clusters > kMaxClusters)
         return fail(FL_INVALID_PARAMETER,
-                    "InvalidParameter: n_clusters: the GPU path supports 1..8192 clusters");
+                    "InvalidParameter: n_clusters: the GPU path supports 1..2^31-65 clusters");
     if (p->batch_size < 0 || p->n_init < 0 || p->max_iter < 0)
         return fail(FL_INVALID_PARAMETER,
                     "InvalidParameter: batch_size/n_init/max_iter: negative");

@@ -434,6 +434,24 @@ static __global__ __attribute__((unused)) __launch_bounds__(256) void km_col_sca
 }
 
 // Exclusive scan of the cluster counts (one 256-thread block).
+// v[0..m) = value.
+static __global__ __attribute__((unused)) void km_fill_i32_kernel(int32_t *__restrict__ v,
+                                                                   int64_t m, int32_t value) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) v[i] = value;
+}
+
+// Members per cluster by global atomics (counts are exact in any order) and
+// the row indices 0..n-1 to be sorted by label.
+static __global__ __attribute__((unused)) void km_count_ids_kernel(
+    const int32_t *__restrict__ labels, int64_t n, int32_t *__restrict__ count,
+    int32_t *__restrict__ ids) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    atomicAdd(&count[labels[i]], 1);
+    ids[i] = (int32_t)i;
+}
+
 static __global__ __attribute__((unused)) __launch_bounds__(256) void km_offsets_kernel(
     const int32_t *__restrict__ count, int64_t k, int64_t *__restrict__ offset) {
     __shared__ int64_t part[256];

@@ -280,3 +280,23 @@ def test_batched_kmeanspp_golden_vectors():
     fitted, _ = F.kmeans_fit_batched([(x, m, i) for x, m, i, _, _ in items])
     for f, (_, _, _, X, case) in zip(fitted, items):
         assert (bits(f.cluster_centers().numpy()) == bits(X[case["picks"]])).all()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "bf16x3", "exact"])
+def test_many_clusters_beyond_lds_counts(precision):
+    # more clusters than the update's LDS counters hold (8192): the member
+    # order comes from a stable radix sort by label; every empty cluster
+    # keeps its centre (kmeans.rs:387-392); no cluster limit but k <= n
+    rng = np.random.default_rng(91)
+    n, d, k = 30_000, 6, 9_000
+    X = rng.normal(size=(n, d)) * 3.0
+    init = X[rng.choice(n, k, replace=False)].copy()
+    c_o, l_o, i_o, it_o = O.kmeans_lloyd(X, init, 4, 0.0)
+    f = fit_lib(X, init, 4, 0.0, precision)
+    assert f.n_iter() == it_o
+    assert (f.labels() == l_o).all()
+    assert (bits(f.cluster_centers()) == bits(c_o)).all()
+    assert abs(f.inertia() - i_o) <= 1e-12 * i_o
+    # predict with the fitted centres against the oracle's assignment
+    lab, _ = O.assign_rows(X, f.cluster_centers())
+    assert (f.predict(X) == lab).all()
