@@ -381,7 +381,19 @@ class Lloyd {
         // exact scan of the uncertified rows; the grid strides over the
         // device-side count (zero rows: every wave exits at once)
         const int t2 = tbegin();
-        if (d_ <= 8192) {
+        if (d_ <= 512) {
+            // eight rows per block share each centre read
+            constexpr int R = 8;
+            double *Ct = ws_.get<double>("ctrans", k_ * d_);
+            hipLaunchKernelGGL(km_transpose_kernel, dim3(grid1(k_ * d_, 256)), dim3(256), 0, st_,
+                               C, k_, d_, Ct);
+            const size_t sm = (size_t)R * d_ * 8;
+            FL_HIP(hipFuncSetAttribute((const void *)km_fallback_rows_kernel<TX, R>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+            hipLaunchKernelGGL((km_fallback_rows_kernel<TX, R>),
+                               dim3((unsigned)std::min<int64_t>(ceil_div(n_, R), 8192)),
+                               dim3(256), sm, st_, X_, Ct, d_, k_, flags, fcnt, labels, best);
+        } else if (d_ <= 8192) {
             double *Ct = ws_.get<double>("ctrans", k_ * d_);
             hipLaunchKernelGGL(km_transpose_kernel, dim3(grid1(k_ * d_, 256)), dim3(256), 0, st_,
                                C, k_, d_, Ct);

@@ -238,6 +238,125 @@ __global__ __launch_bounds__(256) void km_fallback_lds_kernel(
     }
 }
 
+// The same scan for R listed rows per block (R * d doubles staged in LDS):
+// each centre feature read from memory serves the R rows, and the R * 2
+// folds per thread are independent chains.  Per (row, centre) the fold, the
+// visiting order and the tie rule are those of km_fallback_lds_kernel.
+// Slots past the count take the first listed row of the group and are not
+// written.  The grid strides over the count in groups of R.
+template <typename TX, int R>
+__global__ __launch_bounds__(256) void km_fallback_rows_kernel(
+    const TX *__restrict__ X, const double *__restrict__ Ct, int d, int64_t k,
+    const int32_t *__restrict__ list, const int32_t *__restrict__ count,
+    int32_t *__restrict__ labels, double *__restrict__ best) {
+    extern __shared__ double xr[];   // R rows of d
+    __shared__ double wd[R][4];
+    __shared__ int64_t wl[R][4];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int64_t m = *count;
+    for (int64_t e0 = (int64_t)blockIdx.x * R; e0 < m; e0 += (int64_t)gridDim.x * R) {
+        int64_t rows[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) rows[r] = list[e0 + r < m ? e0 + r : e0];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            for (int j = t; j < d; j += 256) xr[r * d + j] = (double)X[rows[r] * d + j];
+        __syncthreads();
+        double bd[R];
+        int64_t bl[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            bd[r] = 1.7976931348623157e308;
+            bl[r] = -1;
+        }
+        int64_t c = t;
+        for (; c + 256 < k; c += 512) {
+            const double *c0 = Ct + c, *c1 = Ct + c + 256;
+            double a0[R], a1[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                a0[r] = 0.0;
+                a1[r] = 0.0;
+            }
+            for (int j = 0; j < d; ++j) {
+                const double y0 = c0[(int64_t)j * k], y1 = c1[(int64_t)j * k];
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const double x = xr[r * d + j];
+                    const double t0 = x - y0;
+                    const double t1 = x - y1;
+                    a0[r] = a0[r] + t0 * t0;
+                    a1[r] = a1[r] + t1 * t1;
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if (a0[r] < bd[r]) {
+                    bd[r] = a0[r];
+                    bl[r] = c;
+                }
+                if (a1[r] < bd[r]) {
+                    bd[r] = a1[r];
+                    bl[r] = c + 256;
+                }
+            }
+        }
+        for (; c < k; c += 256) {
+            const double *c0 = Ct + c;
+            double a0[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) a0[r] = 0.0;
+            for (int j = 0; j < d; ++j) {
+                const double y0 = c0[(int64_t)j * k];
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const double t0 = xr[r * d + j] - y0;
+                    a0[r] = a0[r] + t0 * t0;
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                if (a0[r] < bd[r]) {
+                    bd[r] = a0[r];
+                    bl[r] = c;
+                }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            for (int o = 32; o > 0; o >>= 1) {
+                const double od = __shfl_xor(bd[r], o);
+                const int64_t ol = __shfl_xor(bl[r], o);
+                if (ol >= 0 && (bl[r] < 0 || od < bd[r] || (od == bd[r] && ol < bl[r]))) {
+                    bd[r] = od;
+                    bl[r] = ol;
+                }
+            }
+            if (lane == 0) {
+                wd[r][w] = bd[r];
+                wl[r][w] = bl[r];
+            }
+        }
+        __syncthreads();
+        if (t < R && e0 + t < m) {
+            const int r = t;
+            double qd = wd[r][0];
+            int64_t ql = wl[r][0];
+            for (int v = 1; v < 4; ++v) {
+                const double od = wd[r][v];
+                const int64_t ol = wl[r][v];
+                if (ol >= 0 && (ql < 0 || od < qd || (od == qd && ol < ql))) {
+                    qd = od;
+                    ql = ol;
+                }
+            }
+            const int64_t i = list[e0 + r];
+            labels[i] = ql < 0 ? 0 : (int32_t)ql;
+            best[i] = qd;
+        }
+        __syncthreads();
+    }
+}
+
 // ------------------------------------------------------------- mini-batch --
 // mini_batch_kmeans.rs:381-404 (assign_batch): nearest centre of each batch
 // row, the same sequential scan; one wave per row.
