@@ -601,9 +601,10 @@ class Lloyd {
         double *pre = ws_.get<double>("kps_pre", n_);
         double *nd = ws_.get<double>("kps_nd", n_trials * n_);
         double *pre_t = ws_.get<double>("kps_pret", n_trials * n_);
-        double *thr = ws_.get<double>("kps_thr", n_trials);
         double *pot = ws_.get<double>("kps_pot", n_trials);
-        unsigned long long *cand = ws_.get<unsigned long long>("kps_cand", n_trials);
+        // candidates of odd and even centres (a centre's are set to n - 1
+        // while the previous centre's are still read)
+        unsigned long long *cand = ws_.get<unsigned long long>("kps_cand", 2 * n_trials);
         KppSeqCtl *ctl = ws_.get<KppSeqCtl>("kps_ctl", 1);
         double *r = ws_.get<double>("kps_r", (k_ - 1) * n_trials);
         // Every centre's draws, made now on the assumption that the
@@ -622,18 +623,18 @@ class Lloyd {
                            d_, out, md);
         left_fold(md, n_, n_, 1, nullptr, &ctl->total, pre, ws_, st_);
         const dim3 gn((unsigned)grid1(n_, 256), (unsigned)nt);
+        hipLaunchKernelGGL(kpp_fill_u64_kernel, dim3(1), dim3(64), 0, st_, cand + nt, (int64_t)nt,
+                           (unsigned long long)(n_ - 1));
         for (int64_t c = 1; c < k_; ++c) {
-            hipLaunchKernelGGL(kpp_seq_ctl_kernel, dim3(1), dim3(64), 0, st_, ctl,
-                               r + (c - 1) * nt, nt, (int)c, n_, thr, cand);
-            hipLaunchKernelGGL(kpp_seq_search_pre_kernel, gn, dim3(256), 0, st_, ctl, pre, n_,
-                               thr, cand);
+            unsigned long long *cc = cand + (c & 1) * nt, *cn = cand + ((c + 1) & 1) * nt;
+            hipLaunchKernelGGL(kpp_seq_threshold_kernel, gn, dim3(256), 0, st_, ctl,
+                               r + (c - 1) * nt, (int)c, pre, n_, cc);
             hipLaunchKernelGGL(kpp_seq_trials_kernel<TX>, dim3(grid1(n_, 256)), dim3(256), 0, st_,
-                               ctl, X_, n_, d_, nt, cand, md, nd);
+                               ctl, X_, n_, d_, nt, cc, md, nd);
             left_fold(nd, n_, n_, nt, &ctl->stop, pot, pre_t, ws_, st_);
-            hipLaunchKernelGGL(kpp_seq_select_kernel<TX>, dim3(1), dim3(64), 0, st_, ctl, pot, nt,
-                               cand, X_, d_, out + c * d_);
-            hipLaunchKernelGGL(kpp_seq_adopt_kernel, dim3(grid1(n_, 256)), dim3(256), 0, st_,
-                               ctl, n_, nd, pre_t, md, pre);
+            hipLaunchKernelGGL(kpp_seq_select_adopt_kernel<TX>, dim3(grid1(n_, 256)), dim3(256), 0,
+                               st_, ctl, pot, nt, cc, X_, d_, out + c * d_, n_, nd, pre_t, md, pre,
+                               cn);
             check_launch();
         }
         int32_t stop = 0;

@@ -739,39 +739,6 @@ struct KppSeqCtl {
     int32_t best;
 };
 
-// Centre c (>= 1): thresholds r_t * total for the nt trials, or stop.
-static __global__ __attribute__((unused)) void kpp_seq_ctl_kernel(KppSeqCtl *__restrict__ ctl,
-                                                                  const double *__restrict__ r,
-                                                                  int nt, int c, int64_t n,
-                                                                  double *__restrict__ thr,
-                                                                  unsigned long long *__restrict__ cand) {
-    if (threadIdx.x != 0 || blockIdx.x != 0 || ctl->stop) return;
-    const double total = ctl->total;
-    if (total <= 0.0) {   // kmeans.rs:201
-        ctl->stop = c;
-        return;
-    }
-    for (int t = 0; t < nt; ++t) {
-        thr[t] = r[t] * total;                     // kmeans.rs:209-210
-        cand[t] = (unsigned long long)(n - 1);     // :212, no index reaches it
-    }
-}
-
-// cand[t] = first i with pre[i] >= thr[t] (grid: ceil(n / 256) x nt).  The
-// running sums of non-negative distances never decrease until they turn NaN,
-// after which every comparison is false, so at most one i has pre[i] >=
-// thr[t] and not pre[i-1] >= thr[t]: that thread writes it (no index does
-// when the threshold is never reached, and cand keeps n - 1).
-static __global__ __attribute__((unused)) __launch_bounds__(256) void kpp_seq_search_pre_kernel(
-    const KppSeqCtl *__restrict__ ctl, const double *__restrict__ pre, int64_t n,
-    const double *__restrict__ thr, unsigned long long *__restrict__ cand) {
-    if (ctl->stop) return;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double th = thr[blockIdx.y];
-    if (pre[i] >= th && !(i > 0 && pre[i - 1] >= th)) cand[blockIdx.y] = (unsigned long long)i;
-}
-
 // nd[t*n + i] = min(md[i], fold(x_i, x_cand[t])) with `if d < nd`
 // (:219-226), every trial from one read of the row.
 template <typename TX>
@@ -804,12 +771,40 @@ __global__ __launch_bounds__(256) void kpp_seq_trials_kernel(
     }
 }
 
+// Centre c (>= 1), one launch: stop when the total is <= 0 (kmeans.rs:201;
+// every block returns, block 0 records c), else cand[t] = first i with
+// pre[i] >= r_t * total (:207-213).  The running sums of non-negative
+// distances never decrease until they turn NaN, after which every comparison
+// is false, so at most one i has pre[i] >= thr and not pre[i-1] >= thr: that
+// thread writes it.  cand holds n - 1 beforehand (no index reaches it).  Grid:
+// ceil(n / 256) x nt.
+static __global__ __attribute__((unused)) __launch_bounds__(256) void kpp_seq_threshold_kernel(
+    KppSeqCtl *__restrict__ ctl, const double *__restrict__ r, int c,
+    const double *__restrict__ pre, int64_t n, unsigned long long *__restrict__ cand) {
+    if (ctl->stop) return;
+    const double total = ctl->total;
+    if (total <= 0.0) {
+        if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) ctl->stop = c;
+        return;
+    }
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double th = r[blockIdx.y] * total;   // kmeans.rs:209-210
+    if (pre[i] >= th && !(i > 0 && pre[i - 1] >= th)) cand[blockIdx.y] = (unsigned long long)i;
+}
+
 // The trial with the strictly smallest potential, first trial first
-// (:231-235); its candidate becomes centre c.
+// (:231-235), found by every block; block 0 makes its candidate centre c,
+// keeps its potential as the next total and sets the next centre's
+// candidates to n - 1; every block adopts its distances and running sums
+// (md = nd[best], pre = pre_t[best], :239-241).  Grid: ceil(n / 256).
 template <typename TX>
-__global__ void kpp_seq_select_kernel(KppSeqCtl *__restrict__ ctl, const double *__restrict__ pot,
-                                      int nt, const unsigned long long *__restrict__ cand,
-                                      const TX *__restrict__ X, int d, double *__restrict__ centre) {
+__global__ __launch_bounds__(256) void kpp_seq_select_adopt_kernel(
+    KppSeqCtl *__restrict__ ctl, const double *__restrict__ pot, int nt,
+    const unsigned long long *__restrict__ cand, const TX *__restrict__ X, int d,
+    double *__restrict__ centre, int64_t n, const double *__restrict__ nd,
+    const double *__restrict__ pre_t, double *__restrict__ md, double *__restrict__ pre,
+    unsigned long long *__restrict__ cand_next) {
     if (ctl->stop) return;
     __shared__ int bt;
     if (threadIdx.x == 0) {
@@ -817,84 +812,32 @@ __global__ void kpp_seq_select_kernel(KppSeqCtl *__restrict__ ctl, const double 
         for (int t = 1; t < nt; ++t)
             if (pot[t] < pot[b]) b = t;
         bt = b;
-        ctl->best = b;
-        ctl->total = pot[b];
     }
     __syncthreads();
-    const int64_t row = (int64_t)cand[bt];
-    for (int j = threadIdx.x; j < d; j += blockDim.x) centre[j] = (double)X[row * d + j];
-}
-
-// md = nd[best], pre = the best trial's running sums (:239-241).
-static __global__ __attribute__((unused)) __launch_bounds__(256) void kpp_seq_adopt_kernel(
-    const KppSeqCtl *__restrict__ ctl, int64_t n, const double *__restrict__ nd,
-    const double *__restrict__ pre_t, double *__restrict__ md, double *__restrict__ pre) {
-    if (ctl->stop) return;
+    const int b = bt;
+    if (blockIdx.x == 0) {
+        if (threadIdx.x == 0) {
+            ctl->best = b;
+            ctl->total = pot[b];
+        }
+        const int64_t row = (int64_t)cand[b];
+        for (int j = threadIdx.x; j < d; j += blockDim.x) centre[j] = (double)X[row * d + j];
+        for (int t = threadIdx.x; t < nt; t += blockDim.x)
+            cand_next[t] = (unsigned long long)(n - 1);
+    }
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int64_t o = (int64_t)ctl->best * n + i;
+    const int64_t o = (int64_t)b * n + i;
     md[i] = nd[o];
     pre[i] = pre_t[o];
 }
 
-// k-means++ with the reference's plain left folds (kmeans.rs:200-228): one
-// thread per trial.  Loads are issued 32 ahead of the running sum, which
-// still adds in index order.
-//
-// out[t] = left fold of v[t*n .. t*n + n) for t < nt.
-static __global__ __attribute__((unused)) void seq_folds_kernel(const double *__restrict__ v,
-                                                                int64_t n, int nt,
-                                                                double *__restrict__ out) {
-    const int t = threadIdx.x;
-    if (blockIdx.x != 0 || t >= nt) return;
-    const double *a = v + (int64_t)t * n;
-    double s = 0.0;
-    int64_t i = 0;
-    for (; i + 32 <= n; i += 32) {
-        double b[32];
-#pragma unroll
-        for (int u = 0; u < 32; ++u) b[u] = a[i + u];
-#pragma unroll
-        for (int u = 0; u < 32; ++u) s = s + b[u];
-    }
-    for (; i < n; ++i) s = s + a[i];
-    out[t] = s;
-}
-
-// cand[t] = first i whose running left fold of md reaches thr[t], else n-1.
-static __global__ __attribute__((unused)) void kpp_seq_search_kernel(
-    const double *__restrict__ md, int64_t n, const double *__restrict__ thr, int nt,
-    int64_t *__restrict__ cand) {
-    const int t = threadIdx.x;
-    if (blockIdx.x != 0 || t >= nt) return;
-    const double th = thr[t];
-    double c = 0.0;
-    int64_t found = n - 1;
-    int64_t i = 0;
-    bool done = false;
-    for (; i + 32 <= n && !done; i += 32) {
-        double b[32];
-#pragma unroll
-        for (int u = 0; u < 32; ++u) b[u] = md[i + u];
-#pragma unroll
-        for (int u = 0; u < 32; ++u) {   // no break: b stays in registers
-            if (!done) {
-                c = c + b[u];
-                if (c >= th) {
-                    found = i + u;
-                    done = true;
-                }
-            }
-        }
-    }
-    for (; i < n && !done; ++i) {
-        c = c + md[i];
-        if (c >= th) {
-            found = i;
-            done = true;
-        }
-    }
-    cand[t] = found;
+// v[0..m) = value.
+static __global__ __attribute__((unused)) void kpp_fill_u64_kernel(unsigned long long *__restrict__ v,
+                                                                    int64_t m,
+                                                                    unsigned long long value) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) v[i] = value;
 }
 
 // nd[t*n + i] = min(md[i], fold(x_i, x_cand[t])) for every trial t (grid.y).
