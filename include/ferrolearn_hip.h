@@ -622,8 +622,10 @@ void fl_router_free(fl_router_handle r);
 /* The reference's serial sums -- k-means++'s totals, threshold searches and
  * potentials (kmeans.rs:200,207-213,219-228) and MiniBatch's inertia
  * (mini_batch_kmeans.rs:366-373) are `fold(0, |acc, x| acc + x)` in index
- * order -- computed by one workgroup per sequence, bit for bit the serial
- * result.  count sequences of n device values, sequence b at d_v + b*ld:
+ * order -- bit for bit the serial result: sequences under 8192 values by one
+ * workgroup each, longer ones in 2048-value blocks over many compute units
+ * with an exact walk over the blocks.  count sequences of n device values,
+ * sequence b at d_v + b*ld:
  * d_sums[b] = the fold, and when d_prefix is not NULL, d_prefix[b*ld + i] =
  * the running sum after element i. */
 fl_status fl_left_fold_device(const double *d_v, int64_t n, int64_t count, int64_t ld,
